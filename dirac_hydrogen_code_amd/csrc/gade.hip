@@ -1,0 +1,1448 @@
+// libgade: MI355X-native Crank-Nicolson hot path (fused operator, block-LU
+// preconditioner, device-resident BiCGSTAB).  See include/gade.h for the ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gade_internal.h"
+
+using gade::kChunk;
+using gade::kSoL;
+
+namespace gade {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace gade
+
+extern "C" const char *gade_last_error(void) { return gade::g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 = (re, im))
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// acc + a*b
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+    return acc;
+}
+// acc + conj(a)*b
+__device__ __forceinline__ double2 cfmaconj(double2 a, double2 b, double2 acc) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(-a.y, b.x, acc.y);
+    return acc;
+}
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ bool cnz(double2 a) { return a.x != 0.0 || a.y != 0.0; }
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+// streaming (read-once) load of a complex value
+__device__ __forceinline__ double2 ldnt(const double2 *p) {
+    dvec2 v = __builtin_nontemporal_load((const dvec2 *)p);
+    return make_double2(v.x, v.y);
+}
+
+// ---------------------------------------------------------------------------
+// profiling helper
+// ---------------------------------------------------------------------------
+namespace {
+struct Prof {
+    gade_model *m;
+    const char *name;
+    hipEvent_t a{}, b{};
+    Prof(gade_model *mm, const char *nm) : m(mm), name(nm) {
+        if (m->profile) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, m->stream);
+        }
+    }
+    ~Prof() {
+        if (m->profile) {
+            (void)hipEventRecord(b, m->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            auto &t = m->timers[name];
+            t.total_ms += ms;
+            t.count += 1;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+};
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// kernels: per-step G-sums and stationary diagonal operators
+// ---------------------------------------------------------------------------
+struct Coef6 {
+    double2 c[6];
+};
+
+// GE[l][m][s][a]: m<4: Gs_{l,m} = sum_alpha c_alpha g^m_{alpha,l};
+// m>=4: Gts_{l,m-4} with gt = conj-transpose of g (tools/solver.c:236-287).
+__global__ void k_gsums(const double2 *__restrict__ g, const int *__restrict__ ell_src, const int *__restrict__ tmap,
+                        Coef6 cf, int lmax, int ellw, int n, int nnzb, double2 *__restrict__ GE) {
+    long long total = (long long)lmax * 8 * ellw * n;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        int sa = (int)(idx % ((long long)ellw * n));
+        int lm = (int)(idx / ((long long)ellw * n));
+        int m = lm & 7, l = lm >> 3;
+        int src = ell_src[sa];
+        double2 acc = make_double2(0, 0);
+        if (src >= 0) {
+            int nn = m & 3;
+            bool tr = m >= 4;
+            int k = tr ? tmap[src] : src;
+            for (int a = (l % 2 == 0) ? 1 : 0; a < 6; a += 2) {
+                double2 gv = g[(size_t)(a * 4 * lmax + l * 4 + nn) * nnzb + k];
+                if (tr) gv.y = -gv.y;
+                acc = cadd(acc, cmul(cf.c[a], gv));
+            }
+        }
+        GE[idx] = acc;
+    }
+}
+
+// D[sel][q][s][a]: sel 0: S(k)+ihdt H0(k), 1: S(k)-ihdt H0(k), 2: S(k)  (tools/solver.c:409-458)
+__global__ void k_build_D(const double2 *__restrict__ s, const double2 *__restrict__ h0, const int *__restrict__ ell_src,
+                          const int *__restrict__ kq, int nq, int ellw, int n, int nnzb, double hdt2,
+                          double2 *__restrict__ D) {
+    long long total = (long long)nq * ellw * n;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        int sa = (int)(idx % ((long long)ellw * n));
+        int q = (int)(idx / ((long long)ellw * n));
+        int src = ell_src[sa];
+        double2 S = make_double2(0, 0), H = make_double2(0, 0);
+        if (src >= 0) {
+            double k = kq[q];
+            S = cadd(cadd(s[src], cscale(s[nnzb + src], k)), cscale(s[2 * nnzb + src], k * k));
+            H = cadd(cadd(cadd(h0[src], cscale(h0[nnzb + src], k)), cscale(h0[2 * nnzb + src], k * k)),
+                     cscale(h0[3 * nnzb + src], k * k * k));
+        }
+        // i*hdt2*H
+        double2 iH = make_double2(-hdt2 * H.y, hdt2 * H.x);
+        D[((long long)0 * nq * ellw * n) + idx] = cadd(S, iH);
+        D[((long long)1 * nq * ellw * n) + idx] = csub(S, iH);
+        D[((long long)2 * nq * ellw * n) + idx] = S;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused operator, phase A: angular contraction per row block
+//   Z0_l(r) = sum_c H0l[r,c] x_c      Z2_l(r) = sum_c H0l[r,c] k_c x_c
+//   W0_l(r) = sum_c H1l[r,c] x_c      W1_l(r) = sum_c H1l[r,c] k_c x_c
+// V layout [l][v][r][a]
+// ---------------------------------------------------------------------------
+template <int LM>
+__global__ __launch_bounds__(256) void k_angular(const double2 *__restrict__ x, const int *__restrict__ hAp,
+                                                 const int *__restrict__ hAi, const double2 *__restrict__ coef,
+                                                 const int *__restrict__ kap, int nblk, int n,
+                                                 double2 *__restrict__ V) {
+    const int r = blockIdx.x;
+    const int a = blockIdx.y * blockDim.x + threadIdx.x;
+    double2 z0[LM], z2[LM], w0[LM], w1[LM];
+#pragma unroll
+    for (int l = 0; l < LM; l++) {
+        z0[l] = z2[l] = w0[l] = w1[l] = make_double2(0, 0);
+    }
+    const int pb = hAp[r], pe = hAp[r + 1];
+    for (int p = pb; p < pe; p++) {
+        const int c = hAi[p];
+        const double kc = (double)kap[c];
+        double2 xv = (a < n) ? x[(size_t)c * n + a] : make_double2(0, 0);
+        double2 kx = cscale(xv, kc);
+        const double2 *cp = coef + (size_t)p * LM * 2;
+#pragma unroll
+        for (int l = 0; l < LM; l++) {
+            double2 h0 = cp[2 * l], h1 = cp[2 * l + 1];
+            if (cnz(h0)) {
+                z0[l] = cfma(h0, xv, z0[l]);
+                z2[l] = cfma(h0, kx, z2[l]);
+            }
+            if (cnz(h1)) {
+                w0[l] = cfma(h1, xv, w0[l]);
+                w1[l] = cfma(h1, kx, w1[l]);
+            }
+        }
+    }
+    if (a >= n) return;
+    const size_t plane = (size_t)nblk * n;
+#pragma unroll
+    for (int l = 0; l < LM; l++) {
+        size_t base = (size_t)(l * 4) * plane + (size_t)r * n + a;
+        V[base] = z0[l];
+        V[base + plane] = z2[l];
+        V[base + 2 * plane] = w0[l];
+        V[base + 3 * plane] = w1[l];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused operator, phase B: radial SpMM on a kappa-grouped tile of row blocks
+//   y_r = D_sel(k) x_r + cH * sum_l [ (Gs0+k Gs1) Z0 + (Gs2+k Gs3) Z2 + (Gts0+k Gts2) W0 + (Gts1+k Gts3) W1 ]
+//         + beta y_r
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, const double2 *__restrict__ V,
+                                               const double2 *__restrict__ GE, const double2 *__restrict__ D,
+                                               const int *__restrict__ ell_col, const int *__restrict__ tiles,
+                                               const int *__restrict__ tile_q, const double *__restrict__ tile_k,
+                                               int nblk, int n, int ellw, int lmax, int nq, int sel, double cH,
+                                               double beta, double2 *__restrict__ y) {
+    const int tile = blockIdx.x;
+    const int a = blockIdx.y * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    int rows[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) rows[i] = tiles[tile * R + i];
+    const double k = tile_k[tile];
+    const int q = tile_q[tile];
+    double2 acc[R], dacc[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) acc[i] = dacc[i] = make_double2(0, 0);
+    const size_t plane = (size_t)nblk * n;
+    const size_t slab = (size_t)ellw * n;
+    if (cH != 0.0) {
+        for (int l = 0; l < lmax; l++) {
+            const double2 *G = GE + (size_t)(l * 8) * slab;
+            const double2 *Vl = V + (size_t)(l * 4) * plane;
+            for (int s = 0; s < ellw; s++) {
+                const size_t e = (size_t)s * n + a;
+                const int j = ell_col[e];
+                double2 g0 = G[e], g1 = G[slab + e], g2 = G[2 * slab + e], g3 = G[3 * slab + e];
+                double2 t0 = G[4 * slab + e], t1 = G[5 * slab + e], t2 = G[6 * slab + e], t3 = G[7 * slab + e];
+                double2 M0 = cadd(g0, cscale(g1, k)), M1 = cadd(g2, cscale(g3, k));
+                double2 M2 = cadd(t0, cscale(t2, k)), M3 = cadd(t1, cscale(t3, k));
+#pragma unroll
+                for (int i = 0; i < R; i++) {
+                    if (rows[i] < 0) continue;
+                    const size_t vb = (size_t)rows[i] * n + j;
+                    double2 acc_i = acc[i];
+                    acc_i = cfma(M0, Vl[vb], acc_i);
+                    acc_i = cfma(M1, Vl[plane + vb], acc_i);
+                    acc_i = cfma(M2, Vl[2 * plane + vb], acc_i);
+                    acc_i = cfma(M3, Vl[3 * plane + vb], acc_i);
+                    acc[i] = acc_i;
+                }
+            }
+        }
+    }
+    const double2 *Dq = D + ((size_t)sel * nq + q) * slab;
+    for (int s = 0; s < ellw; s++) {
+        const size_t e = (size_t)s * n + a;
+        const int j = ell_col[e];
+        double2 d = Dq[e];
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            if (rows[i] < 0) continue;
+            dacc[i] = cfma(d, x[(size_t)rows[i] * n + j], dacc[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (rows[i] < 0) continue;
+        const size_t o = (size_t)rows[i] * n + a;
+        double2 res = cadd(dacc[i], cscale(acc[i], cH));
+        if (beta != 0.0) res = cadd(res, cscale(y[o], beta));
+        y[o] = res;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// preconditioner: banded LU per distinct kappa (interleaved radial order)
+// ---------------------------------------------------------------------------
+// AB[q][i][w], w = 2*band+1, A(i,j) at AB[i][j-i+band].  No pivoting: P has a
+// positive-definite Hermitian part (S), so the factorisation exists.
+__global__ __launch_bounds__(256) void k_band_lu(double2 *AB, int n, int band, int *flag) {
+    const int q = blockIdx.x;
+    const int w = 2 * band + 1;
+    double2 *A = AB + (size_t)q * n * w;
+    for (int k = 0; k < n; k++) {
+        const double2 piv = A[(size_t)k * w + band];
+        if (threadIdx.x == 0 && !cnz(piv)) atomicOr(flag, 1);
+        const int iend = min(n - 1, k + band);
+        for (int t = threadIdx.x; t < iend - k; t += blockDim.x) {
+            const int i = k + 1 + t;
+            double2 *lik = A + (size_t)i * w + (k - i + band);
+            *lik = cdiv(*lik, piv);
+        }
+        __syncthreads();
+        const int m = iend - k;
+        for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+            const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+            const double2 lik = A[(size_t)i * w + (k - i + band)];
+            const double2 ukj = A[(size_t)k * w + (j - k + band)];
+            double2 *aij = A + (size_t)i * w + (j - i + band);
+            *aij = csub(*aij, cmul(lik, ukj));
+        }
+        __syncthreads();
+    }
+}
+
+// Per (kappa, chunk): inverses of the 64x64 diagonal chunks of L (unit) and U,
+// stored [col][row], and the band coupling to the neighbouring chunk.
+__global__ __launch_bounds__(64) void k_chunk_inverse(const double2 *AB, int n, int band, int nch, double2 *Linv,
+                                                      double2 *Uinv, double2 *Loff, double2 *Uoff) {
+    const int q = blockIdx.x, ch = blockIdx.y, j = threadIdx.x;
+    const int w = 2 * band + 1;
+    const double2 *A = AB + (size_t)q * n * w;
+    const int base = ch * kChunk;
+    double2 *Li = Linv + ((size_t)q * nch + ch) * kChunk * kChunk + (size_t)j * kChunk;
+    double2 *Ui = Uinv + ((size_t)q * nch + ch) * kChunk * kChunk + (size_t)j * kChunk;
+    auto at = [&](int gi, int gj) -> double2 {  // band entry (gi, gj), zero outside
+        if (gi < 0 || gj < 0 || gi >= n || gj >= n || abs(gi - gj) > band) return make_double2(0, 0);
+        return A[(size_t)gi * w + (gj - gi + band)];
+    };
+    // column j of L_cc^{-1}: forward substitution (rows beyond n act as identity)
+    for (int i = 0; i < kChunk; i++) {
+        double2 v = make_double2(i == j ? 1.0 : 0.0, 0);
+        if (i > j && base + i < n) {
+            for (int m2 = max(j, i - band); m2 < i; m2++) v = csub(v, cmul(at(base + i, base + m2), Li[m2]));
+        }
+        Li[i] = (i < j) ? make_double2(0, 0) : v;
+    }
+    // column j of U_cc^{-1}: back substitution
+    for (int i = kChunk - 1; i >= 0; i--) {
+        double2 v = make_double2(0, 0);
+        if (i <= j) {
+            if (base + i >= n) {
+                v = make_double2(i == j ? 1.0 : 0.0, 0);
+            } else {
+                v = make_double2(i == j ? 1.0 : 0.0, 0);
+                for (int m2 = i + 1; m2 <= min(j, i + band); m2++) v = csub(v, cmul(at(base + i, base + m2), Ui[m2]));
+                v = cdiv(v, at(base + i, base + i));
+            }
+        }
+        Ui[i] = v;
+    }
+    // band couplings: lane j is a row of the chunk
+    const int gi = base + j;
+    double2 *Lo = Loff + ((size_t)q * nch + ch) * band * kChunk;
+    double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * kChunk;
+    for (int t = 0; t < band; t++) {
+        Lo[(size_t)t * kChunk + j] = (gi < n) ? at(gi, base - band + t) : make_double2(0, 0);
+        Uo[(size_t)t * kChunk + j] = (gi < n) ? at(gi, base + kChunk + t) : make_double2(0, 0);
+    }
+}
+
+// x = P^{-1} b per row block: one wave per block, chunk-level scheduling.
+__global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, double2 *__restrict__ xo,
+                                               const int *__restrict__ blk_q, const int *__restrict__ ilv_inv,
+                                               const double2 *__restrict__ Linv, const double2 *__restrict__ Uinv,
+                                               const double2 *__restrict__ Loff, const double2 *__restrict__ Uoff,
+                                               int n, int band, int nch) {
+    extern __shared__ double2 smem[];
+    double2 *yv = smem;                   // nch*64
+    double2 *acc = smem + nch * kChunk;   // 64
+    const int r = blockIdx.x, lane = threadIdx.x;
+    const int q = blk_q[r];
+    const size_t cs = (size_t)kChunk * kChunk;
+    const double2 *bb = b + (size_t)r * n;
+    // forward sweep
+    for (int ch = 0; ch < nch; ch++) {
+        const int gi = ch * kChunk + lane;
+        double2 v = (gi < n) ? bb[ilv_inv[gi]] : make_double2(0, 0);
+        if (ch > 0) {
+            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * kChunk;
+            for (int t = 0; t < band; t++) v = csub(v, cmul(Lo[(size_t)t * kChunk + lane], yv[ch * kChunk - band + t]));
+        }
+        acc[lane] = v;
+        __syncthreads();
+        const double2 *Li = Linv + ((size_t)q * nch + ch) * cs;
+        double2 yi = make_double2(0, 0);
+        for (int j = 0; j <= lane; j++) yi = cfma(Li[(size_t)j * kChunk + lane], acc[j], yi);
+        yv[ch * kChunk + lane] = yi;
+        __syncthreads();
+    }
+    // backward sweep (in place)
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int gi = ch * kChunk + lane;
+        double2 v = yv[gi];
+        if (ch < nch - 1) {
+            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * kChunk;
+            for (int t = 0; t < band; t++)
+                v = csub(v, cmul(Uo[(size_t)t * kChunk + lane], yv[(ch + 1) * kChunk + t]));
+        }
+        acc[lane] = v;
+        __syncthreads();
+        const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs;
+        double2 xi = make_double2(0, 0);
+        for (int j = lane; j < kChunk; j++) xi = cfma(Ui[(size_t)j * kChunk + lane], acc[j], xi);
+        yv[gi] = xi;
+        __syncthreads();
+    }
+    for (int gi = lane; gi < n; gi += kChunk) xo[(size_t)r * n + ilv_inv[gi]] = yv[gi];
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB vector kernels with deterministic last-block reductions
+// ---------------------------------------------------------------------------
+// scalar slots in d_sc
+enum {
+    SC_RHO = 0,        // 2
+    SC_ALPHA = 2,      // 2
+    SC_W = 4,          // 2
+    SC_R0SQ = 6,
+    SC_RHSSQ = 7,
+    SC_RNORM2 = 8,
+    SC_RHONEXT = 9,    // 2
+    SC_COUNT = 12
+};
+enum RedOp { OP_INIT = 0, OP_RESTART, OP_P, OP_ALPHA, OP_S, OP_W, OP_XR };
+
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-reduce NV values; thread 0 returns the sums in out[].
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; i++) v[i] = wave_sum(v[i]);
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < NV; i++) sh[wid * NV + i] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            double s = 0;
+            for (int w2 = 0; w2 < nw; w2++) s += sh[w2 * NV + i];
+            v[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <int OP>
+__global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2 *__restrict__ b, double2 *__restrict__ r,
+                                                       double2 *__restrict__ r0, double2 *__restrict__ p,
+                                                       const double2 *__restrict__ v, double2 *__restrict__ s,
+                                                       const double2 *__restrict__ t, const double2 *__restrict__ yv,
+                                                       const double2 *__restrict__ zv, double2 *__restrict__ x,
+                                                       double *sc, double *part, unsigned *cnt) {
+    __shared__ double sh[4 * 16];
+    __shared__ bool last;
+    double acc[4] = {0, 0, 0, 0};
+    double2 beta = make_double2(0, 0), w = make_double2(0, 0), alpha = make_double2(0, 0);
+    if (OP == OP_P) {
+        // beta = (rho_new / rho_old) * (alpha / w)
+        double2 rho_new = make_double2(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
+        double2 rho_old = make_double2(sc[SC_RHO], sc[SC_RHO + 1]);
+        alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
+        w = make_double2(sc[SC_W], sc[SC_W + 1]);
+        beta = cmul(cdiv(rho_new, rho_old), cdiv(alpha, w));
+    }
+    if (OP == OP_S || OP == OP_XR) alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
+    if (OP == OP_XR) w = make_double2(sc[SC_W], sc[SC_W + 1]);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        if (OP == OP_INIT || OP == OP_RESTART) {
+            double2 ri = csub(b[i], r[i]);
+            r[i] = ri;
+            r0[i] = ri;
+            acc[0] = fma(ri.x, ri.x, fma(ri.y, ri.y, acc[0]));
+            if (OP == OP_INIT) {
+                double2 bi = b[i];
+                acc[1] = fma(bi.x, bi.x, fma(bi.y, bi.y, acc[1]));
+            }
+        } else if (OP == OP_P) {
+            double2 pi = csub(p[i], cmul(w, v[i]));
+            p[i] = cadd(r[i], cmul(beta, pi));
+        } else if (OP == OP_ALPHA) {
+            double2 d = cfmaconj(r0[i], v[i], make_double2(acc[0], acc[1]));
+            acc[0] = d.x;
+            acc[1] = d.y;
+        } else if (OP == OP_S) {
+            s[i] = csub(r[i], cmul(alpha, v[i]));
+        } else if (OP == OP_W) {
+            double2 ti = t[i];
+            acc[0] = fma(ti.x, ti.x, fma(ti.y, ti.y, acc[0]));
+            double2 d = cfmaconj(ti, s[i], make_double2(acc[1], acc[2]));
+            acc[1] = d.x;
+            acc[2] = d.y;
+        } else if (OP == OP_XR) {
+            double2 xi = x[i];
+            xi = cadd(xi, cadd(cmul(alpha, yv[i]), cmul(w, zv[i])));
+            x[i] = xi;
+            double2 ri = csub(s[i], cmul(w, t[i]));
+            r[i] = ri;
+            acc[0] = fma(ri.x, ri.x, fma(ri.y, ri.y, acc[0]));
+            double2 d = cfmaconj(r0[i], ri, make_double2(acc[1], acc[2]));
+            acc[1] = d.x;
+            acc[2] = d.y;
+        }
+    }
+    if (OP == OP_P || OP == OP_S) return;
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < 4; i++)
+            __hip_atomic_store(part + (size_t)blockIdx.x * 4 + i, acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned ticket = atomicAdd(cnt, 1u);
+        last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double tot[4] = {0, 0, 0, 0};
+    for (int bidx = threadIdx.x; bidx < (int)gridDim.x; bidx += blockDim.x)
+        for (int i = 0; i < 4; i++)
+            tot[i] += __hip_atomic_load(part + (size_t)bidx * 4 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sum<4>(tot, sh);
+    if (threadIdx.x == 0) {
+        if (OP == OP_INIT) {
+            sc[SC_R0SQ] = tot[0];
+            sc[SC_RHSSQ] = tot[1];
+            sc[SC_RNORM2] = tot[0];
+            sc[SC_RHONEXT] = tot[0];
+            sc[SC_RHONEXT + 1] = 0;
+            sc[SC_RHO] = 1; sc[SC_RHO + 1] = 0;
+            sc[SC_ALPHA] = 1; sc[SC_ALPHA + 1] = 0;
+            sc[SC_W] = 1; sc[SC_W + 1] = 0;
+        } else if (OP == OP_RESTART) {
+            sc[SC_R0SQ] = tot[0];
+            sc[SC_RNORM2] = tot[0];
+            sc[SC_RHONEXT] = tot[0];
+            sc[SC_RHONEXT + 1] = 0;
+        } else if (OP == OP_ALPHA) {
+            double2 rho = make_double2(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
+            double2 al = cdiv(rho, make_double2(tot[0], tot[1]));
+            sc[SC_ALPHA] = al.x;
+            sc[SC_ALPHA + 1] = al.y;
+            sc[SC_RHO] = rho.x;
+            sc[SC_RHO + 1] = rho.y;
+        } else if (OP == OP_W) {
+            double2 ww = make_double2(0, 0);
+            if (tot[0] > 0) ww = make_double2(tot[1] / tot[0], tot[2] / tot[0]);
+            sc[SC_W] = ww.x;
+            sc[SC_W + 1] = ww.y;
+        } else if (OP == OP_XR) {
+            sc[SC_RNORM2] = tot[0];
+            sc[SC_RHONEXT] = tot[1];
+            sc[SC_RHONEXT + 1] = tot[2];
+        }
+        *cnt = 0;
+    }
+}
+
+// per-block S-norm contributions pop[r] = x_r^H S(k_r) x_r
+__global__ __launch_bounds__(256) void k_snorm(const double2 *__restrict__ x, const double2 *__restrict__ D,
+                                               const int *__restrict__ ell_col, const int *__restrict__ blk_q,
+                                               int n, int ellw, int nq, double *__restrict__ pop) {
+    __shared__ double sh[4 * 16];
+    const int r = blockIdx.x;
+    const double2 *Dq = D + ((size_t)2 * nq + blk_q[r]) * ellw * n;
+    const double2 *xr = x + (size_t)r * n;
+    double acc[1] = {0};
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        double2 sx = make_double2(0, 0);
+        for (int s = 0; s < ellw; s++) sx = cfma(Dq[(size_t)s * n + a], xr[ell_col[(size_t)s * n + a]], sx);
+        double2 xa = xr[a];
+        acc[0] += xa.x * sx.x + xa.y * sx.y;  // Re(conj(x) Sx); S Hermitian
+    }
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) pop[r] = acc[0];
+}
+
+// ---------------------------------------------------------------------------
+// assembled reference formulation
+// ---------------------------------------------------------------------------
+// Hfull row (R, a) has hall_rowlen(R) * sub_rowlen(a) entries ordered by (block col, sub col):
+// Ap = hAp[R]*nnzb + (hAp[R+1]-hAp[R]) * subAp[a]  (closed form of tools/csr.c:653-700)
+__global__ void k_full_Ap(const int *hAp, const int *subAp, int nblk, int n, int nnzb, int *fAp) {
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long N = (long long)nblk * n;
+    if (i > N) return;
+    if (i == N) { fAp[N] = hAp[nblk] * nnzb; return; }
+    int R = (int)(i / n), a = (int)(i % n);
+    fAp[i] = hAp[R] * nnzb + (hAp[R + 1] - hAp[R]) * subAp[a];
+}
+
+__global__ void k_full_Ai(const int *hAp, const int *hAi, const int *subAp, const int *subAi, const int *fAp,
+                          int nblk, int n, int *fAi) {
+    // one block per Hfull row
+    long long row = blockIdx.x + (long long)blockIdx.y * gridDim.x;
+    if (row >= (long long)nblk * n) return;
+    int R = (int)(row / n), a = (int)(row % n);
+    int len = subAp[a + 1] - subAp[a];
+    int nb = hAp[R + 1] - hAp[R];
+    int base = fAp[row];
+    for (int e = threadIdx.x; e < nb * len; e += blockDim.x) {
+        int pos = e / len, t = e % len;
+        fAi[base + e] = hAi[hAp[R] + pos] * n + subAi[subAp[a] + t];
+    }
+}
+
+// Ai_sub_map[p*nnzb + k] = slot of (Hall nz p, sub entry k) in the expanded CSR (tools/csr.c:702-739)
+__global__ void k_full_submap(const int *hAp, const int *subAp, const int *fAp, int nblk, int n, int nnzb,
+                              int *rowof, int *submap) {
+    // one block per Hall nz p; rowof[p] = block row of p
+    int p = blockIdx.x;
+    int R = rowof[p];
+    int pos = p - hAp[R];
+    for (int k = threadIdx.x; k < nnzb; k += blockDim.x) {
+        // row a of sub entry k: binary search in subAp
+        int lo = 0, hi = n;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (subAp[mid] <= k) lo = mid; else hi = mid;
+        }
+        int a = lo, len = subAp[a + 1] - subAp[a];
+        submap[(size_t)p * nnzb + k] = fAp[(size_t)R * n + a] + pos * len + (k - subAp[a]);
+    }
+}
+
+// Hfull.Ax in row order: entry e of row (R, a) -> Hall nz p = hAp[R] + e/len, sub entry k
+// (tools/gpu_solver.cu:84-135 evaluated per destination so the stores are coalesced)
+template <int LM>
+__global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ hAp, const int *__restrict__ hAi,
+                                                  const int *__restrict__ kap, const double2 *__restrict__ coef,
+                                                  const int *__restrict__ diag, const int *__restrict__ subAp,
+                                                  const int *__restrict__ ell_pos, const double2 *__restrict__ GE,
+                                                  const double2 *__restrict__ D, const int *__restrict__ blk_q,
+                                                  const int *__restrict__ fAp, int nblk, int n, int ellw, int nq,
+                                                  double2 *__restrict__ fAx) {
+    long long row = blockIdx.x + (long long)blockIdx.y * gridDim.x;
+    if (row >= (long long)nblk * n) return;
+    const int R = (int)(row / n), a = (int)(row % n);
+    const int len = subAp[a + 1] - subAp[a];
+    const int nb = hAp[R + 1] - hAp[R];
+    const double ki = kap[R];
+    const size_t slab = (size_t)ellw * n;
+    const double2 *Dq = D + ((size_t)0 * nq + blk_q[R]) * slab;  // S + ihdt H0: take the H0 part below
+    const double2 *Sq = D + ((size_t)2 * nq + blk_q[R]) * slab;
+    const int base = fAp[row];
+    for (int e = threadIdx.x; e < nb * len; e += blockDim.x) {
+        const int pos = e / len, t = e % len;
+        const int p = hAp[R] + pos;
+        const double kj = kap[hAi[p]];
+        const int k = subAp[a] + t;
+        const size_t ep = (size_t)ell_pos[k];
+        double2 val = make_double2(0, 0);
+        if (diag[p]) val = csub(Dq[ep], Sq[ep]);  // ihdt*(h00 + k h01 + k^2 h02 + k^3 h03)
+        const double2 *cp = coef + (size_t)p * LM * 2;
+#pragma unroll
+        for (int l = 0; l < LM; l++) {
+            const double2 *G = GE + (size_t)(l * 8) * slab + ep;
+            double2 h0 = cp[2 * l], h1 = cp[2 * l + 1];
+            if (cnz(h0)) {
+                double2 g = cadd(cadd(G[0], cscale(G[slab], ki)), cadd(cscale(G[2 * slab], kj), cscale(G[3 * slab], ki * kj)));
+                val = cfma(h0, g, val);
+            }
+            if (cnz(h1)) {
+                double2 g = cadd(cadd(G[4 * slab], cscale(G[5 * slab], kj)), cadd(cscale(G[6 * slab], ki), cscale(G[7 * slab], ki * kj)));
+                val = cfma(h1, g, val);
+            }
+        }
+        fAx[base + e] = val;
+    }
+}
+
+// y = alpha A x + beta y for a CSR matrix: one wave per row, streaming loads.
+__global__ __launch_bounds__(256) void k_csr_spmv(const int *__restrict__ Ap, const int *__restrict__ Ai,
+                                                  const double2 *__restrict__ Ax, const double2 *__restrict__ x,
+                                                  long long nrows, double2 alpha, double2 beta,
+                                                  double2 *__restrict__ y) {
+    const long long row = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (row >= nrows) return;
+    const int b = Ap[row], e = Ap[row + 1];
+    double2 acc0 = make_double2(0, 0), acc1 = make_double2(0, 0);
+    int i = b + lane;
+    for (; i + 64 < e; i += 128) {
+        double2 a0 = ldnt(Ax + i);
+        int c0 = __builtin_nontemporal_load(Ai + i);
+        double2 a1 = ldnt(Ax + i + 64);
+        int c1 = __builtin_nontemporal_load(Ai + i + 64);
+        acc0 = cfma(a0, x[c0], acc0);
+        acc1 = cfma(a1, x[c1], acc1);
+    }
+    if (i < e) {
+        double2 a0 = ldnt(Ax + i);
+        int c0 = __builtin_nontemporal_load(Ai + i);
+        acc0 = cfma(a0, x[c0], acc0);
+    }
+    double re = wave_sum(acc0.x + acc1.x), im = wave_sum(acc0.y + acc1.y);
+    if (lane == 0) {
+        double2 res = cmul(alpha, make_double2(re, im));
+        if (beta.x != 0.0 || beta.y != 0.0) res = cadd(res, cmul(beta, y[row]));
+        y[row] = res;
+    }
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace {
+
+int hip_alloc(void **p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        gade::set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return GADE_ENOMEM;
+    }
+    return GADE_OK;
+}
+
+template <typename T>
+int upload(T **dst, const T *src, size_t count) {
+    GADE_TRY(hip_alloc((void **)dst, sizeof(T) * count));
+    if (count) GADE_HIP(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return GADE_OK;
+}
+
+int locate(const int *list, int n, int value) {  // first index with list[i] >= value
+    return (int)(std::lower_bound(list, list + n, value) - list);
+}
+
+std::complex<double> csr_value(const int *Ap, const int *Ai, const double *Ax, int row, int col) {
+    int b = Ap[row], n = Ap[row + 1] - b;
+    int cp = b + locate(Ai + b, n, col);
+    if (cp == Ap[row + 1] || Ai[cp] != col) return {0, 0};
+    return {Ax[2 * cp], Ax[2 * cp + 1]};
+}
+
+unsigned grid_for(long long total, int threads, unsigned cap = 65535u * 4) {
+    long long g = (total + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+void free_all(gade_model *m) {
+    void *ptrs[] = {m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
+                    m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
+                    m->d_V, m->d_ilv_inv, m->d_AB, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->d_flag,
+                    m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_cnt,
+                    m->d_fAp, m->d_fAi, m->d_fAx, m->d_SAp, m->d_SAi, m->d_SAx};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (m->h_sc) hipHostFree(m->h_sc);
+}
+
+int launch_angular(gade_model *m, const double2 *x) {
+    dim3 grid(m->nblk, (m->n + 255) / 256);
+#define ANG(LM) \
+    case LM: hipLaunchKernelGGL(k_angular<LM>, grid, dim3(256), 0, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef, m->d_kap, m->nblk, m->n, m->d_V); break;
+    switch (m->lmax) {
+        ANG(1) ANG(2) ANG(3) ANG(4) ANG(5) ANG(6) ANG(7) ANG(8) ANG(9) ANG(10) ANG(11) ANG(12) ANG(13) ANG(14) ANG(15) ANG(16)
+        default:
+            gade::set_error("lmax > 16 is not supported");
+            return GADE_EINVAL;
+    }
+#undef ANG
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta) {
+    dim3 grid(m->ntiles, (m->n + 63) / 64);
+#define RAD(R) \
+    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_col, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, m->lmax, m->nq, sel, cH, beta, y); break;
+    switch (m->tile_r) {
+        RAD(1) RAD(2) RAD(4) RAD(8) RAD(16)
+        default:
+            gade::set_error("unsupported tile width");
+            return GADE_EINVAL;
+    }
+#undef RAD
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+int build_tiles(gade_model *m) {
+    // group blocks by distinct kappa index, split into tiles of tile_r rows
+    std::vector<int> tiles, tq;
+    std::vector<double> tk;
+    for (int q = 0; q < m->nq; q++) {
+        std::vector<int> rows;
+        for (int r = 0; r < m->nblk; r++)
+            if (m->blk_q[r] == q) rows.push_back(r);
+        for (size_t i = 0; i < rows.size(); i += m->tile_r) {
+            for (int j = 0; j < m->tile_r; j++) tiles.push_back(i + j < rows.size() ? rows[i + j] : -1);
+            tq.push_back(q);
+            tk.push_back((double)m->kap_q[q]);
+        }
+    }
+    m->ntiles = (int)tq.size();
+    if (m->d_tiles) hipFree(m->d_tiles);
+    if (m->d_tile_q) hipFree(m->d_tile_q);
+    if (m->d_tile_k) hipFree(m->d_tile_k);
+    m->d_tiles = nullptr; m->d_tile_q = nullptr; m->d_tile_k = nullptr;
+    GADE_TRY(upload(&m->d_tiles, tiles.data(), tiles.size()));
+    GADE_TRY(upload(&m->d_tile_q, tq.data(), tq.size()));
+    GADE_TRY(upload(&m->d_tile_k, tk.data(), tk.size()));
+    return GADE_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int gade_malloc(void **ptr, size_t bytes) { return hip_alloc(ptr, bytes); }
+int gade_free(void *ptr) {
+    GADE_HIP(hipFree(ptr));
+    return GADE_OK;
+}
+int gade_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    GADE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GADE_OK;
+}
+int gade_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    GADE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GADE_OK;
+}
+int gade_memset(void *ptr, int value, size_t bytes) {
+    GADE_HIP(hipMemset(ptr, value, bytes));
+    return GADE_OK;
+}
+int gade_device_sync(void) {
+    GADE_HIP(hipDeviceSynchronize());
+    return GADE_OK;
+}
+
+int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream, gade_model_t **out) {
+    if (!in || !out || in->nblk <= 0 || in->blkdim <= 0 || in->lmax <= 0) {
+        gade::set_error("gade_model_create: invalid arguments");
+        return GADE_EINVAL;
+    }
+    if (in->lmax > gade::kMaxL) {
+        gade::set_error("gade_model_create: lmax > 16 is not supported");
+        return GADE_EINVAL;
+    }
+    gade_model *m = new gade_model();
+    *out = nullptr;
+    auto fail = [&](int rc) {
+        free_all(m);
+        delete m;
+        return rc;
+    };
+    m->nblk = in->nblk;
+    m->n = in->blkdim;
+    m->lmax = in->lmax;
+    m->h = h;
+    m->dt = dt;
+    m->stream = (hipStream_t)stream;
+    m->N = (long long)m->nblk * m->n;
+    const int n = m->n, lmax = m->lmax;
+    m->rad_Ap.assign(in->rad_Ap, in->rad_Ap + n + 1);
+    m->nnzb = m->rad_Ap[n];
+    m->rad_Ai.assign(in->rad_Ai, in->rad_Ai + m->nnzb);
+    m->hall_Ap.assign(in->hall_Ap, in->hall_Ap + m->nblk + 1);
+    m->hnnz = m->hall_Ap[m->nblk];
+    m->hall_Ai.assign(in->hall_Ai, in->hall_Ai + m->hnnz);
+    // structural checks
+    for (int a = 0; a < n; a++) {
+        if (m->rad_Ap[a + 1] < m->rad_Ap[a]) { gade::set_error("radial Ap not monotone"); return fail(GADE_EINVAL); }
+        for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) {
+            if (m->rad_Ai[k] < 0 || m->rad_Ai[k] >= n) { gade::set_error("radial Ai out of range"); return fail(GADE_EINVAL); }
+            if (k > m->rad_Ap[a] && m->rad_Ai[k] <= m->rad_Ai[k - 1]) { gade::set_error("radial Ai not sorted"); return fail(GADE_EINVAL); }
+        }
+    }
+    for (int r = 0; r < m->nblk; r++)
+        for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++)
+            if (m->hall_Ai[p] < 0 || m->hall_Ai[p] >= m->nblk) { gade::set_error("Hall Ai out of range"); return fail(GADE_EINVAL); }
+    // ELL of the radial pattern
+    int ellw = 0;
+    for (int a = 0; a < n; a++) ellw = std::max(ellw, m->rad_Ap[a + 1] - m->rad_Ap[a]);
+    m->ellw = ellw;
+    std::vector<int> ell_col((size_t)ellw * n), ell_src((size_t)ellw * n), ell_pos(m->nnzb), tmap(m->nnzb);
+    for (int a = 0; a < n; a++)
+        for (int s = 0; s < ellw; s++) {
+            int k = m->rad_Ap[a] + s;
+            bool ok = k < m->rad_Ap[a + 1];
+            ell_col[(size_t)s * n + a] = ok ? m->rad_Ai[k] : a;
+            ell_src[(size_t)s * n + a] = ok ? k : -1;
+            if (ok) ell_pos[k] = s * n + a;
+        }
+    // transpose map: entry k=(a,j) -> position of (j,a)  (tools/csr.c:963-972 needs structural symmetry)
+    for (int a = 0; a < n; a++)
+        for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) {
+            int j = m->rad_Ai[k];
+            int b = m->rad_Ap[j], len = m->rad_Ap[j + 1] - b;
+            int cp = b + locate(m->rad_Ai.data() + b, len, a);
+            if (cp == m->rad_Ap[j + 1] || m->rad_Ai[cp] != a) {
+                gade::set_error("radial pattern is not structurally symmetric");
+                return fail(GADE_EINVAL);
+            }
+            tmap[k] = cp;
+        }
+    // kappa per block and distinct kappa values
+    m->kap_blk.resize(m->nblk);
+    m->blk_q.resize(m->nblk);
+    for (int r = 0; r < m->nblk; r++) {
+        int orig = in->hall_perm ? in->hall_perm[r] : r;
+        if (orig < 0 || orig >= in->nang) { gade::set_error("Hall perm out of range"); return fail(GADE_EINVAL); }
+        m->kap_blk[r] = gade_ik(orig);
+        auto it = std::find(m->kap_q.begin(), m->kap_q.end(), m->kap_blk[r]);
+        if (it == m->kap_q.end()) {
+            m->blk_q[r] = (int)m->kap_q.size();
+            m->kap_q.push_back(m->kap_blk[r]);
+        } else {
+            m->blk_q[r] = (int)(it - m->kap_q.begin());
+        }
+    }
+    m->nq = (int)m->kap_q.size();
+    // angular coefficients per Hall nz (tools/gpu_solver.cu:156-188)
+    std::vector<double> coef((size_t)m->hnnz * lmax * 4);
+    std::vector<int> diag(m->hnnz);
+    for (int r = 0; r < m->nblk; r++)
+        for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++) {
+            int c = m->hall_Ai[p];
+            int orow = in->hall_perm ? in->hall_perm[r] : r;
+            int ocol = in->hall_perm ? in->hall_perm[c] : c;
+            diag[p] = (orow == ocol);
+            for (int l = 0; l < lmax; l++) {
+                auto v0 = csr_value(in->H_Ap[2 * l], in->H_Ai[2 * l], in->H_Ax[2 * l], orow, ocol);
+                auto v1 = csr_value(in->H_Ap[2 * l + 1], in->H_Ai[2 * l + 1], in->H_Ax[2 * l + 1], orow, ocol);
+                double *cp = &coef[((size_t)p * lmax + l) * 4];
+                cp[0] = v0.real(); cp[1] = v0.imag(); cp[2] = v1.real(); cp[3] = v1.imag();
+            }
+        }
+    // uploads
+    int rc;
+    std::vector<double> gbuf((size_t)24 * lmax * m->nnzb * 2);
+    for (int i = 0; i < 24 * lmax; i++) memcpy(&gbuf[(size_t)i * m->nnzb * 2], in->g[i], sizeof(double) * 2 * m->nnzb);
+    std::vector<double> sbuf((size_t)3 * m->nnzb * 2), hbuf((size_t)4 * m->nnzb * 2);
+    for (int i = 0; i < 3; i++) memcpy(&sbuf[(size_t)i * m->nnzb * 2], in->s[i], sizeof(double) * 2 * m->nnzb);
+    for (int i = 0; i < 4; i++) memcpy(&hbuf[(size_t)i * m->nnzb * 2], in->h0[i], sizeof(double) * 2 * m->nnzb);
+    if ((rc = upload(&m->d_ell_col, ell_col.data(), ell_col.size())) ||
+        (rc = upload(&m->d_ell_src, ell_src.data(), ell_src.size())) ||
+        (rc = upload(&m->d_ell_pos, ell_pos.data(), ell_pos.size())) ||
+        (rc = upload(&m->d_tmap, tmap.data(), tmap.size())) ||
+        (rc = upload(&m->d_g, (const double2 *)gbuf.data(), gbuf.size() / 2)) ||
+        (rc = upload(&m->d_s, (const double2 *)sbuf.data(), sbuf.size() / 2)) ||
+        (rc = upload(&m->d_h0, (const double2 *)hbuf.data(), hbuf.size() / 2)) ||
+        (rc = upload(&m->d_hAp, m->hall_Ap.data(), m->hall_Ap.size())) ||
+        (rc = upload(&m->d_hAi, m->hall_Ai.data(), m->hall_Ai.size())) ||
+        (rc = upload(&m->d_kap, m->kap_blk.data(), m->kap_blk.size())) ||
+        (rc = upload(&m->d_blk_q, m->blk_q.data(), m->blk_q.size())) ||
+        (rc = upload(&m->d_coef, (const double2 *)coef.data(), coef.size() / 2)) ||
+        (rc = upload(&m->d_diag, diag.data(), diag.size())))
+        return fail(rc);
+    if ((rc = hip_alloc((void **)&m->d_GE, sizeof(double2) * (size_t)lmax * 8 * ellw * n)) ||
+        (rc = hip_alloc((void **)&m->d_D, sizeof(double2) * (size_t)3 * m->nq * ellw * n)) ||
+        (rc = hip_alloc((void **)&m->d_V, sizeof(double2) * (size_t)lmax * 4 * m->N)))
+        return fail(rc);
+    {
+        int *d_kq = nullptr;
+        if ((rc = upload(&d_kq, m->kap_q.data(), m->kap_q.size()))) return fail(rc);
+        long long total = (long long)m->nq * ellw * n;
+        hipLaunchKernelGGL(k_build_D, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_s, m->d_h0,
+                           m->d_ell_src, d_kq, m->nq, ellw, n, m->nnzb, h * dt / 2, m->d_D);
+        hipError_t e = hipStreamSynchronize(m->stream);
+        hipFree(d_kq);
+        if (e != hipSuccess) {
+            gade::set_error(std::string("k_build_D: ") + hipGetErrorString(e));
+            return fail(GADE_EHIP);
+        }
+    }
+    if ((rc = build_tiles(m))) return fail(rc);
+    // BiCGSTAB workspace and scalars
+    double2 **ws[] = {&m->r, &m->r0, &m->v, &m->p, &m->y, &m->z, &m->s, &m->t, &m->tmp};
+    for (auto w : ws) {
+        if ((rc = hip_alloc((void **)w, sizeof(double2) * m->N))) return fail(rc);
+        if (hipMemset(*w, 0, sizeof(double2) * m->N) != hipSuccess) return fail(GADE_EHIP);
+    }
+    m->red_blocks = (int)std::min<long long>(1024, (m->N + kRedThreads - 1) / kRedThreads);
+    if ((rc = hip_alloc((void **)&m->d_sc, sizeof(double) * 64)) ||
+        (rc = hip_alloc((void **)&m->d_part, sizeof(double) * 4 * m->red_blocks)) ||
+        (rc = hip_alloc((void **)&m->d_cnt, sizeof(unsigned) * 4)) ||
+        (rc = hip_alloc((void **)&m->d_flag, sizeof(int) * 4)))
+        return fail(rc);
+    if (hipMemset(m->d_cnt, 0, sizeof(unsigned) * 4) != hipSuccess || hipMemset(m->d_sc, 0, sizeof(double) * 64) != hipSuccess)
+        return fail(GADE_EHIP);
+    if (hipHostMalloc((void **)&m->h_sc, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) {
+        gade::set_error("hipHostMalloc failed");
+        return fail(GADE_ENOMEM);
+    }
+    m->ft.assign(12, 0.0);
+    *out = m;
+    return GADE_OK;
+}
+
+int gade_model_destroy(gade_model_t *m) {
+    if (!m) return GADE_OK;
+    hipDeviceSynchronize();
+    free_all(m);
+    delete m;
+    return GADE_OK;
+}
+
+int gade_model_info(const gade_model_t *m, int *nblk, int *blkdim, int *hall_nnz, int *nnz_b, long long *nnz_full,
+                    int *nkappa) {
+    if (!m) return GADE_EINVAL;
+    if (nblk) *nblk = m->nblk;
+    if (blkdim) *blkdim = m->n;
+    if (hall_nnz) *hall_nnz = m->hnnz;
+    if (nnz_b) *nnz_b = m->nnzb;
+    if (nnz_full) *nnz_full = (long long)m->hnnz * m->nnzb;
+    if (nkappa) *nkappa = m->nq;
+    return GADE_OK;
+}
+
+int gade_set_time(gade_model_t *m, const double *ft) {
+    if (!m || !ft) return GADE_EINVAL;
+    m->ft.assign(ft, ft + 12);
+    Coef6 cf;
+    // c_alpha = SoL * (i h dt/2) * f_alpha
+    const double hd = kSoL * m->h * m->dt / 2;
+    for (int a = 0; a < 6; a++) cf.c[a] = make_double2(-hd * ft[2 * a + 1], hd * ft[2 * a]);
+    long long total = (long long)m->lmax * 8 * m->ellw * m->n;
+    Prof pr(m, "gsums");
+    hipLaunchKernelGGL(k_gsums, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_g, m->d_ell_src, m->d_tmap,
+                       cf, m->lmax, m->ellw, m->n, m->nnzb, m->d_GE);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+static int apply_async(gade_model *m, const double2 *x, double2 *y, double cH, double beta) {
+    int sel;
+    if (cH == 1.0) sel = 0;
+    else if (cH == -1.0) sel = 1;
+    else if (cH == 0.0) sel = 2;
+    else {
+        gade::set_error("gade_apply: cH must be +1, -1 or 0");
+        return GADE_EINVAL;
+    }
+    if (sel != 2) {
+        Prof pr(m, "angular");
+        GADE_TRY(launch_angular(m, x));
+    }
+    Prof pr(m, "radial");
+    return launch_radial(m, x, y, sel, cH, beta);
+}
+
+int gade_apply(gade_model_t *m, const double *x, double *y, double cH, double beta) {
+    if (!m || !x || !y) return GADE_EINVAL;
+    GADE_TRY(apply_async(m, (const double2 *)x, (double2 *)y, cH, beta));
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_precond_setup(gade_model_t *m) {
+    if (!m) return GADE_EINVAL;
+    if (m->pre_ready) return GADE_OK;
+    const int n = m->n, nnzb = m->nnzb;
+    // interleave the two radial halves so that banded quadrants form one band
+    std::vector<int> ilv(n), ilv_inv(n);
+    for (int i = 0; i < n; i++) ilv[i] = (n % 2 == 0) ? (i < n / 2 ? 2 * i : 2 * (i - n / 2) + 1) : i;
+    for (int i = 0; i < n; i++) ilv_inv[ilv[i]] = i;
+    int band = 0;
+    for (int a = 0; a < n; a++)
+        for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) band = std::max(band, std::abs(ilv[a] - ilv[m->rad_Ai[k]]));
+    if (band > gade::kMaxBand) {
+        gade::set_error("preconditioner: radial half-bandwidth " + std::to_string(band) + " exceeds 63");
+        return GADE_EINVAL;
+    }
+    m->band = std::max(band, 1);
+    m->nch = (n + kChunk - 1) / kChunk;
+    const int w = 2 * m->band + 1, nq = m->nq;
+    // P(k) = S(k) + ihdt H0(k) in band storage, from the device D operators (sel 0)
+    std::vector<double2> Dh((size_t)nq * m->ellw * n);
+    GADE_HIP(hipMemcpy(Dh.data(), m->d_D, sizeof(double2) * Dh.size(), hipMemcpyDeviceToHost));
+    std::vector<double2> AB((size_t)nq * n * w, make_double2(0, 0));
+    for (int q = 0; q < nq; q++)
+        for (int a = 0; a < n; a++)
+            for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) {
+                int s = k - m->rad_Ap[a];
+                int i = ilv[a], j = ilv[m->rad_Ai[k]];
+                AB[((size_t)q * n + i) * w + (j - i + m->band)] = Dh[((size_t)q * m->ellw + s) * n + a];
+            }
+    (void)nnzb;
+    GADE_TRY(upload(&m->d_AB, AB.data(), AB.size()));
+    GADE_TRY(upload(&m->d_ilv_inv, ilv_inv.data(), ilv_inv.size()));
+    const size_t cs = (size_t)kChunk * kChunk;
+    GADE_TRY(hip_alloc((void **)&m->d_Linv, sizeof(double2) * nq * m->nch * cs));
+    GADE_TRY(hip_alloc((void **)&m->d_Uinv, sizeof(double2) * nq * m->nch * cs));
+    GADE_TRY(hip_alloc((void **)&m->d_Loff, sizeof(double2) * nq * m->nch * m->band * kChunk));
+    GADE_TRY(hip_alloc((void **)&m->d_Uoff, sizeof(double2) * nq * m->nch * m->band * kChunk));
+    GADE_HIP(hipMemset(m->d_flag, 0, sizeof(int)));
+    hipLaunchKernelGGL(k_band_lu, dim3(nq), dim3(256), 0, m->stream, m->d_AB, n, m->band, m->d_flag);
+    GADE_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_chunk_inverse, dim3(nq, m->nch), dim3(kChunk), 0, m->stream, m->d_AB, n, m->band, m->nch,
+                       m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff);
+    GADE_HIP(hipGetLastError());
+    int flag = 0;
+    GADE_HIP(hipMemcpyAsync(&flag, m->d_flag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    if (flag) {
+        gade::set_error("preconditioner: zero pivot in block LU");
+        return GADE_EPIVOT;
+    }
+    m->pre_ready = true;
+    return GADE_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// preconditioner apply, BiCGSTAB, CN stepping
+// ---------------------------------------------------------------------------
+namespace {
+
+int psolve_async(gade_model *m, const double2 *b, double2 *x) {
+    Prof pr(m, "psolve");
+    size_t shm = sizeof(double2) * ((size_t)m->nch * kChunk + kChunk);
+    hipLaunchKernelGGL(k_psolve, dim3(m->nblk), dim3(kChunk), shm, m->stream, b, x, m->d_blk_q, m->d_ilv_inv,
+                       m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->n, m->band, m->nch);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+template <int OP>
+int bicg_op(gade_model *m, const double2 *b, const double2 *xconst = nullptr, double2 *x = nullptr) {
+    (void)xconst;
+    Prof pr(m, "blas1");
+    hipLaunchKernelGGL(k_bicg<OP>, dim3(m->red_blocks), dim3(kRedThreads), 0, m->stream, m->N, b, m->r, m->r0, m->p,
+                       m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part, m->d_cnt);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+int read_scalars(gade_model *m) {
+    GADE_HIP(hipMemcpyAsync(m->h_sc, m->d_sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, m->stream));
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+// A x into out:  out = (S + Hfull) x
+int applyA(gade_model *m, const double2 *x, double2 *out, int *napply) {
+    if (napply) (*napply)++;
+    return apply_async(m, x, out, 1.0, 0.0);
+}
+
+int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, double *tol_error, int *napply) {
+    const double tol = *tol_error;
+    const int maxit = *iters;
+    GADE_TRY(gade_precond_setup(m));
+    if (napply) *napply = 0;
+    // r = rhs - A x ; r0 = r
+    GADE_TRY(applyA(m, x, m->r, napply));
+    GADE_TRY(bicg_op<OP_INIT>(m, rhs));
+    GADE_TRY(read_scalars(m));
+    double r0_sq = m->h_sc[SC_R0SQ];
+    const double rhs_sq = m->h_sc[SC_RHSSQ];
+    if (rhs_sq == 0) {
+        GADE_HIP(hipMemsetAsync(x, 0, sizeof(double2) * m->N, m->stream));
+        GADE_HIP(hipStreamSynchronize(m->stream));
+        *iters = 0;
+        *tol_error = 0;
+        return GADE_OK;
+    }
+    const double tol2 = tol * tol * rhs_sq;
+    const double eps2 = 1e-12 * 1e-12;
+    if (r0_sq < tol2) {
+        *iters = 0;
+        *tol_error = std::sqrt(r0_sq / rhs_sq);
+        return GADE_OK;
+    }
+    int i = 0;
+    double rnorm2 = r0_sq;
+    double rho_next_abs = r0_sq;
+    while (i < maxit) {
+        if (rho_next_abs < eps2 * r0_sq) {
+            // restart: r = rhs - A x, r0 = r, rho = |r|^2 (tools/gpu_bicgstab.c:222-241)
+            GADE_TRY(applyA(m, x, m->r, napply));
+            GADE_TRY(bicg_op<OP_RESTART>(m, rhs));
+            GADE_TRY(read_scalars(m));
+            r0_sq = m->h_sc[SC_R0SQ];
+        }
+        GADE_TRY(bicg_op<OP_P>(m, rhs));                 // p = r + beta (p - w v)
+        GADE_TRY(psolve_async(m, m->p, m->y));           // y = M^-1 p
+        GADE_TRY(applyA(m, m->y, m->v, napply));         // v = A y
+        GADE_TRY(bicg_op<OP_ALPHA>(m, rhs));             // alpha = rho / (r0 . v)
+        GADE_TRY(bicg_op<OP_S>(m, rhs));                 // s = r - alpha v
+        GADE_TRY(psolve_async(m, m->s, m->z));           // z = M^-1 s
+        GADE_TRY(applyA(m, m->z, m->t, napply));         // t = A z
+        GADE_TRY(bicg_op<OP_W>(m, rhs));                 // w = (t . s) / |t|^2
+        GADE_TRY(bicg_op<OP_XR>(m, rhs, nullptr, x));    // x += alpha y + w z ; r = s - w t
+        GADE_TRY(read_scalars(m));
+        rnorm2 = m->h_sc[SC_RNORM2];
+        rho_next_abs = std::hypot(m->h_sc[SC_RHONEXT], m->h_sc[SC_RHONEXT + 1]);
+        if (!std::isfinite(rnorm2)) {
+            gade::set_error("BiCGSTAB: non-finite residual");
+            return GADE_ENOTCONV;
+        }
+        if (rnorm2 < tol2) break;
+        ++i;
+    }
+    *tol_error = std::sqrt(rnorm2 / rhs_sq);
+    *iters = i;
+    return GADE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gade_psolve(gade_model_t *m, const double *rhs, double *x) {
+    if (!m || !rhs || !x) return GADE_EINVAL;
+    GADE_TRY(gade_precond_setup(m));
+    GADE_TRY(psolve_async(m, (const double2 *)rhs, (double2 *)x));
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_bicgstab(gade_model_t *m, const double *rhs, double *x, int *iters, double *tol_error, int *napply) {
+    if (!m || !rhs || !x || !iters || !tol_error) return GADE_EINVAL;
+    return bicgstab_impl(m, (const double2 *)rhs, (double2 *)x, iters, tol_error, napply);
+}
+
+int gade_reset_workspace(gade_model_t *m) {
+    if (!m) return GADE_EINVAL;
+    double2 *ws[] = {m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t};
+    for (auto w : ws) GADE_HIP(hipMemsetAsync(w, 0, sizeof(double2) * m->N, m->stream));
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_cn_step(gade_model_t *m, double *psi, const double *ft, double tol, int maxit, int *iters, double *tol_error) {
+    if (!m || !psi || !ft) return GADE_EINVAL;
+    GADE_TRY(gade_set_time(m, ft));
+    // rhs = (S - Hfull) psi  (tools/dirac_solver.c:531-536)
+    GADE_TRY(apply_async(m, (const double2 *)psi, m->tmp, -1.0, 0.0));
+    int it = maxit;
+    double te = tol;
+    GADE_TRY(bicgstab_impl(m, m->tmp, (double2 *)psi, &it, &te, nullptr));
+    if (iters) *iters = it;
+    if (tol_error) *tol_error = te;
+    return GADE_OK;
+}
+
+int gade_cn_run(gade_model_t *m, double *psi, double E0, double omega, double cycles, double t0, int nsteps,
+                double tol, int maxit, int *iters, double *tol_out, double *usec) {
+    if (!m || !psi || nsteps < 0) return GADE_EINVAL;
+    GADE_TRY(gade_precond_setup(m));
+    double time = t0;
+    hipEvent_t ea, eb;
+    GADE_HIP(hipEventCreate(&ea));
+    GADE_HIP(hipEventCreate(&eb));
+    for (int step = 0; step < nsteps; step++) {
+        double ft[12];
+        time = time + m->dt;  // tools/dirac_solver.c:509
+        gade_pulse(E0, omega, cycles, time, ft);
+        if (usec) GADE_HIP(hipEventRecord(ea, m->stream));
+        int it = 0;
+        double te = 0;
+        int rc = gade_cn_step(m, psi, ft, tol, maxit, &it, &te);
+        if (rc) {
+            hipEventDestroy(ea);
+            hipEventDestroy(eb);
+            return rc;
+        }
+        if (usec) {
+            GADE_HIP(hipEventRecord(eb, m->stream));
+            GADE_HIP(hipEventSynchronize(eb));
+            float ms = 0;
+            hipEventElapsedTime(&ms, ea, eb);
+            usec[step] = ms * 1000.0;
+        }
+        if (iters) iters[step] = it;
+        if (tol_out) tol_out[step] = te;
+    }
+    hipEventDestroy(ea);
+    hipEventDestroy(eb);
+    return GADE_OK;
+}
+
+int gade_snorm(gade_model_t *m, const double *x, double *norm, double *pop_host) {
+    if (!m || !x) return GADE_EINVAL;
+    double *d_pop = nullptr;
+    GADE_TRY(hip_alloc((void **)&d_pop, sizeof(double) * m->nblk));
+    hipLaunchKernelGGL(k_snorm, dim3(m->nblk), dim3(256), 0, m->stream, (const double2 *)x, m->d_D, m->d_ell_col,
+                       m->d_blk_q, m->n, m->ellw, m->nq, d_pop);
+    std::vector<double> pop(m->nblk);
+    hipError_t e = hipMemcpyAsync(pop.data(), d_pop, sizeof(double) * m->nblk, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    hipFree(d_pop);
+    if (e != hipSuccess) {
+        gade::set_error(std::string("gade_snorm: ") + hipGetErrorString(e));
+        return GADE_EHIP;
+    }
+    double tot = 0;
+    for (double v : pop) tot += v;
+    if (norm) *norm = tot;
+    if (pop_host) memcpy(pop_host, pop.data(), sizeof(double) * m->nblk);
+    return GADE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// assembled reference formulation
+// ---------------------------------------------------------------------------
+int gade_assembled_setup(gade_model_t *m) {
+    if (!m) return GADE_EINVAL;
+    if (m->asm_ready) return GADE_OK;
+    m->nnz_full = (long long)m->hnnz * m->nnzb;
+    if (m->nnz_full > 2147483647LL) {
+        gade::set_error("assembled Hfull exceeds int32 indexing (csr_index_t, tools/types.h:37)");
+        return GADE_EINVAL;
+    }
+    const long long N = m->N;
+    int *d_subAp = nullptr, *d_subAi = nullptr;
+    GADE_TRY(upload(&d_subAp, m->rad_Ap.data(), m->rad_Ap.size()));
+    GADE_TRY(upload(&d_subAi, m->rad_Ai.data(), m->rad_Ai.size()));
+    GADE_TRY(hip_alloc((void **)&m->d_fAp, sizeof(int) * (N + 1)));
+    GADE_TRY(hip_alloc((void **)&m->d_fAi, sizeof(int) * m->nnz_full));
+    GADE_TRY(hip_alloc((void **)&m->d_fAx, sizeof(double2) * m->nnz_full));
+    hipLaunchKernelGGL(k_full_Ap, dim3(grid_for(N + 1, 256, 1u << 30)), dim3(256), 0, m->stream, m->d_hAp, d_subAp,
+                       m->nblk, m->n, m->nnzb, m->d_fAp);
+    GADE_HIP(hipGetLastError());
+    dim3 g2((unsigned)std::min<long long>(N, 65535), (unsigned)((N + 65534) / 65535));
+    hipLaunchKernelGGL(k_full_Ai, g2, dim3(256), 0, m->stream, m->d_hAp, m->d_hAi, d_subAp, d_subAi, m->d_fAp, m->nblk,
+                       m->n, m->d_fAi);
+    GADE_HIP(hipGetLastError());
+    // S: block diagonal, same expansion with a diagonal block pattern
+    std::vector<int> dAp(m->nblk + 1), dAi(m->nblk);
+    for (int i = 0; i < m->nblk; i++) { dAp[i] = i; dAi[i] = i; }
+    dAp[m->nblk] = m->nblk;
+    int *d_dAp = nullptr, *d_dAi = nullptr;
+    GADE_TRY(upload(&d_dAp, dAp.data(), dAp.size()));
+    GADE_TRY(upload(&d_dAi, dAi.data(), dAi.size()));
+    const long long snnz = (long long)m->nblk * m->nnzb;
+    GADE_TRY(hip_alloc((void **)&m->d_SAp, sizeof(int) * (N + 1)));
+    GADE_TRY(hip_alloc((void **)&m->d_SAi, sizeof(int) * snnz));
+    GADE_TRY(hip_alloc((void **)&m->d_SAx, sizeof(double2) * snnz));
+    hipLaunchKernelGGL(k_full_Ap, dim3(grid_for(N + 1, 256, 1u << 30)), dim3(256), 0, m->stream, d_dAp, d_subAp,
+                       m->nblk, m->n, m->nnzb, m->d_SAp);
+    hipLaunchKernelGGL(k_full_Ai, g2, dim3(256), 0, m->stream, d_dAp, d_dAi, d_subAp, d_subAi, m->d_SAp, m->nblk, m->n,
+                       m->d_SAi);
+    GADE_HIP(hipGetLastError());
+    // S values: block r copies S(k_r) in CSR order
+    std::vector<double2> Sh((size_t)snnz);
+    std::vector<double2> Dh((size_t)m->nq * m->ellw * m->n);
+    GADE_HIP(hipMemcpy(Dh.data(), m->d_D + (size_t)2 * m->nq * m->ellw * m->n, sizeof(double2) * Dh.size(),
+                       hipMemcpyDeviceToHost));
+    for (int r = 0; r < m->nblk; r++)
+        for (int a = 0; a < m->n; a++)
+            for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++)
+                Sh[(size_t)r * m->nnzb + k] = Dh[((size_t)m->blk_q[r] * m->ellw + (k - m->rad_Ap[a])) * m->n + a];
+    GADE_HIP(hipMemcpy(m->d_SAx, Sh.data(), sizeof(double2) * snnz, hipMemcpyHostToDevice));
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    hipFree(d_subAp); hipFree(d_subAi); hipFree(d_dAp); hipFree(d_dAi);
+    m->asm_ready = true;
+    return GADE_OK;
+}
+
+int gade_assembled_timedep(gade_model_t *m) {
+    if (!m) return GADE_EINVAL;
+    GADE_TRY(gade_assembled_setup(m));
+    int *d_subAp = nullptr;
+    GADE_TRY(upload(&d_subAp, m->rad_Ap.data(), m->rad_Ap.size()));
+    const long long N = m->N;
+    dim3 g2((unsigned)std::min<long long>(N, 65535), (unsigned)((N + 65534) / 65535));
+    {
+        Prof pr(m, "assemble");
+#define ASM(LM) \
+    case LM: hipLaunchKernelGGL(k_assemble<LM>, g2, dim3(256), 0, m->stream, m->d_hAp, m->d_hAi, m->d_kap, m->d_coef, m->d_diag, d_subAp, m->d_ell_pos, m->d_GE, m->d_D, m->d_blk_q, m->d_fAp, m->nblk, m->n, m->ellw, m->nq, m->d_fAx); break;
+        switch (m->lmax) {
+            ASM(1) ASM(2) ASM(3) ASM(4) ASM(5) ASM(6) ASM(7) ASM(8) ASM(9) ASM(10) ASM(11) ASM(12) ASM(13) ASM(14) ASM(15) ASM(16)
+            default: hipFree(d_subAp); return GADE_EINVAL;
+        }
+#undef ASM
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    hipFree(d_subAp);
+    if (e != hipSuccess) {
+        gade::set_error(std::string("k_assemble: ") + hipGetErrorString(e));
+        return GADE_EHIP;
+    }
+    return GADE_OK;
+}
+
+int gade_assembled_spmv(gade_model_t *m, const double *x, double *y, double alpha, double beta, int with_S) {
+    if (!m || !x || !y || !m->asm_ready) {
+        gade::set_error("gade_assembled_spmv: call gade_assembled_timedep first");
+        return GADE_EINVAL;
+    }
+    const long long N = m->N;
+    unsigned grid = (unsigned)((N * 64 + 255) / 256);
+    {
+        Prof pr(m, "csr_spmv");
+        hipLaunchKernelGGL(k_csr_spmv, dim3(grid), dim3(256), 0, m->stream, m->d_fAp, m->d_fAi, m->d_fAx,
+                           (const double2 *)x, N, make_double2(alpha, 0), make_double2(beta, 0), (double2 *)y);
+    }
+    if (with_S) {
+        Prof pr(m, "csr_spmv_S");
+        hipLaunchKernelGGL(k_csr_spmv, dim3(grid), dim3(256), 0, m->stream, m->d_SAp, m->d_SAi, m->d_SAx,
+                           (const double2 *)x, N, make_double2(1.0, 0), make_double2(1.0, 0), (double2 *)y);
+    }
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, double *Ax) {
+    if (!m || !m->asm_ready) return GADE_EINVAL;
+    if (Ap) GADE_HIP(hipMemcpy(Ap, m->d_fAp, sizeof(int) * (m->N + 1), hipMemcpyDeviceToHost));
+    if (Ai) GADE_HIP(hipMemcpy(Ai, m->d_fAi, sizeof(int) * m->nnz_full, hipMemcpyDeviceToHost));
+    if (Ax) GADE_HIP(hipMemcpy(Ax, m->d_fAx, sizeof(double2) * m->nnz_full, hipMemcpyDeviceToHost));
+    if (Ai_sub_map) {
+        int *d_rowof = nullptr, *d_map = nullptr, *d_subAp = nullptr;
+        std::vector<int> rowof(m->hnnz);
+        for (int r = 0; r < m->nblk; r++)
+            for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++) rowof[p] = r;
+        GADE_TRY(upload(&d_rowof, rowof.data(), rowof.size()));
+        GADE_TRY(upload(&d_subAp, m->rad_Ap.data(), m->rad_Ap.size()));
+        GADE_TRY(hip_alloc((void **)&d_map, sizeof(int) * m->nnz_full));
+        hipLaunchKernelGGL(k_full_submap, dim3(m->hnnz), dim3(256), 0, m->stream, m->d_hAp, d_subAp, m->d_fAp,
+                           m->nblk, m->n, m->nnzb, d_rowof, d_map);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(Ai_sub_map, d_map, sizeof(int) * m->nnz_full, hipMemcpyDeviceToHost);
+        hipFree(d_rowof); hipFree(d_map); hipFree(d_subAp);
+        if (e != hipSuccess) {
+            gade::set_error(std::string("k_full_submap: ") + hipGetErrorString(e));
+            return GADE_EHIP;
+        }
+    }
+    return GADE_OK;
+}
+
+int gade_profile(gade_model_t *m, int enable) {
+    if (!m) return GADE_EINVAL;
+    m->profile = enable != 0;
+    if (!enable) m->timers.clear();
+    return GADE_OK;
+}
+
+int gade_profile_get(gade_model_t *m, const char *name, double *total_ms, long long *count) {
+    if (!m || !name) return GADE_EINVAL;
+    auto it = m->timers.find(name);
+    if (total_ms) *total_ms = it == m->timers.end() ? 0 : it->second.total_ms;
+    if (count) *count = it == m->timers.end() ? 0 : it->second.count;
+    return GADE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Internal definitions shared by the libgade translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "gade.h"
+
+namespace gade {
+
+constexpr double kSoL = 137.035999084;  // tools/solver_common.h:37
+constexpr int kChunk = 64;              // rows per triangular-solve chunk (one wave)
+constexpr int kMaxL = 16;               // largest lmax the angular kernel is instantiated for
+constexpr int kMaxBand = 63;            // largest half-bandwidth the chunked solve supports
+
+void set_error(const std::string &msg);
+
+#define GADE_HIP(expr)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            gade::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));            \
+            return GADE_EHIP;                                                               \
+        }                                                                                   \
+    } while (0)
+
+#define GADE_TRY(expr)                  \
+    do {                                \
+        int rc_ = (expr);               \
+        if (rc_ != GADE_OK) return rc_; \
+    } while (0)
+
+struct Timer {
+    double total_ms = 0;
+    long long count = 0;
+};
+
+}  // namespace gade
+
+struct gade_model {
+    // sizes
+    int nblk = 0, n = 0, lmax = 0, nnzb = 0, ellw = 0, hnnz = 0, nq = 0;
+    long long N = 0;
+    double h = 1, dt = 0;
+    hipStream_t stream = nullptr;
+
+    // host-side structure kept for setup
+    std::vector<int> rad_Ap, rad_Ai, hall_Ap, hall_Ai, kap_blk, kap_q;  // kap_q: kappa of each distinct index
+    std::vector<int> blk_q;   // distinct-kappa index of each block
+    std::vector<double> ft;   // last f(t), 12 doubles
+
+    // radial pattern (ELL, transposed: [slot][row])
+    int *d_ell_col = nullptr, *d_ell_src = nullptr, *d_tmap = nullptr, *d_ell_pos = nullptr;
+    double2 *d_g = nullptr;        // [24*lmax][nnzb]
+    double2 *d_s = nullptr;        // [3][nnzb]
+    double2 *d_h0 = nullptr;       // [4][nnzb]
+    double2 *d_GE = nullptr;       // [lmax][8][ellw][n]
+    double2 *d_D = nullptr;        // [3][nq][ellw][n]: S+ihdtH0, S-ihdtH0, S
+    // Hall and angular coefficients
+    int *d_hAp = nullptr, *d_hAi = nullptr, *d_kap = nullptr, *d_blk_q = nullptr;
+    double2 *d_coef = nullptr;     // [hnnz][lmax][2]
+    int *d_diag = nullptr;         // [hnnz] 1 when orig_row == orig_col
+    // kappa-grouped row tiles for the radial phase
+    int tile_r = 4, ntiles = 0;
+    int *d_tiles = nullptr;        // [ntiles][tile_r]
+    int *d_tile_q = nullptr;       // [ntiles]
+    double *d_tile_k = nullptr;    // [ntiles]
+    double2 *d_V = nullptr;        // [lmax][4][nblk][n]
+    // preconditioner
+    bool pre_ready = false;
+    int band = 0, nch = 0;
+    int *d_ilv_inv = nullptr;      // interleaved -> natural radial index
+    double2 *d_AB = nullptr;       // [nq][n][2*band+1]
+    double2 *d_Linv = nullptr, *d_Uinv = nullptr;  // [nq][nch][64 col][64 row]
+    double2 *d_Loff = nullptr, *d_Uoff = nullptr;  // [nq][nch][band][64 row]
+    int *d_flag = nullptr;
+    // BiCGSTAB workspace (persists across calls, tools/gpu_bicgstab.c:150-167)
+    double2 *r = nullptr, *r0 = nullptr, *v = nullptr, *p = nullptr, *y = nullptr, *z = nullptr,
+            *s = nullptr, *t = nullptr, *tmp = nullptr;
+    double *d_sc = nullptr;        // device scalars
+    double *d_part = nullptr;      // reduction partials
+    unsigned *d_cnt = nullptr;     // last-block ticket
+    double *h_sc = nullptr;        // pinned host mirror
+    int red_blocks = 0;
+    // assembled reference formulation
+    bool asm_ready = false;
+    long long nnz_full = 0;
+    int *d_fAp = nullptr, *d_fAi = nullptr;
+    double2 *d_fAx = nullptr;
+    int *d_SAp = nullptr, *d_SAi = nullptr;
+    double2 *d_SAx = nullptr;
+    // profiling
+    bool profile = false;
+    std::map<std::string, gade::Timer> timers;
+};

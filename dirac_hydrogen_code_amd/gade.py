@@ -1,0 +1,230 @@
+"""Python host mirror of the GaDE CN hot path over libgade.so (ctypes).
+
+The interface follows the reference's driver loop (tools/dirac_solver.c):
+load the coupling files, set f(t) per step, form rhs = (S - H) psi and run
+BiCGSTAB on (S + H).  Everything numeric runs in libgade's HIP kernels; this
+module only moves pointers.  It raises if the HIP library is not built or no
+GPU is visible -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgade.so")
+
+# every entry point declared in include/gade.h
+EXPORTS = (
+    "gade_last_error", "gade_malloc", "gade_free", "gade_memcpy_h2d", "gade_memcpy_d2h", "gade_memset",
+    "gade_device_sync", "gade_model_create", "gade_model_load", "gade_model_destroy", "gade_model_info",
+    "gade_set_time", "gade_apply", "gade_precond_setup", "gade_psolve", "gade_bicgstab", "gade_reset_workspace",
+    "gade_cn_step", "gade_cn_run", "gade_pulse", "gade_ik", "gade_imu", "gade_snorm", "gade_assembled_setup",
+    "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
+)
+
+_lib = None
+
+
+class GadeError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libgade.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GadeError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, f64, sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+        L.gade_last_error.restype = C.c_char_p
+        sig = {
+            "gade_malloc": [C.POINTER(vp), sz], "gade_free": [vp], "gade_memcpy_h2d": [vp, vp, sz],
+            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_device_sync": [],
+            "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
+            "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                C.POINTER(C.c_longlong), C.POINTER(i32)],
+            "gade_set_time": [vp, vp], "gade_apply": [vp, vp, vp, f64, f64], "gade_precond_setup": [vp],
+            "gade_psolve": [vp, vp, vp],
+            "gade_bicgstab": [vp, vp, vp, C.POINTER(i32), C.POINTER(f64), C.POINTER(i32)],
+            "gade_reset_workspace": [vp],
+            "gade_cn_step": [vp, vp, vp, f64, i32, C.POINTER(i32), C.POINTER(f64)],
+            "gade_cn_run": [vp, vp, f64, f64, f64, f64, i32, f64, i32, vp, vp, vp],
+            "gade_snorm": [vp, vp, C.POINTER(f64), vp], "gade_assembled_setup": [vp],
+            "gade_assembled_timedep": [vp], "gade_assembled_spmv": [vp, vp, vp, f64, f64, i32],
+            "gade_assembled_get": [vp, vp, vp, vp, vp], "gade_profile": [vp, i32],
+            "gade_profile_get": [vp, C.c_char_p, C.POINTER(f64), C.POINTER(C.c_longlong)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.gade_pulse.argtypes = [f64, f64, f64, f64, vp]
+        L.gade_pulse.restype = None
+        L.gade_ik.argtypes = [i32]
+        L.gade_ik.restype = i32
+        L.gade_imu.argtypes = [i32]
+        L.gade_imu.restype = f64
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise GadeError(f"libgade error {rc}: {lib().gade_last_error().decode()}")
+
+
+def pulse(E0, omega, cycles, t):
+    """f_alpha(t), alpha=0..5 (tools/solver_common.c:46-65) as complex128[6]."""
+    out = np.zeros(6, np.complex128)
+    lib().gade_pulse(E0, omega, cycles, t, out.ctypes.data)
+    return out
+
+
+class DeviceArray:
+    """A complex128 vector in HBM owned by libgade's allocator."""
+
+    def __init__(self, n, dtype=np.complex128):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().gade_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.size, arr.dtype)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, self.dtype)
+        assert arr.size == self.n
+        check(lib().gade_memcpy_h2d(self.ptr, arr.ctypes.data, self.nbytes))
+
+    def get(self):
+        out = np.empty(self.n, self.dtype)
+        check(lib().gade_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().gade_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().gade_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Propagator:
+    """Device model of one coupling directory: fused A-apply, block-LU, BiCGSTAB, CN."""
+
+    def __init__(self, directory, lmax, dt, h=1.0):
+        self.directory = directory
+        p = C.c_void_p()
+        check(lib().gade_model_load(directory.encode(), lmax, h, dt, None, C.byref(p)))
+        self.ptr = p.value
+        nb, bd, hn, nz, nk = (C.c_int() for _ in range(5))
+        nf = C.c_longlong()
+        check(lib().gade_model_info(self.ptr, C.byref(nb), C.byref(bd), C.byref(hn), C.byref(nz), C.byref(nf),
+                                    C.byref(nk)))
+        self.nblk, self.blkdim, self.hall_nnz, self.nnz_b, self.nnz_full, self.nkappa = (
+            nb.value, bd.value, hn.value, nz.value, nf.value, nk.value)
+        self.n = self.nblk * self.blkdim
+        self.lmax = lmax
+        self.dt = dt
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().gade_model_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_time(self, ft):
+        ft = np.ascontiguousarray(ft, np.complex128)
+        check(lib().gade_set_time(self.ptr, ft.ctypes.data))
+
+    def apply(self, x: DeviceArray, y: DeviceArray, cH=1.0, beta=0.0):
+        check(lib().gade_apply(self.ptr, x.ptr, y.ptr, cH, beta))
+
+    def psolve(self, b: DeviceArray, x: DeviceArray):
+        check(lib().gade_psolve(self.ptr, b.ptr, x.ptr))
+
+    def bicgstab(self, rhs: DeviceArray, x: DeviceArray, tol=1e-12, maxit=500):
+        it, te, na = C.c_int(maxit), C.c_double(tol), C.c_int(0)
+        check(lib().gade_bicgstab(self.ptr, rhs.ptr, x.ptr, C.byref(it), C.byref(te), C.byref(na)))
+        return it.value, te.value, na.value
+
+    def reset_workspace(self):
+        check(lib().gade_reset_workspace(self.ptr))
+
+    def cn_step(self, psi: DeviceArray, ft, tol=1e-12, maxit=500):
+        ft = np.ascontiguousarray(ft, np.complex128)
+        it, te = C.c_int(0), C.c_double(0)
+        check(lib().gade_cn_step(self.ptr, psi.ptr, ft.ctypes.data, tol, maxit, C.byref(it), C.byref(te)))
+        return it.value, te.value
+
+    def cn_run(self, psi: DeviceArray, E0, omega, cycles, nsteps, t0=0.0, tol=1e-12, maxit=500, timing=True):
+        iters = np.zeros(nsteps, np.int32)
+        tols = np.zeros(nsteps, np.float64)
+        usec = np.zeros(nsteps, np.float64)
+        check(lib().gade_cn_run(self.ptr, psi.ptr, E0, omega, cycles, t0, nsteps, tol, maxit, iters.ctypes.data,
+                                tols.ctypes.data, usec.ctypes.data if timing else None))
+        return iters, tols, usec
+
+    def snorm(self, x: DeviceArray, populations=False):
+        nrm = C.c_double()
+        pop = np.zeros(self.nblk) if populations else None
+        check(lib().gade_snorm(self.ptr, x.ptr, C.byref(nrm), pop.ctypes.data if populations else None))
+        return (nrm.value, pop) if populations else nrm.value
+
+    # assembled reference formulation
+    def assembled_timedep(self):
+        check(lib().gade_assembled_timedep(self.ptr))
+
+    def assembled_spmv(self, x: DeviceArray, y: DeviceArray, alpha=1.0, beta=0.0, with_S=False):
+        check(lib().gade_assembled_spmv(self.ptr, x.ptr, y.ptr, alpha, beta, int(with_S)))
+
+    def assembled_get(self, submap=True, values=True):
+        Ap = np.zeros(self.n + 1, np.int32)
+        Ai = np.zeros(self.nnz_full, np.int32)
+        sm = np.zeros(self.nnz_full, np.int32) if submap else None
+        Ax = np.zeros(self.nnz_full, np.complex128) if values else None
+        check(lib().gade_assembled_get(self.ptr, Ap.ctypes.data, Ai.ctypes.data,
+                                       sm.ctypes.data if submap else None, Ax.ctypes.data if values else None))
+        return Ap, Ai, sm, Ax
+
+    def profile(self, enable=True):
+        check(lib().gade_profile(self.ptr, int(enable)))
+
+    def profile_get(self, name):
+        t, c = C.c_double(), C.c_longlong()
+        check(lib().gade_profile_get(self.ptr, name.encode(), C.byref(t), C.byref(c)))
+        return t.value, c.value
+
+
+def read_psi0(directory, n):
+    """psi0.vec: int32 n, complex128[n], int32 perm[n] (tools/dirac_solver.c:68-108)."""
+    raw = open(os.path.join(directory, "psi0.vec"), "rb").read()
+    nn = int(np.frombuffer(raw, np.int32, 1)[0])
+    if nn != n:
+        raise GadeError(f"psi0.vec has {nn} entries, the model has {n}")
+    psi = np.frombuffer(raw, np.complex128, n, 4).copy()
+    perm = np.frombuffer(raw, np.int32, n, 4 + 16 * n).copy()
+    return psi, perm

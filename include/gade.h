@@ -1,0 +1,152 @@
+/*
+ * gade.h -- MI355X-native Crank-Nicolson propagator for the GaDE hot path.
+ *
+ * C ABI of libgade.so: plain pointers and sizes only.  Complex numbers are
+ * interleaved (re, im) doubles, i.e. the layout of C99 `double complex`,
+ * which is the reference's csr_data_t (tools/types.h:28-35).  Vectors are
+ * block-major: psi = [psi_0; ...; psi_{nblk-1}], psi_r of length blkdim.
+ *
+ * Device pointers ("_dev") live in HBM of the current HIP device; the
+ * library runs every kernel on the stream given to gade_model_create
+ * (0 = the null stream) and returns only after the work is complete unless
+ * a function says otherwise.
+ *
+ * Every function returns 0 on success and a negative code on error;
+ * gade_last_error() returns a message for the last failure on this thread.
+ *
+ * Entry points with a counterpart in the reference cite it; those that the
+ * reference lacks say so.
+ */
+#ifndef GADE_H
+#define GADE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GADE_OK 0
+#define GADE_EINVAL -1
+#define GADE_EHIP -2
+#define GADE_ENOMEM -3
+#define GADE_EIO -4
+#define GADE_ENOTCONV -5
+#define GADE_EPIVOT -6
+
+typedef struct gade_model gade_model_t;
+
+/* Host-side description of the coupling files that dumpCouplings and
+ * tools/main.py write (one shared radial pattern for g, h0 and s).
+ * All pointers are host pointers, complex arrays interleaved (re, im). */
+typedef struct {
+    int nblk;            /* Hall block rows kept (H.csr dim) */
+    const int *hall_Ap;  /* [nblk+1] */
+    const int *hall_Ai;  /* [hall_nnz] */
+    const int *hall_perm;/* [nblk] compressed -> original angular index, NULL = identity */
+    int lmax;            /* number of l values (Nl) */
+    int nang;            /* dimension of H0l/H1l (original angular space) */
+    const int *const *H_Ap;     /* [2*lmax]: H0l0, H1l0, H0l1, ... */
+    const int *const *H_Ai;
+    const double *const *H_Ax;
+    int blkdim;          /* radial dimension */
+    const int *rad_Ap;   /* [blkdim+1] shared radial pattern */
+    const int *rad_Ai;   /* [nnz_b] */
+    const double *const *g;  /* [24*lmax] values, index alpha*4*lmax + l*4 + n (tools/dirac_solver.c:303-313) */
+    const double *const *h0; /* [4] */
+    const double *const *s;  /* [3] */
+} gade_inputs_t;
+
+const char *gade_last_error(void);
+
+/* Device helpers so that callers without another GPU runtime can stage data. */
+int gade_malloc(void **ptr_dev, size_t bytes);
+int gade_free(void *ptr_dev);
+int gade_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
+int gade_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
+int gade_memset(void *ptr_dev, int value, size_t bytes);
+int gade_device_sync(void);
+
+/* Builds the device model: uploads couplings once, precomputes the block
+ * gathers (tools/gpu_solver.cu:137-236), the kappa-grouped row tiles and the
+ * per-kappa diagonal operators S(k) +- i h dt/2 H0(k) (tools/solver.c:409-458).
+ * stream: a hipStream_t or NULL. */
+int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream, gade_model_t **out);
+/* Same, reading H.csr, H0l*, H1l*, g*, h0*, s* from a directory
+ * (tools/dirac_solver.c:223-342). */
+int gade_model_load(const char *dir, int lmax, double h, double dt, void *stream, gade_model_t **out);
+int gade_model_destroy(gade_model_t *m);
+/* nblk, blkdim, hall_nnz, nnz_b, nnz of the equivalent assembled Hfull, nkappa (distinct) */
+int gade_model_info(const gade_model_t *m, int *nblk, int *blkdim, int *hall_nnz, int *nnz_b,
+                    long long *nnz_full, int *nkappa);
+
+/* Per-step G-sums (tools/solver.c:236-287; tools/gpu_solver.cu:50-82) for
+ * time coefficients ft[6] (complex, interleaved: 12 doubles). */
+int gade_set_time(gade_model_t *m, const double *ft);
+
+/* Fused operator, no assembly (tools/solver.c:327-395 evaluated matrix-free):
+ *   y = (S + cH * Hfull(t)) x + beta * y     with cH in {+1, -1, 0}.
+ * Hfull includes the stationary i h dt/2 H0 blocks.  beta = 0 never reads y. */
+int gade_apply(gade_model_t *m, const double *x_dev, double *y_dev, double cH, double beta);
+
+/* Preconditioner P = S + i h dt/2 H0 per block (tools/solver.c:460-485):
+ * exact LU of each distinct-kappa block on the GPU (FP64), then x = P^{-1} rhs
+ * with chunked level-scheduled triangular solves.  gade_precond_setup is
+ * called lazily by gade_psolve / gade_bicgstab. */
+int gade_precond_setup(gade_model_t *m);
+int gade_psolve(gade_model_t *m, const double *rhs_dev, double *x_dev);
+
+/* Right-preconditioned BiCGSTAB of tools/gpu_bicgstab.c:133-307 on A = S + Hfull(t),
+ * entirely on the device.  *iters: in max iterations, out iterations
+ * (break-before-increment convention of the reference); *tol_error: in
+ * relative tolerance, out ||r||/||b||.  napply (optional) returns the number
+ * of A applications.  The workspace persists across calls, as the
+ * reference's does. */
+int gade_bicgstab(gade_model_t *m, const double *rhs_dev, double *x_dev, int *iters, double *tol_error,
+                  int *napply);
+int gade_reset_workspace(gade_model_t *m);
+
+/* One CN step (tools/dirac_solver.c:503-577): given f(t_{n+1}) in ft,
+ * rhs = (S - Hfull) psi, psi <- BiCGSTAB(S + Hfull, rhs, x0 = psi). */
+int gade_cn_step(gade_model_t *m, double *psi_dev, const double *ft, double tol, int maxit, int *iters,
+                 double *tol_error);
+
+/* nsteps CN steps of the sin^2 pulse (E0, omega, cycles) starting at time t0
+ * (time accumulated as time += dt, tools/dirac_solver.c:509).  Per-step
+ * iterations, tolerances and microseconds are written when non-NULL. */
+int gade_cn_run(gade_model_t *m, double *psi_dev, double E0, double omega, double cycles, double t0,
+                int nsteps, double tol, int maxit, int *iters, double *tol_out, double *usec);
+
+/* Pulse coefficients f_alpha(t), alpha = 0..5 (tools/solver_common.c:46-65),
+ * as 12 doubles (complex with zero imaginary part). */
+void gade_pulse(double E0, double omega, double cycles, double t, double *ft);
+int gade_ik(int i);       /* tools/solver_common.c:30-37 */
+double gade_imu(int i);   /* tools/solver_common.c:39-44 */
+
+/* S-norm <x|S|x> and per-block populations |x_r|^2_S (FP64 on the device).
+ * pop_host (optional) receives nblk doubles.  No reference counterpart on the
+ * solver side: the reference computes these offline (src/hamiltonian.h:1268-1307). */
+int gade_snorm(gade_model_t *m, const double *x_dev, double *norm, double *pop_host);
+
+/* Assembled reference formulation (kept as the plain-FP64 HBM comparator):
+ * builds Hfull Ap/Ai/Ai_sub_map on the device (tools/csr.c:653-743), assembles
+ * Hfull.Ax each step (tools/gpu_solver.cu:84-135) and multiplies it with a
+ * hand-written CSR SpMV (replacing hipsparseSpMV, tools/gpu_sparse.c:267-313). */
+int gade_assembled_setup(gade_model_t *m);
+int gade_assembled_timedep(gade_model_t *m);  /* uses the ft of the last gade_set_time */
+/* y = alpha * Hfull x + beta * y, then y += S x when with_S
+ * (the rhs of tools/dirac_solver.c:535-536 is alpha = -1, beta = 0, with_S = 1) */
+int gade_assembled_spmv(gade_model_t *m, const double *x_dev, double *y_dev, double alpha, double beta, int with_S);
+/* device views of the assembled structure (int32) for parity checks */
+int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, double *Ax);
+
+/* Timing of the last call of each kernel family, measured with HIP events on
+ * the model's stream when profiling is enabled (gade_profile(m, 1)). */
+int gade_profile(gade_model_t *m, int enable);
+int gade_profile_get(gade_model_t *m, const char *name, double *total_ms, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,201 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) and oracle/_ref/libref.so.
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref = None
+
+cplx_p = np.ctypeslib.ndpointer(np.complex128, flags="C_CONTIGUOUS")
+int_p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+dbl_p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+long_p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: run `make -C {HERE}`")
+        L = C.CDLL(path)
+        L.or_ik.restype = C.c_int
+        L.or_ik.argtypes = [C.c_int]
+        L.or_imu.restype = C.c_double
+        L.or_imu.argtypes = [C.c_int]
+        L.or_pulse.argtypes = [C.c_double] * 4 + [cplx_p]
+        L.or_locate.restype = C.c_int
+        L.or_locate.argtypes = [int_p, C.c_int, C.c_int]
+        L.or_unblock_nnz.restype = C.c_longlong
+        L.or_unblock_nnz.argtypes = [C.c_int, int_p, int_p, C.c_int]
+        L.or_unblock.argtypes = [C.c_int, int_p, int_p, C.c_int, int_p, int_p, int_p, int_p, int_p]
+        L.or_model_load.restype = C.c_void_p
+        L.or_model_load.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int]
+        L.or_model_free.argtypes = [C.c_void_p]
+        L.or_model_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3 + [C.POINTER(C.c_longlong),
+                                                                              C.POINTER(C.c_int)]
+        for name in ("or_model_hfull_Ap", "or_model_hfull_Ai", "or_model_hfull_submap",
+                     "or_model_S_Ap", "or_model_S_Ai"):
+            getattr(L, name).restype = C.POINTER(C.c_int)
+            getattr(L, name).argtypes = [C.c_void_p]
+        for name in ("or_model_hfull_Ax", "or_model_S_Ax"):
+            getattr(L, name).restype = C.POINTER(C.c_double)
+            getattr(L, name).argtypes = [C.c_void_p]
+        L.or_model_row_col.argtypes = [C.c_void_p, int_p, int_p, int_p, int_p, cplx_p, cplx_p]
+        L.or_model_timedep.argtypes = [C.c_void_p, cplx_p]
+        for name in ("or_model_spmv_HS", "or_model_spmv_H", "or_model_spmv_S"):
+            getattr(L, name).argtypes = [C.c_void_p, cplx_p, cplx_p]
+        L.or_model_matfree_H.argtypes = [C.c_void_p, cplx_p, cplx_p, cplx_p]
+        L.or_model_precond_setup.restype = C.c_int
+        L.or_model_precond_setup.argtypes = [C.c_void_p]
+        L.or_model_psolve.argtypes = [C.c_void_p, cplx_p, cplx_p]
+        L.or_bicgstab.argtypes = [C.c_void_p, cplx_p, cplx_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int)]
+        L.or_reset_workspace.argtypes = [C.c_void_p]
+        L.or_cn_run.restype = C.c_int
+        L.or_cn_run.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                C.c_double, C.c_int, cplx_p, int_p, dbl_p, long_p]
+        L.or_vec_read.restype = C.c_int
+        L.or_vec_read.argtypes = [C.c_char_p, C.c_int, cplx_p, int_p]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """oracle/_ref/libref.so built from the reference sources, or None if absent."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libref.so")
+        if not os.path.exists(path):
+            return None
+        # bicgstab.c declares zdotc_ as returning double complex by value: the
+        # MKL runtime must use the GNU Fortran calling convention for that.
+        os.environ.setdefault("MKL_INTERFACE_LAYER", "GNU,LP64")
+        R = C.CDLL(path)
+        R.ik.restype = C.c_int
+        R.ik.argtypes = [C.c_int]
+        R.imu.restype = C.c_double
+        R.imu.argtypes = [C.c_int]
+        R.sorted_list_locate.restype = C.c_int
+        R.sorted_list_locate.argtypes = [int_p, C.c_int, C.c_int]
+        _ref = R
+    return _ref
+
+
+def pulse(E0, omega, cycles, t):
+    out = np.zeros(6, np.complex128)
+    lib().or_pulse(E0, omega, cycles, t, out)
+    return out
+
+
+class Model:
+    """The reference CPU path on one input directory."""
+
+    def __init__(self, directory, lmax, dt, h=1.0, nthreads=1):
+        L = lib()
+        self.ptr = L.or_model_load(directory.encode(), lmax, h, dt, nthreads)
+        if not self.ptr:
+            raise RuntimeError(f"oracle failed to load {directory}")
+        nb, bd, hn, nz = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        nf = C.c_longlong()
+        L.or_model_info(self.ptr, C.byref(nb), C.byref(bd), C.byref(hn), C.byref(nf), C.byref(nz))
+        self.nblk, self.blkdim, self.hall_nnz, self.nnz_full, self.nnz_b = nb.value, bd.value, hn.value, nf.value, nz.value
+        self.n = self.nblk * self.blkdim
+        self.lmax = lmax
+        self.dt = dt
+        self._pre = False
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().or_model_free(self.ptr)
+            self.ptr = None
+
+    def _iarr(self, fn, n):
+        return np.ctypeslib.as_array(fn(self.ptr), shape=(n,)).copy()
+
+    def hfull_structure(self):
+        L = lib()
+        return (self._iarr(L.or_model_hfull_Ap, self.n + 1), self._iarr(L.or_model_hfull_Ai, self.nnz_full),
+                self._iarr(L.or_model_hfull_submap, self.nnz_full))
+
+    def hfull_values(self):
+        raw = np.ctypeslib.as_array(lib().or_model_hfull_Ax(self.ptr), shape=(2 * self.nnz_full,))
+        return raw.view(np.complex128).copy()
+
+    def S_matrix(self):
+        L = lib()
+        nnz = self.nblk * self.nnz_b
+        Ap = self._iarr(L.or_model_S_Ap, self.n + 1)
+        Ai = self._iarr(L.or_model_S_Ai, nnz)
+        Ax = np.ctypeslib.as_array(L.or_model_S_Ax(self.ptr), shape=(2 * nnz,)).view(np.complex128).copy()
+        return Ap, Ai, Ax
+
+    def row_col(self):
+        h = self.hall_nnz
+        a, b, ki, kj = (np.zeros(h, np.int32) for _ in range(4))
+        H0 = np.zeros(h * self.lmax, np.complex128)
+        H1 = np.zeros(h * self.lmax, np.complex128)
+        lib().or_model_row_col(self.ptr, a, b, ki, kj, H0, H1)
+        return a, b, ki, kj, H0, H1
+
+    def timedep(self, ft):
+        lib().or_model_timedep(self.ptr, np.ascontiguousarray(ft, np.complex128))
+
+    def apply(self, x, which="HS"):
+        out = np.zeros(self.n, np.complex128)
+        getattr(lib(), f"or_model_spmv_{which}")(self.ptr, np.ascontiguousarray(x, np.complex128), out)
+        return out
+
+    def matfree_H(self, ft, x):
+        out = np.zeros(self.n, np.complex128)
+        lib().or_model_matfree_H(self.ptr, np.ascontiguousarray(ft, np.complex128),
+                                 np.ascontiguousarray(x, np.complex128), out)
+        return out
+
+    def precond(self):
+        if not self._pre:
+            if lib().or_model_precond_setup(self.ptr):
+                raise RuntimeError("oracle preconditioner setup failed")
+            self._pre = True
+
+    def psolve(self, b):
+        self.precond()
+        x = np.zeros(self.n, np.complex128)
+        lib().or_model_psolve(self.ptr, np.ascontiguousarray(b, np.complex128), x)
+        return x
+
+    def bicgstab(self, rhs, x, tol=1e-12, maxit=500):
+        self.precond()
+        it, te, na = C.c_int(maxit), C.c_double(tol), C.c_int(0)
+        x = np.ascontiguousarray(x, np.complex128).copy()
+        lib().or_bicgstab(self.ptr, np.ascontiguousarray(rhs, np.complex128), x, C.byref(it), C.byref(te),
+                          C.byref(na))
+        return x, it.value, te.value, na.value
+
+    def reset_workspace(self):
+        lib().or_reset_workspace(self.ptr)
+
+    def cn_run(self, psi, E0, omega, cycles, nsteps, t0=0.0, tol=1e-12, maxit=500):
+        self.precond()
+        x = np.ascontiguousarray(psi, np.complex128).copy()
+        iters = np.zeros(nsteps, np.int32)
+        tols = np.zeros(nsteps, np.float64)
+        usec = np.zeros(nsteps, np.int64)
+        lib().or_cn_run(self.ptr, E0, omega, cycles, t0, nsteps, tol, maxit, x, iters, tols, usec)
+        return x, iters, tols, usec
+
+
+def read_psi0_vec(path, n):
+    out = np.zeros(n, np.complex128)
+    perm = np.zeros(n, np.int32)
+    if lib().or_vec_read(path.encode(), n, out, perm):
+        raise RuntimeError(f"cannot read {path}")
+    return out, perm

@@ -1,0 +1,117 @@
+"""The C-ABI library: it loads, exports every entry point that include/*.h
+declares, and its host-side pieces agree with the oracle (no GPU calls)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dirac_hydrogen_code_amd import gade
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        src = re.sub(r"//[^\n]*", "", src)
+        src = re.sub(r"^\s*#.*$", "", src, flags=re.M)
+        for stmt in src.split(";"):
+            stmt = " ".join(stmt.split())
+            if not stmt or stmt.startswith("typedef") or "{" in stmt or "}" in stmt:
+                stmt = stmt.split("}")[-1].strip()
+                if not stmt or stmt.startswith("typedef"):
+                    continue
+            m = re.match(r"^(?:extern\s+)?(?:static\s+inline\s+)?[\w\s\*]+?[\s\*](\w+)\s*\(", stmt)
+            if m and not stmt.startswith("static"):
+                names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = gade.lib()
+    names = declared_functions()
+    assert len(names) > 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(gade.EXPORTS) <= set(names)
+
+
+def test_index_maps_and_pulse_match_oracle():
+    L = gade.lib()
+    for i in range(2 * 41 * 40):
+        assert L.gade_ik(i) == O.lib().or_ik(i)
+        assert L.gade_imu(i) == O.lib().or_imu(i)
+    rng = np.random.default_rng(0)
+    for _ in range(100):
+        E0, w, cyc, t = rng.uniform(1, 100), rng.uniform(25, 75), float(rng.integers(1, 16)), rng.uniform(0, 3)
+        a, b = gade.pulse(E0, w, cyc, t), O.pulse(E0, w, cyc, t)
+        assert np.array_equal(a.view(np.float64), b.view(np.float64))
+
+
+class SparseCSR(C.Structure):
+    """include/csr.h sparse_csr_t"""
+    _fields_ = [(n, C.c_int) for n in ("nrows", "ncols", "nnz", "blk_dim", "blk_nnz", "npart", "row_beg",
+                                       "row_end", "local_offset")] + [
+        ("perm", C.POINTER(C.c_int)), ("row_cpu_dist", C.POINTER(C.c_int)), ("comm_pattern", C.c_void_p),
+        ("comm_pattern_size", C.c_void_p), ("n_comm_entries", C.c_void_p), ("recv_ptr", C.c_void_p),
+        ("send_ptr", C.c_void_p), ("send_vec", C.c_void_p), ("send_type", C.c_void_p),
+        ("Ap", C.POINTER(C.c_int)), ("Ai", C.POINTER(C.c_int)), ("Ax", C.c_void_p),
+        ("Ai_sub_map", C.POINTER(C.c_int)), ("is_link", C.c_int)]
+
+
+@pytest.mark.parametrize("cfg,lmax", [("tiny", 2), ("nondipole_small", 4), ("dipole", 1)])
+def test_host_unblock_bitexact_vs_oracle(datadir, cfg, lmax):
+    """The product's closed-form csr_unblock_matrix equals the reference's two-pass algorithm bit for bit."""
+    L = gade.lib()
+    d = datadir(cfg)
+    Hall, g = SparseCSR(), SparseCSR()
+    L.csr_read.argtypes = [C.c_char_p, C.POINTER(SparseCSR)]
+    L.csr_read((d + "/H.csr").encode(), C.byref(Hall))
+    L.csr_read((d + "/g0a0l0.csr").encode(), C.byref(g))
+    L.csr_block_params.argtypes = [C.POINTER(SparseCSR), C.c_int, C.c_int]
+    L.csr_block_params(C.byref(Hall), g.nrows, g.nnz)
+    out = SparseCSR()
+    L.csr_unblock_matrix.argtypes = [C.POINTER(SparseCSR)] * 3
+    L.csr_unblock_matrix(C.byref(out), C.byref(Hall), C.byref(g))
+    m = O.Model(d, lmax, 1e-3)
+    Ap, Ai, sm = m.hfull_structure()
+    nnz = m.nnz_full
+    assert out.nnz == nnz and out.nrows == m.n
+    assert np.array_equal(np.ctypeslib.as_array(out.Ap, (m.n + 1,)), Ap)
+    assert np.array_equal(np.ctypeslib.as_array(out.Ai, (nnz,)), Ai)
+    assert np.array_equal(np.ctypeslib.as_array(out.Ai_sub_map, (nnz,)), sm)
+    L.csr_free.argtypes = [C.POINTER(SparseCSR)]
+    for s in (out, Hall, g):
+        L.csr_free(C.byref(s))
+
+
+def test_load_rejects_truncated_file(tmp_path, datadir):
+    """A truncated .csr must fail cleanly (error code + message) before any device call."""
+    import shutil
+    d = tmp_path / "bad"
+    shutil.copytree(datadir("tiny"), d)
+    raw = (d / "h01.csr").read_bytes()
+    (d / "h01.csr").write_bytes(raw[:-40])
+    with pytest.raises(gade.GadeError):
+        gade.Propagator(str(d), 2, 1e-3)
+
+
+def test_load_rejects_mismatched_pattern(tmp_path, datadir):
+    """g/h0/s must share the g0a0l0 pattern (the reference indexes them alike without checking)."""
+    import shutil
+    from dirac_hydrogen_code_amd.couplings import csr_read, csr_write
+    d = tmp_path / "bad2"
+    shutil.copytree(datadir("tiny"), d)
+    s = csr_read(str(d / "s1.csr"))
+    Ap, Ai, Ax = s["Ap"].copy(), s["Ai"].copy(), s["Ax"].copy()
+    # drop the last entry of the last row
+    Ap[-1] -= 1
+    csr_write(str(d / "s1.csr"), Ap, Ai[:-1], Ax[:-1])
+    with pytest.raises(gade.GadeError):
+        gade.Propagator(str(d), 2, 1e-3)

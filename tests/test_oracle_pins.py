@@ -1,0 +1,319 @@
+"""Pins of the CPU oracle (oracle/gade_oracle.c) against the reference's own
+sources (oracle/_ref, built from tools/solver_common.c, tools/sorted_list.c,
+tools/bicgstab.c), closed forms, brute force and invariants.
+
+The reference ships no golden vectors or fixtures for this path (it has no
+test suite; tools/matrix_tool.c only prints self-consistency mismatches), so
+every pin here is one of: bit-exact agreement with the compiled reference
+source, a closed form, a brute-force dense computation, or an invariant that
+a plausible indexing mistake breaks (Hermiticity, norm conservation,
+eigenstate phase evolution).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy import sparse
+
+import oracle as O
+from conftest import dt_of
+from dirac_hydrogen_code_amd.couplings import csr_read, SOL
+
+REF = O.ref()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref not built (needs /root/reference)")
+
+
+# ---------------------------------------------------------------- A2: ik / imu
+def test_ik_imu_enumeration():
+    """Blocks of 4|k|: first 2|k| have k<0, next 2|k| k>0, mu = -|k|+1/2 .. |k|-1/2 (tools/solver_common.c:30-44)."""
+    L = O.lib()
+    i = 0
+    for ak in range(1, 41):
+        for sgn in (-1, 1):
+            for j in range(2 * ak):
+                assert L.or_ik(i) == sgn * ak
+                assert L.or_imu(i) == j - ak + 0.5
+                i += 1
+    assert i == 2 * 41 * 40
+
+
+@needs_ref
+def test_ik_imu_bitexact_vs_reference():
+    L = O.lib()
+    for i in range(2 * 65 * 64):
+        assert L.or_ik(i) == REF.ik(i)
+        a, b = L.or_imu(i), REF.imu(i)
+        assert a == b and math.copysign(1, a) == math.copysign(1, b)
+
+
+# ---------------------------------------------------------------- A1: pulse
+def test_pulse_closed_forms():
+    for E0, w, cyc in ((10, 50, 1), (3.3, 25.0, 15), (100, 75.0, 4)):
+        T = cyc * 2 * np.pi / w
+        for t in np.linspace(0, T, 37):
+            f = O.pulse(E0, w, cyc, t)
+            env = np.sin(np.pi * t / T) ** 2
+            assert np.all(f.imag == 0)
+            assert abs(f[0] + f[2] + f[4] - (E0 / w) * env * np.sin(w * t)) < 1e-13 * max(1, E0 / w)
+            assert abs(f[1] + f[3] + f[5] + (E0 / w) * env * np.cos(w * t)) < 1e-13 * max(1, E0 / w)
+    f0 = O.pulse(10, 50, 1, 0.0)
+    assert abs(f0.real - np.array([0, 0.05, 0, 0.05, 0, -0.1])).max() < 1e-16
+
+
+@needs_ref
+def test_pulse_bitexact_vs_reference():
+    class Pulse(C.Structure):
+        _fields_ = [("E0", C.c_double), ("omega", C.c_double), ("T", C.c_double)]
+    REF.beyondDipolePulse_init.argtypes = [C.POINTER(Pulse), C.c_double, C.c_double, C.c_double]
+    REF.beoyndDipolePulse_axialPart.argtypes = [C.POINTER(Pulse), C.c_double, O.cplx_p]
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        E0, w, cyc, t = rng.uniform(1, 100), rng.uniform(25, 75), float(rng.integers(1, 16)), rng.uniform(0, 3)
+        p = Pulse()
+        REF.beyondDipolePulse_init(C.byref(p), E0, w, cyc)
+        ref = np.zeros(6, np.complex128)
+        REF.beoyndDipolePulse_axialPart(C.byref(p), t, ref)
+        assert np.array_equal(O.pulse(E0, w, cyc, t).view(np.float64), ref.view(np.float64))
+
+
+# ---------------------------------------------------------------- sorted-list locate
+def test_locate_bruteforce_and_reference():
+    rng = np.random.default_rng(3)
+    L = O.lib()
+    for n in list(range(0, 70)) + [200, 1000]:
+        lst = np.unique(rng.integers(0, 3 * n + 5, size=n)).astype(np.int32)
+        for v in range(-2, 3 * n + 8, max(1, n // 10)):
+            want = int(np.searchsorted(lst, v, side="left"))
+            got = L.or_locate(lst, len(lst), v)
+            assert got == want
+            if REF is not None:
+                assert REF.sorted_list_locate(lst, len(lst), v) == want
+
+
+# ---------------------------------------------------------------- A3: unblocking
+def _rand_pattern(rng, n, density, sym=True, diag=True):
+    A = rng.random((n, n)) < density
+    if sym:
+        A = A | A.T
+    if diag:
+        np.fill_diagonal(A, True)
+    Ap = np.concatenate([[0], np.cumsum(A.sum(1))]).astype(np.int32)
+    Ai = np.concatenate([np.nonzero(A[i])[0] for i in range(n)]).astype(np.int32)
+    return A, Ap, Ai
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_unblock_bruteforce(seed):
+    rng = np.random.default_rng(seed)
+    nb, b = int(rng.integers(1, 7)), int(rng.integers(1, 6))
+    Hb, hAp, hAi = _rand_pattern(rng, nb, 0.4)
+    Sb, sAp, sAi = _rand_pattern(rng, b, 0.5, sym=False, diag=False)
+    L = O.lib()
+    nnz = L.or_unblock_nnz(nb, hAp, sAp, b)
+    assert nnz == Hb.sum() * Sb.sum()
+    Ap = np.zeros(nb * b + 1, np.int32)
+    Ai = np.zeros(max(nnz, 1), np.int32)
+    sm = np.zeros(max(nnz, 1), np.int32)
+    L.or_unblock(nb, hAp, hAi, b, sAp, sAi, Ap, Ai, sm)
+    dense = np.kron(Hb, Sb)
+    want_Ap = np.concatenate([[0], np.cumsum(dense.sum(1))])
+    assert np.array_equal(Ap, want_Ap)
+    for r in range(nb * b):
+        assert np.array_equal(Ai[Ap[r]:Ap[r + 1]], np.nonzero(dense[r])[0])
+    if nnz:
+        sm = sm[:nnz]
+        assert np.array_equal(np.sort(sm), np.arange(nnz))  # a permutation
+        rows = np.repeat(np.arange(nb * b), np.diff(Ap))
+        loc = 0
+        for R in range(nb):
+            for p in range(hAp[R], hAp[R + 1]):
+                for a in range(b):
+                    for k in range(sAp[a], sAp[a + 1]):
+                        slot = sm[loc]
+                        assert rows[slot] == R * b + a and Ai[slot] == hAi[p] * b + sAi[k]
+                        loc += 1
+
+
+# ---------------------------------------------------------------- A4/A6: assembly
+def _dense(Ap, Ai, Ax, n):
+    return sparse.csr_matrix((Ax, Ai, Ap), shape=(n, n)).toarray()
+
+
+@pytest.mark.parametrize("cfg", ["tiny", "nondipole_small"])
+def test_assembly_hermitian_and_linear(datadir, cfg):
+    """Hfull/(i dt/2) is Hermitian; Hfull is affine in f(t); f=0 gives ihdt*blockdiag(H0(kappa))."""
+    d = datadir(cfg)
+    lmax = {"tiny": 2, "nondipole_small": 4}[cfg]
+    dt = dt_of(cfg)
+    m = O.Model(d, lmax, dt)
+    Ap, Ai, _ = m.hfull_structure()
+    rng = np.random.default_rng(11)
+    f1 = rng.standard_normal(6) + 0j
+    f2 = rng.standard_normal(6) + 0j
+    mats = []
+    for f in (np.zeros(6, complex), f1, f2, f1 + f2):
+        m.timedep(f)
+        mats.append(_dense(Ap, Ai, m.hfull_values(), m.n))
+    H0, H1, H2, H12 = mats
+    for M in mats[1:]:
+        assert np.abs(M + M.conj().T).max() <= 1e-14 * np.abs(M).max()
+    assert np.abs((H12 - H0) - ((H1 - H0) + (H2 - H0))).max() <= 1e-13 * np.abs(H12).max()
+    # field-free part from the h0 files
+    Hall = csr_read(d + "/H.csr")
+    h = [csr_read(d + f"/h0{n}.csr") for n in range(4)]
+    s = [csr_read(d + f"/s{n}.csr") for n in range(3)]
+    b = m.blkdim
+    want = np.zeros((m.n, m.n), complex)
+    wantS = np.zeros((m.n, m.n), complex)
+    for r in range(m.nblk):
+        k = O.lib().or_ik(int(Hall["perm"][r]))
+        Hk = sum(_dense(h[n]["Ap"], h[n]["Ai"], h[n]["Ax"], b) * k ** n for n in range(4))
+        Sk = sum(_dense(s[n]["Ap"], s[n]["Ai"], s[n]["Ax"], b) * k ** n for n in range(3))
+        want[r * b:(r + 1) * b, r * b:(r + 1) * b] = 1j * dt / 2 * Hk
+        wantS[r * b:(r + 1) * b, r * b:(r + 1) * b] = Sk
+    assert np.abs(H0 - want).max() <= 1e-14 * np.abs(want).max()
+    SAp, SAi, SAx = m.S_matrix()
+    assert np.abs(_dense(SAp, SAi, SAx, m.n) - wantS).max() <= 1e-14 * np.abs(wantS).max()
+
+
+@pytest.mark.parametrize("cfg", ["tiny", "nondipole_small", "dipole"])
+def test_matfree_equals_assembled(datadir, cfg):
+    """tools/solver.c:538-759 vs the assembled operator, as tools/matrix_tool.c:470-496 checks."""
+    lmax = {"tiny": 2, "nondipole_small": 4, "dipole": 1}[cfg]
+    m = O.Model(datadir(cfg), lmax, dt_of(cfg))
+    x = np.full(m.n, 1j + 1)  # the reference's probe vector x = (i, i) -> here (1+i)
+    rng = np.random.default_rng(5)
+    xr = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    for f in (O.pulse(10, 50, 1, 0.0123), rng.standard_normal(6) + 0j):
+        m.timedep(f)
+        for v in (x, xr):
+            a, b = m.apply(v, "H"), m.matfree_H(f, v)
+            assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+
+
+def test_row_col_gathers(datadir):
+    """tools/gpu_solver.cu:156-188 gathers: orig indices via the H.csr perm, kappa via ik."""
+    d = datadir("nondipole_small")
+    m = O.Model(d, 4, dt_of("nondipole_small"))
+    ai, aj, ki, kj, H0, H1 = m.row_col()
+    Hall = csr_read(d + "/H.csr")
+    rows = np.repeat(np.arange(m.nblk), np.diff(Hall["Ap"]))
+    assert np.array_equal(ai, Hall["perm"][rows]) and np.array_equal(aj, Hall["perm"][Hall["Ai"]])
+    assert all(O.lib().or_ik(int(a)) == k for a, k in zip(ai, ki))
+    for l in range(4):
+        A = csr_read(d + f"/H0l{l}.csr")
+        D = _dense(A["Ap"], A["Ai"], A["Ax"], A["n"])
+        assert np.array_equal(H0[l::4], D[ai, aj])
+
+
+# ---------------------------------------------------------------- A5/A9: preconditioner
+@pytest.mark.parametrize("cfg", ["tiny", "dipole"])
+def test_psolve_equals_dense_solve(datadir, cfg):
+    lmax = {"tiny": 2, "dipole": 1}[cfg]
+    dt = dt_of(cfg)
+    m = O.Model(datadir(cfg), lmax, dt)
+    m.timedep(np.zeros(6, complex))
+    Ap, Ai, _ = m.hfull_structure()
+    SAp, SAi, SAx = m.S_matrix()
+    P = _dense(SAp, SAi, SAx, m.n) + _dense(Ap, Ai, m.hfull_values(), m.n)  # f=0: S + ihdt H0
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    x = m.psolve(b)
+    want = np.linalg.solve(P, b)
+    assert np.abs(x - want).max() <= 1e-11 * np.abs(want).max()
+
+
+# ---------------------------------------------------------------- A8: BiCGSTAB
+def test_bicgstab_equals_direct_solve(datadir):
+    cfg = "tiny"
+    dt = dt_of(cfg)
+    m = O.Model(datadir(cfg), 2, dt)
+    f = O.pulse(60, 50, 1, 0.05)
+    m.timedep(f)
+    Ap, Ai, _ = m.hfull_structure()
+    SAp, SAi, SAx = m.S_matrix()
+    A = _dense(SAp, SAi, SAx, m.n) + _dense(Ap, Ai, m.hfull_values(), m.n)
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    x, it, te, na = m.bicgstab(b, np.zeros(m.n, complex), tol=1e-13)
+    assert te < 1e-13
+    assert np.linalg.norm(b - A @ x) <= 1e-13 * np.linalg.norm(b) * 10
+    want = np.linalg.solve(A, b)
+    assert np.abs(x - want).max() <= 1e-10 * np.abs(want).max()
+
+
+@needs_ref
+@pytest.mark.parametrize("tol", [1e-12, 1e-16])
+def test_bicgstab_vs_reference_source(datadir, tol):
+    """The reference's own bicgstab() (tools/bicgstab.c, BLAS-1 from MKL) on the oracle operator."""
+    cfg = "dipole"
+    dt = dt_of(cfg)
+    m = O.Model(datadir(cfg), 1, dt)
+    m.precond()
+    psi, _ = O.read_psi0_vec(datadir(cfg) + "/psi0.vec", m.n)
+    rng = np.random.default_rng(4)
+    x0 = psi + 1e-3 * (rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n))
+
+    class WSP(C.Structure):
+        _fields_ = [(f, C.c_void_p) for f in ("r", "r0", "v", "p", "y", "z", "s", "t")]
+
+    REF.bicgstab.argtypes = [C.c_void_p, C.c_void_p, O.cplx_p, O.cplx_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                             C.c_void_p, C.POINTER(WSP), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L = O.lib()
+    w = WSP()
+    xr = x0.copy()
+    xo = x0.copy()
+    m.reset_workspace()
+    for step in range(3):  # the workspace carries p and v across calls in both
+        m.timedep(O.pulse(10, 50, 1, dt * (step + 1)))
+        rhs = m.apply(xr, "S") - m.apply(xr, "H")
+        rhs_o = m.apply(xo, "S") - m.apply(xo, "H")
+        it, te = C.c_int(500), C.c_double(tol)
+        REF.bicgstab(C.cast(L.or_cb_spmv, C.c_void_p).value, m.ptr, rhs, xr, m.n, m.n, 0,
+                     C.cast(L.or_cb_psolve, C.c_void_p).value, m.ptr, C.byref(w), C.byref(it), C.byref(te))
+        xo, it2, te2, _ = m.bicgstab(rhs_o, xo, tol=tol)
+        assert it.value == it2
+        assert abs(te.value - te2) <= 1e-6 * te.value + 1e-300
+        assert np.abs(xo - xr).max() <= 1e-13 * np.abs(xr).max()
+
+
+# ---------------------------------------------------------------- whole CN step
+def test_field_free_eigenstate_phase(datadir):
+    """E0=0, psi0 an eigenvector of (H0, S): psi_n = ((1 - i dt E/2)/(1 + i dt E/2))^n psi0."""
+    cfg = "dipole"
+    d = datadir(cfg)
+    dt = dt_of(cfg)
+    m = O.Model(d, 1, dt)
+    psi, _ = O.read_psi0_vec(d + "/psi0.vec", m.n)
+    import json
+    E = json.load(open(d + "/params.json"))["e0"]
+    n = 25
+    x, iters, tols, _ = m.cn_run(psi, 0.0, 50.0, 1.0, n, tol=1e-14)
+    ratio = (1 - 1j * dt * E / 2) / (1 + 1j * dt * E / 2)
+    want = ratio ** n * psi
+    assert np.abs(x - want).max() <= 1e-10 * np.abs(psi).max()
+
+
+def test_norm_conservation(datadir):
+    """H(t) Hermitian and the same H(t_{n+1}) on both sides: <psi|S|psi> is conserved."""
+    cfg = "dipole"
+    d = datadir(cfg)
+    m = O.Model(d, 1, dt_of(cfg), nthreads=4)
+    psi, _ = O.read_psi0_vec(d + "/psi0.vec", m.n)
+    x, iters, tols, _ = m.cn_run(psi, 10.0, 50.0, 1.0, 200, tol=1e-12)
+    n0 = np.vdot(psi, m.apply(psi, "S")).real
+    n1 = np.vdot(x, m.apply(x, "S")).real
+    assert abs(n0 - 1) < 1e-12
+    assert abs(n1 - n0) < 1e-10
+    assert np.all(tols < 1e-12)
+
+
+def test_oracle_rejects_corrupt_file(tmp_path, datadir):
+    import shutil
+    d = tmp_path / "bad"
+    shutil.copytree(datadir("tiny"), d)
+    raw = (d / "g0a0l0.csr").read_bytes()
+    (d / "g0a0l0.csr").write_bytes(raw[: len(raw) // 2])
+    with pytest.raises(RuntimeError):
+        O.Model(str(d), 2, 1e-3)
