@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,7 +13,6 @@
 
 #include "gade_internal.h"
 
-using gade::kChunk;
 using gade::kSoL;
 
 namespace gade {
@@ -202,61 +202,91 @@ __global__ __launch_bounds__(256) void k_angular(const double2 *__restrict__ x, 
 // fused operator, phase B: radial SpMM on a kappa-grouped tile of row blocks
 //   y_r = D_sel(k) x_r + cH * sum_l [ (Gs0+k Gs1) Z0 + (Gs2+k Gs3) Z2 + (Gts0+k Gts2) W0 + (Gts1+k Gts3) W1 ]
 //         + beta y_r
+// One wave per (tile, 64-row radial chunk).  The chunk's column window of each
+// V vector is staged in LDS once and read by every slot; the chunk index is
+// the fastest-varying part of blockIdx so that with 8 chunks each XCD keeps
+// one chunk's slice of the G-sums resident in its L2.
 // ---------------------------------------------------------------------------
 template <int R>
 __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, const double2 *__restrict__ V,
                                                const double2 *__restrict__ GE, const double2 *__restrict__ D,
-                                               const int *__restrict__ ell_col, const int *__restrict__ tiles,
+                                               const int *__restrict__ ell_lcol, const int *__restrict__ locj,
+                                               const int *__restrict__ nloc, const int *__restrict__ tiles,
                                                const int *__restrict__ tile_q, const double *__restrict__ tile_k,
-                                               int nblk, int n, int ellw, int lmax, int nq, int sel, double cH,
-                                               double beta, double2 *__restrict__ y) {
-    const int tile = blockIdx.x;
-    const int a = blockIdx.y * blockDim.x + threadIdx.x;
-    if (a >= n) return;
+                                               int nblk, int n, int ellw, int lmax, int nq, int nchunk, int maxloc,
+                                               int sel, double cH, double beta, double2 *__restrict__ y) {
+    extern __shared__ double2 sm[];  // [R][maxloc]
+    const int chunk = blockIdx.x % nchunk, tile = blockIdx.x / nchunk;
+    const int lane = threadIdx.x;
+    const int a = chunk * 64 + lane;
+    const bool valid = a < n;
     int rows[R];
 #pragma unroll
     for (int i = 0; i < R; i++) rows[i] = tiles[tile * R + i];
     const double k = tile_k[tile];
     const int q = tile_q[tile];
-    double2 acc[R], dacc[R];
-#pragma unroll
-    for (int i = 0; i < R; i++) acc[i] = dacc[i] = make_double2(0, 0);
+    const int nl = nloc[chunk];
+    const int *lj = locj + (size_t)chunk * maxloc;
     const size_t plane = (size_t)nblk * n;
     const size_t slab = (size_t)ellw * n;
+    double2 acc[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) acc[i] = make_double2(0, 0);
     if (cH != 0.0) {
         for (int l = 0; l < lmax; l++) {
-            const double2 *G = GE + (size_t)(l * 8) * slab;
-            const double2 *Vl = V + (size_t)(l * 4) * plane;
-            for (int s = 0; s < ellw; s++) {
-                const size_t e = (size_t)s * n + a;
-                const int j = ell_col[e];
-                double2 g0 = G[e], g1 = G[slab + e], g2 = G[2 * slab + e], g3 = G[3 * slab + e];
-                double2 t0 = G[4 * slab + e], t1 = G[5 * slab + e], t2 = G[6 * slab + e], t3 = G[7 * slab + e];
-                double2 M0 = cadd(g0, cscale(g1, k)), M1 = cadd(g2, cscale(g3, k));
-                double2 M2 = cadd(t0, cscale(t2, k)), M3 = cadd(t1, cscale(t3, k));
+#pragma unroll 1
+            for (int v = 0; v < 4; v++) {
+                const double2 *Vv = V + (size_t)(l * 4 + v) * plane;
 #pragma unroll
                 for (int i = 0; i < R; i++) {
                     if (rows[i] < 0) continue;
-                    const size_t vb = (size_t)rows[i] * n + j;
-                    double2 acc_i = acc[i];
-                    acc_i = cfma(M0, Vl[vb], acc_i);
-                    acc_i = cfma(M1, Vl[plane + vb], acc_i);
-                    acc_i = cfma(M2, Vl[2 * plane + vb], acc_i);
-                    acc_i = cfma(M3, Vl[3 * plane + vb], acc_i);
-                    acc[i] = acc_i;
+                    const double2 *src = Vv + (size_t)rows[i] * n;
+                    for (int idx = lane; idx < nl; idx += 64) sm[i * maxloc + idx] = src[lj[idx]];
                 }
+                __syncthreads();
+                if (valid) {
+                    // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
+                    const int m1 = (v < 2) ? 2 * v : 4 + (v - 2);
+                    const int m2 = (v < 2) ? 2 * v + 1 : 6 + (v - 2);
+                    const double2 *G1 = GE + (size_t)(l * 8 + m1) * slab;
+                    const double2 *G2 = GE + (size_t)(l * 8 + m2) * slab;
+                    for (int s = 0; s < ellw; s++) {
+                        const size_t e = (size_t)s * n + a;
+                        const int lc = ell_lcol[e];
+                        const double2 g1 = G1[e], g2 = G2[e];
+                        const double2 M = make_double2(fma(k, g2.x, g1.x), fma(k, g2.y, g1.y));
+#pragma unroll
+                        for (int i = 0; i < R; i++) {
+                            if (rows[i] < 0) continue;
+                            acc[i] = cfma(M, sm[i * maxloc + lc], acc[i]);
+                        }
+                    }
+                }
+                __syncthreads();
             }
         }
     }
+    // stationary diagonal operator on x_r
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (rows[i] < 0) continue;
+        const double2 *src = x + (size_t)rows[i] * n;
+        for (int idx = lane; idx < nl; idx += 64) sm[i * maxloc + idx] = src[lj[idx]];
+    }
+    __syncthreads();
+    if (!valid) return;
     const double2 *Dq = D + ((size_t)sel * nq + q) * slab;
+    double2 dacc[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) dacc[i] = make_double2(0, 0);
     for (int s = 0; s < ellw; s++) {
         const size_t e = (size_t)s * n + a;
-        const int j = ell_col[e];
-        double2 d = Dq[e];
+        const int lc = ell_lcol[e];
+        const double2 d = Dq[e];
 #pragma unroll
         for (int i = 0; i < R; i++) {
             if (rows[i] < 0) continue;
-            dacc[i] = cfma(d, x[(size_t)rows[i] * n + j], dacc[i]);
+            dacc[i] = cfma(d, sm[i * maxloc + lc], dacc[i]);
         }
     }
 #pragma unroll
@@ -300,22 +330,23 @@ __global__ __launch_bounds__(256) void k_band_lu(double2 *AB, int n, int band, i
     }
 }
 
-// Per (kappa, chunk): inverses of the 64x64 diagonal chunks of L (unit) and U,
+// Per (kappa, chunk): inverses of the CHxCH diagonal chunks of L (unit) and U,
 // stored [col][row], and the band coupling to the neighbouring chunk.
-__global__ __launch_bounds__(64) void k_chunk_inverse(const double2 *AB, int n, int band, int nch, double2 *Linv,
+template <int CH>
+__global__ __launch_bounds__(CH) void k_chunk_inverse(const double2 *AB, int n, int band, int nch, double2 *Linv,
                                                       double2 *Uinv, double2 *Loff, double2 *Uoff) {
     const int q = blockIdx.x, ch = blockIdx.y, j = threadIdx.x;
     const int w = 2 * band + 1;
     const double2 *A = AB + (size_t)q * n * w;
-    const int base = ch * kChunk;
-    double2 *Li = Linv + ((size_t)q * nch + ch) * kChunk * kChunk + (size_t)j * kChunk;
-    double2 *Ui = Uinv + ((size_t)q * nch + ch) * kChunk * kChunk + (size_t)j * kChunk;
+    const int base = ch * CH;
+    double2 *Li = Linv + ((size_t)q * nch + ch) * CH * CH + (size_t)j * CH;
+    double2 *Ui = Uinv + ((size_t)q * nch + ch) * CH * CH + (size_t)j * CH;
     auto at = [&](int gi, int gj) -> double2 {  // band entry (gi, gj), zero outside
         if (gi < 0 || gj < 0 || gi >= n || gj >= n || abs(gi - gj) > band) return make_double2(0, 0);
         return A[(size_t)gi * w + (gj - gi + band)];
     };
     // column j of L_cc^{-1}: forward substitution (rows beyond n act as identity)
-    for (int i = 0; i < kChunk; i++) {
+    for (int i = 0; i < CH; i++) {
         double2 v = make_double2(i == j ? 1.0 : 0.0, 0);
         if (i > j && base + i < n) {
             for (int m2 = max(j, i - band); m2 < i; m2++) v = csub(v, cmul(at(base + i, base + m2), Li[m2]));
@@ -323,13 +354,11 @@ __global__ __launch_bounds__(64) void k_chunk_inverse(const double2 *AB, int n, 
         Li[i] = (i < j) ? make_double2(0, 0) : v;
     }
     // column j of U_cc^{-1}: back substitution
-    for (int i = kChunk - 1; i >= 0; i--) {
+    for (int i = CH - 1; i >= 0; i--) {
         double2 v = make_double2(0, 0);
         if (i <= j) {
-            if (base + i >= n) {
-                v = make_double2(i == j ? 1.0 : 0.0, 0);
-            } else {
-                v = make_double2(i == j ? 1.0 : 0.0, 0);
+            v = make_double2(i == j ? 1.0 : 0.0, 0);
+            if (base + i < n) {
                 for (int m2 = i + 1; m2 <= min(j, i + band); m2++) v = csub(v, cmul(at(base + i, base + m2), Ui[m2]));
                 v = cdiv(v, at(base + i, base + i));
             }
@@ -338,61 +367,69 @@ __global__ __launch_bounds__(64) void k_chunk_inverse(const double2 *AB, int n, 
     }
     // band couplings: lane j is a row of the chunk
     const int gi = base + j;
-    double2 *Lo = Loff + ((size_t)q * nch + ch) * band * kChunk;
-    double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * kChunk;
+    double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH;
+    double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH;
     for (int t = 0; t < band; t++) {
-        Lo[(size_t)t * kChunk + j] = (gi < n) ? at(gi, base - band + t) : make_double2(0, 0);
-        Uo[(size_t)t * kChunk + j] = (gi < n) ? at(gi, base + kChunk + t) : make_double2(0, 0);
+        Lo[(size_t)t * CH + j] = (gi < n) ? at(gi, base - band + t) : make_double2(0, 0);
+        Uo[(size_t)t * CH + j] = (gi < n) ? at(gi, base + CH + t) : make_double2(0, 0);
     }
 }
 
-// x = P^{-1} b per row block: one wave per block, chunk-level scheduling.
+// x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =
+// (block slot, row of the chunk).  Levels are chunks of CH rows: each chunk
+// takes the band coupling to the previous chunk, then its diagonal inverse.
+template <int CH>
 __global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, double2 *__restrict__ xo,
-                                               const int *__restrict__ blk_q, const int *__restrict__ ilv_inv,
-                                               const double2 *__restrict__ Linv, const double2 *__restrict__ Uinv,
-                                               const double2 *__restrict__ Loff, const double2 *__restrict__ Uoff,
-                                               int n, int band, int nch) {
+                                               const int *__restrict__ wblocks, const int *__restrict__ blk_q,
+                                               const int *__restrict__ ilv_inv, const double2 *__restrict__ Linv,
+                                               const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
+                                               const double2 *__restrict__ Uoff, int n, int band, int nch) {
+    constexpr int G = 64 / CH;
     extern __shared__ double2 smem[];
-    double2 *yv = smem;                   // nch*64
-    double2 *acc = smem + nch * kChunk;   // 64
-    const int r = blockIdx.x, lane = threadIdx.x;
-    const int q = blk_q[r];
-    const size_t cs = (size_t)kChunk * kChunk;
-    const double2 *bb = b + (size_t)r * n;
+    const int lane = threadIdx.x, g = lane / CH, i = lane % CH;
+    const int r = wblocks[blockIdx.x * G + g];
+    const int q = r >= 0 ? blk_q[r] : 0;
+    double2 *yv = smem + (size_t)g * (nch * CH + CH);
+    double2 *acc = yv + nch * CH;
+    const size_t cs = (size_t)CH * CH;
+    const double2 *bb = b + (size_t)(r >= 0 ? r : 0) * n;
     // forward sweep
     for (int ch = 0; ch < nch; ch++) {
-        const int gi = ch * kChunk + lane;
-        double2 v = (gi < n) ? bb[ilv_inv[gi]] : make_double2(0, 0);
+        const int gi = ch * CH + i;
+        double2 v = (r >= 0 && gi < n) ? bb[ilv_inv[gi]] : make_double2(0, 0);
         if (ch > 0) {
-            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * kChunk;
-            for (int t = 0; t < band; t++) v = csub(v, cmul(Lo[(size_t)t * kChunk + lane], yv[ch * kChunk - band + t]));
+            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH;
+            for (int t = 0; t < band; t++) v = csub(v, cmul(Lo[(size_t)t * CH + i], yv[ch * CH - band + t]));
         }
-        acc[lane] = v;
+        acc[i] = v;
         __syncthreads();
         const double2 *Li = Linv + ((size_t)q * nch + ch) * cs;
         double2 yi = make_double2(0, 0);
-        for (int j = 0; j <= lane; j++) yi = cfma(Li[(size_t)j * kChunk + lane], acc[j], yi);
-        yv[ch * kChunk + lane] = yi;
+#pragma unroll 4
+        for (int j = 0; j < CH; j++) yi = cfma(Li[(size_t)j * CH + i], acc[j], yi);
+        yv[gi] = yi;
         __syncthreads();
     }
     // backward sweep (in place)
     for (int ch = nch - 1; ch >= 0; ch--) {
-        const int gi = ch * kChunk + lane;
+        const int gi = ch * CH + i;
         double2 v = yv[gi];
         if (ch < nch - 1) {
-            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * kChunk;
-            for (int t = 0; t < band; t++)
-                v = csub(v, cmul(Uo[(size_t)t * kChunk + lane], yv[(ch + 1) * kChunk + t]));
+            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH;
+            for (int t = 0; t < band; t++) v = csub(v, cmul(Uo[(size_t)t * CH + i], yv[(ch + 1) * CH + t]));
         }
-        acc[lane] = v;
+        acc[i] = v;
         __syncthreads();
         const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs;
         double2 xi = make_double2(0, 0);
-        for (int j = lane; j < kChunk; j++) xi = cfma(Ui[(size_t)j * kChunk + lane], acc[j], xi);
+#pragma unroll 4
+        for (int j = 0; j < CH; j++) xi = cfma(Ui[(size_t)j * CH + i], acc[j], xi);
+        __syncthreads();
         yv[gi] = xi;
         __syncthreads();
     }
-    for (int gi = lane; gi < n; gi += kChunk) xo[(size_t)r * n + ilv_inv[gi]] = yv[gi];
+    if (r < 0) return;
+    for (int gi = i; gi < n; gi += CH) xo[(size_t)r * n + ilv_inv[gi]] = yv[gi];
 }
 
 // ---------------------------------------------------------------------------
@@ -738,7 +775,7 @@ unsigned grid_for(long long total, int threads, unsigned cap = 65535u * 4) {
 }
 
 void free_all(gade_model *m) {
-    void *ptrs[] = {m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
+    void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_wblocks, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
                     m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
                     m->d_V, m->d_ilv_inv, m->d_AB, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->d_flag,
                     m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_cnt,
@@ -764,11 +801,12 @@ int launch_angular(gade_model *m, const double2 *x) {
 }
 
 int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta) {
-    dim3 grid(m->ntiles, (m->n + 63) / 64);
+    dim3 grid(m->ntiles * m->nchunk);
+    size_t shm = sizeof(double2) * (size_t)m->tile_r * m->maxloc;
 #define RAD(R) \
-    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_col, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, m->lmax, m->nq, sel, cH, beta, y); break;
+    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), shm, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, m->lmax, m->nq, m->nchunk, m->maxloc, sel, cH, beta, y); break;
     switch (m->tile_r) {
-        RAD(1) RAD(2) RAD(4) RAD(8) RAD(16)
+        RAD(1) RAD(2) RAD(4) RAD(8)
         default:
             gade::set_error("unsupported tile width");
             return GADE_EINVAL;
@@ -886,6 +924,30 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
             ell_src[(size_t)s * n + a] = ok ? k : -1;
             if (ok) ell_pos[k] = s * n + a;
         }
+    // per 64-row chunk: distinct columns (staged in LDS by the radial kernel)
+    m->nchunk = (n + 63) / 64;
+    std::vector<std::vector<int>> cols(m->nchunk);
+    for (int c = 0; c < m->nchunk; c++) {
+        for (int a = c * 64; a < std::min(n, c * 64 + 64); a++)
+            for (int s2 = 0; s2 < ellw; s2++) cols[c].push_back(ell_col[(size_t)s2 * n + a]);
+        std::sort(cols[c].begin(), cols[c].end());
+        cols[c].erase(std::unique(cols[c].begin(), cols[c].end()), cols[c].end());
+        m->maxloc = std::max(m->maxloc, (int)cols[c].size());
+    }
+    std::vector<int> locj((size_t)m->nchunk * m->maxloc, 0), nloc(m->nchunk), ell_lcol((size_t)ellw * n);
+    for (int c = 0; c < m->nchunk; c++) {
+        nloc[c] = (int)cols[c].size();
+        std::copy(cols[c].begin(), cols[c].end(), locj.begin() + (size_t)c * m->maxloc);
+    }
+    for (int a = 0; a < n; a++)
+        for (int s2 = 0; s2 < ellw; s2++) {
+            const auto &cc = cols[a / 64];
+            ell_lcol[(size_t)s2 * n + a] = (int)(std::lower_bound(cc.begin(), cc.end(), ell_col[(size_t)s2 * n + a]) - cc.begin());
+        }
+    if ((size_t)8 * 16 * m->maxloc > 64 * 1024) {
+        gade::set_error("radial pattern too wide for the LDS window of the fused operator");
+        return fail(GADE_EINVAL);
+    }
     // transpose map: entry k=(a,j) -> position of (j,a)  (tools/csr.c:963-972 needs structural symmetry)
     for (int a = 0; a < n; a++)
         for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) {
@@ -938,6 +1000,9 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
     for (int i = 0; i < 3; i++) memcpy(&sbuf[(size_t)i * m->nnzb * 2], in->s[i], sizeof(double) * 2 * m->nnzb);
     for (int i = 0; i < 4; i++) memcpy(&hbuf[(size_t)i * m->nnzb * 2], in->h0[i], sizeof(double) * 2 * m->nnzb);
     if ((rc = upload(&m->d_ell_col, ell_col.data(), ell_col.size())) ||
+        (rc = upload(&m->d_ell_lcol, ell_lcol.data(), ell_lcol.size())) ||
+        (rc = upload(&m->d_locj, locj.data(), locj.size())) ||
+        (rc = upload(&m->d_nloc, nloc.data(), nloc.size())) ||
         (rc = upload(&m->d_ell_src, ell_src.data(), ell_src.size())) ||
         (rc = upload(&m->d_ell_pos, ell_pos.data(), ell_pos.size())) ||
         (rc = upload(&m->d_tmap, tmap.data(), tmap.size())) ||
@@ -1067,7 +1132,9 @@ int gade_precond_setup(gade_model_t *m) {
         return GADE_EINVAL;
     }
     m->band = std::max(band, 1);
-    m->nch = (n + kChunk - 1) / kChunk;
+    m->ch = m->band < 16 ? 16 : (m->band < 32 ? 32 : 64);
+    const int CH = m->ch;
+    m->nch = (n + CH - 1) / CH;
     const int w = 2 * m->band + 1, nq = m->nq;
     // P(k) = S(k) + ihdt H0(k) in band storage, from the device D operators (sel 0)
     std::vector<double2> Dh((size_t)nq * m->ellw * n);
@@ -1076,23 +1143,25 @@ int gade_precond_setup(gade_model_t *m) {
     for (int q = 0; q < nq; q++)
         for (int a = 0; a < n; a++)
             for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) {
-                int s = k - m->rad_Ap[a];
+                int s2 = k - m->rad_Ap[a];
                 int i = ilv[a], j = ilv[m->rad_Ai[k]];
-                AB[((size_t)q * n + i) * w + (j - i + m->band)] = Dh[((size_t)q * m->ellw + s) * n + a];
+                AB[((size_t)q * n + i) * w + (j - i + m->band)] = Dh[((size_t)q * m->ellw + s2) * n + a];
             }
     (void)nnzb;
     GADE_TRY(upload(&m->d_AB, AB.data(), AB.size()));
     GADE_TRY(upload(&m->d_ilv_inv, ilv_inv.data(), ilv_inv.size()));
-    const size_t cs = (size_t)kChunk * kChunk;
+    const size_t cs = (size_t)CH * CH;
     GADE_TRY(hip_alloc((void **)&m->d_Linv, sizeof(double2) * nq * m->nch * cs));
     GADE_TRY(hip_alloc((void **)&m->d_Uinv, sizeof(double2) * nq * m->nch * cs));
-    GADE_TRY(hip_alloc((void **)&m->d_Loff, sizeof(double2) * nq * m->nch * m->band * kChunk));
-    GADE_TRY(hip_alloc((void **)&m->d_Uoff, sizeof(double2) * nq * m->nch * m->band * kChunk));
+    GADE_TRY(hip_alloc((void **)&m->d_Loff, sizeof(double2) * nq * m->nch * m->band * CH));
+    GADE_TRY(hip_alloc((void **)&m->d_Uoff, sizeof(double2) * nq * m->nch * m->band * CH));
     GADE_HIP(hipMemset(m->d_flag, 0, sizeof(int)));
     hipLaunchKernelGGL(k_band_lu, dim3(nq), dim3(256), 0, m->stream, m->d_AB, n, m->band, m->d_flag);
     GADE_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_chunk_inverse, dim3(nq, m->nch), dim3(kChunk), 0, m->stream, m->d_AB, n, m->band, m->nch,
-                       m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff);
+#define INV(C) \
+    case C: hipLaunchKernelGGL(k_chunk_inverse<C>, dim3(nq, m->nch), dim3(C), 0, m->stream, m->d_AB, n, m->band, m->nch, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff); break;
+    switch (CH) { INV(16) INV(32) INV(64) }
+#undef INV
     GADE_HIP(hipGetLastError());
     int flag = 0;
     GADE_HIP(hipMemcpyAsync(&flag, m->d_flag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
@@ -1101,6 +1170,18 @@ int gade_precond_setup(gade_model_t *m) {
         gade::set_error("preconditioner: zero pivot in block LU");
         return GADE_EPIVOT;
     }
+    // waves of 64/CH blocks, blocks of one kappa together so their factor loads coincide
+    const int G = 64 / CH;
+    std::vector<int> wb;
+    for (int q = 0; q < nq; q++) {
+        std::vector<int> rows;
+        for (int r = 0; r < m->nblk; r++)
+            if (m->blk_q[r] == q) rows.push_back(r);
+        for (size_t i = 0; i < rows.size(); i += G)
+            for (int j = 0; j < G; j++) wb.push_back(i + j < rows.size() ? rows[i + j] : -1);
+    }
+    m->npw = (int)wb.size() / G;
+    GADE_TRY(upload(&m->d_wblocks, wb.data(), wb.size()));
     m->pre_ready = true;
     return GADE_OK;
 }
@@ -1114,9 +1195,12 @@ namespace {
 
 int psolve_async(gade_model *m, const double2 *b, double2 *x) {
     Prof pr(m, "psolve");
-    size_t shm = sizeof(double2) * ((size_t)m->nch * kChunk + kChunk);
-    hipLaunchKernelGGL(k_psolve, dim3(m->nblk), dim3(kChunk), shm, m->stream, b, x, m->d_blk_q, m->d_ilv_inv,
-                       m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->n, m->band, m->nch);
+    const int G = 64 / m->ch;
+    size_t shm = sizeof(double2) * (size_t)G * ((size_t)m->nch * m->ch + m->ch);
+#define PS(C) \
+    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(m->npw), dim3(64), shm, m->stream, b, x, m->d_wblocks, m->d_blk_q, m->d_ilv_inv, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->n, m->band, m->nch); break;
+    switch (m->ch) { PS(16) PS(32) PS(64) default: return GADE_EINVAL; }
+#undef PS
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -1428,6 +1512,31 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
         }
     }
     return GADE_OK;
+}
+
+int gade_set_param(gade_model_t *m, const char *name, int value) {
+    if (!m || !name) return GADE_EINVAL;
+    if (!strcmp(name, "tile_r")) {
+        if (value != 1 && value != 2 && value != 4 && value != 8) {
+            gade::set_error("tile_r must be 1, 2, 4 or 8");
+            return GADE_EINVAL;
+        }
+        GADE_HIP(hipStreamSynchronize(m->stream));
+        m->tile_r = value;
+        return build_tiles(m);
+    }
+    gade::set_error(std::string("unknown parameter ") + name);
+    return GADE_EINVAL;
+}
+
+int gade_get_param(const gade_model_t *m, const char *name, int *value) {
+    if (!m || !name || !value) return GADE_EINVAL;
+    if (!strcmp(name, "tile_r")) {
+        *value = m->tile_r;
+        return GADE_OK;
+    }
+    gade::set_error(std::string("unknown parameter ") + name);
+    return GADE_EINVAL;
 }
 
 int gade_profile(gade_model_t *m, int enable) {

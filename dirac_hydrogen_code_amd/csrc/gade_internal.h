@@ -54,6 +54,9 @@ struct gade_model {
 
     // radial pattern (ELL, transposed: [slot][row])
     int *d_ell_col = nullptr, *d_ell_src = nullptr, *d_tmap = nullptr, *d_ell_pos = nullptr;
+    // per 64-row radial chunk: distinct columns (LDS window) and ELL columns local to it
+    int nchunk = 0, maxloc = 0;
+    int *d_ell_lcol = nullptr, *d_locj = nullptr, *d_nloc = nullptr;
     double2 *d_g = nullptr;        // [24*lmax][nnzb]
     double2 *d_s = nullptr;        // [3][nnzb]
     double2 *d_h0 = nullptr;       // [4][nnzb]
@@ -71,7 +74,8 @@ struct gade_model {
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     // preconditioner
     bool pre_ready = false;
-    int band = 0, nch = 0;
+    int band = 0, nch = 0, ch = 16, npw = 0;   // half-bandwidth, chunks, chunk rows, psolve waves
+    int *d_wblocks = nullptr;      // [npw][64/ch] blocks served by each psolve wave (-1 = none)
     int *d_ilv_inv = nullptr;      // interleaved -> natural radial index
     double2 *d_AB = nullptr;       // [nq][n][2*band+1]
     double2 *d_Linv = nullptr, *d_Uinv = nullptr;  // [nq][nch][64 col][64 row]

@@ -23,6 +23,7 @@ EXPORTS = (
     "gade_set_time", "gade_apply", "gade_precond_setup", "gade_psolve", "gade_bicgstab", "gade_reset_workspace",
     "gade_cn_step", "gade_cn_run", "gade_pulse", "gade_ik", "gade_imu", "gade_snorm", "gade_assembled_setup",
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
+    "gade_set_param", "gade_get_param",
 )
 
 _lib = None
@@ -57,6 +58,7 @@ def lib():
             "gade_assembled_timedep": [vp], "gade_assembled_spmv": [vp, vp, vp, f64, f64, i32],
             "gade_assembled_get": [vp, vp, vp, vp, vp], "gade_profile": [vp, i32],
             "gade_profile_get": [vp, C.c_char_p, C.POINTER(f64), C.POINTER(C.c_longlong)],
+            "gade_set_param": [vp, C.c_char_p, i32], "gade_get_param": [vp, C.c_char_p, C.POINTER(i32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -209,6 +211,14 @@ class Propagator:
         check(lib().gade_assembled_get(self.ptr, Ap.ctypes.data, Ai.ctypes.data,
                                        sm.ctypes.data if submap else None, Ax.ctypes.data if values else None))
         return Ap, Ai, sm, Ax
+
+    def set_param(self, name, value):
+        check(lib().gade_set_param(self.ptr, name.encode(), int(value)))
+
+    def get_param(self, name):
+        v = C.c_int()
+        check(lib().gade_get_param(self.ptr, name.encode(), C.byref(v)))
+        return v.value
 
     def profile(self, enable=True):
         check(lib().gade_profile(self.ptr, int(enable)))

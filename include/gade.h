@@ -141,6 +141,12 @@ int gade_assembled_spmv(gade_model_t *m, const double *x_dev, double *y_dev, dou
 /* device views of the assembled structure (int32) for parity checks */
 int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, double *Ax);
 
+/* Tunable launch parameters (no reference counterpart; used by the autotuner):
+ *   "tile_r"  row blocks per radial tile (1, 2, 4, 8).
+ * Results do not depend on the choice. */
+int gade_set_param(gade_model_t *m, const char *name, int value);
+int gade_get_param(const gade_model_t *m, const char *name, int *value);
+
 /* Timing of the last call of each kernel family, measured with HIP events on
  * the model's stream when profiling is enabled (gade_profile(m, 1)). */
 int gade_profile(gade_model_t *m, int enable);
