@@ -152,45 +152,74 @@ __global__ void k_build_D(const double2 *__restrict__ s, const double2 *__restri
 // fused operator, phase A: angular contraction per row block
 //   Z0_l(r) = sum_c H0l[r,c] x_c      Z2_l(r) = sum_c H0l[r,c] k_c x_c
 //   W0_l(r) = sum_c H1l[r,c] x_c      W1_l(r) = sum_c H1l[r,c] k_c x_c
-// V layout [l][v][r][a]
+// V layout [l][v][r][a].  blockIdx.z selects a group of LB multipoles so the
+// accumulators stay small; the row's coefficients are staged in LDS and the
+// x loads of four block columns are in flight together.
 // ---------------------------------------------------------------------------
-template <int LM>
-__global__ __launch_bounds__(256) void k_angular(const double2 *__restrict__ x, const int *__restrict__ hAp,
-                                                 const int *__restrict__ hAi, const double2 *__restrict__ coef,
-                                                 const int *__restrict__ kap, int nblk, int n,
-                                                 double2 *__restrict__ V) {
+constexpr int kAngThreads = 256;
+constexpr int kAngLB = 4;
+
+template <int LB>
+__global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restrict__ x, const int *__restrict__ hAp,
+                                                         const int *__restrict__ hAi,
+                                                         const double2 *__restrict__ coef,
+                                                         const int *__restrict__ kap, int nblk, int n, int lmax,
+                                                         int maxrow, double2 *__restrict__ V) {
+    extern __shared__ double2 shc[];  // [maxrow][LB][2] coefficients, then [maxrow] (col, kappa) packed
     const int r = blockIdx.x;
     const int a = blockIdx.y * blockDim.x + threadIdx.x;
-    double2 z0[LM], z2[LM], w0[LM], w1[LM];
-#pragma unroll
-    for (int l = 0; l < LM; l++) {
-        z0[l] = z2[l] = w0[l] = w1[l] = make_double2(0, 0);
-    }
+    const int l0 = blockIdx.z * LB;
+    const int nlb = min(LB, lmax - l0);
     const int pb = hAp[r], pe = hAp[r + 1];
-    for (int p = pb; p < pe; p++) {
-        const int c = hAi[p];
-        const double kc = (double)kap[c];
-        double2 xv = (a < n) ? x[(size_t)c * n + a] : make_double2(0, 0);
-        double2 kx = cscale(xv, kc);
-        const double2 *cp = coef + (size_t)p * LM * 2;
+    const int len = pe - pb;
+    int2 *shk = (int2 *)(shc + (size_t)maxrow * LB * 2);
+    for (int e = threadIdx.x; e < len * LB * 2; e += blockDim.x) {
+        const int p = e / (LB * 2), rem = e % (LB * 2), l = rem / 2, w = rem % 2;
+        shc[e] = (l < nlb) ? coef[((size_t)(pb + p) * lmax + l0 + l) * 2 + w] : make_double2(0, 0);
+    }
+    for (int e = threadIdx.x; e < len; e += blockDim.x) {
+        const int c = hAi[pb + e];
+        shk[e] = make_int2(c, kap[c]);
+    }
+    __syncthreads();
+    double2 z0[LB], z2[LB], w0[LB], w1[LB];
 #pragma unroll
-        for (int l = 0; l < LM; l++) {
-            double2 h0 = cp[2 * l], h1 = cp[2 * l + 1];
-            if (cnz(h0)) {
-                z0[l] = cfma(h0, xv, z0[l]);
-                z2[l] = cfma(h0, kx, z2[l]);
-            }
-            if (cnz(h1)) {
-                w0[l] = cfma(h1, xv, w0[l]);
-                w1[l] = cfma(h1, kx, w1[l]);
+    for (int l = 0; l < LB; l++) z0[l] = z2[l] = w0[l] = w1[l] = make_double2(0, 0);
+    const int ac = a < n ? a : n - 1;
+    constexpr int U = 4;
+    for (int p0 = 0; p0 < len; p0 += U) {
+        double2 xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int p = min(p0 + u, len - 1);
+            xv[u] = x[(size_t)shk[p].x * n + ac];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int p = p0 + u;
+            if (p >= len) break;
+            const double2 kx = cscale(xv[u], (double)shk[p].y);
+            const double2 *cp = shc + (size_t)p * LB * 2;
+#pragma unroll
+            for (int l = 0; l < LB; l++) {
+                const double2 h0 = cp[2 * l], h1 = cp[2 * l + 1];
+                if (cnz(h0)) {
+                    z0[l] = cfma(h0, xv[u], z0[l]);
+                    z2[l] = cfma(h0, kx, z2[l]);
+                }
+                if (cnz(h1)) {
+                    w0[l] = cfma(h1, xv[u], w0[l]);
+                    w1[l] = cfma(h1, kx, w1[l]);
+                }
             }
         }
     }
     if (a >= n) return;
     const size_t plane = (size_t)nblk * n;
 #pragma unroll
-    for (int l = 0; l < LM; l++) {
-        size_t base = (size_t)(l * 4) * plane + (size_t)r * n + a;
+    for (int l = 0; l < LB; l++) {
+        if (l >= nlb) break;
+        size_t base = (size_t)((l0 + l) * 4) * plane + (size_t)r * n + a;
         V[base] = z0[l];
         V[base + plane] = z2[l];
         V[base + 2 * plane] = w0[l];
@@ -203,10 +232,15 @@ __global__ __launch_bounds__(256) void k_angular(const double2 *__restrict__ x, 
 //   y_r = D_sel(k) x_r + cH * sum_l [ (Gs0+k Gs1) Z0 + (Gs2+k Gs3) Z2 + (Gts0+k Gts2) W0 + (Gts1+k Gts3) W1 ]
 //         + beta y_r
 // One wave per (tile, 64-row radial chunk).  The chunk's column window of each
-// V vector is staged in LDS once and read by every slot; the chunk index is
-// the fastest-varying part of blockIdx so that with 8 chunks each XCD keeps
-// one chunk's slice of the G-sums resident in its L2.
+// V vector is staged in LDS once and read by every slot; the slot loads are
+// issued eight at a time.  The chunk index is the fastest-varying part of
+// blockIdx so that with 8 chunks each XCD keeps one chunk's slice of the
+// G-sums resident in its L2.
 // ---------------------------------------------------------------------------
+constexpr int kMaxSlots = 32;   // ELL width supported by the fused radial kernel
+constexpr int kMaxLocW = 4;     // LDS window <= 64 * kMaxLocW columns per chunk
+constexpr int kSlotBatch = 8;
+
 template <int R>
 __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, const double2 *__restrict__ V,
                                                const double2 *__restrict__ GE, const double2 *__restrict__ D,
@@ -220,46 +254,72 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
     const int lane = threadIdx.x;
     const int a = chunk * 64 + lane;
     const bool valid = a < n;
+    const int ac = valid ? a : n - 1;
     int rows[R];
+    bool live[R];
 #pragma unroll
-    for (int i = 0; i < R; i++) rows[i] = tiles[tile * R + i];
+    for (int i = 0; i < R; i++) {
+        const int rr = tiles[tile * R + i];
+        live[i] = rr >= 0;
+        rows[i] = rr >= 0 ? rr : tiles[tile * R];  // padded rows re-read row 0 and are never stored
+    }
     const double k = tile_k[tile];
     const int q = tile_q[tile];
     const int nl = nloc[chunk];
-    const int *lj = locj + (size_t)chunk * maxloc;
+    int ljr[kMaxLocW];
+#pragma unroll
+    for (int t = 0; t < kMaxLocW; t++) {
+        const int idx = lane + 64 * t;
+        ljr[t] = idx < nl ? locj[(size_t)chunk * maxloc + idx] : -1;
+    }
+    const int *lc = ell_lcol + ac;
     const size_t plane = (size_t)nblk * n;
     const size_t slab = (size_t)ellw * n;
     double2 acc[R];
 #pragma unroll
     for (int i = 0; i < R; i++) acc[i] = make_double2(0, 0);
+
+    auto stage = [&](const double2 *base) {
+        double2 tmp[R][kMaxLocW];
+#pragma unroll
+        for (int i = 0; i < R; i++)
+#pragma unroll
+            for (int t = 0; t < kMaxLocW; t++)
+                if (ljr[t] >= 0) tmp[i][t] = base[(size_t)rows[i] * n + ljr[t]];
+#pragma unroll
+        for (int i = 0; i < R; i++)
+#pragma unroll
+            for (int t = 0; t < kMaxLocW; t++)
+                if (ljr[t] >= 0) sm[i * maxloc + lane + 64 * t] = tmp[i][t];
+    };
+
     if (cH != 0.0) {
         for (int l = 0; l < lmax; l++) {
 #pragma unroll 1
             for (int v = 0; v < 4; v++) {
-                const double2 *Vv = V + (size_t)(l * 4 + v) * plane;
-#pragma unroll
-                for (int i = 0; i < R; i++) {
-                    if (rows[i] < 0) continue;
-                    const double2 *src = Vv + (size_t)rows[i] * n;
-                    for (int idx = lane; idx < nl; idx += 64) sm[i * maxloc + idx] = src[lj[idx]];
-                }
+                stage(V + (size_t)(l * 4 + v) * plane);
                 __syncthreads();
-                if (valid) {
-                    // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
-                    const int m1 = (v < 2) ? 2 * v : 4 + (v - 2);
-                    const int m2 = (v < 2) ? 2 * v + 1 : 6 + (v - 2);
-                    const double2 *G1 = GE + (size_t)(l * 8 + m1) * slab;
-                    const double2 *G2 = GE + (size_t)(l * 8 + m2) * slab;
-                    for (int s = 0; s < ellw; s++) {
-                        const size_t e = (size_t)s * n + a;
-                        const int lc = ell_lcol[e];
-                        const double2 g1 = G1[e], g2 = G2[e];
-                        const double2 M = make_double2(fma(k, g2.x, g1.x), fma(k, g2.y, g1.y));
+                // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
+                const int m1 = (v < 2) ? 2 * v : 4 + (v - 2);
+                const int m2 = (v < 2) ? 2 * v + 1 : 6 + (v - 2);
+                const double2 *G1 = GE + (size_t)(l * 8 + m1) * slab + ac;
+                const double2 *G2 = GE + (size_t)(l * 8 + m2) * slab + ac;
+#pragma unroll 1
+                for (int sb = 0; sb < ellw; sb += kSlotBatch) {
+                    double2 g1[kSlotBatch], g2[kSlotBatch];
+                    int lcol[kSlotBatch];
 #pragma unroll
-                        for (int i = 0; i < R; i++) {
-                            if (rows[i] < 0) continue;
-                            acc[i] = cfma(M, sm[i * maxloc + lc], acc[i]);
-                        }
+                    for (int u = 0; u < kSlotBatch; u++) {
+                        const int s2 = sb + u;
+                        lcol[u] = s2 < ellw ? lc[(size_t)s2 * n] : 0;
+                        g1[u] = s2 < ellw ? G1[(size_t)s2 * n] : make_double2(0, 0);
+                        g2[u] = s2 < ellw ? G2[(size_t)s2 * n] : make_double2(0, 0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kSlotBatch; u++) {
+                        const double2 M = make_double2(fma(k, g2[u].x, g1[u].x), fma(k, g2[u].y, g1[u].y));
+#pragma unroll
+                        for (int i = 0; i < R; i++) acc[i] = cfma(M, sm[i * maxloc + lcol[u]], acc[i]);
                     }
                 }
                 __syncthreads();
@@ -267,31 +327,30 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
         }
     }
     // stationary diagonal operator on x_r
-#pragma unroll
-    for (int i = 0; i < R; i++) {
-        if (rows[i] < 0) continue;
-        const double2 *src = x + (size_t)rows[i] * n;
-        for (int idx = lane; idx < nl; idx += 64) sm[i * maxloc + idx] = src[lj[idx]];
-    }
+    stage(x);
     __syncthreads();
-    if (!valid) return;
-    const double2 *Dq = D + ((size_t)sel * nq + q) * slab;
+    const double2 *Dq = D + ((size_t)sel * nq + q) * slab + ac;
     double2 dacc[R];
 #pragma unroll
     for (int i = 0; i < R; i++) dacc[i] = make_double2(0, 0);
-    for (int s = 0; s < ellw; s++) {
-        const size_t e = (size_t)s * n + a;
-        const int lc = ell_lcol[e];
-        const double2 d = Dq[e];
+#pragma unroll 1
+    for (int sb = 0; sb < ellw; sb += kSlotBatch) {
+        double2 d[kSlotBatch];
+        int lcol[kSlotBatch];
 #pragma unroll
-        for (int i = 0; i < R; i++) {
-            if (rows[i] < 0) continue;
-            dacc[i] = cfma(d, sm[i * maxloc + lc], dacc[i]);
+        for (int u = 0; u < kSlotBatch; u++) {
+            lcol[u] = (sb + u) < ellw ? lc[(size_t)(sb + u) * n] : 0;
+            d[u] = (sb + u) < ellw ? Dq[(size_t)(sb + u) * n] : make_double2(0, 0);
         }
+#pragma unroll
+        for (int u = 0; u < kSlotBatch; u++)
+#pragma unroll
+            for (int i = 0; i < R; i++) dacc[i] = cfma(d[u], sm[i * maxloc + lcol[u]], dacc[i]);
     }
+    if (!valid) return;
 #pragma unroll
     for (int i = 0; i < R; i++) {
-        if (rows[i] < 0) continue;
+        if (!live[i]) continue;
         const size_t o = (size_t)rows[i] * n + a;
         double2 res = cadd(dacc[i], cscale(acc[i], cH));
         if (beta != 0.0) res = cadd(res, cscale(y[o], beta));
@@ -375,16 +434,23 @@ __global__ __launch_bounds__(CH) void k_chunk_inverse(const double2 *AB, int n, 
     }
 }
 
+// radial index of interleaved position gi (the two halves of the radial basis alternate)
+__device__ __forceinline__ int natural_index(int gi, int n, int ilv) {
+    return ilv ? ((gi & 1) ? (n >> 1) + (gi >> 1) : (gi >> 1)) : gi;
+}
+
 // x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =
 // (block slot, row of the chunk).  Levels are chunks of CH rows: each chunk
 // takes the band coupling to the previous chunk, then its diagonal inverse.
+// Factor values are fetched 16 at a time so their loads overlap.
 template <int CH>
 __global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, double2 *__restrict__ xo,
                                                const int *__restrict__ wblocks, const int *__restrict__ blk_q,
-                                               const int *__restrict__ ilv_inv, const double2 *__restrict__ Linv,
+                                               int ilv, const double2 *__restrict__ Linv,
                                                const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
                                                const double2 *__restrict__ Uoff, int n, int band, int nch) {
     constexpr int G = 64 / CH;
+    constexpr int B = 16;
     extern __shared__ double2 smem[];
     const int lane = threadIdx.x, g = lane / CH, i = lane % CH;
     const int r = wblocks[blockIdx.x * G + g];
@@ -396,17 +462,30 @@ __global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, do
     // forward sweep
     for (int ch = 0; ch < nch; ch++) {
         const int gi = ch * CH + i;
-        double2 v = (r >= 0 && gi < n) ? bb[ilv_inv[gi]] : make_double2(0, 0);
+        double2 v = (r >= 0 && gi < n) ? bb[natural_index(gi, n, ilv)] : make_double2(0, 0);
         if (ch > 0) {
-            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH;
-            for (int t = 0; t < band; t++) v = csub(v, cmul(Lo[(size_t)t * CH + i], yv[ch * CH - band + t]));
+            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH + i;
+            for (int t0 = 0; t0 < band; t0 += B) {
+                double2 f[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Lo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
+#pragma unroll
+                for (int u = 0; u < B; u++)
+                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[ch * CH - band + t0 + u]));
+            }
         }
         acc[i] = v;
         __syncthreads();
-        const double2 *Li = Linv + ((size_t)q * nch + ch) * cs;
+        const double2 *Li = Linv + ((size_t)q * nch + ch) * cs + i;
         double2 yi = make_double2(0, 0);
-#pragma unroll 4
-        for (int j = 0; j < CH; j++) yi = cfma(Li[(size_t)j * CH + i], acc[j], yi);
+#pragma unroll
+        for (int j0 = 0; j0 < CH; j0 += B) {
+            double2 f[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) f[u] = Li[(size_t)(j0 + u) * CH];
+#pragma unroll
+            for (int u = 0; u < B; u++) yi = cfma(f[u], acc[j0 + u], yi);
+        }
         yv[gi] = yi;
         __syncthreads();
     }
@@ -415,21 +494,33 @@ __global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, do
         const int gi = ch * CH + i;
         double2 v = yv[gi];
         if (ch < nch - 1) {
-            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH;
-            for (int t = 0; t < band; t++) v = csub(v, cmul(Uo[(size_t)t * CH + i], yv[(ch + 1) * CH + t]));
+            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH + i;
+            for (int t0 = 0; t0 < band; t0 += B) {
+                double2 f[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Uo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
+#pragma unroll
+                for (int u = 0; u < B; u++)
+                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[(ch + 1) * CH + t0 + u]));
+            }
         }
         acc[i] = v;
         __syncthreads();
-        const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs;
+        const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs + i;
         double2 xi = make_double2(0, 0);
-#pragma unroll 4
-        for (int j = 0; j < CH; j++) xi = cfma(Ui[(size_t)j * CH + i], acc[j], xi);
-        __syncthreads();
+#pragma unroll
+        for (int j0 = 0; j0 < CH; j0 += B) {
+            double2 f[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) f[u] = Ui[(size_t)(j0 + u) * CH];
+#pragma unroll
+            for (int u = 0; u < B; u++) xi = cfma(f[u], acc[j0 + u], xi);
+        }
         yv[gi] = xi;
         __syncthreads();
     }
     if (r < 0) return;
-    for (int gi = i; gi < n; gi += CH) xo[(size_t)r * n + ilv_inv[gi]] = yv[gi];
+    for (int gi = i; gi < n; gi += CH) xo[(size_t)r * n + natural_index(gi, n, ilv)] = yv[gi];
 }
 
 // ---------------------------------------------------------------------------
@@ -484,9 +575,8 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2
                                                        const double2 *__restrict__ v, double2 *__restrict__ s,
                                                        const double2 *__restrict__ t, const double2 *__restrict__ yv,
                                                        const double2 *__restrict__ zv, double2 *__restrict__ x,
-                                                       double *sc, double *part, unsigned *cnt) {
+                                                       const double *__restrict__ sc, double *__restrict__ part) {
     __shared__ double sh[4 * 16];
-    __shared__ bool last;
     double acc[4] = {0, 0, 0, 0};
     double2 beta = make_double2(0, 0), w = make_double2(0, 0), alpha = make_double2(0, 0);
     if (OP == OP_P) {
@@ -538,54 +628,49 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2
     }
     if (OP == OP_P || OP == OP_S) return;
     block_sum<4>(acc, sh);
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < 4; i++)
-            __hip_atomic_store(part + (size_t)blockIdx.x * 4 + i, acc[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned ticket = atomicAdd(cnt, 1u);
-        last = (ticket == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; i++) part[(size_t)blockIdx.x * 4 + i] = acc[i];
+}
+
+// Sums the partials in a fixed order and updates the scalars (one block).
+template <int OP>
+__global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const double *__restrict__ part, double *sc) {
+    __shared__ double sh[4 * 16];
     double tot[4] = {0, 0, 0, 0};
-    for (int bidx = threadIdx.x; bidx < (int)gridDim.x; bidx += blockDim.x)
-        for (int i = 0; i < 4; i++)
-            tot[i] += __hip_atomic_load(part + (size_t)bidx * 4 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int bidx = threadIdx.x; bidx < nparts; bidx += blockDim.x)
+        for (int i = 0; i < 4; i++) tot[i] += part[(size_t)bidx * 4 + i];
     block_sum<4>(tot, sh);
-    if (threadIdx.x == 0) {
-        if (OP == OP_INIT) {
-            sc[SC_R0SQ] = tot[0];
-            sc[SC_RHSSQ] = tot[1];
-            sc[SC_RNORM2] = tot[0];
-            sc[SC_RHONEXT] = tot[0];
-            sc[SC_RHONEXT + 1] = 0;
-            sc[SC_RHO] = 1; sc[SC_RHO + 1] = 0;
-            sc[SC_ALPHA] = 1; sc[SC_ALPHA + 1] = 0;
-            sc[SC_W] = 1; sc[SC_W + 1] = 0;
-        } else if (OP == OP_RESTART) {
-            sc[SC_R0SQ] = tot[0];
-            sc[SC_RNORM2] = tot[0];
-            sc[SC_RHONEXT] = tot[0];
-            sc[SC_RHONEXT + 1] = 0;
-        } else if (OP == OP_ALPHA) {
-            double2 rho = make_double2(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
-            double2 al = cdiv(rho, make_double2(tot[0], tot[1]));
-            sc[SC_ALPHA] = al.x;
-            sc[SC_ALPHA + 1] = al.y;
-            sc[SC_RHO] = rho.x;
-            sc[SC_RHO + 1] = rho.y;
-        } else if (OP == OP_W) {
-            double2 ww = make_double2(0, 0);
-            if (tot[0] > 0) ww = make_double2(tot[1] / tot[0], tot[2] / tot[0]);
-            sc[SC_W] = ww.x;
-            sc[SC_W + 1] = ww.y;
-        } else if (OP == OP_XR) {
-            sc[SC_RNORM2] = tot[0];
-            sc[SC_RHONEXT] = tot[1];
-            sc[SC_RHONEXT + 1] = tot[2];
-        }
-        *cnt = 0;
+    if (threadIdx.x != 0) return;
+    if (OP == OP_INIT) {
+        sc[SC_R0SQ] = tot[0];
+        sc[SC_RHSSQ] = tot[1];
+        sc[SC_RNORM2] = tot[0];
+        sc[SC_RHONEXT] = tot[0];
+        sc[SC_RHONEXT + 1] = 0;
+        sc[SC_RHO] = 1; sc[SC_RHO + 1] = 0;
+        sc[SC_ALPHA] = 1; sc[SC_ALPHA + 1] = 0;
+        sc[SC_W] = 1; sc[SC_W + 1] = 0;
+    } else if (OP == OP_RESTART) {
+        sc[SC_R0SQ] = tot[0];
+        sc[SC_RNORM2] = tot[0];
+        sc[SC_RHONEXT] = tot[0];
+        sc[SC_RHONEXT + 1] = 0;
+    } else if (OP == OP_ALPHA) {
+        double2 rho = make_double2(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
+        double2 al = cdiv(rho, make_double2(tot[0], tot[1]));
+        sc[SC_ALPHA] = al.x;
+        sc[SC_ALPHA + 1] = al.y;
+        sc[SC_RHO] = rho.x;
+        sc[SC_RHO + 1] = rho.y;
+    } else if (OP == OP_W) {
+        double2 ww = make_double2(0, 0);
+        if (tot[0] > 0) ww = make_double2(tot[1] / tot[0], tot[2] / tot[0]);
+        sc[SC_W] = ww.x;
+        sc[SC_W + 1] = ww.y;
+    } else if (OP == OP_XR) {
+        sc[SC_RNORM2] = tot[0];
+        sc[SC_RHONEXT] = tot[1];
+        sc[SC_RHONEXT + 1] = tot[2];
     }
 }
 
@@ -786,16 +871,10 @@ void free_all(gade_model *m) {
 }
 
 int launch_angular(gade_model *m, const double2 *x) {
-    dim3 grid(m->nblk, (m->n + 255) / 256);
-#define ANG(LM) \
-    case LM: hipLaunchKernelGGL(k_angular<LM>, grid, dim3(256), 0, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef, m->d_kap, m->nblk, m->n, m->d_V); break;
-    switch (m->lmax) {
-        ANG(1) ANG(2) ANG(3) ANG(4) ANG(5) ANG(6) ANG(7) ANG(8) ANG(9) ANG(10) ANG(11) ANG(12) ANG(13) ANG(14) ANG(15) ANG(16)
-        default:
-            gade::set_error("lmax > 16 is not supported");
-            return GADE_EINVAL;
-    }
-#undef ANG
+    dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
+    size_t shm = sizeof(double2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
+    hipLaunchKernelGGL(k_angular<kAngLB>, grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef,
+                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -944,7 +1023,7 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
             const auto &cc = cols[a / 64];
             ell_lcol[(size_t)s2 * n + a] = (int)(std::lower_bound(cc.begin(), cc.end(), ell_col[(size_t)s2 * n + a]) - cc.begin());
         }
-    if ((size_t)8 * 16 * m->maxloc > 64 * 1024) {
+    if (m->maxloc > 64 * kMaxLocW) {
         gade::set_error("radial pattern too wide for the LDS window of the fused operator");
         return fail(GADE_EINVAL);
     }
@@ -960,6 +1039,15 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
             }
             tmap[k] = cp;
         }
+    if (ellw > kMaxSlots) {
+        gade::set_error("radial rows with more than 32 non-zeros are not supported by the fused operator");
+        return fail(GADE_EINVAL);
+    }
+    for (int r = 0; r < m->nblk; r++) m->maxrow = std::max(m->maxrow, m->hall_Ap[r + 1] - m->hall_Ap[r]);
+    if ((size_t)m->maxrow * (kAngLB * 2 * sizeof(double2) + sizeof(int2)) > 60 * 1024) {
+        gade::set_error("Hall rows too long for the angular kernel's LDS staging");
+        return fail(GADE_EINVAL);
+    }
     // kappa per block and distinct kappa values
     m->kap_blk.resize(m->nblk);
     m->blk_q.resize(m->nblk);
@@ -1198,7 +1286,7 @@ int psolve_async(gade_model *m, const double2 *b, double2 *x) {
     const int G = 64 / m->ch;
     size_t shm = sizeof(double2) * (size_t)G * ((size_t)m->nch * m->ch + m->ch);
 #define PS(C) \
-    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(m->npw), dim3(64), shm, m->stream, b, x, m->d_wblocks, m->d_blk_q, m->d_ilv_inv, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->n, m->band, m->nch); break;
+    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(m->npw), dim3(64), shm, m->stream, b, x, m->d_wblocks, m->d_blk_q, (m->n % 2 == 0) ? 1 : 0, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->n, m->band, m->nch); break;
     switch (m->ch) { PS(16) PS(32) PS(64) default: return GADE_EINVAL; }
 #undef PS
     GADE_HIP(hipGetLastError());
@@ -1206,11 +1294,12 @@ int psolve_async(gade_model *m, const double2 *b, double2 *x) {
 }
 
 template <int OP>
-int bicg_op(gade_model *m, const double2 *b, const double2 *xconst = nullptr, double2 *x = nullptr) {
-    (void)xconst;
+int bicg_op(gade_model *m, const double2 *b, double2 *x = nullptr) {
     Prof pr(m, "blas1");
     hipLaunchKernelGGL(k_bicg<OP>, dim3(m->red_blocks), dim3(kRedThreads), 0, m->stream, m->N, b, m->r, m->r0, m->p,
-                       m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part, m->d_cnt);
+                       m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part);
+    if (OP != OP_P && OP != OP_S)
+        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(1), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part, m->d_sc);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -1271,7 +1360,7 @@ int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, dou
         GADE_TRY(psolve_async(m, m->s, m->z));           // z = M^-1 s
         GADE_TRY(applyA(m, m->z, m->t, napply));         // t = A z
         GADE_TRY(bicg_op<OP_W>(m, rhs));                 // w = (t . s) / |t|^2
-        GADE_TRY(bicg_op<OP_XR>(m, rhs, nullptr, x));    // x += alpha y + w z ; r = s - w t
+        GADE_TRY(bicg_op<OP_XR>(m, rhs, x));    // x += alpha y + w z ; r = s - w t
         GADE_TRY(read_scalars(m));
         rnorm2 = m->h_sc[SC_RNORM2];
         rho_next_abs = std::hypot(m->h_sc[SC_RHONEXT], m->h_sc[SC_RHONEXT + 1]);

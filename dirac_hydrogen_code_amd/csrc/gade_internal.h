@@ -55,7 +55,7 @@ struct gade_model {
     // radial pattern (ELL, transposed: [slot][row])
     int *d_ell_col = nullptr, *d_ell_src = nullptr, *d_tmap = nullptr, *d_ell_pos = nullptr;
     // per 64-row radial chunk: distinct columns (LDS window) and ELL columns local to it
-    int nchunk = 0, maxloc = 0;
+    int nchunk = 0, maxloc = 0, maxrow = 0;
     int *d_ell_lcol = nullptr, *d_locj = nullptr, *d_nloc = nullptr;
     double2 *d_g = nullptr;        // [24*lmax][nnzb]
     double2 *d_s = nullptr;        // [3][nnzb]
