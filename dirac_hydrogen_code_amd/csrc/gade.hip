@@ -322,171 +322,6 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
     }
 }
 
-// ---------------------------------------------------------------------------
-// preconditioner: banded LU per distinct kappa (interleaved radial order)
-// ---------------------------------------------------------------------------
-// AB[q][i][w], w = 2*band+1, A(i,j) at AB[i][j-i+band].  No pivoting: P has a
-// positive-definite Hermitian part (S), so the factorisation exists.
-__global__ __launch_bounds__(256) void k_band_lu(double2 *AB, int n, int band, int *flag) {
-    const int q = blockIdx.x;
-    const int w = 2 * band + 1;
-    double2 *A = AB + (size_t)q * n * w;
-    for (int k = 0; k < n; k++) {
-        const double2 piv = A[(size_t)k * w + band];
-        if (threadIdx.x == 0 && !cnz(piv)) atomicOr(flag, 1);
-        const int iend = min(n - 1, k + band);
-        for (int t = threadIdx.x; t < iend - k; t += blockDim.x) {
-            const int i = k + 1 + t;
-            double2 *lik = A + (size_t)i * w + (k - i + band);
-            *lik = cdiv(*lik, piv);
-        }
-        __syncthreads();
-        const int m = iend - k;
-        for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
-            const int i = k + 1 + idx / m, j = k + 1 + idx % m;
-            const double2 lik = A[(size_t)i * w + (k - i + band)];
-            const double2 ukj = A[(size_t)k * w + (j - k + band)];
-            double2 *aij = A + (size_t)i * w + (j - i + band);
-            *aij = csub(*aij, cmul(lik, ukj));
-        }
-        __syncthreads();
-    }
-}
-
-// Per (kappa, chunk): inverses of the CHxCH diagonal chunks of L (unit) and U,
-// stored [col][row], and the band coupling to the neighbouring chunk.
-template <int CH>
-__global__ __launch_bounds__(CH) void k_chunk_inverse(const double2 *AB, int n, int band, int nch, double2 *Linv,
-                                                      double2 *Uinv, double2 *Loff, double2 *Uoff) {
-    const int q = blockIdx.x, ch = blockIdx.y, j = threadIdx.x;
-    const int w = 2 * band + 1;
-    const double2 *A = AB + (size_t)q * n * w;
-    const int base = ch * CH;
-    double2 *Li = Linv + ((size_t)q * nch + ch) * CH * CH + (size_t)j * CH;
-    double2 *Ui = Uinv + ((size_t)q * nch + ch) * CH * CH + (size_t)j * CH;
-    auto at = [&](int gi, int gj) -> double2 {  // band entry (gi, gj), zero outside
-        if (gi < 0 || gj < 0 || gi >= n || gj >= n || abs(gi - gj) > band) return make_double2(0, 0);
-        return A[(size_t)gi * w + (gj - gi + band)];
-    };
-    // column j of L_cc^{-1}: forward substitution (rows beyond n act as identity)
-    for (int i = 0; i < CH; i++) {
-        double2 v = make_double2(i == j ? 1.0 : 0.0, 0);
-        if (i > j && base + i < n) {
-            for (int m2 = max(j, i - band); m2 < i; m2++) v = csub(v, cmul(at(base + i, base + m2), Li[m2]));
-        }
-        Li[i] = (i < j) ? make_double2(0, 0) : v;
-    }
-    // column j of U_cc^{-1}: back substitution
-    for (int i = CH - 1; i >= 0; i--) {
-        double2 v = make_double2(0, 0);
-        if (i <= j) {
-            v = make_double2(i == j ? 1.0 : 0.0, 0);
-            if (base + i < n) {
-                for (int m2 = i + 1; m2 <= min(j, i + band); m2++) v = csub(v, cmul(at(base + i, base + m2), Ui[m2]));
-                v = cdiv(v, at(base + i, base + i));
-            }
-        }
-        Ui[i] = v;
-    }
-    // band couplings: lane j is a row of the chunk
-    const int gi = base + j;
-    double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH;
-    double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH;
-    for (int t = 0; t < band; t++) {
-        Lo[(size_t)t * CH + j] = (gi < n) ? at(gi, base - band + t) : make_double2(0, 0);
-        Uo[(size_t)t * CH + j] = (gi < n) ? at(gi, base + CH + t) : make_double2(0, 0);
-    }
-}
-
-// radial index of interleaved position gi (the two halves of the radial basis alternate)
-__device__ __forceinline__ int natural_index(int gi, int n, int ilv) {
-    return ilv ? ((gi & 1) ? (n >> 1) + (gi >> 1) : (gi >> 1)) : gi;
-}
-
-// x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =
-// (block slot, row of the chunk).  Levels are chunks of CH rows: each chunk
-// takes the band coupling to the previous chunk, then its diagonal inverse.
-// Factor values are fetched 16 at a time so their loads overlap.
-template <int CH>
-__global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, double2 *__restrict__ xo,
-                                               const int *__restrict__ wblocks, const int *__restrict__ blk_q,
-                                               int ilv, const double2 *__restrict__ Linv,
-                                               const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
-                                               const double2 *__restrict__ Uoff, int n, int band, int nch) {
-    constexpr int G = 64 / CH;
-    constexpr int B = 16;
-    extern __shared__ double2 smem[];
-    const int lane = threadIdx.x, g = lane / CH, i = lane % CH;
-    const int r = wblocks[blockIdx.x * G + g];
-    const int q = r >= 0 ? blk_q[r] : 0;
-    double2 *yv = smem + (size_t)g * (nch * CH + CH);
-    double2 *acc = yv + nch * CH;
-    const size_t cs = (size_t)CH * CH;
-    const double2 *bb = b + (size_t)(r >= 0 ? r : 0) * n;
-    // forward sweep
-    for (int ch = 0; ch < nch; ch++) {
-        const int gi = ch * CH + i;
-        double2 v = (r >= 0 && gi < n) ? bb[natural_index(gi, n, ilv)] : make_double2(0, 0);
-        if (ch > 0) {
-            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH + i;
-            for (int t0 = 0; t0 < band; t0 += B) {
-                double2 f[B];
-#pragma unroll
-                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Lo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
-#pragma unroll
-                for (int u = 0; u < B; u++)
-                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[ch * CH - band + t0 + u]));
-            }
-        }
-        acc[i] = v;
-        __syncthreads();
-        const double2 *Li = Linv + ((size_t)q * nch + ch) * cs + i;
-        double2 yi = make_double2(0, 0);
-#pragma unroll
-        for (int j0 = 0; j0 < CH; j0 += B) {
-            double2 f[B];
-#pragma unroll
-            for (int u = 0; u < B; u++) f[u] = Li[(size_t)(j0 + u) * CH];
-#pragma unroll
-            for (int u = 0; u < B; u++) yi = cfma(f[u], acc[j0 + u], yi);
-        }
-        yv[gi] = yi;
-        __syncthreads();
-    }
-    // backward sweep (in place)
-    for (int ch = nch - 1; ch >= 0; ch--) {
-        const int gi = ch * CH + i;
-        double2 v = yv[gi];
-        if (ch < nch - 1) {
-            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH + i;
-            for (int t0 = 0; t0 < band; t0 += B) {
-                double2 f[B];
-#pragma unroll
-                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Uo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
-#pragma unroll
-                for (int u = 0; u < B; u++)
-                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[(ch + 1) * CH + t0 + u]));
-            }
-        }
-        acc[i] = v;
-        __syncthreads();
-        const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs + i;
-        double2 xi = make_double2(0, 0);
-#pragma unroll
-        for (int j0 = 0; j0 < CH; j0 += B) {
-            double2 f[B];
-#pragma unroll
-            for (int u = 0; u < B; u++) f[u] = Ui[(size_t)(j0 + u) * CH];
-#pragma unroll
-            for (int u = 0; u < B; u++) xi = cfma(f[u], acc[j0 + u], xi);
-        }
-        yv[gi] = xi;
-        __syncthreads();
-    }
-    if (r < 0) return;
-    for (int gi = i; gi < n; gi += CH) xo[(size_t)r * n + natural_index(gi, n, ilv)] = yv[gi];
-}
-
 // per-block S-norm contributions pop[r] = x_r^H S(k_r) x_r
 __global__ __launch_bounds__(256) void k_snorm(const double2 *__restrict__ x, const double2 *__restrict__ D,
                                                const int *__restrict__ ell_col, const int *__restrict__ blk_q,
@@ -605,7 +440,9 @@ __global__ __launch_bounds__(256) void k_assemble(const int *__restrict__ hAp, c
 // ===========================================================================
 namespace {
 
-int hip_alloc(void **p, size_t bytes) {
+}  // namespace
+
+int gade::hip_alloc(void **p, size_t bytes) {
     if (bytes == 0) bytes = 16;
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
@@ -614,6 +451,9 @@ int hip_alloc(void **p, size_t bytes) {
     }
     return GADE_OK;
 }
+
+namespace {
+using gade::hip_alloc;
 
 template <typename T>
 int upload(T **dst, const T *src, size_t count) {
@@ -641,9 +481,11 @@ unsigned grid_for(long long total, int threads, unsigned cap = 65535u * 4) {
 }
 
 void free_all(gade_model *m) {
-    void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_wblocks, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
+    gade::blocklu_free(m->pre);
+    m->pre = nullptr;
+    void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
                     m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
-                    m->d_V, m->d_ilv_inv, m->d_AB, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->d_flag,
+                    m->d_V, m->d_flag,
                     m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_cnt,
                     m->d_fAp, m->d_fAi, m->d_fAx, m->d_SAp, m->d_SAi, m->d_SAx};
     for (void *p : ptrs)
@@ -987,72 +829,18 @@ int gade_apply(gade_model_t *m, const double *x, double *y, double cH, double be
 
 int gade_precond_setup(gade_model_t *m) {
     if (!m) return GADE_EINVAL;
-    if (m->pre_ready) return GADE_OK;
-    const int n = m->n, nnzb = m->nnzb;
-    // interleave the two radial halves so that banded quadrants form one band
-    std::vector<int> ilv(n), ilv_inv(n);
-    for (int i = 0; i < n; i++) ilv[i] = (n % 2 == 0) ? (i < n / 2 ? 2 * i : 2 * (i - n / 2) + 1) : i;
-    for (int i = 0; i < n; i++) ilv_inv[ilv[i]] = i;
-    int band = 0;
-    for (int a = 0; a < n; a++)
-        for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) band = std::max(band, std::abs(ilv[a] - ilv[m->rad_Ai[k]]));
-    if (band > gade::kMaxBand) {
-        gade::set_error("preconditioner: radial half-bandwidth " + std::to_string(band) + " exceeds 63");
-        return GADE_EINVAL;
-    }
-    m->band = std::max(band, 1);
-    m->ch = m->band < 16 ? 16 : (m->band < 32 ? 32 : 64);
-    const int CH = m->ch;
-    m->nch = (n + CH - 1) / CH;
-    const int w = 2 * m->band + 1, nq = m->nq;
-    // P(k) = S(k) + ihdt H0(k) in band storage, from the device D operators (sel 0)
+    if (m->pre) return GADE_OK;
+    const int n = m->n, nq = m->nq;
+    // P(k) = S(k) + ihdt H0(k) per distinct kappa, from the device D operators (sel 0), in CSR order
     std::vector<double2> Dh((size_t)nq * m->ellw * n);
     GADE_HIP(hipMemcpy(Dh.data(), m->d_D, sizeof(double2) * Dh.size(), hipMemcpyDeviceToHost));
-    std::vector<double2> AB((size_t)nq * n * w, make_double2(0, 0));
+    std::vector<double2> Pq((size_t)nq * m->nnzb);
     for (int q = 0; q < nq; q++)
         for (int a = 0; a < n; a++)
-            for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++) {
-                int s2 = k - m->rad_Ap[a];
-                int i = ilv[a], j = ilv[m->rad_Ai[k]];
-                AB[((size_t)q * n + i) * w + (j - i + m->band)] = Dh[((size_t)q * m->ellw + s2) * n + a];
-            }
-    (void)nnzb;
-    GADE_TRY(upload(&m->d_AB, AB.data(), AB.size()));
-    GADE_TRY(upload(&m->d_ilv_inv, ilv_inv.data(), ilv_inv.size()));
-    const size_t cs = (size_t)CH * CH;
-    GADE_TRY(hip_alloc((void **)&m->d_Linv, sizeof(double2) * nq * m->nch * cs));
-    GADE_TRY(hip_alloc((void **)&m->d_Uinv, sizeof(double2) * nq * m->nch * cs));
-    GADE_TRY(hip_alloc((void **)&m->d_Loff, sizeof(double2) * nq * m->nch * m->band * CH));
-    GADE_TRY(hip_alloc((void **)&m->d_Uoff, sizeof(double2) * nq * m->nch * m->band * CH));
-    GADE_HIP(hipMemset(m->d_flag, 0, sizeof(int)));
-    hipLaunchKernelGGL(k_band_lu, dim3(nq), dim3(256), 0, m->stream, m->d_AB, n, m->band, m->d_flag);
-    GADE_HIP(hipGetLastError());
-#define INV(C) \
-    case C: hipLaunchKernelGGL(k_chunk_inverse<C>, dim3(nq, m->nch), dim3(C), 0, m->stream, m->d_AB, n, m->band, m->nch, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff); break;
-    switch (CH) { INV(16) INV(32) INV(64) }
-#undef INV
-    GADE_HIP(hipGetLastError());
-    int flag = 0;
-    GADE_HIP(hipMemcpyAsync(&flag, m->d_flag, sizeof(int), hipMemcpyDeviceToHost, m->stream));
-    GADE_HIP(hipStreamSynchronize(m->stream));
-    if (flag) {
-        gade::set_error("preconditioner: zero pivot in block LU");
-        return GADE_EPIVOT;
-    }
-    // waves of 64/CH blocks, blocks of one kappa together so their factor loads coincide
-    const int G = 64 / CH;
-    std::vector<int> wb;
-    for (int q = 0; q < nq; q++) {
-        std::vector<int> rows;
-        for (int r = 0; r < m->nblk; r++)
-            if (m->blk_q[r] == q) rows.push_back(r);
-        for (size_t i = 0; i < rows.size(); i += G)
-            for (int j = 0; j < G; j++) wb.push_back(i + j < rows.size() ? rows[i + j] : -1);
-    }
-    m->npw = (int)wb.size() / G;
-    GADE_TRY(upload(&m->d_wblocks, wb.data(), wb.size()));
-    m->pre_ready = true;
-    return GADE_OK;
+            for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++)
+                Pq[(size_t)q * m->nnzb + k] = Dh[((size_t)q * m->ellw + (k - m->rad_Ap[a])) * n + a];
+    return gade::blocklu_build(m->nblk, n, m->rad_Ap.data(), m->rad_Ai.data(), nq, m->blk_q.data(), Pq, m->stream,
+                               &m->pre);
 }
 
 }  // extern "C"
@@ -1064,14 +852,7 @@ namespace {
 
 int psolve_async(gade_model *m, const double2 *b, double2 *x) {
     Prof pr(m, "psolve");
-    const int G = 64 / m->ch;
-    size_t shm = sizeof(double2) * (size_t)G * ((size_t)m->nch * m->ch + m->ch);
-#define PS(C) \
-    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(m->npw), dim3(64), shm, m->stream, b, x, m->d_wblocks, m->d_blk_q, (m->n % 2 == 0) ? 1 : 0, m->d_Linv, m->d_Uinv, m->d_Loff, m->d_Uoff, m->n, m->band, m->nch); break;
-    switch (m->ch) { PS(16) PS(32) PS(64) default: return GADE_EINVAL; }
-#undef PS
-    GADE_HIP(hipGetLastError());
-    return GADE_OK;
+    return gade::blocklu_solve_async(m->pre, b, x, m->stream);
 }
 
 template <int OP>
@@ -1309,9 +1090,8 @@ int gade_assembled_setup(gade_model_t *m) {
     return GADE_OK;
 }
 
-int gade_assembled_timedep(gade_model_t *m) {
-    if (!m) return GADE_EINVAL;
-    GADE_TRY(gade_assembled_setup(m));
+int gade_assembled_timedep_into(gade_model_t *m, const int *Ap_dev, double *Ax_dev) {
+    if (!m || !Ap_dev || !Ax_dev) return GADE_EINVAL;
     int *d_subAp = nullptr;
     GADE_TRY(upload(&d_subAp, m->rad_Ap.data(), m->rad_Ap.size()));
     const long long N = m->N;
@@ -1319,21 +1099,27 @@ int gade_assembled_timedep(gade_model_t *m) {
     {
         Prof pr(m, "assemble");
 #define ASM(LM) \
-    case LM: hipLaunchKernelGGL(k_assemble<LM>, g2, dim3(256), 0, m->stream, m->d_hAp, m->d_hAi, m->d_kap, m->d_coef, m->d_diag, d_subAp, m->d_ell_pos, m->d_GE, m->d_D, m->d_blk_q, m->d_fAp, m->nblk, m->n, m->ellw, m->nq, m->d_fAx); break;
+    case LM: hipLaunchKernelGGL(k_assemble<LM>, g2, dim3(256), 0, m->stream, m->d_hAp, m->d_hAi, m->d_kap, m->d_coef, m->d_diag, d_subAp, m->d_ell_pos, m->d_GE, m->d_D, m->d_blk_q, Ap_dev, m->nblk, m->n, m->ellw, m->nq, (double2 *)Ax_dev); break;
         switch (m->lmax) {
             ASM(1) ASM(2) ASM(3) ASM(4) ASM(5) ASM(6) ASM(7) ASM(8) ASM(9) ASM(10) ASM(11) ASM(12) ASM(13) ASM(14) ASM(15) ASM(16)
-            default: hipFree(d_subAp); return GADE_EINVAL;
+            default: (void)hipFree(d_subAp); return GADE_EINVAL;
         }
 #undef ASM
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
-    hipFree(d_subAp);
+    (void)hipFree(d_subAp);
     if (e != hipSuccess) {
         gade::set_error(std::string("k_assemble: ") + hipGetErrorString(e));
         return GADE_EHIP;
     }
     return GADE_OK;
+}
+
+int gade_assembled_timedep(gade_model_t *m) {
+    if (!m) return GADE_EINVAL;
+    GADE_TRY(gade_assembled_setup(m));
+    return gade_assembled_timedep_into(m, m->d_fAp, (double *)m->d_fAx);
 }
 
 int gade_assembled_spmv(gade_model_t *m, const double *x, double *y, double alpha, double beta, int with_S) {
@@ -1425,3 +1211,112 @@ int gade_profile_get(gade_model_t *m, const char *name, double *total_ms, long l
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// block-diagonal banded LU (shared by the model and the drop-in compute_preconditioner)
+// ---------------------------------------------------------------------------
+namespace gade {
+
+void blocklu_free(gade_blocklu *lu) {
+    if (!lu) return;
+    void *ptrs[] = {lu->d_AB, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->d_blk_q, lu->d_wblocks, lu->d_flag};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete lu;
+}
+
+int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
+                  const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out) {
+    const int nnzb = rAp[n];
+    // interleave the two radial halves so that banded quadrants form one band
+    std::vector<int> ilv(n);
+    const int inter = (n % 2 == 0) ? 1 : 0;
+    for (int i = 0; i < n; i++) ilv[i] = inter ? (i < n / 2 ? 2 * i : 2 * (i - n / 2) + 1) : i;
+    int band = 0;
+    for (int a = 0; a < n; a++)
+        for (int k = rAp[a]; k < rAp[a + 1]; k++) band = std::max(band, std::abs(ilv[a] - ilv[rAi[k]]));
+    if (band > gade::kMaxBand) {
+        gade::set_error("preconditioner: radial half-bandwidth " + std::to_string(band) + " exceeds 63");
+        return GADE_EINVAL;
+    }
+    gade_blocklu *lu = new gade_blocklu();
+    auto fail = [&](int rc) {
+        blocklu_free(lu);
+        return rc;
+    };
+    lu->n = n;
+    lu->nblk = nblk;
+    lu->nq = nq;
+    lu->ilv = inter;
+    lu->band = std::max(band, 1);
+    lu->ch = lu->band < 16 ? 16 : (lu->band < 32 ? 32 : 64);
+    const int CH = lu->ch;
+    lu->nch = (n + CH - 1) / CH;
+    const int w = 2 * lu->band + 1;
+    std::vector<double2> AB((size_t)nq * n * w, make_double2(0, 0));
+    for (int q = 0; q < nq; q++)
+        for (int a = 0; a < n; a++)
+            for (int k = rAp[a]; k < rAp[a + 1]; k++) {
+                const int i = ilv[a], j = ilv[rAi[k]];
+                AB[((size_t)q * n + i) * w + (j - i + lu->band)] = Pq[(size_t)q * nnzb + k];
+            }
+    int rc;
+    if ((rc = upload(&lu->d_AB, AB.data(), AB.size()))) return fail(rc);
+    const size_t cs = (size_t)CH * CH;
+    if ((rc = hip_alloc((void **)&lu->d_Linv, sizeof(double2) * nq * lu->nch * cs)) ||
+        (rc = hip_alloc((void **)&lu->d_Uinv, sizeof(double2) * nq * lu->nch * cs)) ||
+        (rc = hip_alloc((void **)&lu->d_Loff, sizeof(double2) * nq * lu->nch * lu->band * CH)) ||
+        (rc = hip_alloc((void **)&lu->d_Uoff, sizeof(double2) * nq * lu->nch * lu->band * CH)) ||
+        (rc = hip_alloc((void **)&lu->d_flag, sizeof(int))))
+        return fail(rc);
+    if (hipMemsetAsync(lu->d_flag, 0, sizeof(int), stream) != hipSuccess) return fail(GADE_EHIP);
+    hipLaunchKernelGGL(k_band_lu, dim3(nq), dim3(256), 0, stream, lu->d_AB, n, lu->band, lu->d_flag);
+#define INV(C) \
+    case C: hipLaunchKernelGGL(k_chunk_inverse<C>, dim3(nq, lu->nch), dim3(C), 0, stream, lu->d_AB, n, lu->band, lu->nch, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff); break;
+    switch (CH) { INV(16) INV(32) INV(64) }
+#undef INV
+    int flag = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, lu->d_flag, sizeof(int), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        gade::set_error(std::string("block LU: ") + hipGetErrorString(e));
+        return fail(GADE_EHIP);
+    }
+    if (flag) {
+        gade::set_error("preconditioner: zero pivot in block LU");
+        return fail(GADE_EPIVOT);
+    }
+    // waves of 64/CH blocks, blocks of one distinct factor together so their loads coincide
+    const int G = 64 / CH;
+    std::vector<int> wb;
+    for (int q = 0; q < nq; q++) {
+        std::vector<int> rows;
+        for (int r = 0; r < nblk; r++)
+            if (blk_q[r] == q) rows.push_back(r);
+        for (size_t i = 0; i < rows.size(); i += G)
+            for (int j = 0; j < G; j++) wb.push_back(i + j < rows.size() ? rows[i + j] : -1);
+    }
+    lu->npw = (int)wb.size() / G;
+    if ((rc = upload(&lu->d_wblocks, wb.data(), wb.size())) || (rc = upload(&lu->d_blk_q, blk_q, (size_t)nblk)))
+        return fail(rc);
+    *out = lu;
+    return GADE_OK;
+}
+
+int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream) {
+    if (!lu) {
+        gade::set_error("preconditioner not set up");
+        return GADE_EINVAL;
+    }
+    const int G = 64 / lu->ch;
+    size_t shm = sizeof(double2) * (size_t)G * ((size_t)lu->nch * lu->ch + lu->ch);
+#define PS(C) \
+    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(lu->npw), dim3(64), shm, stream, b, x, lu->d_wblocks, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch); break;
+    switch (lu->ch) { PS(16) PS(32) PS(64) default: return GADE_EINVAL; }
+#undef PS
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+}  // namespace gade

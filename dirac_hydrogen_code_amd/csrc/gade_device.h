@@ -224,3 +224,169 @@ static __global__ __launch_bounds__(256) void k_csr_spmv(const int *__restrict__
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// preconditioner: banded LU per distinct kappa (interleaved radial order)
+// ---------------------------------------------------------------------------
+// AB[q][i][w], w = 2*band+1, A(i,j) at AB[i][j-i+band].  No pivoting: P has a
+// positive-definite Hermitian part (S), so the factorisation exists.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_band_lu(double2 *AB, int n, int band, int *flag) {
+    const int q = blockIdx.x;
+    const int w = 2 * band + 1;
+    double2 *A = AB + (size_t)q * n * w;
+    for (int k = 0; k < n; k++) {
+        const double2 piv = A[(size_t)k * w + band];
+        if (threadIdx.x == 0 && !cnz(piv)) atomicOr(flag, 1);
+        const int iend = min(n - 1, k + band);
+        for (int t = threadIdx.x; t < iend - k; t += blockDim.x) {
+            const int i = k + 1 + t;
+            double2 *lik = A + (size_t)i * w + (k - i + band);
+            *lik = cdiv(*lik, piv);
+        }
+        __syncthreads();
+        const int m = iend - k;
+        for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+            const int i = k + 1 + idx / m, j = k + 1 + idx % m;
+            const double2 lik = A[(size_t)i * w + (k - i + band)];
+            const double2 ukj = A[(size_t)k * w + (j - k + band)];
+            double2 *aij = A + (size_t)i * w + (j - i + band);
+            *aij = csub(*aij, cmul(lik, ukj));
+        }
+        __syncthreads();
+    }
+}
+
+// Per (kappa, chunk): inverses of the CHxCH diagonal chunks of L (unit) and U,
+// stored [col][row], and the band coupling to the neighbouring chunk.
+template <int CH>
+__global__ __launch_bounds__(CH) void k_chunk_inverse(const double2 *AB, int n, int band, int nch, double2 *Linv,
+                                                      double2 *Uinv, double2 *Loff, double2 *Uoff) {
+    const int q = blockIdx.x, ch = blockIdx.y, j = threadIdx.x;
+    const int w = 2 * band + 1;
+    const double2 *A = AB + (size_t)q * n * w;
+    const int base = ch * CH;
+    double2 *Li = Linv + ((size_t)q * nch + ch) * CH * CH + (size_t)j * CH;
+    double2 *Ui = Uinv + ((size_t)q * nch + ch) * CH * CH + (size_t)j * CH;
+    auto at = [&](int gi, int gj) -> double2 {  // band entry (gi, gj), zero outside
+        if (gi < 0 || gj < 0 || gi >= n || gj >= n || abs(gi - gj) > band) return make_double2(0, 0);
+        return A[(size_t)gi * w + (gj - gi + band)];
+    };
+    // column j of L_cc^{-1}: forward substitution (rows beyond n act as identity)
+    for (int i = 0; i < CH; i++) {
+        double2 v = make_double2(i == j ? 1.0 : 0.0, 0);
+        if (i > j && base + i < n) {
+            for (int m2 = max(j, i - band); m2 < i; m2++) v = csub(v, cmul(at(base + i, base + m2), Li[m2]));
+        }
+        Li[i] = (i < j) ? make_double2(0, 0) : v;
+    }
+    // column j of U_cc^{-1}: back substitution
+    for (int i = CH - 1; i >= 0; i--) {
+        double2 v = make_double2(0, 0);
+        if (i <= j) {
+            v = make_double2(i == j ? 1.0 : 0.0, 0);
+            if (base + i < n) {
+                for (int m2 = i + 1; m2 <= min(j, i + band); m2++) v = csub(v, cmul(at(base + i, base + m2), Ui[m2]));
+                v = cdiv(v, at(base + i, base + i));
+            }
+        }
+        Ui[i] = v;
+    }
+    // band couplings: lane j is a row of the chunk
+    const int gi = base + j;
+    double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH;
+    double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH;
+    for (int t = 0; t < band; t++) {
+        Lo[(size_t)t * CH + j] = (gi < n) ? at(gi, base - band + t) : make_double2(0, 0);
+        Uo[(size_t)t * CH + j] = (gi < n) ? at(gi, base + CH + t) : make_double2(0, 0);
+    }
+}
+
+// radial index of interleaved position gi (the two halves of the radial basis alternate)
+static __device__ __forceinline__ int natural_index(int gi, int n, int ilv) {
+    return ilv ? ((gi & 1) ? (n >> 1) + (gi >> 1) : (gi >> 1)) : gi;
+}
+
+// x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =
+// (block slot, row of the chunk).  Levels are chunks of CH rows: each chunk
+// takes the band coupling to the previous chunk, then its diagonal inverse.
+// Factor values are fetched 16 at a time so their loads overlap.
+template <int CH>
+__global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, double2 *__restrict__ xo,
+                                               const int *__restrict__ wblocks, const int *__restrict__ blk_q,
+                                               int ilv, const double2 *__restrict__ Linv,
+                                               const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
+                                               const double2 *__restrict__ Uoff, int n, int band, int nch) {
+    constexpr int G = 64 / CH;
+    constexpr int B = 16;
+    extern __shared__ double2 smem[];
+    const int lane = threadIdx.x, g = lane / CH, i = lane % CH;
+    const int r = wblocks[blockIdx.x * G + g];
+    const int q = r >= 0 ? blk_q[r] : 0;
+    double2 *yv = smem + (size_t)g * (nch * CH + CH);
+    double2 *acc = yv + nch * CH;
+    const size_t cs = (size_t)CH * CH;
+    const double2 *bb = b + (size_t)(r >= 0 ? r : 0) * n;
+    // forward sweep
+    for (int ch = 0; ch < nch; ch++) {
+        const int gi = ch * CH + i;
+        double2 v = (r >= 0 && gi < n) ? bb[natural_index(gi, n, ilv)] : make_double2(0, 0);
+        if (ch > 0) {
+            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH + i;
+            for (int t0 = 0; t0 < band; t0 += B) {
+                double2 f[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Lo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
+#pragma unroll
+                for (int u = 0; u < B; u++)
+                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[ch * CH - band + t0 + u]));
+            }
+        }
+        acc[i] = v;
+        __syncthreads();
+        const double2 *Li = Linv + ((size_t)q * nch + ch) * cs + i;
+        double2 yi = make_double2(0, 0);
+#pragma unroll
+        for (int j0 = 0; j0 < CH; j0 += B) {
+            double2 f[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) f[u] = Li[(size_t)(j0 + u) * CH];
+#pragma unroll
+            for (int u = 0; u < B; u++) yi = cfma(f[u], acc[j0 + u], yi);
+        }
+        yv[gi] = yi;
+        __syncthreads();
+    }
+    // backward sweep (in place)
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int gi = ch * CH + i;
+        double2 v = yv[gi];
+        if (ch < nch - 1) {
+            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH + i;
+            for (int t0 = 0; t0 < band; t0 += B) {
+                double2 f[B];
+#pragma unroll
+                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Uo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
+#pragma unroll
+                for (int u = 0; u < B; u++)
+                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[(ch + 1) * CH + t0 + u]));
+            }
+        }
+        acc[i] = v;
+        __syncthreads();
+        const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs + i;
+        double2 xi = make_double2(0, 0);
+#pragma unroll
+        for (int j0 = 0; j0 < CH; j0 += B) {
+            double2 f[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) f[u] = Ui[(size_t)(j0 + u) * CH];
+#pragma unroll
+            for (int u = 0; u < B; u++) xi = cfma(f[u], acc[j0 + u], xi);
+        }
+        yv[gi] = xi;
+        __syncthreads();
+    }
+    if (r < 0) return;
+    for (int gi = i; gi < n; gi += CH) xo[(size_t)r * n + natural_index(gi, n, ilv)] = yv[gi];
+}
+

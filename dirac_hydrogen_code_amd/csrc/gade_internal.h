@@ -40,6 +40,23 @@ struct Timer {
 
 }  // namespace gade
 
+// Block-diagonal preconditioner: exact banded LU of each distinct diagonal
+// block (interleaved radial order) with chunk-level-scheduled triangular solves.
+struct gade_blocklu {
+    int n = 0, nblk = 0, nq = 0, band = 0, ch = 16, nch = 0, npw = 0, ilv = 0;
+    double2 *d_AB = nullptr, *d_Linv = nullptr, *d_Uinv = nullptr, *d_Loff = nullptr, *d_Uoff = nullptr;
+    int *d_blk_q = nullptr, *d_wblocks = nullptr, *d_flag = nullptr;
+};
+
+namespace gade {
+// Pq: values of each distinct block in the CSR order of (rAp, rAi), [nq][nnz]
+int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
+                  const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out);
+int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream);
+void blocklu_free(gade_blocklu *lu);
+int hip_alloc(void **p, size_t bytes);
+}  // namespace gade
+
 struct gade_model {
     // sizes
     int nblk = 0, n = 0, lmax = 0, nnzb = 0, ellw = 0, hnnz = 0, nq = 0;
@@ -73,13 +90,7 @@ struct gade_model {
     double *d_tile_k = nullptr;    // [ntiles]
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     // preconditioner
-    bool pre_ready = false;
-    int band = 0, nch = 0, ch = 16, npw = 0;   // half-bandwidth, chunks, chunk rows, psolve waves
-    int *d_wblocks = nullptr;      // [npw][64/ch] blocks served by each psolve wave (-1 = none)
-    int *d_ilv_inv = nullptr;      // interleaved -> natural radial index
-    double2 *d_AB = nullptr;       // [nq][n][2*band+1]
-    double2 *d_Linv = nullptr, *d_Uinv = nullptr;  // [nq][nch][64 col][64 row]
-    double2 *d_Loff = nullptr, *d_Uoff = nullptr;  // [nq][nch][band][64 row]
+    gade_blocklu *pre = nullptr;
     int *d_flag = nullptr;
     // BiCGSTAB workspace (persists across calls, tools/gpu_bicgstab.c:150-167)
     double2 *r = nullptr, *r0 = nullptr, *v = nullptr, *p = nullptr, *y = nullptr, *z = nullptr,

@@ -15,6 +15,8 @@
 #include "types.h"
 
 #ifdef __cplusplus
+/* csr_data_t is std::complex<double> in C++: same layout and return convention as double complex */
+#pragma clang diagnostic ignored "-Wreturn-type-c-linkage"
 extern "C" {
 #endif
 

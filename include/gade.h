@@ -135,6 +135,9 @@ int gade_snorm(gade_model_t *m, const double *x_dev, double *norm, double *pop_h
  * hand-written CSR SpMV (replacing hipsparseSpMV, tools/gpu_sparse.c:267-313). */
 int gade_assembled_setup(gade_model_t *m);
 int gade_assembled_timedep(gade_model_t *m);  /* uses the ft of the last gade_set_time */
+/* Same, writing Hfull.Ax into a caller's device CSR with the expanded row
+ * pointers Ap_dev (the drop-in gpu_compute_timedep_matrices uses it). */
+int gade_assembled_timedep_into(gade_model_t *m, const int *Ap_dev, double *Ax_dev);
 /* y = alpha * Hfull x + beta * y, then y += S x when with_S
  * (the rhs of tools/dirac_solver.c:535-536 is alpha = -1, beta = 0, with_S = 1) */
 int gade_assembled_spmv(gade_model_t *m, const double *x_dev, double *y_dev, double alpha, double beta, int with_S);
