@@ -203,7 +203,6 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restri
 // ---------------------------------------------------------------------------
 constexpr int kMaxSlots = 32;   // ELL width supported by the fused radial kernel
 constexpr int kMaxLocW = 4;     // LDS window <= 64 * kMaxLocW columns per chunk
-constexpr int kSlotBatch = 8;
 
 template <int R>
 __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, const double2 *__restrict__ V,
@@ -211,10 +210,13 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
                                                const int *__restrict__ ell_lcol, const int *__restrict__ locj,
                                                const int *__restrict__ nloc, const int *__restrict__ tiles,
                                                const int *__restrict__ tile_q, const double *__restrict__ tile_k,
-                                               int nblk, int n, int ellw, int lmax, int nq, int nchunk, int maxloc,
-                                               int sel, double cH, double beta, double2 *__restrict__ y) {
-    extern __shared__ double2 sm[];  // [R][maxloc]
-    const int chunk = blockIdx.x % nchunk, tile = blockIdx.x / nchunk;
+                                               int nblk, int n, int ellw, int lgrid, int nq, int nchunk, int maxloc,
+                                               int sel, double cH, double beta, double2 *__restrict__ y,
+                                               double2 *__restrict__ ypart) {
+    extern __shared__ double2 sm[];  // [4][R][maxloc]
+    const int chunk = blockIdx.x % nchunk;
+    const int rest = blockIdx.x / nchunk;
+    const int l = rest % lgrid, tile = rest / lgrid;
     const int lane = threadIdx.x;
     const int a = chunk * 64 + lane;
     const bool valid = a < n;
@@ -243,73 +245,94 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
 #pragma unroll
     for (int i = 0; i < R; i++) acc[i] = make_double2(0, 0);
 
-    auto stage = [&](const double2 *base) {
-        double2 tmp[R][kMaxLocW];
-#pragma unroll
-        for (int i = 0; i < R; i++)
-#pragma unroll
-            for (int t = 0; t < kMaxLocW; t++)
-                if (ljr[t] >= 0) tmp[i][t] = base[(size_t)rows[i] * n + ljr[t]];
-#pragma unroll
-        for (int i = 0; i < R; i++)
-#pragma unroll
-            for (int t = 0; t < kMaxLocW; t++)
-                if (ljr[t] >= 0) sm[i * maxloc + lane + 64 * t] = tmp[i][t];
-    };
-
     if (cH != 0.0) {
-        for (int l = 0; l < lmax; l++) {
+        // stage the chunk windows of Z0, Z2, W0, W1 of this multipole for every row of the tile
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double2 tmp[R][kMaxLocW];
+#pragma unroll
+            for (int i = 0; i < R; i++)
+#pragma unroll
+                for (int t = 0; t < kMaxLocW; t++)
+                    if (ljr[t] >= 0) tmp[i][t] = V[(size_t)(l * 4 + v) * plane + (size_t)rows[i] * n + ljr[t]];
+#pragma unroll
+            for (int i = 0; i < R; i++)
+#pragma unroll
+                for (int t = 0; t < kMaxLocW; t++)
+                    if (ljr[t] >= 0) sm[(v * R + i) * maxloc + lane + 64 * t] = tmp[i][t];
+        }
+        __syncthreads();
+        // flattened (v, slot batch) loop; the next batch's loads are in flight while this one computes
+        constexpr int SB = 4;
+        const int nbv = (ellw + SB - 1) / SB;
+        const int nb = 4 * nbv;
+        int lcN[SB];
+        double2 g1N[SB], g2N[SB];
+        auto fetch = [&](int j) {
+            const int v = j / nbv, s0 = (j % nbv) * SB;
+            // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
+            const int m1 = (v < 2) ? 2 * v : 4 + (v - 2);
+            const int m2 = (v < 2) ? 2 * v + 1 : 6 + (v - 2);
+            const double2 *G1 = GE + (size_t)(l * 8 + m1) * slab + ac;
+            const double2 *G2 = GE + (size_t)(l * 8 + m2) * slab + ac;
+#pragma unroll
+            for (int u = 0; u < SB; u++) {
+                const int s2 = s0 + u;
+                const bool ok = s2 < ellw;
+                lcN[u] = ok ? lc[(size_t)s2 * n] : 0;
+                g1N[u] = ok ? G1[(size_t)s2 * n] : make_double2(0, 0);
+                g2N[u] = ok ? G2[(size_t)s2 * n] : make_double2(0, 0);
+            }
+        };
+        fetch(0);
 #pragma unroll 1
-            for (int v = 0; v < 4; v++) {
-                stage(V + (size_t)(l * 4 + v) * plane);
-                __syncthreads();
-                // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
-                const int m1 = (v < 2) ? 2 * v : 4 + (v - 2);
-                const int m2 = (v < 2) ? 2 * v + 1 : 6 + (v - 2);
-                const double2 *G1 = GE + (size_t)(l * 8 + m1) * slab + ac;
-                const double2 *G2 = GE + (size_t)(l * 8 + m2) * slab + ac;
-#pragma unroll 1
-                for (int sb = 0; sb < ellw; sb += kSlotBatch) {
-                    double2 g1[kSlotBatch], g2[kSlotBatch];
-                    int lcol[kSlotBatch];
+        for (int j = 0; j < nb; j++) {
+            int lcC[SB];
+            double2 g1C[SB], g2C[SB];
 #pragma unroll
-                    for (int u = 0; u < kSlotBatch; u++) {
-                        const int s2 = sb + u;
-                        lcol[u] = s2 < ellw ? lc[(size_t)s2 * n] : 0;
-                        g1[u] = s2 < ellw ? G1[(size_t)s2 * n] : make_double2(0, 0);
-                        g2[u] = s2 < ellw ? G2[(size_t)s2 * n] : make_double2(0, 0);
-                    }
+            for (int u = 0; u < SB; u++) {
+                lcC[u] = lcN[u];
+                g1C[u] = g1N[u];
+                g2C[u] = g2N[u];
+            }
+            if (j + 1 < nb) fetch(j + 1);
+            const double2 *smv = sm + (size_t)(j / nbv) * R * maxloc;
 #pragma unroll
-                    for (int u = 0; u < kSlotBatch; u++) {
-                        const double2 M = make_double2(fma(k, g2[u].x, g1[u].x), fma(k, g2[u].y, g1[u].y));
+            for (int u = 0; u < SB; u++) {
+                const double2 M = make_double2(fma(k, g2C[u].x, g1C[u].x), fma(k, g2C[u].y, g1C[u].y));
 #pragma unroll
-                        for (int i = 0; i < R; i++) acc[i] = cfma(M, sm[i * maxloc + lcol[u]], acc[i]);
-                    }
-                }
-                __syncthreads();
+                for (int i = 0; i < R; i++) acc[i] = cfma(M, smv[i * maxloc + lcC[u]], acc[i]);
             }
         }
     }
-    // stationary diagonal operator on x_r
-    stage(x);
-    __syncthreads();
-    const double2 *Dq = D + ((size_t)sel * nq + q) * slab + ac;
     double2 dacc[R];
 #pragma unroll
     for (int i = 0; i < R; i++) dacc[i] = make_double2(0, 0);
-#pragma unroll 1
-    for (int sb = 0; sb < ellw; sb += kSlotBatch) {
-        double2 d[kSlotBatch];
-        int lcol[kSlotBatch];
+    if (l == 0) {
+        // stationary diagonal operator on x_r (carried by the first multipole slice)
+        __syncthreads();
+        {
+            double2 tmp[R][kMaxLocW];
 #pragma unroll
-        for (int u = 0; u < kSlotBatch; u++) {
-            lcol[u] = (sb + u) < ellw ? lc[(size_t)(sb + u) * n] : 0;
-            d[u] = (sb + u) < ellw ? Dq[(size_t)(sb + u) * n] : make_double2(0, 0);
+            for (int i = 0; i < R; i++)
+#pragma unroll
+                for (int t = 0; t < kMaxLocW; t++)
+                    if (ljr[t] >= 0) tmp[i][t] = x[(size_t)rows[i] * n + ljr[t]];
+#pragma unroll
+            for (int i = 0; i < R; i++)
+#pragma unroll
+                for (int t = 0; t < kMaxLocW; t++)
+                    if (ljr[t] >= 0) sm[i * maxloc + lane + 64 * t] = tmp[i][t];
         }
+        __syncthreads();
+        const double2 *Dq = D + ((size_t)sel * nq + q) * slab + ac;
+#pragma unroll 2
+        for (int s2 = 0; s2 < ellw; s2++) {
+            const int lcol = lc[(size_t)s2 * n];
+            const double2 d = Dq[(size_t)s2 * n];
 #pragma unroll
-        for (int u = 0; u < kSlotBatch; u++)
-#pragma unroll
-            for (int i = 0; i < R; i++) dacc[i] = cfma(d[u], sm[i * maxloc + lcol[u]], dacc[i]);
+            for (int i = 0; i < R; i++) dacc[i] = cfma(d, sm[i * maxloc + lcol], dacc[i]);
+        }
     }
     if (!valid) return;
 #pragma unroll
@@ -317,8 +340,23 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
         if (!live[i]) continue;
         const size_t o = (size_t)rows[i] * n + a;
         double2 res = cadd(dacc[i], cscale(acc[i], cH));
-        if (beta != 0.0) res = cadd(res, cscale(y[o], beta));
-        y[o] = res;
+        if (ypart) {
+            ypart[(size_t)l * plane + o] = res;  // summed over l by k_radial_sum
+        } else {
+            if (beta != 0.0) res = cadd(res, cscale(y[o], beta));
+            y[o] = res;
+        }
+    }
+}
+
+// y = sum_l ypart[l] + beta y  (fixed summation order)
+__global__ __launch_bounds__(256) void k_radial_sum(const double2 *__restrict__ ypart, int lgrid, long long N,
+                                                    double beta, double2 *__restrict__ y) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        double2 s = ypart[i];
+        for (int l = 1; l < lgrid; l++) s = cadd(s, ypart[(size_t)l * N + i]);
+        if (beta != 0.0) s = cadd(s, cscale(y[i], beta));
+        y[i] = s;
     }
 }
 
@@ -485,7 +523,7 @@ void free_all(gade_model *m) {
     m->pre = nullptr;
     void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
                     m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
-                    m->d_V, m->d_flag,
+                    m->d_V, m->d_ypart, m->d_flag,
                     m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_cnt,
                     m->d_fAp, m->d_fAi, m->d_fAx, m->d_SAp, m->d_SAi, m->d_SAx};
     for (void *p : ptrs)
@@ -503,10 +541,13 @@ int launch_angular(gade_model *m, const double2 *x) {
 }
 
 int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta) {
-    dim3 grid(m->ntiles * m->nchunk);
-    size_t shm = sizeof(double2) * (size_t)m->tile_r * m->maxloc;
+    const bool inter = cH != 0.0;
+    const int lgrid = inter ? m->lmax : 1;
+    dim3 grid(m->ntiles * lgrid * m->nchunk);
+    size_t shm = sizeof(double2) * 4 * (size_t)m->tile_r * m->maxloc;
+    double2 *part = inter ? m->d_ypart : nullptr;
 #define RAD(R) \
-    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), shm, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, m->lmax, m->nq, m->nchunk, m->maxloc, sel, cH, beta, y); break;
+    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), shm, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, lgrid, m->nq, m->nchunk, m->maxloc, sel, cH, beta, y, part); break;
     switch (m->tile_r) {
         RAD(1) RAD(2) RAD(4) RAD(8)
         default:
@@ -515,6 +556,11 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
     }
 #undef RAD
     GADE_HIP(hipGetLastError());
+    if (inter) {
+        hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart, lgrid,
+                           m->N, beta, y);
+        GADE_HIP(hipGetLastError());
+    }
     return GADE_OK;
 }
 
@@ -729,7 +775,8 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
         return fail(rc);
     if ((rc = hip_alloc((void **)&m->d_GE, sizeof(double2) * (size_t)lmax * 8 * ellw * n)) ||
         (rc = hip_alloc((void **)&m->d_D, sizeof(double2) * (size_t)3 * m->nq * ellw * n)) ||
-        (rc = hip_alloc((void **)&m->d_V, sizeof(double2) * (size_t)lmax * 4 * m->N)))
+        (rc = hip_alloc((void **)&m->d_V, sizeof(double2) * (size_t)lmax * 4 * m->N)) ||
+        (rc = hip_alloc((void **)&m->d_ypart, sizeof(double2) * (size_t)lmax * m->N)))
         return fail(rc);
     {
         int *d_kq = nullptr;

@@ -89,6 +89,7 @@ struct gade_model {
     int *d_tile_q = nullptr;       // [ntiles]
     double *d_tile_k = nullptr;    // [ntiles]
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
+    double2 *d_ypart = nullptr;    // [lmax][nblk][n] per-multipole partial outputs of the radial phase
     // preconditioner
     gade_blocklu *pre = nullptr;
     int *d_flag = nullptr;
