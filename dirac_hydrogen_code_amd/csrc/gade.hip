@@ -30,29 +30,53 @@ namespace {
 struct Prof {
     gade_model *m;
     const char *name;
+    bool on;
     hipEvent_t a{}, b{};
-    Prof(gade_model *mm, const char *nm) : m(mm), name(nm) {
-        if (m->profile) {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, m->stream);
+    static hipEvent_t take(gade_model *m) {
+        hipEvent_t e{};
+        if (!m->event_pool.empty()) {
+            e = m->event_pool.back();
+            m->event_pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
+    }
+    Prof(gade_model *mm, const char *nm, int bit) : m(mm), name(nm), on((mm->profile & bit) != 0) {
+        if (on) {
+            a = take(m);
+            b = take(m);
+            on = a && b;
+            if (on) (void)hipEventRecord(a, m->stream);
         }
     }
     ~Prof() {
-        if (m->profile) {
+        if (on) {
             (void)hipEventRecord(b, m->stream);
-            (void)hipEventSynchronize(b);
-            float ms = 0;
-            (void)hipEventElapsedTime(&ms, a, b);
-            auto &t = m->timers[name];
-            t.total_ms += ms;
-            t.count += 1;
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
+            m->pending.push_back({name, a, b});
         }
     }
 };
+
+void resolve_pending(gade_model *m) {
+    for (auto &p : m->pending) {
+        (void)hipEventSynchronize(p.b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        auto &t = m->timers[p.name];
+        t.total_ms += ms;
+        t.count += 1;
+        m->event_pool.push_back(p.a);
+        m->event_pool.push_back(p.b);
+    }
+    m->pending.clear();
+}
 }  // namespace
+
+__global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
 
 // ---------------------------------------------------------------------------
 // kernels: per-step G-sums and stationary diagonal operators
@@ -360,6 +384,255 @@ __global__ __launch_bounds__(256) void k_radial_sum(const double2 *__restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------
+// banded (DIA) form of the fused radial phase
+//
+// The radial pattern of the DKB basis is a 2x2 block of banded matrices: with
+// nh = n/2, row h*nh+i couples to columns h'*nh+i+d-BW (d = 0..2BW) of both
+// halves h'.  Coefficients are stored per quadrant qd = 2h+h' and diagonal d
+// as contiguous runs over i (padded to nhp, zeros outside the band), so a
+// radial product is a sum over d of coefficient[d][i] * window[i+d]: no column
+// indices, and the window of a wave is one contiguous run staged in LDS.
+//
+// One wave owns kDiaR row blocks of one kappa, one output half h and
+// kDiaSeg = 64*kDiaP consecutive outputs (kDiaP per lane, so each staged
+// window value is read once per lane and reused across the diagonals), and
+// one multipole l (the fastest part of blockIdx, so with 8 multipoles each
+// XCD streams a single multipole's coefficients from its own L2).  The
+// per-multipole results go to ypart and are summed in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kDiaR = 8;
+constexpr int kDiaP = 2;
+constexpr int kDiaSeg = 64 * kDiaP;
+constexpr int kDiaBW = 7;
+
+// LDS slot of window position p: swaps neighbours on alternate 8-slot rows so
+// that lanes reading positions 2*lane+t hit distinct banks
+__device__ __forceinline__ int dia_swz(int p) { return p ^ (((p >> 3) ^ (p >> 4)) & 1); }
+
+// GD[l][v][qd][d][ip] = (G_m1, G_m2) of the kappa-folded pair of v, from the
+// DIA-ordered couplings gs[l][j][nn][qd][d][ip] (g of alpha = 2j + (l even))
+// and gts (conjugated transposes).  One thread per (l, qd, d, ip).
+__global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ gs, const double2 *__restrict__ gts,
+                                                   Coef6 cf, int lmax, int nd, int nhp, double2 *__restrict__ GD) {
+    const int S = 4 * nd * nhp;  // entries per (l, j, nn) plane
+    const long long total = (long long)lmax * S;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int l = (int)(idx / S);
+        const int e = (int)(idx - (long long)l * S);  // (qd, d, ip)
+        double2 G[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) G[m] = make_double2(0, 0);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double2 c = cf.c[2 * j + ((l % 2 == 0) ? 1 : 0)];
+#pragma unroll
+            for (int nn = 0; nn < 4; nn++) {
+                const size_t o = ((size_t)(l * 3 + j) * 4 + nn) * S + e;
+                G[nn] = cfma(c, gs[o], G[nn]);
+                G[4 + nn] = cfma(c, gts[o], G[4 + nn]);
+            }
+        }
+        // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
+        const int m1[4] = {0, 2, 4, 5}, m2[4] = {1, 3, 6, 7};
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const size_t o = ((size_t)(l * 4 + v) * S + e) * 2;
+            GD[o] = G[m1[v]];
+            GD[o + 1] = G[m2[v]];
+        }
+    }
+}
+
+// gs/gts gathered into DIA order once at setup; dia_src[(qd*nd+d)*nhp+ip] is
+// the radial CSR index of that entry or -1.
+__global__ void k_build_gdia(const double2 *__restrict__ g, const int *__restrict__ dia_src,
+                             const int *__restrict__ tmap, int lmax, int nd, int nhp, int nnzb,
+                             double2 *__restrict__ gs, double2 *__restrict__ gts) {
+    const int S = 4 * nd * nhp;
+    const long long total = (long long)lmax * 12 * S;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(idx % S);
+        const int ljn = (int)(idx / S);  // (l*3 + j)*4 + nn
+        const int nn = ljn % 4, j = (ljn / 4) % 3, l = ljn / 12;
+        const int alpha = 2 * j + ((l % 2 == 0) ? 1 : 0);
+        const int k = dia_src[e];
+        double2 a = make_double2(0, 0), b = make_double2(0, 0);
+        if (k >= 0) {
+            const size_t plane = (size_t)(alpha * 4 * lmax + l * 4 + nn) * nnzb;
+            a = g[plane + k];
+            b = g[plane + tmap[k]];
+            b.y = -b.y;
+        }
+        gs[idx] = a;
+        gts[idx] = b;
+    }
+}
+
+// DD[sel][q][qd][d][ip] from the ELL-layout diagonal operators
+__global__ void k_build_DD(const double2 *__restrict__ D, const int *__restrict__ dia_src,
+                           const int *__restrict__ ell_pos, int nq, int ellw, int n, int nd, int nhp,
+                           double2 *__restrict__ DD) {
+    const int S = 4 * nd * nhp;
+    const long long total = (long long)3 * nq * S;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int e = (int)(idx % S);
+        const long long sq = idx / S;  // sel*nq + q
+        const int k = dia_src[e];
+        DD[idx] = k >= 0 ? D[(size_t)sq * ellw * n + ell_pos[k]] : make_double2(0, 0);
+    }
+}
+
+template <int BW>
+__global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x, const double2 *__restrict__ V,
+                                                   const double2 *__restrict__ GD, const double2 *__restrict__ DD,
+                                                   const int *__restrict__ tiles, const int *__restrict__ tile_q,
+                                                   const double *__restrict__ tile_k, int nblk, int n, int nh,
+                                                   int nhp, int nseg, int lgrid, int nq, int sel, double cH,
+                                                   double beta, double2 *__restrict__ y,
+                                                   double2 *__restrict__ ypart) {
+    constexpr int R = kDiaR, P = kDiaP, ND = 2 * BW + 1, W = kDiaSeg + 2 * BW;
+    __shared__ double2 sm[R * W];
+    const int lane = threadIdx.x;
+    int rest = blockIdx.x;
+    const int l = rest % lgrid;
+    rest /= lgrid;
+    const int seg = rest % nseg;
+    rest /= nseg;
+    const int h = rest & 1;
+    const int tile = rest >> 1;
+    const double kap = tile_k[tile];
+    const int q = tile_q[tile];
+    const int i0 = seg * kDiaSeg;
+    int rows[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) rows[i] = tiles[tile * R + i];
+    const size_t plane = (size_t)nblk * n;
+    const int S = 4 * ND * nhp;
+
+    double2 acc[R][P];
+#pragma unroll
+    for (int i = 0; i < R; i++)
+#pragma unroll
+        for (int j = 0; j < P; j++) acc[i][j] = make_double2(0, 0);
+
+    // window of half hp of 'src' for every row of the tile: position p <-> half index i0 - BW + p.
+    // All loads of the window are issued before the first LDS write.
+    auto stage = [&](const double2 *src, int hp) {
+        constexpr int NC = (W + 63) / 64;
+        double2 t[R][NC];
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            const double2 *rp = src + (size_t)(rows[i] >= 0 ? rows[i] : 0) * n + hp * nh;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int p = c * 64 + lane;
+                const int b = i0 - BW + p;
+                t[i][c] = make_double2(0, 0);
+                if (rows[i] >= 0 && p < W && b >= 0 && b < nh) t[i][c] = rp[b];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < R; i++)
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                const int p = c * 64 + lane;
+                if (p < W) sm[i * W + dia_swz(p)] = t[i][c];
+            }
+        __syncthreads();
+    };
+    // acc[i][j] += sum_d M_d[i0 + P*lane + j] * window[P*lane + j + d]; with two coefficient
+    // streams M = C1 + kap*C2 (pairs), else M = C1.  Coefficients are fetched PF diagonals ahead.
+    auto sweep = [&](const double2 *C, bool pair) {
+        constexpr int PF = 3;
+        const int stride = pair ? 2 : 1;
+        const double2 *cp = C + (size_t)(i0 + P * lane) * stride;
+        double2 w[R][P];
+#pragma unroll
+        for (int i = 0; i < R; i++)
+#pragma unroll
+            for (int j = 0; j < P; j++) w[i][j] = sm[i * W + dia_swz(P * lane + j)];
+        double2 c1[PF][P], c2[PF][P];
+#pragma unroll
+        for (int d = 0; d < PF - 1; d++)
+#pragma unroll
+            for (int j = 0; j < P; j++) {
+                const double2 *np = cp + (size_t)d * nhp * stride;
+                c1[d][j] = np[j * stride];
+                c2[d][j] = pair ? np[j * stride + 1] : make_double2(0, 0);
+            }
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            if (d + PF - 1 < ND) {
+                const double2 *np = cp + (size_t)(d + PF - 1) * nhp * stride;
+#pragma unroll
+                for (int j = 0; j < P; j++) {
+                    c1[(d + PF - 1) % PF][j] = np[j * stride];
+                    c2[(d + PF - 1) % PF][j] = pair ? np[j * stride + 1] : make_double2(0, 0);
+                }
+            }
+            double2 M[P];
+#pragma unroll
+            for (int j = 0; j < P; j++) {
+                const double2 a = c1[d % PF][j], g = c2[d % PF][j];
+                M[j] = pair ? make_double2(fma(kap, g.x, a.x), fma(kap, g.y, a.y)) : a;
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+#pragma unroll
+                for (int j = 0; j < P; j++) acc[i][j] = cfma(M[j], w[i][j], acc[i][j]);
+                if (d + 1 < ND) {
+#pragma unroll
+                    for (int j = 0; j + 1 < P; j++) w[i][j] = w[i][j + 1];
+                    w[i][P - 1] = sm[i * W + dia_swz(P * lane + P + d)];
+                }
+            }
+        }
+    };
+
+    if (cH != 0.0) {
+        for (int v = 0; v < 4; v++) {
+            for (int hp = 0; hp < 2; hp++) {
+                stage(V + (size_t)(l * 4 + v) * plane, hp);
+                sweep(GD + ((size_t)(l * 4 + v) * S + (size_t)(2 * h + hp) * ND * nhp) * 2, true);
+            }
+        }
+        if (cH != 1.0) {
+#pragma unroll
+            for (int i = 0; i < R; i++)
+#pragma unroll
+                for (int j = 0; j < P; j++) acc[i][j] = cscale(acc[i][j], cH);
+        }
+    }
+    if (l == 0) {
+        for (int hp = 0; hp < 2; hp++) {
+            stage(x, hp);
+            sweep(DD + ((size_t)(sel * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (rows[i] < 0) continue;
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const int ip = i0 + P * lane + j;
+            if (ip >= nh) continue;
+            const size_t o = (size_t)rows[i] * n + h * nh + ip;
+            if (ypart) {
+                ypart[(size_t)l * plane + o] = acc[i][j];
+            } else {
+                double2 r = acc[i][j];
+                if (beta != 0.0) r = cadd(r, cscale(y[o], beta));
+                y[o] = r;
+            }
+        }
+    }
+}
+
 // per-block S-norm contributions pop[r] = x_r^H S(k_r) x_r
 __global__ __launch_bounds__(256) void k_snorm(const double2 *__restrict__ x, const double2 *__restrict__ D,
                                                const int *__restrict__ ell_col, const int *__restrict__ blk_q,
@@ -523,12 +796,15 @@ void free_all(gade_model *m) {
     m->pre = nullptr;
     void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
                     m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
-                    m->d_V, m->d_ypart, m->d_flag,
+                    m->d_V, m->d_ypart, m->d_flag, m->d_dia_src, m->d_gs, m->d_gts, m->d_GD, m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k,
                     m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_cnt,
                     m->d_fAp, m->d_fAi, m->d_fAx, m->d_SAp, m->d_SAi, m->d_SAx};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (m->h_sc) hipHostFree(m->h_sc);
+    resolve_pending(m);
+    for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
+    m->event_pool.clear();
 }
 
 int launch_angular(gade_model *m, const double2 *x) {
@@ -542,21 +818,43 @@ int launch_angular(gade_model *m, const double2 *x) {
 
 int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta) {
     const bool inter = cH != 0.0;
+    if (m->radial_kind == 2) {
+        const int lgrid = inter ? m->lmax : 1;
+        dim3 grid((unsigned)m->ndtiles * 2 * m->nseg * lgrid);
+        {
+            Prof pr(m, "radial", GADE_PROF_RADIAL);
+            hipLaunchKernelGGL(k_radial_dia<kDiaBW>, grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD, m->d_DD,
+                               m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, m->nseg, lgrid,
+                               m->nq, sel, cH, beta, y, inter ? m->d_ypart : nullptr);
+            GADE_HIP(hipGetLastError());
+        }
+        if (inter) {
+            Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
+            hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart,
+                               lgrid, m->N, beta, y);
+            GADE_HIP(hipGetLastError());
+        }
+        return GADE_OK;
+    }
     const int lgrid = inter ? m->lmax : 1;
     dim3 grid(m->ntiles * lgrid * m->nchunk);
     size_t shm = sizeof(double2) * 4 * (size_t)m->tile_r * m->maxloc;
     double2 *part = inter ? m->d_ypart : nullptr;
 #define RAD(R) \
     case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), shm, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, lgrid, m->nq, m->nchunk, m->maxloc, sel, cH, beta, y, part); break;
-    switch (m->tile_r) {
-        RAD(1) RAD(2) RAD(4) RAD(8)
-        default:
-            gade::set_error("unsupported tile width");
-            return GADE_EINVAL;
+    {
+        Prof pr(m, "radial", GADE_PROF_RADIAL);
+        switch (m->tile_r) {
+            RAD(1) RAD(2) RAD(4) RAD(8)
+            default:
+                gade::set_error("unsupported tile width");
+                return GADE_EINVAL;
+        }
     }
 #undef RAD
     GADE_HIP(hipGetLastError());
     if (inter) {
+        Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
         hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart, lgrid,
                            m->N, beta, y);
         GADE_HIP(hipGetLastError());
@@ -564,28 +862,95 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
     return GADE_OK;
 }
 
-int build_tiles(gade_model *m) {
-    // group blocks by distinct kappa index, split into tiles of tile_r rows
+// group blocks by distinct kappa index and split each group into tiles of R rows (-1 pads)
+int kappa_tiles(const gade_model *m, int R, int **d_tiles, int **d_tq, double **d_tk, int *ntiles) {
     std::vector<int> tiles, tq;
     std::vector<double> tk;
     for (int q = 0; q < m->nq; q++) {
         std::vector<int> rows;
         for (int r = 0; r < m->nblk; r++)
             if (m->blk_q[r] == q) rows.push_back(r);
-        for (size_t i = 0; i < rows.size(); i += m->tile_r) {
-            for (int j = 0; j < m->tile_r; j++) tiles.push_back(i + j < rows.size() ? rows[i + j] : -1);
+        for (size_t i = 0; i < rows.size(); i += R) {
+            for (int j = 0; j < R; j++) tiles.push_back(i + j < rows.size() ? rows[i + j] : -1);
             tq.push_back(q);
             tk.push_back((double)m->kap_q[q]);
         }
     }
-    m->ntiles = (int)tq.size();
-    if (m->d_tiles) hipFree(m->d_tiles);
-    if (m->d_tile_q) hipFree(m->d_tile_q);
-    if (m->d_tile_k) hipFree(m->d_tile_k);
-    m->d_tiles = nullptr; m->d_tile_q = nullptr; m->d_tile_k = nullptr;
-    GADE_TRY(upload(&m->d_tiles, tiles.data(), tiles.size()));
-    GADE_TRY(upload(&m->d_tile_q, tq.data(), tq.size()));
-    GADE_TRY(upload(&m->d_tile_k, tk.data(), tk.size()));
+    *ntiles = (int)tq.size();
+    for (void *p : {(void *)*d_tiles, (void *)*d_tq, (void *)*d_tk})
+        if (p) (void)hipFree(p);
+    *d_tiles = nullptr;
+    *d_tq = nullptr;
+    *d_tk = nullptr;
+    GADE_TRY(upload(d_tiles, tiles.data(), tiles.size()));
+    GADE_TRY(upload(d_tq, tq.data(), tq.size()));
+    GADE_TRY(upload(d_tk, tk.data(), tk.size()));
+    return GADE_OK;
+}
+
+int build_tiles(gade_model *m) { return kappa_tiles(m, m->tile_r, &m->d_tiles, &m->d_tile_q, &m->d_tile_k, &m->ntiles); }
+
+// Banded form of the radial pattern: n even, every entry (h*nh+i, h'*nh+j)
+// with |i-j| <= kDiaBW.  Builds the DIA maps and the static DIA operators.
+int setup_dia(gade_model *m) {
+    const int n = m->n;
+    m->dia_ok = false;
+    if (n % 2) return GADE_OK;
+    const int nh = n / 2;
+    for (int a = 0; a < n; a++)
+        for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++)
+            if (std::abs(m->rad_Ai[k] % nh - a % nh) > kDiaBW) return GADE_OK;
+    m->nh = nh;
+    m->nd = 2 * kDiaBW + 1;
+    m->nseg = (nh + kDiaSeg - 1) / kDiaSeg;
+    m->nhp = m->nseg * kDiaSeg;
+    const int S = 4 * m->nd * m->nhp;
+    std::vector<int> dia_src(S, -1);
+    for (int qd = 0; qd < 4; qd++)
+        for (int d = 0; d < m->nd; d++)
+            for (int ip = 0; ip < nh; ip++) {
+                const int a = (qd >> 1) * nh + ip, jc = ip + d - kDiaBW;
+                if (jc < 0 || jc >= nh) continue;
+                const int col = (qd & 1) * nh + jc;
+                const int b = m->rad_Ap[a], len = m->rad_Ap[a + 1] - b;
+                const int p = b + locate(m->rad_Ai.data() + b, len, col);
+                if (p < m->rad_Ap[a + 1] && m->rad_Ai[p] == col) dia_src[((size_t)qd * m->nd + d) * m->nhp + ip] = p;
+            }
+    const int lmax = m->lmax;
+    GADE_TRY(upload(&m->d_dia_src, dia_src.data(), dia_src.size()));
+    GADE_TRY(hip_alloc((void **)&m->d_gs, sizeof(double2) * (size_t)lmax * 12 * S));
+    GADE_TRY(hip_alloc((void **)&m->d_gts, sizeof(double2) * (size_t)lmax * 12 * S));
+    GADE_TRY(hip_alloc((void **)&m->d_GD, sizeof(double2) * (size_t)lmax * 4 * S * 2));
+    GADE_TRY(hip_alloc((void **)&m->d_DD, sizeof(double2) * (size_t)3 * m->nq * S));
+    const long long tg = (long long)lmax * 12 * S, td = (long long)3 * m->nq * S;
+    hipLaunchKernelGGL(k_build_gdia, dim3(grid_for(tg, 256)), dim3(256), 0, m->stream, m->d_g, m->d_dia_src, m->d_tmap,
+                       lmax, m->nd, m->nhp, m->nnzb, m->d_gs, m->d_gts);
+    hipLaunchKernelGGL(k_build_DD, dim3(grid_for(td, 256)), dim3(256), 0, m->stream, m->d_D, m->d_dia_src,
+                       m->d_ell_pos, m->nq, m->ellw, n, m->nd, m->nhp, m->d_DD);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    GADE_TRY(kappa_tiles(m, kDiaR, &m->d_dtiles, &m->d_dtile_q, &m->d_dtile_k, &m->ndtiles));
+    m->dia_ok = true;
+    return GADE_OK;
+}
+
+// G-sums of the current f(t) in the layout of the active radial kernel
+int launch_gsums(gade_model *m, bool ell) {
+    Coef6 cf;
+    // c_alpha = SoL * (i h dt/2) * f_alpha
+    const double hd = kSoL * m->h * m->dt / 2;
+    for (int a = 0; a < 6; a++) cf.c[a] = make_double2(-hd * m->ft[2 * a + 1], hd * m->ft[2 * a]);
+    Prof pr(m, "gsums", GADE_PROF_GSUMS);
+    if (ell) {
+        long long total = (long long)m->lmax * 8 * m->ellw * m->n;
+        hipLaunchKernelGGL(k_gsums, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_g, m->d_ell_src,
+                           m->d_tmap, cf, m->lmax, m->ellw, m->n, m->nnzb, m->d_GE);
+    } else {
+        long long total = (long long)m->lmax * 4 * m->nd * m->nhp;
+        hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_gs, m->d_gts, cf,
+                           m->lmax, m->nd, m->nhp, m->d_GD);
+    }
+    GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
 
@@ -607,6 +972,20 @@ int gade_memcpy_h2d(void *dst, const void *src, size_t bytes) {
 }
 int gade_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     GADE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GADE_OK;
+}
+int gade_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    if (!dst || !src) return GADE_EINVAL;
+    if (bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0) {
+        // plain streaming copy: reads and writes exactly 'bytes' (used to calibrate traffic counters)
+        const long long n = (long long)(bytes / 16);
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256, 8192)), dim3(256), 0, nullptr, (const double2 *)src,
+                           (double2 *)dst, n);
+        GADE_HIP(hipGetLastError());
+        GADE_HIP(hipDeviceSynchronize());
+    } else {
+        GADE_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    }
     return GADE_OK;
 }
 int gade_memset(void *ptr, int value, size_t bytes) {
@@ -792,6 +1171,8 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
         }
     }
     if ((rc = build_tiles(m))) return fail(rc);
+    if ((rc = setup_dia(m))) return fail(rc);
+    m->radial_kind = m->dia_ok ? 2 : 1;
     // BiCGSTAB workspace and scalars
     double2 **ws[] = {&m->r, &m->r0, &m->v, &m->p, &m->y, &m->z, &m->s, &m->t, &m->tmp};
     for (auto w : ws) {
@@ -838,16 +1219,7 @@ int gade_model_info(const gade_model_t *m, int *nblk, int *blkdim, int *hall_nnz
 int gade_set_time(gade_model_t *m, const double *ft) {
     if (!m || !ft) return GADE_EINVAL;
     m->ft.assign(ft, ft + 12);
-    Coef6 cf;
-    // c_alpha = SoL * (i h dt/2) * f_alpha
-    const double hd = kSoL * m->h * m->dt / 2;
-    for (int a = 0; a < 6; a++) cf.c[a] = make_double2(-hd * ft[2 * a + 1], hd * ft[2 * a]);
-    long long total = (long long)m->lmax * 8 * m->ellw * m->n;
-    Prof pr(m, "gsums");
-    hipLaunchKernelGGL(k_gsums, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_g, m->d_ell_src, m->d_tmap,
-                       cf, m->lmax, m->ellw, m->n, m->nnzb, m->d_GE);
-    GADE_HIP(hipGetLastError());
-    return GADE_OK;
+    return launch_gsums(m, m->radial_kind != 2);
 }
 
 static int apply_async(gade_model *m, const double2 *x, double2 *y, double cH, double beta) {
@@ -860,10 +1232,9 @@ static int apply_async(gade_model *m, const double2 *x, double2 *y, double cH, d
         return GADE_EINVAL;
     }
     if (sel != 2) {
-        Prof pr(m, "angular");
+        Prof pr(m, "angular", GADE_PROF_ANGULAR);
         GADE_TRY(launch_angular(m, x));
     }
-    Prof pr(m, "radial");
     return launch_radial(m, x, y, sel, cH, beta);
 }
 
@@ -898,13 +1269,13 @@ int gade_precond_setup(gade_model_t *m) {
 namespace {
 
 int psolve_async(gade_model *m, const double2 *b, double2 *x) {
-    Prof pr(m, "psolve");
+    Prof pr(m, "psolve", GADE_PROF_PSOLVE);
     return gade::blocklu_solve_async(m->pre, b, x, m->stream);
 }
 
 template <int OP>
 int bicg_op(gade_model *m, const double2 *b, double2 *x = nullptr) {
-    Prof pr(m, "blas1");
+    Prof pr(m, "blas1", GADE_PROF_BLAS1);
     hipLaunchKernelGGL(k_bicg<OP>, dim3(m->red_blocks), dim3(kRedThreads), 0, m->stream, m->N, b, m->r, m->r0, m->p,
                        m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part);
     if (OP != OP_P && OP != OP_S)
@@ -1139,12 +1510,13 @@ int gade_assembled_setup(gade_model_t *m) {
 
 int gade_assembled_timedep_into(gade_model_t *m, const int *Ap_dev, double *Ax_dev) {
     if (!m || !Ap_dev || !Ax_dev) return GADE_EINVAL;
+    if (m->radial_kind == 2) GADE_TRY(launch_gsums(m, true));  // the fused path keeps only the DIA layout
     int *d_subAp = nullptr;
     GADE_TRY(upload(&d_subAp, m->rad_Ap.data(), m->rad_Ap.size()));
     const long long N = m->N;
     dim3 g2((unsigned)std::min<long long>(N, 65535), (unsigned)((N + 65534) / 65535));
     {
-        Prof pr(m, "assemble");
+        Prof pr(m, "assemble", GADE_PROF_ASSEMBLE);
 #define ASM(LM) \
     case LM: hipLaunchKernelGGL(k_assemble<LM>, g2, dim3(256), 0, m->stream, m->d_hAp, m->d_hAi, m->d_kap, m->d_coef, m->d_diag, d_subAp, m->d_ell_pos, m->d_GE, m->d_D, m->d_blk_q, Ap_dev, m->nblk, m->n, m->ellw, m->nq, (double2 *)Ax_dev); break;
         switch (m->lmax) {
@@ -1177,12 +1549,12 @@ int gade_assembled_spmv(gade_model_t *m, const double *x, double *y, double alph
     const long long N = m->N;
     unsigned grid = (unsigned)((N * 64 + 255) / 256);
     {
-        Prof pr(m, "csr_spmv");
+        Prof pr(m, "csr_spmv", GADE_PROF_CSR_SPMV);
         hipLaunchKernelGGL(k_csr_spmv, dim3(grid), dim3(256), 0, m->stream, m->d_fAp, m->d_fAi, m->d_fAx,
                            (const double2 *)x, N, make_double2(alpha, 0), make_double2(beta, 0), (double2 *)y);
     }
     if (with_S) {
-        Prof pr(m, "csr_spmv_S");
+        Prof pr(m, "csr_spmv_S", GADE_PROF_CSR_SPMV);
         hipLaunchKernelGGL(k_csr_spmv, dim3(grid), dim3(256), 0, m->stream, m->d_SAp, m->d_SAi, m->d_SAx,
                            (const double2 *)x, N, make_double2(1.0, 0), make_double2(1.0, 0), (double2 *)y);
     }
@@ -1228,6 +1600,19 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->tile_r = value;
         return build_tiles(m);
     }
+    if (!strcmp(name, "radial")) {
+        // 0: banded kernel when the pattern allows it, 1: ELL kernel, 2: banded kernel
+        if (value < 0 || value > 2 || (value == 2 && !m->dia_ok)) {
+            gade::set_error(value == 2 ? "the radial pattern is not banded: no DIA kernel" : "radial must be 0, 1 or 2");
+            return GADE_EINVAL;
+        }
+        const int kind = value == 0 ? (m->dia_ok ? 2 : 1) : value;
+        if (kind != m->radial_kind) {
+            m->radial_kind = kind;
+            GADE_TRY(launch_gsums(m, kind != 2));  // coefficients of the current f(t) in the new layout
+        }
+        return GADE_OK;
+    }
     gade::set_error(std::string("unknown parameter ") + name);
     return GADE_EINVAL;
 }
@@ -1238,19 +1623,27 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
         *value = m->tile_r;
         return GADE_OK;
     }
+    if (!strcmp(name, "radial")) {
+        *value = m->radial_kind;
+        return GADE_OK;
+    }
     gade::set_error(std::string("unknown parameter ") + name);
     return GADE_EINVAL;
 }
 
-int gade_profile(gade_model_t *m, int enable) {
+int gade_profile(gade_model_t *m, int mask) {
     if (!m) return GADE_EINVAL;
-    m->profile = enable != 0;
-    if (!enable) m->timers.clear();
+    m->profile = mask;
+    if (!mask) {
+        resolve_pending(m);
+        m->timers.clear();
+    }
     return GADE_OK;
 }
 
 int gade_profile_get(gade_model_t *m, const char *name, double *total_ms, long long *count) {
     if (!m || !name) return GADE_EINVAL;
+    resolve_pending(m);
     auto it = m->timers.find(name);
     if (total_ms) *total_ms = it == m->timers.end() ? 0 : it->second.total_ms;
     if (count) *count = it == m->timers.end() ? 0 : it->second.count;

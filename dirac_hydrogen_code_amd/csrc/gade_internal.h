@@ -38,6 +38,12 @@ struct Timer {
     long long count = 0;
 };
 
+// one kernel-family launch bracketed by two events, resolved when queried
+struct PendingEvent {
+    std::string name;
+    hipEvent_t a, b;
+};
+
 }  // namespace gade
 
 // Block-diagonal preconditioner: exact banded LU of each distinct diagonal
@@ -84,10 +90,20 @@ struct gade_model {
     double2 *d_coef = nullptr;     // [hnnz][lmax][2]
     int *d_diag = nullptr;         // [hnnz] 1 when orig_row == orig_col
     // kappa-grouped row tiles for the radial phase
-    int tile_r = 4, ntiles = 0;
+    int tile_r = 2, ntiles = 0;
     int *d_tiles = nullptr;        // [ntiles][tile_r]
     int *d_tile_q = nullptr;       // [ntiles]
     double *d_tile_k = nullptr;    // [ntiles]
+    // banded (DIA) radial phase: quadrant/diagonal-major coefficients, tiles of 8 rows of one kappa
+    bool dia_ok = false;           // radial pattern is a 2x2 block of bands of half-width <= 7
+    int radial_kind = 0;           // 1: ELL kernel, 2: DIA kernel (active)
+    int nh = 0, nhp = 0, nd = 0, nseg = 0, ndtiles = 0;
+    int *d_dia_src = nullptr;      // [4][nd][nhp] radial CSR index or -1
+    double2 *d_gs = nullptr, *d_gts = nullptr;  // [lmax][3][4][4][nd][nhp] DIA-ordered couplings
+    double2 *d_GD = nullptr;       // [lmax][4][4][nd][nhp][2] kappa-folded G-sum pairs
+    double2 *d_DD = nullptr;       // [3][nq][4][nd][nhp] diagonal operators
+    int *d_dtiles = nullptr, *d_dtile_q = nullptr;
+    double *d_dtile_k = nullptr;
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     double2 *d_ypart = nullptr;    // [lmax][nblk][n] per-multipole partial outputs of the radial phase
     // preconditioner
@@ -108,7 +124,10 @@ struct gade_model {
     double2 *d_fAx = nullptr;
     int *d_SAp = nullptr, *d_SAi = nullptr;
     double2 *d_SAx = nullptr;
-    // profiling
-    bool profile = false;
+    // profiling: event pairs are recorded without synchronizing and resolved
+    // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families
+    int profile = 0;
     std::map<std::string, gade::Timer> timers;
+    std::vector<gade::PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool;
 };

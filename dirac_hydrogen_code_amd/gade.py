@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libgade.so")
 # every entry point declared in include/gade.h
 EXPORTS = (
     "gade_last_error", "gade_malloc", "gade_free", "gade_memcpy_h2d", "gade_memcpy_d2h", "gade_memset",
-    "gade_device_sync", "gade_model_create", "gade_model_load", "gade_model_destroy", "gade_model_info",
+    "gade_memcpy_d2d", "gade_device_sync", "gade_model_create", "gade_model_load", "gade_model_destroy", "gade_model_info",
     "gade_set_time", "gade_apply", "gade_precond_setup", "gade_psolve", "gade_bicgstab", "gade_reset_workspace",
     "gade_cn_step", "gade_cn_run", "gade_pulse", "gade_ik", "gade_imu", "gade_snorm", "gade_assembled_setup",
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
@@ -44,7 +44,7 @@ def lib():
         L.gade_last_error.restype = C.c_char_p
         sig = {
             "gade_malloc": [C.POINTER(vp), sz], "gade_free": [vp], "gade_memcpy_h2d": [vp, vp, sz],
-            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_device_sync": [],
+            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_device_sync": [],
             "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
             "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                 C.POINTER(C.c_longlong), C.POINTER(i32)],
@@ -84,6 +84,10 @@ def pulse(E0, omega, cycles, t):
     out = np.zeros(6, np.complex128)
     lib().gade_pulse(E0, omega, cycles, t, out.ctypes.data)
     return out
+
+
+PROF = {"gsums": 1, "angular": 2, "radial": 4, "psolve": 8, "blas1": 16, "assemble": 32, "csr_spmv": 64, "radial_sum": 128}
+PROF_ALL = 0x7fffffff
 
 
 class DeviceArray:
@@ -220,8 +224,17 @@ class Propagator:
         check(lib().gade_get_param(self.ptr, name.encode(), C.byref(v)))
         return v.value
 
-    def profile(self, enable=True):
-        check(lib().gade_profile(self.ptr, int(enable)))
+    def profile(self, families="all"):
+        """Enable per-family event timing: 'all', a list of family names, or a false value to stop."""
+        if not families:
+            mask = 0
+        elif families == "all" or families is True:
+            mask = PROF_ALL
+        else:
+            mask = 0
+            for f in ([families] if isinstance(families, str) else families):
+                mask |= PROF[f]
+        check(lib().gade_profile(self.ptr, mask))
 
     def profile_get(self, name):
         t, c = C.c_double(), C.c_longlong()

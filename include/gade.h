@@ -66,6 +66,7 @@ int gade_free(void *ptr_dev);
 int gade_memcpy_h2d(void *dst_dev, const void *src, size_t bytes);
 int gade_memcpy_d2h(void *dst, const void *src_dev, size_t bytes);
 int gade_memset(void *ptr_dev, int value, size_t bytes);
+int gade_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes);
 int gade_device_sync(void);
 
 /* Builds the device model: uploads couplings once, precomputes the block
@@ -145,14 +146,23 @@ int gade_assembled_spmv(gade_model_t *m, const double *x_dev, double *y_dev, dou
 int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, double *Ax);
 
 /* Tunable launch parameters (no reference counterpart; used by the autotuner):
- *   "tile_r"  row blocks per radial tile (1, 2, 4, 8).
+ *   "tile_r"  row blocks per radial tile of the ELL kernel (1, 2, 4, 8);
+ *   "radial"  radial-phase kernel: 0 banded when the pattern allows (default),
+ *             1 general ELL kernel, 2 banded (DIA) kernel; reads back 1 or 2.
  * Results do not depend on the choice. */
 int gade_set_param(gade_model_t *m, const char *name, int value);
 int gade_get_param(const gade_model_t *m, const char *name, int *value);
 
-/* Timing of the last call of each kernel family, measured with HIP events on
- * the model's stream when profiling is enabled (gade_profile(m, 1)). */
-int gade_profile(gade_model_t *m, int enable);
+/* Per-family kernel timing with HIP events on the model's stream.  'mask'
+ * selects the families (GADE_PROF_*; GADE_PROF_ALL for every one, 0 stops and
+ * clears).  Events are recorded without synchronizing; gade_profile_get
+ * waits for the recorded launches and returns their accumulated time. */
+enum {
+    GADE_PROF_GSUMS = 1, GADE_PROF_ANGULAR = 2, GADE_PROF_RADIAL = 4, GADE_PROF_PSOLVE = 8, GADE_PROF_BLAS1 = 16,
+    GADE_PROF_ASSEMBLE = 32, GADE_PROF_CSR_SPMV = 64, GADE_PROF_RADIAL_SUM = 128,
+    GADE_PROF_ALL = 0x7fffffff
+};
+int gade_profile(gade_model_t *m, int mask);
 int gade_profile_get(gade_model_t *m, const char *name, double *total_ms, long long *count);
 
 #ifdef __cplusplus
