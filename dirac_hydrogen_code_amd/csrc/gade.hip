@@ -43,6 +43,7 @@ struct Prof {
         return e;
     }
     Prof(gade_model *mm, const char *nm, int bit) : m(mm), name(nm), on((mm->profile & bit) != 0) {
+        if (on && m->prof_period > 1) on = (m->prof_seen[nm]++ % m->prof_period) == 0;
         if (on) {
             a = take(m);
             b = take(m);
@@ -88,7 +89,9 @@ struct Coef6 {
 // GE[l][m][s][a]: m<4: Gs_{l,m} = sum_alpha c_alpha g^m_{alpha,l};
 // m>=4: Gts_{l,m-4} with gt = conj-transpose of g (tools/solver.c:236-287).
 __global__ void k_gsums(const double2 *__restrict__ g, const int *__restrict__ ell_src, const int *__restrict__ tmap,
-                        Coef6 cf, int lmax, int ellw, int n, int nnzb, double2 *__restrict__ GE) {
+                        Coef6 cf, int lmax, int ellw, int n, int nnzb, double2 *__restrict__ GE,
+                  const int *__restrict__ gate) {
+    GADE_GATED(gate);
     long long total = (long long)lmax * 8 * ellw * n;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
@@ -152,7 +155,9 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restri
                                                          const int *__restrict__ hAi,
                                                          const double2 *__restrict__ coef,
                                                          const int *__restrict__ kap, int nblk, int n, int lmax,
-                                                         int maxrow, double2 *__restrict__ V) {
+                                                         int maxrow, double2 *__restrict__ V,
+                  const int *__restrict__ gate) {
+    GADE_GATED(gate);
     extern __shared__ double2 shc[];  // [maxrow][LB][2] coefficients, then [maxrow] (col, kappa) packed
     const int r = blockIdx.x;
     const int a = blockIdx.y * blockDim.x + threadIdx.x;
@@ -236,7 +241,9 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
                                                const int *__restrict__ tile_q, const double *__restrict__ tile_k,
                                                int nblk, int n, int ellw, int lgrid, int nq, int nchunk, int maxloc,
                                                int sel, double cH, double beta, double2 *__restrict__ y,
-                                               double2 *__restrict__ ypart) {
+                                               double2 *__restrict__ ypart,
+                  const int *__restrict__ gate) {
+    GADE_GATED(gate);
     extern __shared__ double2 sm[];  // [4][R][maxloc]
     const int chunk = blockIdx.x % nchunk;
     const int rest = blockIdx.x / nchunk;
@@ -375,7 +382,9 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
 
 // y = sum_l ypart[l] + beta y  (fixed summation order)
 __global__ __launch_bounds__(256) void k_radial_sum(const double2 *__restrict__ ypart, int lgrid, long long N,
-                                                    double beta, double2 *__restrict__ y) {
+                                                    double beta, double2 *__restrict__ y,
+                  const int *__restrict__ gate) {
+    GADE_GATED(gate);
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
         double2 s = ypart[i];
         for (int l = 1; l < lgrid; l++) s = cadd(s, ypart[(size_t)l * N + i]);
@@ -414,7 +423,9 @@ __device__ __forceinline__ int dia_swz(int p) { return p ^ (((p >> 3) ^ (p >> 4)
 // DIA-ordered couplings gs[l][j][nn][qd][d][ip] (g of alpha = 2j + (l even))
 // and gts (conjugated transposes).  One thread per (l, qd, d, ip).
 __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ gs, const double2 *__restrict__ gts,
-                                                   Coef6 cf, int lmax, int nd, int nhp, double2 *__restrict__ GD) {
+                                                   Coef6 cf, int lmax, int nd, int nhp, double2 *__restrict__ GD,
+                  const int *__restrict__ gate) {
+    GADE_GATED(gate);
     const int S = 4 * nd * nhp;  // entries per (l, j, nn) plane
     const long long total = (long long)lmax * S;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
@@ -493,7 +504,9 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
                                                    const double *__restrict__ tile_k, int nblk, int n, int nh,
                                                    int nhp, int nseg, int lgrid, int nq, int sel, double cH,
                                                    double beta, double2 *__restrict__ y,
-                                                   double2 *__restrict__ ypart) {
+                                                   double2 *__restrict__ ypart,
+                  const int *__restrict__ gate) {
+    GADE_GATED(gate);
     constexpr int R = kDiaR, P = kDiaP, ND = 2 * BW + 1, W = kDiaSeg + 2 * BW;
     __shared__ double2 sm[R * W];
     const int lane = threadIdx.x;
@@ -802,6 +815,9 @@ void free_all(gade_model *m) {
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (m->h_sc) hipHostFree(m->h_sc);
+    if (m->h_ctl) hipHostFree(m->h_ctl);
+    for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols})
+        if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
     m->event_pool.clear();
@@ -811,7 +827,7 @@ int launch_angular(gade_model *m, const double2 *x) {
     dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
     size_t shm = sizeof(double2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
     hipLaunchKernelGGL(k_angular<kAngLB>, grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef,
-                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V);
+                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V, m->gate);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -825,13 +841,13 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
             Prof pr(m, "radial", GADE_PROF_RADIAL);
             hipLaunchKernelGGL(k_radial_dia<kDiaBW>, grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD, m->d_DD,
                                m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, m->nseg, lgrid,
-                               m->nq, sel, cH, beta, y, inter ? m->d_ypart : nullptr);
+                               m->nq, sel, cH, beta, y, inter ? m->d_ypart : nullptr, m->gate);
             GADE_HIP(hipGetLastError());
         }
         if (inter) {
             Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
             hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart,
-                               lgrid, m->N, beta, y);
+                               lgrid, m->N, beta, y, m->gate);
             GADE_HIP(hipGetLastError());
         }
         return GADE_OK;
@@ -841,7 +857,7 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
     size_t shm = sizeof(double2) * 4 * (size_t)m->tile_r * m->maxloc;
     double2 *part = inter ? m->d_ypart : nullptr;
 #define RAD(R) \
-    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), shm, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, lgrid, m->nq, m->nchunk, m->maxloc, sel, cH, beta, y, part); break;
+    case R: hipLaunchKernelGGL(k_radial<R>, grid, dim3(64), shm, m->stream, x, m->d_V, m->d_GE, m->d_D, m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_tiles, m->d_tile_q, m->d_tile_k, m->nblk, m->n, m->ellw, lgrid, m->nq, m->nchunk, m->maxloc, sel, cH, beta, y, part, m->gate); break;
     {
         Prof pr(m, "radial", GADE_PROF_RADIAL);
         switch (m->tile_r) {
@@ -856,7 +872,7 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
     if (inter) {
         Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
         hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart, lgrid,
-                           m->N, beta, y);
+                           m->N, beta, y, m->gate);
         GADE_HIP(hipGetLastError());
     }
     return GADE_OK;
@@ -944,11 +960,11 @@ int launch_gsums(gade_model *m, bool ell) {
     if (ell) {
         long long total = (long long)m->lmax * 8 * m->ellw * m->n;
         hipLaunchKernelGGL(k_gsums, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_g, m->d_ell_src,
-                           m->d_tmap, cf, m->lmax, m->ellw, m->n, m->nnzb, m->d_GE);
+                           m->d_tmap, cf, m->lmax, m->ellw, m->n, m->nnzb, m->d_GE, m->gate);
     } else {
         long long total = (long long)m->lmax * 4 * m->nd * m->nhp;
         hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_gs, m->d_gts, cf,
-                           m->lmax, m->nd, m->nhp, m->d_GD);
+                           m->lmax, m->nd, m->nhp, m->d_GD, m->gate);
     }
     GADE_HIP(hipGetLastError());
     return GADE_OK;
@@ -1187,6 +1203,12 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
         return fail(rc);
     if (hipMemset(m->d_cnt, 0, sizeof(unsigned) * 4) != hipSuccess || hipMemset(m->d_sc, 0, sizeof(double) * 64) != hipSuccess)
         return fail(GADE_EHIP);
+    if ((rc = hip_alloc((void **)&m->d_ctl, sizeof(int) * CT_COUNT))) return fail(rc);
+    if (hipMemset(m->d_ctl, 0, sizeof(int) * CT_COUNT) != hipSuccess) return fail(GADE_EHIP);
+    if (hipHostMalloc((void **)&m->h_ctl, sizeof(int) * CT_COUNT, hipHostMallocDefault) != hipSuccess) {
+        gade::set_error("hipHostMalloc failed");
+        return fail(GADE_ENOMEM);
+    }
     if (hipHostMalloc((void **)&m->h_sc, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) {
         gade::set_error("hipHostMalloc failed");
         return fail(GADE_ENOMEM);
@@ -1270,16 +1292,18 @@ namespace {
 
 int psolve_async(gade_model *m, const double2 *b, double2 *x) {
     Prof pr(m, "psolve", GADE_PROF_PSOLVE);
-    return gade::blocklu_solve_async(m->pre, b, x, m->stream);
+    return gade::blocklu_solve_async(m->pre, b, x, m->stream, m->gate);
 }
 
 template <int OP>
 int bicg_op(gade_model *m, const double2 *b, double2 *x = nullptr) {
     Prof pr(m, "blas1", GADE_PROF_BLAS1);
+    int *ctl = m->ctl_on ? m->d_ctl : nullptr;
     hipLaunchKernelGGL(k_bicg<OP>, dim3(m->red_blocks), dim3(kRedThreads), 0, m->stream, m->N, b, m->r, m->r0, m->p,
-                       m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part);
+                       m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part, m->gate, ctl);
     if (OP != OP_P && OP != OP_S)
-        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(1), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part, m->d_sc);
+        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(1), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part, m->d_sc,
+                           m->gate, ctl);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -1296,36 +1320,22 @@ int applyA(gade_model *m, const double2 *x, double2 *out, int *napply) {
     return apply_async(m, x, out, 1.0, 0.0);
 }
 
-int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, double *tol_error, int *napply) {
-    const double tol = *tol_error;
-    const int maxit = *iters;
-    GADE_TRY(gade_precond_setup(m));
-    if (napply) *napply = 0;
-    // r = rhs - A x ; r0 = r
-    GADE_TRY(applyA(m, x, m->r, napply));
-    GADE_TRY(bicg_op<OP_INIT>(m, rhs));
+// Host-driven BiCGSTAB iterations from iteration i, with r, r0, p, v, x and the
+// scalars as they stand on the device (tools/gpu_bicgstab.c:133-307).
+int bicg_host_loop(gade_model *m, const double2 *rhs, double2 *x, int i, double tol, int maxit, int *iters,
+                   double *tol_error, int *napply) {
     GADE_TRY(read_scalars(m));
     double r0_sq = m->h_sc[SC_R0SQ];
     const double rhs_sq = m->h_sc[SC_RHSSQ];
-    if (rhs_sq == 0) {
-        GADE_HIP(hipMemsetAsync(x, 0, sizeof(double2) * m->N, m->stream));
-        GADE_HIP(hipStreamSynchronize(m->stream));
-        *iters = 0;
-        *tol_error = 0;
-        return GADE_OK;
-    }
     const double tol2 = tol * tol * rhs_sq;
-    const double eps2 = 1e-12 * 1e-12;
-    if (r0_sq < tol2) {
-        *iters = 0;
-        *tol_error = std::sqrt(r0_sq / rhs_sq);
-        return GADE_OK;
+    double rnorm2 = m->h_sc[SC_RNORM2];
+    double rho_next_abs = std::hypot(m->h_sc[SC_RHONEXT], m->h_sc[SC_RHONEXT + 1]);
+    if (!std::isfinite(rnorm2)) {
+        gade::set_error("BiCGSTAB: non-finite residual");
+        return GADE_ENOTCONV;
     }
-    int i = 0;
-    double rnorm2 = r0_sq;
-    double rho_next_abs = r0_sq;
     while (i < maxit) {
-        if (rho_next_abs < eps2 * r0_sq) {
+        if (rho_next_abs < kRestartEps2 * r0_sq) {
             // restart: r = rhs - A x, r0 = r, rho = |r|^2 (tools/gpu_bicgstab.c:222-241)
             GADE_TRY(applyA(m, x, m->r, napply));
             GADE_TRY(bicg_op<OP_RESTART>(m, rhs));
@@ -1340,7 +1350,7 @@ int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, dou
         GADE_TRY(psolve_async(m, m->s, m->z));           // z = M^-1 s
         GADE_TRY(applyA(m, m->z, m->t, napply));         // t = A z
         GADE_TRY(bicg_op<OP_W>(m, rhs));                 // w = (t . s) / |t|^2
-        GADE_TRY(bicg_op<OP_XR>(m, rhs, x));    // x += alpha y + w z ; r = s - w t
+        GADE_TRY(bicg_op<OP_XR>(m, rhs, x));             // x += alpha y + w z ; r = s - w t
         GADE_TRY(read_scalars(m));
         rnorm2 = m->h_sc[SC_RNORM2];
         rho_next_abs = std::hypot(m->h_sc[SC_RHONEXT], m->h_sc[SC_RHONEXT + 1]);
@@ -1353,6 +1363,179 @@ int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, dou
     }
     *tol_error = std::sqrt(rnorm2 / rhs_sq);
     *iters = i;
+    return GADE_OK;
+}
+
+int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, double *tol_error, int *napply) {
+    const double tol = *tol_error;
+    const int maxit = *iters;
+    GADE_TRY(gade_precond_setup(m));
+    if (napply) *napply = 0;
+    // r = rhs - A x ; r0 = r
+    GADE_TRY(applyA(m, x, m->r, napply));
+    GADE_TRY(bicg_op<OP_INIT>(m, rhs));
+    GADE_TRY(read_scalars(m));
+    const double r0_sq = m->h_sc[SC_R0SQ];
+    const double rhs_sq = m->h_sc[SC_RHSSQ];
+    if (rhs_sq == 0) {
+        GADE_HIP(hipMemsetAsync(x, 0, sizeof(double2) * m->N, m->stream));
+        GADE_HIP(hipStreamSynchronize(m->stream));
+        *iters = 0;
+        *tol_error = 0;
+        return GADE_OK;
+    }
+    if (r0_sq < tol * tol * rhs_sq) {
+        *iters = 0;
+        *tol_error = std::sqrt(r0_sq / rhs_sq);
+        return GADE_OK;
+    }
+    return bicg_host_loop(m, rhs, x, 0, tol, maxit, iters, tol_error, napply);
+}
+
+// ---------------------------------------------------------------------------
+// device-driven CN stepping
+//
+// A step is enqueued as f(t) -> G-sums, rhs = (S - H) psi, r = rhs - A psi and
+// K BiCGSTAB iterations whose launches are gated on the device control words:
+// the reductions take the host loop's decisions (converged, maxit, restart,
+// non-finite) on the device, and once a solve is decided the remaining
+// iterations of the step return at once.  k_step_end records iterations and
+// tolerance, or marks the run stalled when the step needs more than K
+// iterations or the host (restart, non-finite, zero rhs); then every later
+// launch is void until the host has finished that step with bicg_host_loop.
+// The host synchronizes once per window of steps.
+// ---------------------------------------------------------------------------
+__global__ void k_ctl_set(int *ctl, double *sc, double tol, int maxit) {
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < CT_COUNT; k++) ctl[k] = 0;
+    ctl[CT_MAXIT] = maxit;
+    sc[SC_TOL] = tol;
+}
+
+struct GateScope {
+    gade_model *m;
+    explicit GateScope(gade_model *mm) : m(mm) { m->ctl_on = true; }
+    ~GateScope() {
+        m->gate = nullptr;
+        m->ctl_on = false;
+    }
+};
+
+int ensure_step_buffers(gade_model *m, int nsteps) {
+    if (nsteps <= m->steps_cap) return GADE_OK;
+    if (m->d_iters) (void)hipFree(m->d_iters);
+    if (m->d_tols) (void)hipFree(m->d_tols);
+    m->d_iters = nullptr;
+    m->d_tols = nullptr;
+    m->steps_cap = 0;
+    GADE_TRY(hip_alloc((void **)&m->d_iters, sizeof(int) * nsteps));
+    GADE_TRY(hip_alloc((void **)&m->d_tols, sizeof(double) * nsteps));
+    m->steps_cap = nsteps;
+    return GADE_OK;
+}
+
+int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K) {
+    GateScope gs(m);
+    m->ft.assign(ft, ft + 12);
+    m->gate = m->d_ctl + CT_STALLED;
+    GADE_TRY(launch_gsums(m, m->radial_kind != 2));
+    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
+    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0, x0 = psi
+    GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
+    m->gate = m->d_ctl + CT_SKIP;
+    for (int k = 0; k < K; k++) {
+        GADE_TRY(bicg_op<OP_P>(m, m->tmp));
+        GADE_TRY(psolve_async(m, m->p, m->y));
+        GADE_TRY(apply_async(m, m->y, m->v, 1.0, 0.0));
+        GADE_TRY(bicg_op<OP_ALPHA>(m, m->tmp));
+        GADE_TRY(bicg_op<OP_S>(m, m->tmp));
+        GADE_TRY(psolve_async(m, m->s, m->z));
+        GADE_TRY(apply_async(m, m->z, m->t, 1.0, 0.0));
+        GADE_TRY(bicg_op<OP_W>(m, m->tmp));
+        GADE_TRY(bicg_op<OP_XR>(m, m->tmp, psi));
+    }
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->d_sc, step, m->d_iters, m->d_tols);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// Steps 0..nsteps-1 at the given times (f(t) from gade_pulse); psi is x0 and result.
+int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, double E0, double omega,
+                  double cycles, double tol, int maxit, int *iters, double *tol_out, double *usec) {
+    if (nsteps == 0) return GADE_OK;
+    GADE_TRY(gade_precond_setup(m));
+    GADE_TRY(ensure_step_buffers(m, nsteps));
+    std::vector<int> it_h(nsteps, 0);
+    std::vector<double> tol_h(nsteps, 0.0);
+    std::vector<hipEvent_t> ev;
+    auto free_events = [&]() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+    };
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->d_sc, tol, maxit);
+    GADE_HIP(hipGetLastError());
+    int s = 0;
+    while (s < nsteps) {
+        const int W = std::min(m->window, nsteps - s);
+        const int K = m->iter_batch > 0 ? m->iter_batch : m->last_k;
+        if (usec) {
+            free_events();
+            ev.resize(W + 1);
+            for (auto &e : ev) GADE_HIP(hipEventCreate(&e));
+            GADE_HIP(hipEventRecord(ev[0], m->stream));
+        }
+        for (int w = 0; w < W; w++) {
+            double ft[12];
+            gade_pulse(E0, omega, cycles, times[s + w], ft);
+            int rc = enqueue_step(m, psi, ft, s + w, K);
+            if (rc == GADE_OK && usec) rc = hipEventRecord(ev[w + 1], m->stream) == hipSuccess ? GADE_OK : GADE_EHIP;
+            if (rc) {
+                free_events();
+                return rc;
+            }
+        }
+        GADE_HIP(hipMemcpyAsync(m->h_ctl, m->d_ctl, sizeof(int) * CT_COUNT, hipMemcpyDeviceToHost, m->stream));
+        GADE_HIP(hipMemcpyAsync(it_h.data() + s, m->d_iters + s, sizeof(int) * W, hipMemcpyDeviceToHost, m->stream));
+        GADE_HIP(hipMemcpyAsync(tol_h.data() + s, m->d_tols + s, sizeof(double) * W, hipMemcpyDeviceToHost,
+                                m->stream));
+        GADE_HIP(hipStreamSynchronize(m->stream));
+        int done = s + W;
+        if (m->h_ctl[CT_STALLED]) {
+            done = m->h_ctl[CT_STALL_STEP];
+            // finish the stalled step on the host from the device state
+            GADE_TRY(read_scalars(m));
+            int it = 0;
+            double te = 0;
+            if (m->h_sc[SC_RHSSQ] == 0) {
+                GADE_HIP(hipMemsetAsync(psi, 0, sizeof(double2) * m->N, m->stream));
+            } else {
+                GADE_TRY(bicg_host_loop(m, m->tmp, psi, m->h_ctl[CT_ITER], tol, maxit, &it, &te, nullptr));
+            }
+            it_h[done] = it;
+            tol_h[done] = te;
+            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->d_sc, tol, maxit);
+            GADE_HIP(hipGetLastError());
+            done += 1;
+        }
+        int kmax = 0;
+        for (int w = s; w < done; w++) kmax = std::max(kmax, it_h[w] + 1);
+        m->last_k = std::min(std::max(kmax + 1, 2), 16);
+        if (usec) {
+            for (int w = 0; w < done - s; w++) {
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, ev[w], ev[w + 1]);
+                usec[s + w] = ms * 1000.0;
+            }
+            if (done - s < W) usec[done - 1] = -1;  // finished on the host: no device timing
+        }
+        s = done;
+    }
+    free_events();
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    for (int w = 0; w < nsteps; w++) {
+        if (iters) iters[w] = it_h[w];
+        if (tol_out) tol_out[w] = tol_h[w];
+    }
     return GADE_OK;
 }
 
@@ -1393,19 +1576,26 @@ int gade_cn_step(gade_model_t *m, double *psi, const double *ft, double tol, int
     if (tol_error) *tol_error = te;
     return GADE_OK;
 }
-
 int gade_cn_run(gade_model_t *m, double *psi, double E0, double omega, double cycles, double t0, int nsteps,
                 double tol, int maxit, int *iters, double *tol_out, double *usec) {
     if (!m || !psi || nsteps < 0) return GADE_EINVAL;
     GADE_TRY(gade_precond_setup(m));
+    // step times accumulate as in the reference loop (tools/dirac_solver.c:509)
+    std::vector<double> times(nsteps);
     double time = t0;
+    for (int step = 0; step < nsteps; step++) {
+        time = time + m->dt;
+        times[step] = time;
+    }
+    if (m->device_loop)
+        return cn_run_device(m, (double2 *)psi, times.data(), nsteps, E0, omega, cycles, tol, maxit, iters, tol_out,
+                             usec);
     hipEvent_t ea, eb;
     GADE_HIP(hipEventCreate(&ea));
     GADE_HIP(hipEventCreate(&eb));
     for (int step = 0; step < nsteps; step++) {
         double ft[12];
-        time = time + m->dt;  // tools/dirac_solver.c:509
-        gade_pulse(E0, omega, cycles, time, ft);
+        gade_pulse(E0, omega, cycles, times[step], ft);
         if (usec) GADE_HIP(hipEventRecord(ea, m->stream));
         int it = 0;
         double te = 0;
@@ -1600,6 +1790,26 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->tile_r = value;
         return build_tiles(m);
     }
+    if (!strcmp(name, "device_loop")) {
+        m->device_loop = value != 0;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "window") || !strcmp(name, "iter_batch")) {
+        if (value < (name[0] == 'w' ? 1 : 0) || value > 4096) {
+            gade::set_error(std::string(name) + " out of range");
+            return GADE_EINVAL;
+        }
+        (name[0] == 'w' ? m->window : m->iter_batch) = value;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "profile_period")) {
+        if (value < 1) {
+            gade::set_error("profile_period must be >= 1");
+            return GADE_EINVAL;
+        }
+        m->prof_period = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "radial")) {
         // 0: banded kernel when the pattern allows it, 1: ELL kernel, 2: banded kernel
         if (value < 0 || value > 2 || (value == 2 && !m->dia_ok)) {
@@ -1625,6 +1835,22 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "radial")) {
         *value = m->radial_kind;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "device_loop")) {
+        *value = m->device_loop;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "window")) {
+        *value = m->window;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "iter_batch")) {
+        *value = m->iter_batch;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "profile_period")) {
+        *value = m->prof_period;
         return GADE_OK;
     }
     gade::set_error(std::string("unknown parameter ") + name);
@@ -1705,8 +1931,8 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
     const size_t cs = (size_t)CH * CH;
     if ((rc = hip_alloc((void **)&lu->d_Linv, sizeof(double2) * nq * lu->nch * cs)) ||
         (rc = hip_alloc((void **)&lu->d_Uinv, sizeof(double2) * nq * lu->nch * cs)) ||
-        (rc = hip_alloc((void **)&lu->d_Loff, sizeof(double2) * nq * lu->nch * lu->band * CH)) ||
-        (rc = hip_alloc((void **)&lu->d_Uoff, sizeof(double2) * nq * lu->nch * lu->band * CH)) ||
+        (rc = hip_alloc((void **)&lu->d_Loff, sizeof(double2) * nq * lu->nch * cs)) ||
+        (rc = hip_alloc((void **)&lu->d_Uoff, sizeof(double2) * nq * lu->nch * cs)) ||
         (rc = hip_alloc((void **)&lu->d_flag, sizeof(int))))
         return fail(rc);
     if (hipMemsetAsync(lu->d_flag, 0, sizeof(int), stream) != hipSuccess) return fail(GADE_EHIP);
@@ -1744,7 +1970,7 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
     return GADE_OK;
 }
 
-int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream) {
+int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream, const int *gate) {
     if (!lu) {
         gade::set_error("preconditioner not set up");
         return GADE_EINVAL;
@@ -1752,7 +1978,7 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
     const int G = 64 / lu->ch;
     size_t shm = sizeof(double2) * (size_t)G * ((size_t)lu->nch * lu->ch + lu->ch);
 #define PS(C) \
-    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(lu->npw), dim3(64), shm, stream, b, x, lu->d_wblocks, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch); break;
+    case C: hipLaunchKernelGGL(k_psolve<C>, dim3(lu->npw), dim3(64), shm, stream, b, x, lu->d_wblocks, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch, gate); break;
     switch (lu->ch) { PS(16) PS(32) PS(64) default: return GADE_EINVAL; }
 #undef PS
     GADE_HIP(hipGetLastError());

@@ -42,6 +42,27 @@ __device__ __forceinline__ double2 ldnt(const double2 *p) {
 }
 
 // ---------------------------------------------------------------------------
+// gated launches: a kernel given a non-null 'gate' returns at once when the
+// device word it points at is non-zero.  The device-driven CN stepping uses
+// this to enqueue a step's iterations ahead of knowing how many it needs.
+// ---------------------------------------------------------------------------
+#define GADE_GATED(gate)                          \
+    do {                                          \
+        if ((gate) != nullptr && *(gate) != 0) return; \
+    } while (0)
+
+// control words of the device-driven solve (int, in d_ctl)
+enum {
+    CT_STATUS = 0,   // 0 iterating, 1 converged, 2 needs the host loop, 3 maxit reached
+    CT_ITER = 1,     // completed iterations of the current solve
+    CT_STALLED = 2,  // a step ended unfinished: every later gated launch is void
+    CT_SKIP = 3,     // CT_STATUS != 0 or CT_STALLED: the iteration gate
+    CT_MAXIT = 5,
+    CT_STALL_STEP = 6,
+    CT_COUNT = 8
+};
+
+// ---------------------------------------------------------------------------
 // BiCGSTAB vector kernels with deterministic last-block reductions
 // ---------------------------------------------------------------------------
 // scalar slots in d_sc
@@ -53,8 +74,11 @@ enum {
     SC_RHSSQ = 7,
     SC_RNORM2 = 8,
     SC_RHONEXT = 9,    // 2
-    SC_COUNT = 12
+    SC_TOL = 11,       // requested relative tolerance (device-driven solve)
+    SC_TOL2 = 12,      // tol^2 |rhs|^2
+    SC_COUNT = 16
 };
+constexpr double kRestartEps2 = 1e-24;  // restart when |rho| < 1e-24 |r0|^2 (tools/gpu_bicgstab.c:195)
 enum RedOp { OP_INIT = 0, OP_RESTART, OP_P, OP_ALPHA, OP_S, OP_W, OP_XR };
 
 constexpr int kRedThreads = 256;
@@ -93,9 +117,22 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2
                                                        const double2 *__restrict__ v, double2 *__restrict__ s,
                                                        const double2 *__restrict__ t, const double2 *__restrict__ yv,
                                                        const double2 *__restrict__ zv, double2 *__restrict__ x,
-                                                       const double *__restrict__ sc, double *__restrict__ part) {
+                                                       const double *__restrict__ sc, double *__restrict__ part,
+                                                       const int *gate = nullptr, int *ctl = nullptr) {
+    GADE_GATED(gate);
     __shared__ double sh[4 * 16];
     double acc[4] = {0, 0, 0, 0};
+    if (OP == OP_P && ctl != nullptr) {
+        // top of an iteration of the device-driven solve: a restart is left to the host loop
+        const double rn = hypot(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
+        if (rn < kRestartEps2 * sc[SC_R0SQ]) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ctl[CT_STATUS] = 2;
+                ctl[CT_SKIP] = 1;
+            }
+            return;
+        }
+    }
     double2 beta = make_double2(0, 0), w = make_double2(0, 0), alpha = make_double2(0, 0);
     if (OP == OP_P) {
         // beta = (rho_new / rho_old) * (alpha / w)
@@ -152,7 +189,9 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2
 
 // Sums the partials in a fixed order and updates the scalars (one block).
 template <int OP>
-__global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const double *__restrict__ part, double *sc) {
+__global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const double *__restrict__ part, double *sc,
+                                                           const int *gate = nullptr, int *ctl = nullptr) {
+    GADE_GATED(gate);
     __shared__ double sh[4 * 16];
     double tot[4] = {0, 0, 0, 0};
     for (int bidx = threadIdx.x; bidx < nparts; bidx += blockDim.x)
@@ -189,6 +228,45 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
         sc[SC_RNORM2] = tot[0];
         sc[SC_RHONEXT] = tot[1];
         sc[SC_RHONEXT + 1] = tot[2];
+    }
+    if (ctl == nullptr) return;
+    // device-driven solve: the same decisions as the host loop (break-before-increment)
+    if (OP == OP_INIT) {
+        const double tol = sc[SC_TOL];
+        const double tol2 = tol * tol * tot[1];
+        sc[SC_TOL2] = tol2;
+        const int st = (tot[1] == 0) ? 2 : (tot[0] < tol2 ? 1 : 0);
+        ctl[CT_STATUS] = st;
+        ctl[CT_ITER] = 0;
+        ctl[CT_SKIP] = st != 0;
+    } else if (OP == OP_XR) {
+        int st = 0;
+        if (!isfinite(tot[0])) {
+            st = 2;
+        } else if (tot[0] < sc[SC_TOL2]) {
+            st = 1;
+        } else {
+            const int it = ctl[CT_ITER] + 1;
+            ctl[CT_ITER] = it;
+            if (it >= ctl[CT_MAXIT]) st = 3;
+        }
+        ctl[CT_STATUS] = st;
+        ctl[CT_SKIP] = st != 0;
+    }
+}
+
+// End of a device-driven CN step: record iterations and tolerance of a
+// finished solve, or mark the run stalled at this step so the host continues it.
+[[maybe_unused]] static __global__ void k_step_end(int *ctl, const double *__restrict__ sc, int step, int *iters, double *tols) {
+    if (threadIdx.x != 0 || ctl[CT_STALLED] != 0) return;
+    const int st = ctl[CT_STATUS];
+    if (st == 1 || st == 3) {
+        iters[step] = ctl[CT_ITER];
+        tols[step] = sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]);
+    } else {
+        ctl[CT_STALLED] = 1;
+        ctl[CT_SKIP] = 1;
+        ctl[CT_STALL_STEP] = step;
     }
 }
 
@@ -291,13 +369,14 @@ __global__ __launch_bounds__(CH) void k_chunk_inverse(const double2 *AB, int n, 
         }
         Ui[i] = v;
     }
-    // band couplings: lane j is a row of the chunk
+    // band couplings [t][row], CH rows of t with zeros for t >= band: lane j is a row of the chunk
     const int gi = base + j;
-    double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH;
-    double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH;
-    for (int t = 0; t < band; t++) {
-        Lo[(size_t)t * CH + j] = (gi < n) ? at(gi, base - band + t) : make_double2(0, 0);
-        Uo[(size_t)t * CH + j] = (gi < n) ? at(gi, base + CH + t) : make_double2(0, 0);
+    double2 *Lo = Loff + ((size_t)q * nch + ch) * CH * CH;
+    double2 *Uo = Uoff + ((size_t)q * nch + ch) * CH * CH;
+    for (int t = 0; t < CH; t++) {
+        const bool in = gi < n && t < band;
+        Lo[(size_t)t * CH + j] = in ? at(gi, base - band + t) : make_double2(0, 0);
+        Uo[(size_t)t * CH + j] = in ? at(gi, base + CH + t) : make_double2(0, 0);
     }
 }
 
@@ -308,16 +387,20 @@ static __device__ __forceinline__ int natural_index(int gi, int n, int ilv) {
 
 // x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =
 // (block slot, row of the chunk).  Levels are chunks of CH rows: each chunk
-// takes the band coupling to the previous chunk, then its diagonal inverse.
-// Factor values are fetched 16 at a time so their loads overlap.
+// takes the band coupling to the neighbouring chunk, then its diagonal-block
+// inverse.  b is staged in LDS before the sweeps, and the factors of the next
+// level are loaded one phase ahead (coupling factors during the inverse phase,
+// inverse factors during the coupling phase) so their latency overlaps work.
 template <int CH>
 __global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, double2 *__restrict__ xo,
                                                const int *__restrict__ wblocks, const int *__restrict__ blk_q,
                                                int ilv, const double2 *__restrict__ Linv,
                                                const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
-                                               const double2 *__restrict__ Uoff, int n, int band, int nch) {
+                                               const double2 *__restrict__ Uoff, int n, int band, int nch,
+                                               const int *__restrict__ gate = nullptr) {
+    GADE_GATED(gate);
     constexpr int G = 64 / CH;
-    constexpr int B = 16;
+    constexpr int NB = 16;  // b loads in flight while staging
     extern __shared__ double2 smem[];
     const int lane = threadIdx.x, g = lane / CH, i = lane % CH;
     const int r = wblocks[blockIdx.x * G + g];
@@ -326,65 +409,138 @@ __global__ __launch_bounds__(64) void k_psolve(const double2 *__restrict__ b, do
     double2 *acc = yv + nch * CH;
     const size_t cs = (size_t)CH * CH;
     const double2 *bb = b + (size_t)(r >= 0 ? r : 0) * n;
-    // forward sweep
-    for (int ch = 0; ch < nch; ch++) {
-        const int gi = ch * CH + i;
-        double2 v = (r >= 0 && gi < n) ? bb[natural_index(gi, n, ilv)] : make_double2(0, 0);
-        if (ch > 0) {
-            const double2 *Lo = Loff + ((size_t)q * nch + ch) * band * CH + i;
-            for (int t0 = 0; t0 < band; t0 += B) {
-                double2 f[B];
+    for (int c0 = 0; c0 < nch; c0 += NB) {
+        double2 t[NB];
 #pragma unroll
-                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Lo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
-#pragma unroll
-                for (int u = 0; u < B; u++)
-                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[ch * CH - band + t0 + u]));
-            }
+        for (int u = 0; u < NB; u++) {
+            const int gi = (c0 + u) * CH + i;
+            t[u] = (r >= 0 && c0 + u < nch && gi < n) ? bb[natural_index(gi, n, ilv)] : make_double2(0, 0);
         }
-        acc[i] = v;
-        __syncthreads();
-        const double2 *Li = Linv + ((size_t)q * nch + ch) * cs + i;
-        double2 yi = make_double2(0, 0);
 #pragma unroll
-        for (int j0 = 0; j0 < CH; j0 += B) {
-            double2 f[B];
-#pragma unroll
-            for (int u = 0; u < B; u++) f[u] = Li[(size_t)(j0 + u) * CH];
-#pragma unroll
-            for (int u = 0; u < B; u++) yi = cfma(f[u], acc[j0 + u], yi);
-        }
-        yv[gi] = yi;
-        __syncthreads();
+        for (int u = 0; u < NB; u++)
+            if (c0 + u < nch) yv[(c0 + u) * CH + i] = t[u];
     }
-    // backward sweep (in place)
-    for (int ch = nch - 1; ch >= 0; ch--) {
-        const int gi = ch * CH + i;
-        double2 v = yv[gi];
-        if (ch < nch - 1) {
-            const double2 *Uo = Uoff + ((size_t)q * nch + ch) * band * CH + i;
-            for (int t0 = 0; t0 < band; t0 += B) {
+    __syncthreads();
+    if constexpr (CH <= 16) {
+        double2 fc[CH], fi[CH];
+        auto load_cpl = [&](const double2 *Off, int ch) {
+            const double2 *p = Off + ((size_t)q * nch + ch) * cs + i;
+#pragma unroll
+            for (int t = 0; t < CH; t++) fc[t] = p[(size_t)t * CH];
+        };
+        auto load_inv = [&](const double2 *Inv, int ch) {
+            const double2 *p = Inv + ((size_t)q * nch + ch) * cs + i;
+#pragma unroll
+            for (int j = 0; j < CH; j++) fi[j] = p[(size_t)j * CH];
+        };
+        // v - sum_t fc[t] * y[base + t] over all CH rows of the coupling (rows t >= band are zero and
+        // meet finite staged values), two interleaved partial sums
+        auto coupling = [&](double2 v, int base) {
+            double2 s0 = make_double2(0, 0), s1 = make_double2(0, 0);
+#pragma unroll
+            for (int t = 0; t < CH; t += 2) {
+                s0 = cfma(fc[t], yv[base + t], s0);
+                s1 = cfma(fc[t + 1], yv[base + t + 1], s1);
+            }
+            return csub(v, cadd(s0, s1));
+        };
+        // sum_j fi[j] * acc[j], four interleaved partial sums
+        auto inverse = [&]() {
+            double2 s[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) s[k] = make_double2(0, 0);
+#pragma unroll
+            for (int j = 0; j < CH; j++) s[j % 4] = cfma(fi[j], acc[j], s[j % 4]);
+            return cadd(cadd(s[0], s[1]), cadd(s[2], s[3]));
+        };
+        // forward sweep: L y = b.  Level ch uses fc = coupling of ch and fi = inverse of ch;
+        // each is reloaded for ch+1 as soon as its phase of level ch is done.
+        load_inv(Linv, 0);
+        for (int ch = 0; ch < nch; ch++) {
+            const int gi = ch * CH + i;
+            double2 v = yv[gi];
+            if (ch > 0) v = coupling(v, ch * CH - band);
+            acc[i] = v;
+            __syncthreads();
+            if (ch + 1 < nch) load_cpl(Loff, ch + 1);
+            const double2 yi = inverse();
+            if (ch + 1 < nch) load_inv(Linv, ch + 1);
+            yv[gi] = yi;
+            __syncthreads();
+        }
+        // backward sweep (in place): U x = y; chunk ch couples to chunk ch+1
+        load_inv(Uinv, nch - 1);
+        for (int ch = nch - 1; ch >= 0; ch--) {
+            const int gi = ch * CH + i;
+            double2 v = yv[gi];
+            if (ch < nch - 1) v = coupling(v, (ch + 1) * CH);
+            acc[i] = v;
+            __syncthreads();
+            if (ch >= 1) load_cpl(Uoff, ch - 1);
+            const double2 xi = inverse();
+            if (ch >= 1) load_inv(Uinv, ch - 1);
+            yv[gi] = xi;
+            __syncthreads();
+        }
+    } else {
+        // wide chunks: factors streamed from memory 16 at a time (no register prefetch)
+        constexpr int B = 16;
+        for (int ch = 0; ch < nch; ch++) {
+            const int gi = ch * CH + i;
+            double2 v = yv[gi];
+            if (ch > 0) {
+                const double2 *Lo = Loff + ((size_t)q * nch + ch) * cs + i;
+                for (int t0 = 0; t0 < band; t0 += B) {
+                    double2 f[B];
+#pragma unroll
+                    for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Lo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
+#pragma unroll
+                    for (int u = 0; u < B; u++)
+                        if (t0 + u < band) v = csub(v, cmul(f[u], yv[ch * CH - band + t0 + u]));
+                }
+            }
+            acc[i] = v;
+            __syncthreads();
+            const double2 *Li = Linv + ((size_t)q * nch + ch) * cs + i;
+            double2 yi = make_double2(0, 0);
+            for (int j0 = 0; j0 < CH; j0 += B) {
                 double2 f[B];
 #pragma unroll
-                for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Uo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
+                for (int u = 0; u < B; u++) f[u] = Li[(size_t)(j0 + u) * CH];
 #pragma unroll
-                for (int u = 0; u < B; u++)
-                    if (t0 + u < band) v = csub(v, cmul(f[u], yv[(ch + 1) * CH + t0 + u]));
+                for (int u = 0; u < B; u++) yi = cfma(f[u], acc[j0 + u], yi);
             }
+            yv[gi] = yi;
+            __syncthreads();
         }
-        acc[i] = v;
-        __syncthreads();
-        const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs + i;
-        double2 xi = make_double2(0, 0);
+        for (int ch = nch - 1; ch >= 0; ch--) {
+            const int gi = ch * CH + i;
+            double2 v = yv[gi];
+            if (ch < nch - 1) {
+                const double2 *Uo = Uoff + ((size_t)q * nch + ch) * cs + i;
+                for (int t0 = 0; t0 < band; t0 += B) {
+                    double2 f[B];
 #pragma unroll
-        for (int j0 = 0; j0 < CH; j0 += B) {
-            double2 f[B];
+                    for (int u = 0; u < B; u++) f[u] = (t0 + u < band) ? Uo[(size_t)(t0 + u) * CH] : make_double2(0, 0);
 #pragma unroll
-            for (int u = 0; u < B; u++) f[u] = Ui[(size_t)(j0 + u) * CH];
+                    for (int u = 0; u < B; u++)
+                        if (t0 + u < band) v = csub(v, cmul(f[u], yv[(ch + 1) * CH + t0 + u]));
+                }
+            }
+            acc[i] = v;
+            __syncthreads();
+            const double2 *Ui = Uinv + ((size_t)q * nch + ch) * cs + i;
+            double2 xi = make_double2(0, 0);
+            for (int j0 = 0; j0 < CH; j0 += B) {
+                double2 f[B];
 #pragma unroll
-            for (int u = 0; u < B; u++) xi = cfma(f[u], acc[j0 + u], xi);
+                for (int u = 0; u < B; u++) f[u] = Ui[(size_t)(j0 + u) * CH];
+#pragma unroll
+                for (int u = 0; u < B; u++) xi = cfma(f[u], acc[j0 + u], xi);
+            }
+            yv[gi] = xi;
+            __syncthreads();
         }
-        yv[gi] = xi;
-        __syncthreads();
     }
     if (r < 0) return;
     for (int gi = i; gi < n; gi += CH) xo[(size_t)r * n + natural_index(gi, n, ilv)] = yv[gi];

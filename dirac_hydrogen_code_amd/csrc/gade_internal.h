@@ -58,7 +58,8 @@ namespace gade {
 // Pq: values of each distinct block in the CSR order of (rAp, rAi), [nq][nnz]
 int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
                   const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out);
-int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream);
+int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream,
+                        const int *gate = nullptr);
 void blocklu_free(gade_blocklu *lu);
 int hip_alloc(void **p, size_t bytes);
 }  // namespace gade
@@ -124,10 +125,24 @@ struct gade_model {
     double2 *d_fAx = nullptr;
     int *d_SAp = nullptr, *d_SAi = nullptr;
     double2 *d_SAx = nullptr;
+    // device-driven CN stepping: control words, per-step outputs, launch gate
+    int *d_ctl = nullptr;          // CT_* words
+    int *h_ctl = nullptr;          // pinned mirror
+    int *d_iters = nullptr;        // [steps_cap]
+    double *d_tols = nullptr;      // [steps_cap]
+    int steps_cap = 0;
+    const int *gate = nullptr;     // gate passed to every model launch (null: ungated)
+    bool ctl_on = false;           // reductions update d_ctl
+    bool device_loop = true;       // CN steps driven on the device (false: host loop per iteration)
+    int window = 8;                // steps enqueued per host synchronization
+    int iter_batch = 0;            // iterations enqueued per step (0: adaptive)
+    int last_k = 2;                // adaptive iterations per step
     // profiling: event pairs are recorded without synchronizing and resolved
     // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families
     int profile = 0;
     std::map<std::string, gade::Timer> timers;
     std::vector<gade::PendingEvent> pending;
     std::vector<hipEvent_t> event_pool;
+    int prof_period = 1;                       // time every prof_period-th launch of a family
+    std::map<std::string, long long> prof_seen;
 };

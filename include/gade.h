@@ -148,7 +148,14 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
 /* Tunable launch parameters (no reference counterpart; used by the autotuner):
  *   "tile_r"  row blocks per radial tile of the ELL kernel (1, 2, 4, 8);
  *   "radial"  radial-phase kernel: 0 banded when the pattern allows (default),
- *             1 general ELL kernel, 2 banded (DIA) kernel; reads back 1 or 2.
+ *             1 general ELL kernel, 2 banded (DIA) kernel; reads back 1 or 2;
+ *   "device_loop"  1 (default): gade_cn_run decides convergence on the device and
+ *             synchronizes once per window of steps; 0: one host round trip per
+ *             BiCGSTAB iteration (the reference's control flow);
+ *   "window"  steps enqueued per host synchronization (default 8);
+ *   "iter_batch"  BiCGSTAB iterations enqueued per step, 0 = adaptive (default);
+ *             a step needing more is finished by the host loop, same result;
+ *   "profile_period"  time every n-th launch of a profiled family (default 1).
  * Results do not depend on the choice. */
 int gade_set_param(gade_model_t *m, const char *name, int value);
 int gade_get_param(const gade_model_t *m, const char *name, int *value);

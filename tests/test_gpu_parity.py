@@ -208,3 +208,39 @@ def test_radial_kernels_agree_and_switch(datadir):
         assert relmax(a[0], b) <= 1e-14, cH
     om.timedep(f)
     assert relmax(res[("dia", 1.0)][0], om.apply(x, "S") + om.apply(x, "H")) <= 1e-13
+
+
+@pytest.mark.parametrize("window,batch", [(8, 0), (3, 1), (1, 0)])
+def test_device_loop_matches_host_loop(datadir, window, batch):
+    """The device-driven stepping (decisions on the device, one host round trip per window) gives the
+    host loop's trajectory bit for bit, including steps it has to hand back to the host (iter_batch 1)."""
+    cfg = "nondipole_small"
+    d, dt, om, gp = pair(datadir, cfg)
+    psi0, _ = gade.read_psi0(d, gp.n)
+    say(f"device loop window={window} batch={batch}")
+    out = {}
+    for dev in (0, 1):
+        gp.set_param("device_loop", dev)
+        gp.set_param("window", window)
+        gp.set_param("iter_batch", batch)
+        gp.reset_workspace()
+        dpsi = gade.DeviceArray.from_host(psi0)
+        it, tols, us = gp.cn_run(dpsi, 10.0, 50.0, 1.0, 12, t0=0.0, tol=1e-12, timing=True)
+        out[dev] = (dpsi.get(), it, tols)
+    assert np.array_equal(out[0][0].view(np.float64), out[1][0].view(np.float64))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+    assert out[1][1].max() >= 1  # the window exercised real iterations
+
+
+def test_device_loop_zero_rhs(datadir):
+    """psi = 0 gives rhs = 0: both loops return x = 0 with 0 iterations and tolerance 0."""
+    cfg = "tiny"
+    d, dt, om, gp = pair(datadir, cfg)
+    say("device loop zero rhs")
+    for dev in (0, 1):
+        gp.set_param("device_loop", dev)
+        dpsi = gade.DeviceArray.from_host(np.zeros(gp.n, complex))
+        it, tols, _ = gp.cn_run(dpsi, 10.0, 50.0, 1.0, 3, t0=0.0, tol=1e-12, timing=False)
+        assert np.all(it == 0) and np.all(tols == 0)
+        assert not np.any(dpsi.get())
