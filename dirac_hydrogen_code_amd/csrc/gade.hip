@@ -816,7 +816,7 @@ void free_all(gade_model *m) {
         if (p) hipFree(p);
     if (m->h_sc) hipHostFree(m->h_sc);
     if (m->h_ctl) hipHostFree(m->h_ctl);
-    for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols})
+    for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -1885,7 +1885,8 @@ namespace gade {
 
 void blocklu_free(gade_blocklu *lu) {
     if (!lu) return;
-    void *ptrs[] = {lu->d_AB, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->d_blk_q, lu->d_wblocks, lu->d_flag};
+    void *ptrs[] = {lu->d_AB, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->d_blk_q, lu->d_wblocks, lu->d_flag,
+                    lu->d_order};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete lu;
@@ -1964,7 +1965,11 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
             for (int j = 0; j < G; j++) wb.push_back(i + j < rows.size() ? rows[i + j] : -1);
     }
     lu->npw = (int)wb.size() / G;
-    if ((rc = upload(&lu->d_wblocks, wb.data(), wb.size())) || (rc = upload(&lu->d_blk_q, blk_q, (size_t)nblk)))
+    std::vector<int> order;
+    for (int v : wb)
+        if (v >= 0) order.push_back(v);
+    if ((rc = upload(&lu->d_wblocks, wb.data(), wb.size())) || (rc = upload(&lu->d_blk_q, blk_q, (size_t)nblk)) ||
+        (rc = upload(&lu->d_order, order.data(), order.size())))
         return fail(rc);
     *out = lu;
     return GADE_OK;
@@ -1974,6 +1979,14 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
     if (!lu) {
         gade::set_error("preconditioner not set up");
         return GADE_EINVAL;
+    }
+    if (lu->ch == 16) {
+        // one block per wave, four lanes per row
+        const size_t shq = sizeof(double2) * ((size_t)lu->nch * 16 + 16);
+        hipLaunchKernelGGL(k_psolve_q<4>, dim3(lu->nblk), dim3(64), shq, stream, b, x, lu->d_order, lu->d_blk_q, lu->ilv,
+                           lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch, gate);
+        GADE_HIP(hipGetLastError());
+        return GADE_OK;
     }
     const int G = 64 / lu->ch;
     size_t shm = sizeof(double2) * (size_t)G * ((size_t)lu->nch * lu->ch + lu->ch);

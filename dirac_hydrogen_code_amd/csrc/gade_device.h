@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
 }
 
 // y = alpha A x + beta y for a CSR matrix: one wave per row, streaming loads.
-static __global__ __launch_bounds__(256) void k_csr_spmv(const int *__restrict__ Ap, const int *__restrict__ Ai,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_csr_spmv(const int *__restrict__ Ap, const int *__restrict__ Ai,
                                                   const double2 *__restrict__ Ax, const double2 *__restrict__ x,
                                                   long long nrows, double2 alpha, double2 beta,
                                                   double2 *__restrict__ y) {
@@ -383,6 +383,124 @@ __global__ __launch_bounds__(CH) void k_chunk_inverse(const double2 *AB, int n, 
 // radial index of interleaved position gi (the two halves of the radial basis alternate)
 static __device__ __forceinline__ int natural_index(int gi, int n, int ilv) {
     return ilv ? ((gi & 1) ? (n >> 1) + (gi >> 1) : (gi >> 1)) : gi;
+}
+
+// sum of a complex value over the four lanes of a quad (DPP quad permutes), in every lane
+__device__ __forceinline__ double dpp_quad_swap(double v, int ctrl_is_xor2) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    int l2, h2;
+    if (ctrl_is_xor2) {
+        l2 = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        h2 = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xF, 0xF, false);
+    } else {
+        l2 = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        h2 = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xF, 0xF, false);
+    }
+    return __hiloint2double(h2, l2);
+}
+__device__ __forceinline__ double2 quad_sum(double2 v) {
+    v.x += dpp_quad_swap(v.x, 0);
+    v.y += dpp_quad_swap(v.y, 0);
+    v.x += dpp_quad_swap(v.x, 1);
+    v.y += dpp_quad_swap(v.y, 1);
+    return v;
+}
+
+// x = P^{-1} b with 16-row levels and one block per wave: lane = 4*row + k,
+// and each of the four lanes of a row takes a quarter of the band coupling
+// and of the level inverse (terms 4k..4k+3), summed over the quad with DPP.
+// b is staged in LDS; the factors of level ch+PF are loaded as soon as the
+// register slot of level ch is free, so PF levels of loads are in flight.
+// Requires CH = 16 and band <= 16 (coupling rows padded to 16, zero beyond band).
+template <int PF>
+__global__ __launch_bounds__(64) void k_psolve_q(const double2 *__restrict__ b, double2 *__restrict__ xo,
+                                                 const int *__restrict__ order, const int *__restrict__ blk_q,
+                                                 int ilv, const double2 *__restrict__ Linv,
+                                                 const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
+                                                 const double2 *__restrict__ Uoff, int n, int band, int nch,
+                                                 const int *__restrict__ gate = nullptr) {
+    GADE_GATED(gate);
+    constexpr int CH = 16;
+    extern __shared__ double2 smem[];
+    const int lane = threadIdx.x, i = lane >> 2, k = lane & 3;
+    const int r = order[blockIdx.x];
+    const int q = blk_q[r];
+    double2 *yv = smem;              // [nch*CH]: b, then y, then x (interleaved order)
+    double2 *acc = smem + nch * CH;  // [CH]
+    const double2 *bb = b + (size_t)r * n;
+    const int ntot = nch * CH;
+    for (int c0 = 0; c0 < ntot; c0 += 64 * 8) {
+        double2 t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = c0 + 64 * u + lane;
+            t[u] = (c < n) ? bb[natural_index(c, n, ilv)] : make_double2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = c0 + 64 * u + lane;
+            if (c < ntot) yv[c] = t[u];
+        }
+    }
+    __syncthreads();
+    const size_t cs = (size_t)CH * CH;
+    double2 fc[PF][4], fv[PF][4];
+    // factors of level ch into slot s: coupling rows 4k..4k+3 and inverse columns 4k..4k+3 of row i
+    auto load = [&](const double2 *Off, const double2 *Inv, int ch, double2 (&c)[4], double2 (&v)[4]) {
+        const double2 *po = Off + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
+        const double2 *pi = Inv + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            c[u] = po[(size_t)u * CH];
+            v[u] = pi[(size_t)u * CH];
+        }
+    };
+    // one level: val = b_i - sum_t C[t][i] y[cbase + t];  y_i = sum_j Inv[j][i] val_j
+    auto level = [&](int ch, int cbase, bool couple, const double2 (&c)[4], const double2 (&v)[4]) {
+        const int gi = ch * CH + i;
+        double2 s = make_double2(0, 0);
+        if (couple) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) s = cfma(c[u], yv[cbase + 4 * k + u], s);
+            s = quad_sum(s);
+        }
+        const double2 val = csub(yv[gi], s);
+        if (k == 0) acc[i] = val;
+        __syncthreads();
+        double2 y = make_double2(0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) y = cfma(v[u], acc[4 * k + u], y);
+        y = quad_sum(y);
+        if (k == 0) yv[gi] = y;
+        __syncthreads();
+    };
+    // forward sweep: L y = b
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (u < nch) load(Loff, Linv, u, fc[u], fv[u]);
+    for (int ch0 = 0; ch0 < nch; ch0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int ch = ch0 + u;
+            if (ch >= nch) break;
+            level(ch, ch * CH - band, ch > 0, fc[u], fv[u]);
+            if (ch + PF < nch) load(Loff, Linv, ch + PF, fc[u], fv[u]);
+        }
+    }
+    // backward sweep: U x = y, level ch couples to level ch+1
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (nch - 1 - u >= 0) load(Uoff, Uinv, nch - 1 - u, fc[u], fv[u]);
+    for (int ch0 = nch - 1; ch0 >= 0; ch0 -= PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int ch = ch0 - u;
+            if (ch < 0) break;
+            level(ch, (ch + 1) * CH, ch < nch - 1, fc[u], fv[u]);
+            if (ch - PF >= 0) load(Uoff, Uinv, ch - PF, fc[u], fv[u]);
+        }
+    }
+    for (int c = lane; c < n; c += 64) xo[(size_t)r * n + natural_index(c, n, ilv)] = yv[c];
 }
 
 // x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =

@@ -52,6 +52,7 @@ struct gade_blocklu {
     int n = 0, nblk = 0, nq = 0, band = 0, ch = 16, nch = 0, npw = 0, ilv = 0;
     double2 *d_AB = nullptr, *d_Linv = nullptr, *d_Uinv = nullptr, *d_Loff = nullptr, *d_Uoff = nullptr;
     int *d_blk_q = nullptr, *d_wblocks = nullptr, *d_flag = nullptr;
+    int *d_order = nullptr;  // blocks sorted by distinct factor (one block per wave)
 };
 
 namespace gade {
@@ -125,6 +126,10 @@ struct gade_model {
     double2 *d_fAx = nullptr;
     int *d_SAp = nullptr, *d_SAi = nullptr;
     double2 *d_SAx = nullptr;
+    // eigen-projection: per distinct kappa, S-orthonormal eigenvectors (conjugate-transposed,
+    // [q][i][j] = conj(v_j[i])) and energies [q][j] of H0(k) v = E S(k) v
+    double2 *d_evec = nullptr;
+    double *d_eval = nullptr;
     // device-driven CN stepping: control words, per-step outputs, launch gate
     int *d_ctl = nullptr;          // CT_* words
     int *h_ctl = nullptr;          // pinned mirror

@@ -23,7 +23,7 @@ EXPORTS = (
     "gade_set_time", "gade_apply", "gade_precond_setup", "gade_psolve", "gade_bicgstab", "gade_reset_workspace",
     "gade_cn_step", "gade_cn_run", "gade_pulse", "gade_ik", "gade_imu", "gade_snorm", "gade_assembled_setup",
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
-    "gade_set_param", "gade_get_param",
+    "gade_set_param", "gade_get_param", "gade_eig_setup", "gade_eig_values", "gade_populations",
 )
 
 _lib = None
@@ -44,7 +44,8 @@ def lib():
         L.gade_last_error.restype = C.c_char_p
         sig = {
             "gade_malloc": [C.POINTER(vp), sz], "gade_free": [vp], "gade_memcpy_h2d": [vp, vp, sz],
-            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_device_sync": [],
+            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_eig_values": [vp, vp, vp, vp],
+            "gade_populations": [vp, vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), vp], "gade_device_sync": [],
             "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
             "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                 C.POINTER(C.c_longlong), C.POINTER(i32)],
@@ -199,6 +200,26 @@ class Propagator:
         pop = np.zeros(self.nblk) if populations else None
         check(lib().gade_snorm(self.ptr, x.ptr, C.byref(nrm), pop.ctypes.data if populations else None))
         return (nrm.value, pop) if populations else nrm.value
+
+    def populations(self, x: DeviceArray, per_block=False):
+        """Eigen-projection populations: dict(norm, bound, ionised[, per_block [nblk][4]])."""
+        nrm, b, i = C.c_double(), C.c_double(), C.c_double()
+        pb = np.zeros((self.nblk, 4)) if per_block else None
+        check(lib().gade_populations(self.ptr, x.ptr, C.byref(nrm), C.byref(b), C.byref(i),
+                                     pb.ctypes.data if per_block else None))
+        out = dict(norm=nrm.value, bound=b.value, ionised=i.value)
+        if per_block:
+            out["per_block"] = pb
+        return out
+
+    def eig_values(self):
+        """(kappa[nkappa], energies[nkappa][blkdim]) of H0(k) v = E S(k) v."""
+        check(lib().gade_eig_setup(self.ptr))
+        kap = np.zeros(self.nkappa, np.int32)
+        ev = np.zeros((self.nkappa, self.blkdim))
+        nk = C.c_int()
+        check(lib().gade_eig_values(self.ptr, C.byref(nk), kap.ctypes.data, ev.ctypes.data))
+        return kap[:nk.value], ev[:nk.value]
 
     # assembled reference formulation
     def assembled_timedep(self):

@@ -130,6 +130,20 @@ double gade_imu(int i);   /* tools/solver_common.c:39-44 */
  * solver side: the reference computes these offline (src/hamiltonian.h:1268-1307). */
 int gade_snorm(gade_model_t *m, const double *x_dev, double *norm, double *pop_host);
 
+/* Populations by eigen-projection (src/hamiltonian.h:872-943, 1268-1307; driven
+ * by src/process_output.cpp:332-355).  For every distinct kappa the generalized
+ * eigenpairs H0(k) v = E S(k) v with v^H S v = 1 are computed once per model on
+ * the device (gade_eig_setup, done on first use).  gade_populations projects
+ * S(k_r) psi_r of every block r on them: population_j = |v_j^H S psi_r|^2,
+ * summed into bound (-c^2 <= E < 0) and ionised (E >= 0); norm = psi^H S psi.
+ * per_block (optional) receives [nblk][4]: negative continuum (E < -c^2),
+ * bound, ionised, psi_r^H S psi_r. */
+int gade_eig_setup(gade_model_t *m);
+/* distinct kappa values [nkappa] and energies [nkappa][blkdim] (ascending per kappa) */
+int gade_eig_values(gade_model_t *m, int *nkappa, int *kappa, double *energies);
+int gade_populations(gade_model_t *m, const double *x_dev, double *norm, double *bound, double *ionised,
+                     double *per_block);
+
 /* Assembled reference formulation (kept as the plain-FP64 HBM comparator):
  * builds Hfull Ap/Ai/Ai_sub_map on the device (tools/csr.c:653-743), assembles
  * Hfull.Ax each step (tools/gpu_solver.cu:84-135) and multiplies it with a

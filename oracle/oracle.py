@@ -199,3 +199,58 @@ def read_psi0_vec(path, n):
     if lib().or_vec_read(path.encode(), n, out, perm):
         raise RuntimeError(f"cannot read {path}")
     return out, perm
+
+
+def _read_csr(path):
+    """The .csr layout of tools/csr.c:262-318: int32 dim, nnz, Ap[dim+1], Ai[nnz], complex128 Ax[nnz],
+    then optionally int32 perm[dim], npart, part[npart+1]."""
+    raw = open(path, "rb").read()
+    n, nnz = np.frombuffer(raw, np.int32, 2)
+    off = 8
+    Ap = np.frombuffer(raw, np.int32, n + 1, off)
+    off += 4 * (n + 1)
+    Ai = np.frombuffer(raw, np.int32, nnz, off)
+    off += 4 * nnz
+    Ax = np.frombuffer(raw, np.complex128, nnz, off)
+    off += 16 * nnz
+    perm = np.frombuffer(raw, np.int32, n, off) if len(raw) - off >= 4 * n else None
+    return int(n), Ap, Ai, Ax, perm
+
+
+def _dense(path):
+    n, Ap, Ai, Ax, _ = _read_csr(path)
+    M = np.zeros((n, n), np.complex128)
+    rows = np.repeat(np.arange(n), np.diff(Ap))
+    M[rows, Ai] = Ax
+    return M
+
+
+def populations(directory, psi, csq=137.035999084 ** 2):
+    """Eigen-projection populations (src/hamiltonian.h:872-943, 1268-1307; src/process_output.cpp:332-355).
+
+    For each block r of psi (Hall order, kappa from the H.csr permutation and ik of tools/solver_common.c:30-37):
+    S = s0 + k s1 + k^2 s2, H0 = h00 + k h01 + k^2 h02 + k^3 h03 (tools/solver.c:409-458), generalized
+    eigenpairs H0 v = E S v with v^H S v = 1 (LAPACK through scipy), population_j = |v_j^H S psi_r|^2.
+    Returns dict(norm, bound, ionised, negative, per_block [nblk][4], energies {kappa: E})."""
+    import scipy.linalg as sl
+    s = [_dense(os.path.join(directory, f"s{i}.csr")) for i in range(3)]
+    h = [_dense(os.path.join(directory, f"h0{i}.csr")) for i in range(4)]
+    nblk, _, _, _, perm = _read_csr(os.path.join(directory, "H.csr"))
+    perm = np.arange(nblk) if perm is None else perm
+    n = s[0].shape[0]
+    psi = np.asarray(psi).reshape(nblk, n)
+    cache = {}
+    out = np.zeros((nblk, 4))
+    for r in range(nblk):
+        k = lib().or_ik(int(perm[r]))
+        if k not in cache:
+            S = s[0] + k * s[1] + k * k * s[2]
+            H = h[0] + k * h[1] + k * k * h[2] + k ** 3 * h[3]
+            E, V = sl.eigh(H, S)
+            cache[k] = (E, V, S)
+        E, V, S = cache[k]
+        w = S @ psi[r]
+        p = np.abs(V.conj().T @ w) ** 2
+        out[r] = [p[E < -csq].sum(), p[(E >= -csq) & (E < 0)].sum(), p[E >= 0].sum(), np.vdot(psi[r], w).real]
+    return dict(norm=out[:, 3].sum(), bound=out[:, 1].sum(), ionised=out[:, 2].sum(), negative=out[:, 0].sum(),
+                per_block=out, energies={k: v[0] for k, v in cache.items()})

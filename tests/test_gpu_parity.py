@@ -244,3 +244,26 @@ def test_device_loop_zero_rhs(datadir):
         it, tols, _ = gp.cn_run(dpsi, 10.0, 50.0, 1.0, 3, t0=0.0, tol=1e-12, timing=False)
         assert np.all(it == 0) and np.all(tols == 0)
         assert not np.any(dpsi.get())
+
+
+@pytest.mark.parametrize("cfg", ["tiny", "nondipole_small"])
+def test_populations_match_oracle(datadir, cfg):
+    """A11: device eigenpairs (per distinct kappa) and eigen-projection populations against the oracle's
+    scipy eigh + dense projection: per-block classes to 1e-10 of the block norm, energies to 1e-10."""
+    d, dt, om, gp = pair(datadir, cfg)
+    say(f"populations {cfg}")
+    rng = np.random.default_rng(21)
+    psi0, _ = gade.read_psi0(d, gp.n)
+    x = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
+    kap, ev = gp.eig_values()
+    for v in (psi0, x):
+        ref = O.populations(d, v)
+        got = gp.populations(gade.DeviceArray.from_host(v), per_block=True)
+        scale = np.maximum(ref["per_block"][:, 3], 1e-300)[:, None]
+        assert np.abs(got["per_block"] - ref["per_block"]).max(axis=None) <= 1e-10 * max(ref["norm"], 1.0)
+        assert np.all(np.abs(got["per_block"] - ref["per_block"]) <= 1e-9 * scale + 1e-14)
+        for key in ("norm", "bound", "ionised"):
+            assert abs(got[key] - ref[key]) <= 1e-10 * ref["norm"], key
+    for k, E in zip(kap, ev):
+        Eref = ref["energies"][int(k)]
+        assert np.abs(E - Eref).max() <= 1e-10 * np.abs(Eref).max()

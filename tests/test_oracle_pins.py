@@ -11,6 +11,7 @@ eigenstate phase evolution).
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -317,3 +318,23 @@ def test_oracle_rejects_corrupt_file(tmp_path, datadir):
     (d / "g0a0l0.csr").write_bytes(raw[: len(raw) // 2])
     with pytest.raises(RuntimeError):
         O.Model(str(d), 2, 1e-3)
+
+
+def test_populations_ground_state_and_completeness(datadir):
+    """A11 pins: the 1s state is one S-normalised eigenvector of (H0(-1), S(-1)), so its bound population is its
+    norm (1) and nothing is ionised; its energy is the generator's ground-state energy; for any vector the three
+    classes sum to psi^H S psi (completeness of the S-orthonormal eigenbasis)."""
+    import json
+    d = datadir("dipole")
+    nblk = O._read_csr(os.path.join(d, "H.csr"))[0]
+    nr = O._read_csr(os.path.join(d, "s0.csr"))[0]
+    psi, _ = O.read_psi0_vec(os.path.join(d, "psi0.vec"), nblk * nr)
+    p = O.populations(d, psi)
+    assert abs(p["norm"] - 1) < 1e-10 and abs(p["bound"] - 1) < 1e-10 and p["ionised"] < 1e-14
+    e0 = json.load(open(os.path.join(d, "params.json")))["e0"]
+    assert abs(min(e for e in p["energies"][-1] if e > -1e4) - e0) < 1e-9
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(nblk * nr) + 1j * rng.standard_normal(nblk * nr)
+    q = O.populations(d, x)
+    assert abs(q["bound"] + q["ionised"] + q["negative"] - q["norm"]) <= 1e-11 * q["norm"]
+    assert q["ionised"] > 0 and q["negative"] > 0
