@@ -816,7 +816,8 @@ void free_all(gade_model *m) {
         if (p) hipFree(p);
     if (m->h_sc) hipHostFree(m->h_sc);
     if (m->h_ctl) hipHostFree(m->h_ctl);
-    for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval})
+    for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval,
+                    (void *)m->d_hist})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -1407,7 +1408,8 @@ int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, dou
 // ---------------------------------------------------------------------------
 __global__ void k_ctl_set(int *ctl, double *sc, double tol, int maxit) {
     if (threadIdx.x != 0) return;
-    for (int k = 0; k < CT_COUNT; k++) ctl[k] = 0;
+    for (int k = 0; k < CT_COUNT; k++)
+        if (k != CT_HIST) ctl[k] = 0;
     ctl[CT_MAXIT] = maxit;
     sc[SC_TOL] = tol;
 }
@@ -1434,13 +1436,25 @@ int ensure_step_buffers(gade_model *m, int nsteps) {
     return GADE_OK;
 }
 
+// x0 by extrapolation of the history (order m->guess_order); no-op for order 0
+int launch_guess(gade_model *m, double2 *psi) {
+    if (m->guess_order <= 0) return GADE_OK;
+    if (!m->d_hist) GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+    hipLaunchKernelGGL(k_extrap, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->N, psi, m->d_hist,
+                       m->guess_order, m->d_ctl, m->gate);
+    hipLaunchKernelGGL(k_hist_count, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->gate);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
 int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K) {
     GateScope gs(m);
     m->ft.assign(ft, ft + 12);
     m->gate = m->d_ctl + CT_STALLED;
     GADE_TRY(launch_gsums(m, m->radial_kind != 2));
     GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
-    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0, x0 = psi
+    GADE_TRY(launch_guess(m, psi));                    // x0 (psi_n unless extrapolating)
+    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0
     GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
     m->gate = m->d_ctl + CT_SKIP;
     for (int k = 0; k < K; k++) {
@@ -1560,6 +1574,18 @@ int gade_reset_workspace(gade_model_t *m) {
     if (!m) return GADE_EINVAL;
     double2 *ws[] = {m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t};
     for (auto w : ws) GADE_HIP(hipMemsetAsync(w, 0, sizeof(double2) * m->N, m->stream));
+    GADE_HIP(hipMemsetAsync(m->d_ctl + CT_HIST, 0, sizeof(int), m->stream));  // the guess history restarts
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_extrapolate(gade_model_t *m, double *psi, int order) {
+    if (!m || !psi || order < 0 || order > kMaxGuessOrder) return GADE_EINVAL;
+    const int saved = m->guess_order;
+    m->guess_order = order;
+    int rc = order > 0 ? launch_guess(m, (double2 *)psi) : GADE_OK;
+    m->guess_order = saved;
+    GADE_TRY(rc);
     GADE_HIP(hipStreamSynchronize(m->stream));
     return GADE_OK;
 }
@@ -1569,6 +1595,7 @@ int gade_cn_step(gade_model_t *m, double *psi, const double *ft, double tol, int
     GADE_TRY(gade_set_time(m, ft));
     // rhs = (S - Hfull) psi  (tools/dirac_solver.c:531-536)
     GADE_TRY(apply_async(m, (const double2 *)psi, m->tmp, -1.0, 0.0));
+    GADE_TRY(launch_guess(m, (double2 *)psi));
     int it = maxit;
     double te = tol;
     GADE_TRY(bicgstab_impl(m, m->tmp, (double2 *)psi, &it, &te, nullptr));
@@ -1794,6 +1821,14 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->device_loop = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "guess_order")) {
+        if (value < 0 || value > kMaxGuessOrder) {
+            gade::set_error("guess_order must be 0..3");
+            return GADE_EINVAL;
+        }
+        m->guess_order = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "window") || !strcmp(name, "iter_batch")) {
         if (value < (name[0] == 'w' ? 1 : 0) || value > 4096) {
             gade::set_error(std::string(name) + " out of range");
@@ -1839,6 +1874,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "device_loop")) {
         *value = m->device_loop;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "guess_order")) {
+        *value = m->guess_order;
         return GADE_OK;
     }
     if (!strcmp(name, "window")) {

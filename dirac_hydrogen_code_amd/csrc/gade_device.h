@@ -59,8 +59,52 @@ enum {
     CT_SKIP = 3,     // CT_STATUS != 0 or CT_STALLED: the iteration gate
     CT_MAXIT = 5,
     CT_STALL_STEP = 6,
+    CT_HIST = 7,     // previous solutions held in the history ring
     CT_COUNT = 8
 };
+
+constexpr int kMaxGuessOrder = 3;  // polynomial extrapolation order of the initial guess
+constexpr int kHistSlots = kMaxGuessOrder;
+
+// Initial guess by polynomial extrapolation of the step history:
+// x0 = sum_{k=0..p} c_k psi_{n-k}, c_k = (-1)^k binom(p+1, k+1) (Lagrange at
+// t_{n+1} through equally spaced t_n..t_{n-p}), with p = min(order, history).
+// psi holds psi_n on entry and x0 on exit; psi_n enters the history ring.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_extrap(long long N, double2 *__restrict__ psi, double2 *__restrict__ hist,
+                                                int order, int *ctl, const int *__restrict__ gate) {
+    GADE_GATED(gate);
+    const int have = ctl[CT_HIST];
+    const int p = order < have ? order : have;
+    // ring slot of psi_{n-k} before the push: (have - k) mod kHistSlots; psi_n goes to slot have mod kHistSlots
+    const int slot0 = have % kHistSlots;
+    double c[kMaxGuessOrder + 1];
+    {
+        // binom(p+1, k+1) with alternating signs
+        double bnm = p + 1;
+        for (int k = 0; k <= kMaxGuessOrder; k++) {
+            c[k] = (k <= p) ? ((k % 2) ? -bnm : bnm) : 0.0;
+            bnm = bnm * (p - k) / (k + 2);
+        }
+    }
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+        const double2 xn = psi[i];
+        double2 x = cscale(xn, c[0]);
+#pragma unroll
+        for (int k = 1; k <= kMaxGuessOrder; k++) {
+            if (k > p) break;
+            const int sl = ((slot0 - k) % kHistSlots + kHistSlots) % kHistSlots;
+            x = cadd(x, cscale(hist[(size_t)sl * N + i], c[k]));
+        }
+        hist[(size_t)slot0 * N + i] = xn;
+        psi[i] = x;
+    }
+}
+
+// history count update after k_extrap (one thread, ordered after it on the stream)
+[[maybe_unused]] static __global__ void k_hist_count(int *ctl, const int *__restrict__ gate) {
+    GADE_GATED(gate);
+    if (threadIdx.x == 0) ctl[CT_HIST] = ctl[CT_HIST] + 1;
+}
 
 // ---------------------------------------------------------------------------
 // BiCGSTAB vector kernels with deterministic last-block reductions

@@ -142,6 +142,8 @@ struct gade_model {
     int window = 8;                // steps enqueued per host synchronization
     int iter_batch = 0;            // iterations enqueued per step (0: adaptive)
     int last_k = 2;                // adaptive iterations per step
+    int guess_order = 0;           // initial guess: 0 psi_n (the reference), p: extrapolation of order p
+    double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions
     // profiling: event pairs are recorded without synchronizing and resolved
     // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families
     int profile = 0;

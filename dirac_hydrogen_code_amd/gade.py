@@ -24,6 +24,7 @@ EXPORTS = (
     "gade_cn_step", "gade_cn_run", "gade_pulse", "gade_ik", "gade_imu", "gade_snorm", "gade_assembled_setup",
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
     "gade_set_param", "gade_get_param", "gade_eig_setup", "gade_eig_values", "gade_populations",
+    "gade_extrapolate",
 )
 
 _lib = None
@@ -44,7 +45,7 @@ def lib():
         L.gade_last_error.restype = C.c_char_p
         sig = {
             "gade_malloc": [C.POINTER(vp), sz], "gade_free": [vp], "gade_memcpy_h2d": [vp, vp, sz],
-            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_eig_values": [vp, vp, vp, vp],
+            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_extrapolate": [vp, vp, i32], "gade_eig_values": [vp, vp, vp, vp],
             "gade_populations": [vp, vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), vp], "gade_device_sync": [],
             "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
             "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -200,6 +201,9 @@ class Propagator:
         pop = np.zeros(self.nblk) if populations else None
         check(lib().gade_snorm(self.ptr, x.ptr, C.byref(nrm), pop.ctypes.data if populations else None))
         return (nrm.value, pop) if populations else nrm.value
+
+    def extrapolate(self, psi: DeviceArray, order):
+        check(lib().gade_extrapolate(self.ptr, psi.ptr, order))
 
     def populations(self, x: DeviceArray, per_block=False):
         """Eigen-projection populations: dict(norm, bound, ionised[, per_block [nblk][4]])."""

@@ -130,6 +130,15 @@ double gade_imu(int i);   /* tools/solver_common.c:39-44 */
  * solver side: the reference computes these offline (src/hamiltonian.h:1268-1307). */
 int gade_snorm(gade_model_t *m, const double *x_dev, double *norm, double *pop_host);
 
+/* Initial guess by polynomial extrapolation (no reference counterpart; the
+ * reference starts each solve from psi_n, tools/dirac_solver.c:548): pushes
+ * psi_n (in psi_dev) into the model's history and overwrites psi_dev with
+ * sum_{k=0..p} (-1)^k binom(p+1,k+1) psi_{n-k}, p = min(order, history length),
+ * order <= 3.  With set_param("guess_order", p) every CN step does this before
+ * r = b - A x0; the converged solution at the same tolerance is unchanged in
+ * meaning, only the iteration count moves.  gade_reset_workspace clears the history. */
+int gade_extrapolate(gade_model_t *m, double *psi_dev, int order);
+
 /* Populations by eigen-projection (src/hamiltonian.h:872-943, 1268-1307; driven
  * by src/process_output.cpp:332-355).  For every distinct kappa the generalized
  * eigenpairs H0(k) v = E S(k) v with v^H S v = 1 are computed once per model on
@@ -169,7 +178,9 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
  *   "window"  steps enqueued per host synchronization (default 8);
  *   "iter_batch"  BiCGSTAB iterations enqueued per step, 0 = adaptive (default);
  *             a step needing more is finished by the host loop, same result;
- *   "profile_period"  time every n-th launch of a profiled family (default 1).
+ *   "profile_period"  time every n-th launch of a profiled family (default 1);
+ *   "guess_order"  initial guess of each CN step: 0 psi_n (default, the
+ *             reference), 1..3 polynomial extrapolation (gade_extrapolate).
  * Results do not depend on the choice. */
 int gade_set_param(gade_model_t *m, const char *name, int value);
 int gade_get_param(const gade_model_t *m, const char *name, int *value);

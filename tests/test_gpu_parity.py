@@ -267,3 +267,48 @@ def test_populations_match_oracle(datadir, cfg):
     for k, E in zip(kap, ev):
         Eref = ref["energies"][int(k)]
         assert np.abs(E - Eref).max() <= 1e-10 * np.abs(Eref).max()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_extrapolation_polynomial_exact(datadir, order):
+    """The initial-guess extrapolation of order p reproduces any vector polynomial of degree <= p in t at the next
+    step (equally spaced history), to rounding."""
+    cfg = "tiny"
+    d, dt, om, gp = pair(datadir, cfg)
+    say(f"extrapolation order {order}")
+    rng = np.random.default_rng(order)
+    coef = [rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n) for _ in range(order + 1)]
+    traj = lambda t: sum(c * t ** k for k, c in enumerate(coef))
+    gp.reset_workspace()
+    d_x = gade.DeviceArray(gp.n)
+    h = 0.37
+    for k in range(order + 1):
+        d_x.upload(traj(k * h))
+        gp.extrapolate(d_x, order)
+    want = traj((order + 1) * h)
+    assert relmax(d_x.get(), want) <= 1e-12
+
+
+def test_extrapolated_guess_same_solution_fewer_iterations(datadir):
+    """guess_order 2: the device and host loops agree bit for bit, the trajectory equals the plain one to the
+    solver tolerance, and the total iteration count does not grow."""
+    cfg = "nondipole_small"
+    d, dt, om, gp = pair(datadir, cfg)
+    psi0, _ = gade.read_psi0(d, gp.n)
+    say("extrapolated guess")
+    out = {}
+    for go, dev in ((0, 1), (2, 0), (2, 1)):
+        gp.set_param("guess_order", go)
+        gp.set_param("device_loop", dev)
+        gp.reset_workspace()
+        dpsi = gade.DeviceArray.from_host(psi0)
+        it, tols, _ = gp.cn_run(dpsi, 10.0, 50.0, 2.0, 40, t0=0.0, tol=1e-12, timing=False)
+        out[(go, dev)] = (dpsi.get(), it, tols)
+        assert tols.max() < 1e-12
+    gp.set_param("guess_order", 0)
+    a, b = out[(2, 0)], out[(2, 1)]
+    assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64)) and np.array_equal(a[1], b[1])
+    p0, p2 = out[(0, 1)][0], out[(2, 1)][0]
+    assert np.linalg.norm(p2 - p0) / np.linalg.norm(p0) <= 1e-10
+    say(f"  iterations plain {out[(0, 1)][1].sum()} extrapolated {out[(2, 1)][1].sum()}")
+    assert out[(2, 1)][1].sum() <= out[(0, 1)][1].sum()
