@@ -65,8 +65,10 @@ void resolve_pending(gade_model *m) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, p.a, p.b);
         auto &t = m->timers[p.name];
-        t.total_ms += ms;
-        t.count += 1;
+        if ((double)ms * 1e6 >= (double)m->prof_floor_ns) {  // launches ended by a device gate are dropped
+            t.total_ms += ms;
+            t.count += 1;
+        }
         m->event_pool.push_back(p.a);
         m->event_pool.push_back(p.b);
     }
@@ -1837,6 +1839,14 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         (name[0] == 'w' ? m->window : m->iter_batch) = value;
         return GADE_OK;
     }
+    if (!strcmp(name, "profile_floor_ns")) {
+        if (value < 0) {
+            gade::set_error("profile_floor_ns must be >= 0");
+            return GADE_EINVAL;
+        }
+        m->prof_floor_ns = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "profile_period")) {
         if (value < 1) {
             gade::set_error("profile_period must be >= 1");
@@ -2004,9 +2014,29 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
             for (int j = 0; j < G; j++) wb.push_back(i + j < rows.size() ? rows[i + j] : -1);
     }
     lu->npw = (int)wb.size() / G;
-    std::vector<int> order;
-    for (int v : wb)
-        if (v >= 0) order.push_back(v);
+    // one block per wave: the blocks of one distinct factor go to one XCD (workgroups are dealt to the 8
+    // XCDs round-robin), so each XCD's L2 holds the factors of its own kappa groups only
+    constexpr int kXcd = 8;
+    std::vector<std::vector<int>> bins(kXcd);
+    {
+        std::vector<std::vector<int>> groups(nq);
+        for (int r = 0; r < nblk; r++) groups[blk_q[r]].push_back(r);
+        std::vector<int> gi(nq);
+        for (int q = 0; q < nq; q++) gi[q] = q;
+        std::stable_sort(gi.begin(), gi.end(), [&](int a, int b) { return groups[a].size() > groups[b].size(); });
+        for (int q : gi) {
+            int best = 0;
+            for (int x = 1; x < kXcd; x++)
+                if (bins[x].size() < bins[best].size()) best = x;
+            bins[best].insert(bins[best].end(), groups[q].begin(), groups[q].end());
+        }
+    }
+    size_t maxlen = 0;
+    for (auto &b : bins) maxlen = std::max(maxlen, b.size());
+    std::vector<int> order(maxlen * kXcd, -1);
+    for (int x = 0; x < kXcd; x++)
+        for (size_t k = 0; k < bins[x].size(); k++) order[k * kXcd + x] = bins[x][k];
+    lu->norder = (int)order.size();
     if ((rc = upload(&lu->d_wblocks, wb.data(), wb.size())) || (rc = upload(&lu->d_blk_q, blk_q, (size_t)nblk)) ||
         (rc = upload(&lu->d_order, order.data(), order.size())))
         return fail(rc);
@@ -2022,7 +2052,7 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
     if (lu->ch == 16) {
         // one block per wave, four lanes per row
         const size_t shq = sizeof(double2) * ((size_t)lu->nch * 16 + 16);
-        hipLaunchKernelGGL(k_psolve_q<4>, dim3(lu->nblk), dim3(64), shq, stream, b, x, lu->d_order, lu->d_blk_q, lu->ilv,
+        hipLaunchKernelGGL(k_psolve_q<4>, dim3(lu->norder), dim3(64), shq, stream, b, x, lu->d_order, lu->d_blk_q, lu->ilv,
                            lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch, gate);
         GADE_HIP(hipGetLastError());
         return GADE_OK;

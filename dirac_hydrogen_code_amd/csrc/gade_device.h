@@ -468,6 +468,7 @@ __global__ __launch_bounds__(64) void k_psolve_q(const double2 *__restrict__ b, 
     extern __shared__ double2 smem[];
     const int lane = threadIdx.x, i = lane >> 2, k = lane & 3;
     const int r = order[blockIdx.x];
+    if (r < 0) return;  // padding slot (whole workgroup)
     const int q = blk_q[r];
     double2 *yv = smem;              // [nch*CH]: b, then y, then x (interleaved order)
     double2 *acc = smem + nch * CH;  // [CH]

@@ -52,7 +52,8 @@ struct gade_blocklu {
     int n = 0, nblk = 0, nq = 0, band = 0, ch = 16, nch = 0, npw = 0, ilv = 0;
     double2 *d_AB = nullptr, *d_Linv = nullptr, *d_Uinv = nullptr, *d_Loff = nullptr, *d_Uoff = nullptr;
     int *d_blk_q = nullptr, *d_wblocks = nullptr, *d_flag = nullptr;
-    int *d_order = nullptr;  // blocks sorted by distinct factor (one block per wave)
+    int *d_order = nullptr;  // one block per wave, -1 pads; blocks of one factor on one XCD
+    int norder = 0;
 };
 
 namespace gade {
@@ -151,5 +152,6 @@ struct gade_model {
     std::vector<gade::PendingEvent> pending;
     std::vector<hipEvent_t> event_pool;
     int prof_period = 1;                       // time every prof_period-th launch of a family
+    int prof_floor_ns = 0;                     // timed launches shorter than this are not counted
     std::map<std::string, long long> prof_seen;
 };

@@ -179,6 +179,8 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
  *   "iter_batch"  BiCGSTAB iterations enqueued per step, 0 = adaptive (default);
  *             a step needing more is finished by the host loop, same result;
  *   "profile_period"  time every n-th launch of a profiled family (default 1);
+ *   "profile_floor_ns"  timed launches shorter than this (those a device-side
+ *             gate ended at once) are left out of the family totals (default 0);
  *   "guess_order"  initial guess of each CN step: 0 psi_n (default, the
  *             reference), 1..3 polynomial extrapolation (gade_extrapolate).
  * Results do not depend on the choice. */
