@@ -87,6 +87,11 @@ __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, d
 struct Coef6 {
     double2 c[6];
 };
+// the pulse coefficients of up to kCoefGroup instances, passed by value to one launch
+constexpr int kCoefGroup = 32;
+struct CoefGroup {
+    Coef6 c[kCoefGroup];
+};
 
 // GE[l][m][s][a]: m<4: Gs_{l,m} = sum_alpha c_alpha g^m_{alpha,l};
 // m>=4: Gts_{l,m-4} with gt = conj-transpose of g (tools/solver.c:236-287).
@@ -158,10 +163,10 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restri
                                                          const double2 *__restrict__ coef,
                                                          const int *__restrict__ kap, int nblk, int n, int lmax,
                                                          int maxrow, double2 *__restrict__ V,
-                  const int *__restrict__ gate) {
-    GADE_GATED(gate);
-    extern __shared__ double2 shc[];  // [maxrow][LB][2] coefficients, then [maxrow] (col, kappa) packed
+                                                         const int *gate, int gstride, int nblk1) {
     const int r = blockIdx.x;
+    GADE_GATED_B(gate, gstride, r / nblk1);
+    extern __shared__ double2 shc[];  // [maxrow][LB][2] coefficients, then [maxrow] (col, kappa) packed
     const int a = blockIdx.y * blockDim.x + threadIdx.x;
     const int l0 = blockIdx.z * LB;
     const int nlb = min(LB, lmax - l0);
@@ -382,12 +387,13 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
     }
 }
 
-// y = sum_l ypart[l] + beta y  (fixed summation order)
+// y = sum_l ypart[l] + beta y  (fixed summation order); blockIdx.y = instance of N1 entries
 __global__ __launch_bounds__(256) void k_radial_sum(const double2 *__restrict__ ypart, int lgrid, long long N,
-                                                    double beta, double2 *__restrict__ y,
-                  const int *__restrict__ gate) {
-    GADE_GATED(gate);
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+                                                    double beta, double2 *__restrict__ y, const int *gate,
+                                                    int gstride, long long N1) {
+    GADE_GATED_B(gate, gstride, blockIdx.y);
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)blockIdx.y * N1 + j;
         double2 s = ypart[i];
         for (int l = 1; l < lgrid; l++) s = cadd(s, ypart[(size_t)l * N + i]);
         if (beta != 0.0) s = cadd(s, cscale(y[i], beta));
@@ -425,9 +431,12 @@ __device__ __forceinline__ int dia_swz(int p) { return p ^ (((p >> 3) ^ (p >> 4)
 // DIA-ordered couplings gs[l][j][nn][qd][d][ip] (g of alpha = 2j + (l even))
 // and gts (conjugated transposes).  One thread per (l, qd, d, ip).
 __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ gs, const double2 *__restrict__ gts,
-                                                   Coef6 cf, int lmax, int nd, int nhp, double2 *__restrict__ GD,
-                  const int *__restrict__ gate) {
-    GADE_GATED(gate);
+                                                   CoefGroup cg, int b0, size_t gd_stride, int lmax, int nd, int nhp,
+                                                   double2 *__restrict__ GD0, const int *gate, int gstride) {
+    const int b = b0 + blockIdx.y;
+    GADE_GATED_B(gate, gstride, b);
+    const Coef6 &cf = cg.c[blockIdx.y];
+    double2 *__restrict__ GD = GD0 + (size_t)b * gd_stride;
     const int S = 4 * nd * nhp;  // entries per (l, j, nn) plane
     const long long total = (long long)lmax * S;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
@@ -506,9 +515,8 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
                                                    const double *__restrict__ tile_k, int nblk, int n, int nh,
                                                    int nhp, int nseg, int lgrid, int nq, int sel, double cH,
                                                    double beta, double2 *__restrict__ y,
-                                                   double2 *__restrict__ ypart,
-                  const int *__restrict__ gate) {
-    GADE_GATED(gate);
+                                                   double2 *__restrict__ ypart, const int *gate, int gstride,
+                                                   const int *__restrict__ tile_b, size_t gd_stride) {
     constexpr int R = kDiaR, P = kDiaP, ND = 2 * BW + 1, W = kDiaSeg + 2 * BW;
     __shared__ double2 sm[R * W];
     const int lane = threadIdx.x;
@@ -519,6 +527,9 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
     rest /= nseg;
     const int h = rest & 1;
     const int tile = rest >> 1;
+    const int inst = tile_b[tile];
+    GADE_GATED_B(gate, gstride, inst);
+    GD += (size_t)inst * gd_stride;  // this instance's G-sums
     const double kap = tile_k[tile];
     const int q = tile_q[tile];
     const int i0 = seg * kDiaSeg;
@@ -818,8 +829,9 @@ void free_all(gade_model *m) {
         if (p) hipFree(p);
     if (m->h_sc) hipHostFree(m->h_sc);
     if (m->h_ctl) hipHostFree(m->h_ctl);
+    if (m->h_gctl) hipHostFree(m->h_gctl);
     for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval,
-                    (void *)m->d_hist})
+                    (void *)m->d_hist, (void *)m->d_gctl, (void *)m->d_nsteps, (void *)m->d_dtile_b})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -830,7 +842,7 @@ int launch_angular(gade_model *m, const double2 *x) {
     dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
     size_t shm = sizeof(double2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
     hipLaunchKernelGGL(k_angular<kAngLB>, grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef,
-                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V, m->gate);
+                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V, m->gate, m->gstride, m->nblk1);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -844,13 +856,14 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
             Prof pr(m, "radial", GADE_PROF_RADIAL);
             hipLaunchKernelGGL(k_radial_dia<kDiaBW>, grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD, m->d_DD,
                                m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, m->nseg, lgrid,
-                               m->nq, sel, cH, beta, y, inter ? m->d_ypart : nullptr, m->gate);
+                               m->nq, sel, cH, beta, y, inter ? m->d_ypart : nullptr, m->gate, m->gstride,
+                               m->d_dtile_b, m->gd_stride);
             GADE_HIP(hipGetLastError());
         }
         if (inter) {
             Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
-            hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart,
-                               lgrid, m->N, beta, y, m->gate);
+            hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
+                               m->d_ypart, lgrid, m->N, beta, y, m->gate, m->gstride, m->N1);
             GADE_HIP(hipGetLastError());
         }
         return GADE_OK;
@@ -874,29 +887,34 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
     GADE_HIP(hipGetLastError());
     if (inter) {
         Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
-        hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->d_ypart, lgrid,
-                           m->N, beta, y, m->gate);
+        hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream, m->d_ypart,
+                           lgrid, m->N, beta, y, m->gate, m->gstride, m->N1);
         GADE_HIP(hipGetLastError());
     }
     return GADE_OK;
 }
 
 // group blocks by distinct kappa index and split each group into tiles of R rows (-1 pads)
-int kappa_tiles(const gade_model *m, int R, int **d_tiles, int **d_tq, double **d_tk, int *ntiles) {
-    std::vector<int> tiles, tq;
+// per instance, group its blocks by distinct kappa index and split each group into tiles of R rows (-1 pads);
+// a tile never mixes instances
+int kappa_tiles(const gade_model *m, int R, int **d_tiles, int **d_tq, double **d_tk, int *ntiles,
+                int **d_tb = nullptr) {
+    std::vector<int> tiles, tq, tb;
     std::vector<double> tk;
-    for (int q = 0; q < m->nq; q++) {
-        std::vector<int> rows;
-        for (int r = 0; r < m->nblk; r++)
-            if (m->blk_q[r] == q) rows.push_back(r);
-        for (size_t i = 0; i < rows.size(); i += R) {
-            for (int j = 0; j < R; j++) tiles.push_back(i + j < rows.size() ? rows[i + j] : -1);
-            tq.push_back(q);
-            tk.push_back((double)m->kap_q[q]);
+    for (int b = 0; b < m->B; b++)
+        for (int q = 0; q < m->nq; q++) {
+            std::vector<int> rows;
+            for (int r = b * m->nblk1; r < (b + 1) * m->nblk1; r++)
+                if (m->blk_q[r] == q) rows.push_back(r);
+            for (size_t i = 0; i < rows.size(); i += R) {
+                for (int j = 0; j < R; j++) tiles.push_back(i + j < rows.size() ? rows[i + j] : -1);
+                tq.push_back(q);
+                tk.push_back((double)m->kap_q[q]);
+                tb.push_back(b);
+            }
         }
-    }
     *ntiles = (int)tq.size();
-    for (void *p : {(void *)*d_tiles, (void *)*d_tq, (void *)*d_tk})
+    for (void *p : {(void *)*d_tiles, (void *)*d_tq, (void *)*d_tk, d_tb ? (void *)*d_tb : nullptr})
         if (p) (void)hipFree(p);
     *d_tiles = nullptr;
     *d_tq = nullptr;
@@ -904,6 +922,10 @@ int kappa_tiles(const gade_model *m, int R, int **d_tiles, int **d_tq, double **
     GADE_TRY(upload(d_tiles, tiles.data(), tiles.size()));
     GADE_TRY(upload(d_tq, tq.data(), tq.size()));
     GADE_TRY(upload(d_tk, tk.data(), tk.size()));
+    if (d_tb) {
+        *d_tb = nullptr;
+        GADE_TRY(upload(d_tb, tb.data(), tb.size()));
+    }
     return GADE_OK;
 }
 
@@ -939,7 +961,8 @@ int setup_dia(gade_model *m) {
     GADE_TRY(upload(&m->d_dia_src, dia_src.data(), dia_src.size()));
     GADE_TRY(hip_alloc((void **)&m->d_gs, sizeof(double2) * (size_t)lmax * 12 * S));
     GADE_TRY(hip_alloc((void **)&m->d_gts, sizeof(double2) * (size_t)lmax * 12 * S));
-    GADE_TRY(hip_alloc((void **)&m->d_GD, sizeof(double2) * (size_t)lmax * 4 * S * 2));
+    m->gd_stride = (size_t)lmax * 4 * S * 2;
+    GADE_TRY(hip_alloc((void **)&m->d_GD, sizeof(double2) * m->gd_stride * m->B));
     GADE_TRY(hip_alloc((void **)&m->d_DD, sizeof(double2) * (size_t)3 * m->nq * S));
     const long long tg = (long long)lmax * 12 * S, td = (long long)3 * m->nq * S;
     hipLaunchKernelGGL(k_build_gdia, dim3(grid_for(tg, 256)), dim3(256), 0, m->stream, m->d_g, m->d_dia_src, m->d_tmap,
@@ -948,26 +971,37 @@ int setup_dia(gade_model *m) {
                        m->d_ell_pos, m->nq, m->ellw, n, m->nd, m->nhp, m->d_DD);
     GADE_HIP(hipGetLastError());
     GADE_HIP(hipStreamSynchronize(m->stream));
-    GADE_TRY(kappa_tiles(m, kDiaR, &m->d_dtiles, &m->d_dtile_q, &m->d_dtile_k, &m->ndtiles));
+    GADE_TRY(kappa_tiles(m, kDiaR, &m->d_dtiles, &m->d_dtile_q, &m->d_dtile_k, &m->ndtiles, &m->d_dtile_b));
     m->dia_ok = true;
     return GADE_OK;
 }
 
 // G-sums of the current f(t) in the layout of the active radial kernel
-int launch_gsums(gade_model *m, bool ell) {
+// pulse coefficients c_alpha = SoL * (i h dt/2) * f_alpha of instance b
+Coef6 coef_of(const gade_model *m, int b) {
     Coef6 cf;
-    // c_alpha = SoL * (i h dt/2) * f_alpha
     const double hd = kSoL * m->h * m->dt / 2;
-    for (int a = 0; a < 6; a++) cf.c[a] = make_double2(-hd * m->ft[2 * a + 1], hd * m->ft[2 * a]);
+    const double *ft = m->ft.data() + (size_t)12 * b;
+    for (int a = 0; a < 6; a++) cf.c[a] = make_double2(-hd * ft[2 * a + 1], hd * ft[2 * a]);
+    return cf;
+}
+
+// G-sums of the current f(t) of every instance in the layout of the active radial kernel
+int launch_gsums(gade_model *m, bool ell) {
     Prof pr(m, "gsums", GADE_PROF_GSUMS);
     if (ell) {
         long long total = (long long)m->lmax * 8 * m->ellw * m->n;
         hipLaunchKernelGGL(k_gsums, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_g, m->d_ell_src,
-                           m->d_tmap, cf, m->lmax, m->ellw, m->n, m->nnzb, m->d_GE, m->gate);
+                           m->d_tmap, coef_of(m, 0), m->lmax, m->ellw, m->n, m->nnzb, m->d_GE, m->gate);
     } else {
         long long total = (long long)m->lmax * 4 * m->nd * m->nhp;
-        hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_gs, m->d_gts, cf,
-                           m->lmax, m->nd, m->nhp, m->d_GD, m->gate);
+        for (int b0 = 0; b0 < m->B; b0 += kCoefGroup) {
+            const int nb = std::min(kCoefGroup, m->B - b0);
+            CoefGroup cg{};
+            for (int k = 0; k < nb; k++) cg.c[k] = coef_of(m, b0 + k);
+            hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), nb), dim3(256), 0, m->stream, m->d_gs,
+                               m->d_gts, cg, b0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride);
+        }
     }
     GADE_HIP(hipGetLastError());
     return GADE_OK;
@@ -1017,7 +1051,11 @@ int gade_device_sync(void) {
 }
 
 int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream, gade_model_t **out) {
-    if (!in || !out || in->nblk <= 0 || in->blkdim <= 0 || in->lmax <= 0) {
+    return gade_model_create_batch(in, 1, h, dt, stream, out);
+}
+
+int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt, void *stream, gade_model_t **out) {
+    if (!in || !out || in->nblk <= 0 || in->blkdim <= 0 || in->lmax <= 0 || B < 1) {
         gade::set_error("gade_model_create: invalid arguments");
         return GADE_EINVAL;
     }
@@ -1032,7 +1070,9 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
         delete m;
         return rc;
     };
-    m->nblk = in->nblk;
+    m->nblk = in->nblk;  // one instance until the structure is replicated below
+    m->nblk1 = in->nblk;
+    m->B = B;
     m->n = in->blkdim;
     m->lmax = in->lmax;
     m->h = h;
@@ -1147,6 +1187,36 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
                 cp[0] = v0.real(); cp[1] = v0.imag(); cp[2] = v1.real(); cp[3] = v1.imag();
             }
         }
+    // B instances: the Hall structure, kappa maps and angular coefficients repeat with row offsets
+    m->N1 = (long long)m->nblk1 * n;
+    if (B > 1) {
+        const int nb1 = m->nblk1, hn1 = m->hnnz;
+        std::vector<int> Ap((size_t)B * nb1 + 1), Ai((size_t)B * hn1), kb((size_t)B * nb1), bq((size_t)B * nb1);
+        std::vector<double> cf((size_t)B * coef.size());
+        std::vector<int> dg((size_t)B * hn1);
+        for (int b = 0; b < B; b++) {
+            for (int r = 0; r < nb1; r++) {
+                Ap[(size_t)b * nb1 + r] = b * hn1 + m->hall_Ap[r];
+                kb[(size_t)b * nb1 + r] = m->kap_blk[r];
+                bq[(size_t)b * nb1 + r] = m->blk_q[r];
+            }
+            for (int p = 0; p < hn1; p++) {
+                Ai[(size_t)b * hn1 + p] = b * nb1 + m->hall_Ai[p];
+                dg[(size_t)b * hn1 + p] = diag[p];
+            }
+            std::copy(coef.begin(), coef.end(), cf.begin() + (size_t)b * coef.size());
+        }
+        Ap[(size_t)B * nb1] = B * hn1;
+        m->hall_Ap.swap(Ap);
+        m->hall_Ai.swap(Ai);
+        m->kap_blk.swap(kb);
+        m->blk_q.swap(bq);
+        coef.swap(cf);
+        diag.swap(dg);
+        m->nblk = B * nb1;
+        m->hnnz = B * hn1;
+    }
+    m->N = (long long)m->nblk * n;
     // uploads
     int rc;
     std::vector<double> gbuf((size_t)24 * lmax * m->nnzb * 2);
@@ -1192,31 +1262,36 @@ int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream
     if ((rc = build_tiles(m))) return fail(rc);
     if ((rc = setup_dia(m))) return fail(rc);
     m->radial_kind = m->dia_ok ? 2 : 1;
+    if (B > 1 && !m->dia_ok) {
+        gade::set_error("batched models need a banded radial pattern (2x2 bands of half-width <= 7)");
+        return fail(GADE_EINVAL);
+    }
     // BiCGSTAB workspace and scalars
     double2 **ws[] = {&m->r, &m->r0, &m->v, &m->p, &m->y, &m->z, &m->s, &m->t, &m->tmp};
     for (auto w : ws) {
         if ((rc = hip_alloc((void **)w, sizeof(double2) * m->N))) return fail(rc);
         if (hipMemset(*w, 0, sizeof(double2) * m->N) != hipSuccess) return fail(GADE_EHIP);
     }
-    m->red_blocks = (int)std::min<long long>(1024, (m->N + kRedThreads - 1) / kRedThreads);
-    if ((rc = hip_alloc((void **)&m->d_sc, sizeof(double) * 64)) ||
-        (rc = hip_alloc((void **)&m->d_part, sizeof(double) * 4 * m->red_blocks)) ||
+    m->red_blocks = (int)std::min<long long>(1024, (m->N1 + kRedThreads - 1) / kRedThreads);
+    const size_t nsc = (size_t)B * SC_STRIDE, nct = (size_t)B * CT_STRIDE;
+    if ((rc = hip_alloc((void **)&m->d_sc, sizeof(double) * nsc)) ||
+        (rc = hip_alloc((void **)&m->d_part, sizeof(double) * 4 * (size_t)B * m->red_blocks)) ||
         (rc = hip_alloc((void **)&m->d_cnt, sizeof(unsigned) * 4)) ||
-        (rc = hip_alloc((void **)&m->d_flag, sizeof(int) * 4)))
+        (rc = hip_alloc((void **)&m->d_flag, sizeof(int) * 4)) ||
+        (rc = hip_alloc((void **)&m->d_ctl, sizeof(int) * nct)) ||
+        (rc = hip_alloc((void **)&m->d_gctl, sizeof(int) * GC_COUNT)) ||
+        (rc = hip_alloc((void **)&m->d_nsteps, sizeof(int) * B)))
         return fail(rc);
-    if (hipMemset(m->d_cnt, 0, sizeof(unsigned) * 4) != hipSuccess || hipMemset(m->d_sc, 0, sizeof(double) * 64) != hipSuccess)
+    if (hipMemset(m->d_cnt, 0, sizeof(unsigned) * 4) != hipSuccess || hipMemset(m->d_sc, 0, sizeof(double) * nsc) != hipSuccess ||
+        hipMemset(m->d_ctl, 0, sizeof(int) * nct) != hipSuccess || hipMemset(m->d_gctl, 0, sizeof(int) * GC_COUNT) != hipSuccess)
         return fail(GADE_EHIP);
-    if ((rc = hip_alloc((void **)&m->d_ctl, sizeof(int) * CT_COUNT))) return fail(rc);
-    if (hipMemset(m->d_ctl, 0, sizeof(int) * CT_COUNT) != hipSuccess) return fail(GADE_EHIP);
-    if (hipHostMalloc((void **)&m->h_ctl, sizeof(int) * CT_COUNT, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&m->h_ctl, sizeof(int) * nct, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&m->h_gctl, sizeof(int) * GC_COUNT, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&m->h_sc, sizeof(double) * nsc, hipHostMallocDefault) != hipSuccess) {
         gade::set_error("hipHostMalloc failed");
         return fail(GADE_ENOMEM);
     }
-    if (hipHostMalloc((void **)&m->h_sc, sizeof(double) * 64, hipHostMallocDefault) != hipSuccess) {
-        gade::set_error("hipHostMalloc failed");
-        return fail(GADE_ENOMEM);
-    }
-    m->ft.assign(12, 0.0);
+    m->ft.assign((size_t)12 * B, 0.0);
     *out = m;
     return GADE_OK;
 }
@@ -1232,18 +1307,23 @@ int gade_model_destroy(gade_model_t *m) {
 int gade_model_info(const gade_model_t *m, int *nblk, int *blkdim, int *hall_nnz, int *nnz_b, long long *nnz_full,
                     int *nkappa) {
     if (!m) return GADE_EINVAL;
-    if (nblk) *nblk = m->nblk;
+    if (nblk) *nblk = m->nblk1;  // blocks of one instance
     if (blkdim) *blkdim = m->n;
-    if (hall_nnz) *hall_nnz = m->hnnz;
+    if (hall_nnz) *hall_nnz = m->hnnz / m->B;
     if (nnz_b) *nnz_b = m->nnzb;
-    if (nnz_full) *nnz_full = (long long)m->hnnz * m->nnzb;
+    if (nnz_full) *nnz_full = (long long)(m->hnnz / m->B) * m->nnzb;
     if (nkappa) *nkappa = m->nq;
     return GADE_OK;
 }
 
 int gade_set_time(gade_model_t *m, const double *ft) {
     if (!m || !ft) return GADE_EINVAL;
-    m->ft.assign(ft, ft + 12);
+    for (int b = 0; b < m->B; b++) std::copy(ft, ft + 12, m->ft.begin() + (size_t)12 * b);
+    return launch_gsums(m, m->radial_kind != 2);
+}
+int gade_set_time_batch(gade_model_t *m, const double *ft) {
+    if (!m || !ft) return GADE_EINVAL;
+    std::copy(ft, ft + (size_t)12 * m->B, m->ft.begin());
     return launch_gsums(m, m->radial_kind != 2);
 }
 
@@ -1295,18 +1375,18 @@ namespace {
 
 int psolve_async(gade_model *m, const double2 *b, double2 *x) {
     Prof pr(m, "psolve", GADE_PROF_PSOLVE);
-    return gade::blocklu_solve_async(m->pre, b, x, m->stream, m->gate);
+    return gade::blocklu_solve_async(m->pre, b, x, m->stream, m->gate, m->gstride, m->nblk1);
 }
 
 template <int OP>
 int bicg_op(gade_model *m, const double2 *b, double2 *x = nullptr) {
     Prof pr(m, "blas1", GADE_PROF_BLAS1);
     int *ctl = m->ctl_on ? m->d_ctl : nullptr;
-    hipLaunchKernelGGL(k_bicg<OP>, dim3(m->red_blocks), dim3(kRedThreads), 0, m->stream, m->N, b, m->r, m->r0, m->p,
-                       m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part, m->gate, ctl);
+    hipLaunchKernelGGL(k_bicg<OP>, dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream, m->N1, b, m->r, m->r0,
+                       m->p, m->v, m->s, m->t, m->y, m->z, x, m->d_sc, m->d_part, m->gate, m->gstride);
     if (OP != OP_P && OP != OP_S)
-        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(1), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part, m->d_sc,
-                           m->gate, ctl);
+        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part,
+                           m->d_sc, m->gate, m->gstride, ctl);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -1370,6 +1450,10 @@ int bicg_host_loop(gade_model *m, const double2 *rhs, double2 *x, int i, double 
 }
 
 int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, double *tol_error, int *napply) {
+    if (m->B != 1) {
+        gade::set_error("the host-driven BiCGSTAB serves one instance; batched models step with gade_cn_run_batch");
+        return GADE_EINVAL;
+    }
     const double tol = *tol_error;
     const int maxit = *iters;
     GADE_TRY(gade_precond_setup(m));
@@ -1398,29 +1482,54 @@ int bicgstab_impl(gade_model *m, const double2 *rhs, double2 *x, int *iters, dou
 // ---------------------------------------------------------------------------
 // device-driven CN stepping
 //
-// A step is enqueued as f(t) -> G-sums, rhs = (S - H) psi, r = rhs - A psi and
-// K BiCGSTAB iterations whose launches are gated on the device control words:
-// the reductions take the host loop's decisions (converged, maxit, restart,
-// non-finite) on the device, and once a solve is decided the remaining
-// iterations of the step return at once.  k_step_end records iterations and
-// tolerance, or marks the run stalled when the step needs more than K
-// iterations or the host (restart, non-finite, zero rhs); then every later
-// launch is void until the host has finished that step with bicg_host_loop.
-// The host synchronizes once per window of steps.
+// A step is enqueued for every instance as f(t) -> G-sums, rhs = (S - H) psi,
+// [x0 by extrapolation], r = rhs - A x0 and K BiCGSTAB iterations whose
+// launches are gated on each instance's control words: the reductions take
+// the host loop's decisions (converged, maxit, restart, non-finite, zero rhs)
+// on the device, and once an instance's solve is decided its remaining
+// launches return at once.  k_step_end records iterations and tolerance, or
+// marks the run stalled when an instance needs more than K iterations or the
+// host (restart, non-finite, zero rhs); every later launch is then void until
+// the host has finished that step: it restarts or zeroes the instances that
+// need it and enqueues more iterations.  The host synchronizes once per window
+// of steps.
 // ---------------------------------------------------------------------------
-__global__ void k_ctl_set(int *ctl, double *sc, double tol, int maxit) {
+__global__ void k_hist_reset(int *ctl0) {
+    if (threadIdx.x == 0) ctl0[(size_t)blockIdx.x * CT_STRIDE + CT_HIST] = 0;
+}
+
+__global__ void k_ctl_set(int *ctl0, int *gctl, double *sc0, double tol, int maxit) {
     if (threadIdx.x != 0) return;
-    for (int k = 0; k < CT_COUNT; k++)
+    int *ctl = ctl0 + (size_t)blockIdx.x * CT_STRIDE;
+    for (int k = 0; k < CT_STRIDE; k++)
         if (k != CT_HIST) ctl[k] = 0;
     ctl[CT_MAXIT] = maxit;
-    sc[SC_TOL] = tol;
+    ctl[CT_HSEL] = 1;
+    sc0[(size_t)blockIdx.x * SC_STRIDE + SC_TOL] = tol;
+    if (blockIdx.x == 0)
+        for (int k = 0; k < GC_COUNT; k++) gctl[k] = 0;
+}
+
+// x0 by extrapolation of each instance's history (order m->guess_order); no-op for order 0
+int launch_guess(gade_model *m, double2 *psi) {
+    if (m->guess_order <= 0) return GADE_OK;
+    if (!m->d_hist) GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+    hipLaunchKernelGGL(k_extrap, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream, m->N1, m->N, psi,
+                       m->d_hist, m->guess_order, m->d_ctl, m->gate, m->gstride);
+    hipLaunchKernelGGL(k_hist_count, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->gate, m->gstride);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
 }
 
 struct GateScope {
     gade_model *m;
-    explicit GateScope(gade_model *mm) : m(mm) { m->ctl_on = true; }
+    explicit GateScope(gade_model *mm) : m(mm) {
+        m->ctl_on = true;
+        m->gstride = CT_STRIDE;
+    }
     ~GateScope() {
         m->gate = nullptr;
+        m->gstride = 0;
         m->ctl_on = false;
     }
 };
@@ -1432,32 +1541,14 @@ int ensure_step_buffers(gade_model *m, int nsteps) {
     m->d_iters = nullptr;
     m->d_tols = nullptr;
     m->steps_cap = 0;
-    GADE_TRY(hip_alloc((void **)&m->d_iters, sizeof(int) * nsteps));
-    GADE_TRY(hip_alloc((void **)&m->d_tols, sizeof(double) * nsteps));
+    GADE_TRY(hip_alloc((void **)&m->d_iters, sizeof(int) * (size_t)m->B * nsteps));
+    GADE_TRY(hip_alloc((void **)&m->d_tols, sizeof(double) * (size_t)m->B * nsteps));
     m->steps_cap = nsteps;
     return GADE_OK;
 }
 
-// x0 by extrapolation of the history (order m->guess_order); no-op for order 0
-int launch_guess(gade_model *m, double2 *psi) {
-    if (m->guess_order <= 0) return GADE_OK;
-    if (!m->d_hist) GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
-    hipLaunchKernelGGL(k_extrap, dim3(grid_for(m->N, 256, 2048)), dim3(256), 0, m->stream, m->N, psi, m->d_hist,
-                       m->guess_order, m->d_ctl, m->gate);
-    hipLaunchKernelGGL(k_hist_count, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->gate);
-    GADE_HIP(hipGetLastError());
-    return GADE_OK;
-}
-
-int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K) {
-    GateScope gs(m);
-    m->ft.assign(ft, ft + 12);
-    m->gate = m->d_ctl + CT_STALLED;
-    GADE_TRY(launch_gsums(m, m->radial_kind != 2));
-    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
-    GADE_TRY(launch_guess(m, psi));                    // x0 (psi_n unless extrapolating)
-    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0
-    GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
+// K gated BiCGSTAB iterations on the current step of every instance
+int enqueue_iterations(gade_model *m, double2 *psi, int K) {
     m->gate = m->d_ctl + CT_SKIP;
     for (int k = 0; k < K; k++) {
         GADE_TRY(bicg_op<OP_P>(m, m->tmp));
@@ -1470,26 +1561,131 @@ int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K)
         GADE_TRY(bicg_op<OP_W>(m, m->tmp));
         GADE_TRY(bicg_op<OP_XR>(m, m->tmp, psi));
     }
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->d_sc, step, m->d_iters, m->d_tols);
+    return GADE_OK;
+}
+
+int enqueue_step_end(gade_model *m, int step) {
+    hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, step,
+                       m->steps_cap, m->d_iters, m->d_tols);
+    hipLaunchKernelGGL(k_stall_spread, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
 
-// Steps 0..nsteps-1 at the given times (f(t) from gade_pulse); psi is x0 and result.
-int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, double E0, double omega,
-                  double cycles, double tol, int maxit, int *iters, double *tol_out, double *usec) {
+// ft: [B][12] pulse coefficients of this step
+int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K, bool per_instance_steps) {
+    GateScope gs(m);
+    std::copy(ft, ft + (size_t)12 * m->B, m->ft.begin());
+    hipLaunchKernelGGL(k_step_begin, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, step,
+                       per_instance_steps ? m->d_nsteps : nullptr);
+    GADE_HIP(hipGetLastError());
+    m->gate = m->d_ctl + CT_PSKIP;
+    GADE_TRY(launch_gsums(m, m->radial_kind != 2));
+    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
+    GADE_TRY(launch_guess(m, psi));                    // x0 (psi_n unless extrapolating)
+    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0
+    GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
+    GADE_TRY(enqueue_iterations(m, psi, K));
+    return enqueue_step_end(m, step);
+}
+
+// The run stalled at 'step': finish it for every instance from the device state.
+// it_h/tol_h: [B][steps_cap] host copies of the per-step outputs.
+int finish_stalled_step(gade_model *m, double2 *psi, int step, int K, std::vector<int> &it_h,
+                        std::vector<double> &tol_h) {
+    const int B = m->B;
+    for (int round = 0;; round++) {
+        GADE_HIP(hipMemcpyAsync(m->h_ctl, m->d_ctl, sizeof(int) * B * CT_STRIDE, hipMemcpyDeviceToHost, m->stream));
+        GADE_HIP(hipMemcpyAsync(m->h_sc, m->d_sc, sizeof(double) * B * SC_STRIDE, hipMemcpyDeviceToHost, m->stream));
+        GADE_HIP(hipStreamSynchronize(m->stream));
+        // instances idle at this step (pulse over) keep PSKIP set by k_step_begin; k_stall_spread set it for
+        // the others, so activity is read from the step's status instead
+        bool more = false, restart = false;
+        for (int b = 0; b < B; b++) {
+            int *c = m->h_ctl + (size_t)b * CT_STRIDE;
+            const double *sc = m->h_sc + (size_t)b * SC_STRIDE;
+            const int st = c[CT_STATUS];
+            c[CT_HSEL] = 1;
+            c[CT_PSKIP] = 1;
+            if (st == 2) {
+                if (sc[SC_RHSSQ] == 0) {
+                    // zero rhs: x = 0 (tools/gpu_bicgstab.c:183-189)
+                    GADE_HIP(hipMemsetAsync(psi + (size_t)b * m->N1, 0, sizeof(double2) * m->N1, m->stream));
+                    c[CT_STATUS] = 1;
+                    c[CT_ITER] = 0;
+                } else if (!std::isfinite(sc[SC_RNORM2])) {
+                    gade::set_error("BiCGSTAB: non-finite residual");
+                    return GADE_ENOTCONV;
+                } else {
+                    // restart: r = rhs - A x, r0 = r, rho = |r|^2 (tools/gpu_bicgstab.c:222-241)
+                    c[CT_HSEL] = 0;
+                    c[CT_STATUS] = 0;
+                    restart = more = true;
+                }
+            } else if (st == 0) {
+                more = true;
+            }
+            c[CT_SKIP] = c[CT_STATUS] != 0;
+        }
+        if (round > 4096) {
+            gade::set_error("device-driven solve does not finish");
+            return GADE_ENOTCONV;
+        }
+        // the decisions above go back to the device in every round, the last one included, and the
+        // run is no longer stalled
+        m->h_gctl[GC_STALLED] = 0;
+        m->h_gctl[GC_STALL_STEP] = 0;
+        GADE_HIP(hipMemcpyAsync(m->d_ctl, m->h_ctl, sizeof(int) * B * CT_STRIDE, hipMemcpyHostToDevice, m->stream));
+        GADE_HIP(hipMemcpyAsync(m->d_gctl, m->h_gctl, sizeof(int) * GC_COUNT, hipMemcpyHostToDevice, m->stream));
+        if (more) {
+            GateScope gs(m);
+            if (restart) {
+                m->gate = m->d_ctl + CT_HSEL;
+                GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));
+                GADE_TRY(bicg_op<OP_RESTART>(m, m->tmp));
+            }
+            GADE_TRY(enqueue_iterations(m, psi, K));
+        }
+        // record every decided solve of this step (zero-rhs instances with 0 iterations, tolerance 0)
+        hipLaunchKernelGGL(k_step_end_resume, dim3(B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, step,
+                           m->steps_cap, m->d_iters, m->d_tols);
+        GADE_HIP(hipGetLastError());
+        GADE_HIP(hipMemcpyAsync(it_h.data(), m->d_iters, sizeof(int) * B * m->steps_cap, hipMemcpyDeviceToHost,
+                                m->stream));
+        GADE_HIP(hipMemcpyAsync(tol_h.data(), m->d_tols, sizeof(double) * B * m->steps_cap, hipMemcpyDeviceToHost,
+                                m->stream));
+        if (!more) {
+            GADE_HIP(hipStreamSynchronize(m->stream));
+            break;
+        }
+    }
+    // normal stepping resumes with the next step: k_step_begin recomputes every instance's gates
+    return GADE_OK;
+}
+
+// Steps 0..nsteps-1 for every instance at the given times; params [B][4] = (E0, omega, cycles, cep) of
+// each instance's pulse; nsteps_b [B] (null: every instance runs all steps).  psi [B][N1] is x0 and
+// result.  iters/tol_out [B][nsteps] (entries past an instance's pulse are left untouched).
+int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, const double *params,
+                  const int *nsteps_b, double tol, int maxit, int *iters, double *tol_out, double *usec) {
     if (nsteps == 0) return GADE_OK;
+    const int B = m->B;
     GADE_TRY(gade_precond_setup(m));
     GADE_TRY(ensure_step_buffers(m, nsteps));
-    std::vector<int> it_h(nsteps, 0);
-    std::vector<double> tol_h(nsteps, 0.0);
+    const size_t cap = (size_t)m->steps_cap;
+    std::vector<int> it_h((size_t)B * cap, 0);
+    std::vector<double> tol_h((size_t)B * cap, 0.0);
+    GADE_HIP(hipMemsetAsync(m->d_iters, 0, sizeof(int) * B * cap, m->stream));
+    GADE_HIP(hipMemsetAsync(m->d_tols, 0, sizeof(double) * B * cap, m->stream));
+    if (nsteps_b) GADE_HIP(hipMemcpyAsync(m->d_nsteps, nsteps_b, sizeof(int) * B, hipMemcpyHostToDevice, m->stream));
     std::vector<hipEvent_t> ev;
     auto free_events = [&]() {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         ev.clear();
     };
-    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->d_sc, tol, maxit);
+    hipLaunchKernelGGL(k_ctl_set, dim3(B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, tol, maxit);
     GADE_HIP(hipGetLastError());
+    std::vector<double> ft((size_t)12 * B);
     int s = 0;
     while (s < nsteps) {
         const int W = std::min(m->window, nsteps - s);
@@ -1501,40 +1697,35 @@ int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, 
             GADE_HIP(hipEventRecord(ev[0], m->stream));
         }
         for (int w = 0; w < W; w++) {
-            double ft[12];
-            gade_pulse(E0, omega, cycles, times[s + w], ft);
-            int rc = enqueue_step(m, psi, ft, s + w, K);
+            for (int b = 0; b < B; b++) {
+                const double *pp = params + (size_t)4 * b;
+                gade_pulse_cep(pp[0], pp[1], pp[2], pp[3], times[s + w], ft.data() + (size_t)12 * b);
+            }
+            int rc = enqueue_step(m, psi, ft.data(), s + w, K, nsteps_b != nullptr);
             if (rc == GADE_OK && usec) rc = hipEventRecord(ev[w + 1], m->stream) == hipSuccess ? GADE_OK : GADE_EHIP;
             if (rc) {
                 free_events();
                 return rc;
             }
         }
-        GADE_HIP(hipMemcpyAsync(m->h_ctl, m->d_ctl, sizeof(int) * CT_COUNT, hipMemcpyDeviceToHost, m->stream));
-        GADE_HIP(hipMemcpyAsync(it_h.data() + s, m->d_iters + s, sizeof(int) * W, hipMemcpyDeviceToHost, m->stream));
-        GADE_HIP(hipMemcpyAsync(tol_h.data() + s, m->d_tols + s, sizeof(double) * W, hipMemcpyDeviceToHost,
-                                m->stream));
+        GADE_HIP(hipMemcpyAsync(m->h_gctl, m->d_gctl, sizeof(int) * GC_COUNT, hipMemcpyDeviceToHost, m->stream));
+        GADE_HIP(hipMemcpyAsync(it_h.data(), m->d_iters, sizeof(int) * B * cap, hipMemcpyDeviceToHost, m->stream));
+        GADE_HIP(hipMemcpyAsync(tol_h.data(), m->d_tols, sizeof(double) * B * cap, hipMemcpyDeviceToHost, m->stream));
         GADE_HIP(hipStreamSynchronize(m->stream));
         int done = s + W;
-        if (m->h_ctl[CT_STALLED]) {
-            done = m->h_ctl[CT_STALL_STEP];
-            // finish the stalled step on the host from the device state
-            GADE_TRY(read_scalars(m));
-            int it = 0;
-            double te = 0;
-            if (m->h_sc[SC_RHSSQ] == 0) {
-                GADE_HIP(hipMemsetAsync(psi, 0, sizeof(double2) * m->N, m->stream));
-            } else {
-                GADE_TRY(bicg_host_loop(m, m->tmp, psi, m->h_ctl[CT_ITER], tol, maxit, &it, &te, nullptr));
+        if (m->h_gctl[GC_STALLED]) {
+            done = m->h_gctl[GC_STALL_STEP];
+            if (done < s || done >= s + W) {
+                gade::set_error("device-driven stepping: inconsistent stall record");
+                return GADE_EHIP;
             }
-            it_h[done] = it;
-            tol_h[done] = te;
-            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, m->stream, m->d_ctl, m->d_sc, tol, maxit);
-            GADE_HIP(hipGetLastError());
+            GADE_TRY(finish_stalled_step(m, psi, done, K, it_h, tol_h));
+            GADE_HIP(hipStreamSynchronize(m->stream));
             done += 1;
         }
         int kmax = 0;
-        for (int w = s; w < done; w++) kmax = std::max(kmax, it_h[w] + 1);
+        for (int b = 0; b < B; b++)
+            for (int w = s; w < done; w++) kmax = std::max(kmax, it_h[(size_t)b * cap + w] + 1);
         m->last_k = std::min(std::max(kmax + 1, 2), 16);
         if (usec) {
             for (int w = 0; w < done - s; w++) {
@@ -1548,9 +1739,12 @@ int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, 
     }
     free_events();
     GADE_HIP(hipStreamSynchronize(m->stream));
-    for (int w = 0; w < nsteps; w++) {
-        if (iters) iters[w] = it_h[w];
-        if (tol_out) tol_out[w] = tol_h[w];
+    for (int b = 0; b < B; b++) {
+        const int nb = nsteps_b ? std::min(nsteps_b[b], nsteps) : nsteps;
+        for (int w = 0; w < nb; w++) {
+            if (iters) iters[(size_t)b * nsteps + w] = it_h[(size_t)b * cap + w];
+            if (tol_out) tol_out[(size_t)b * nsteps + w] = tol_h[(size_t)b * cap + w];
+        }
     }
     return GADE_OK;
 }
@@ -1576,7 +1770,8 @@ int gade_reset_workspace(gade_model_t *m) {
     if (!m) return GADE_EINVAL;
     double2 *ws[] = {m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t};
     for (auto w : ws) GADE_HIP(hipMemsetAsync(w, 0, sizeof(double2) * m->N, m->stream));
-    GADE_HIP(hipMemsetAsync(m->d_ctl + CT_HIST, 0, sizeof(int), m->stream));  // the guess history restarts
+    hipLaunchKernelGGL(k_hist_reset, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl);  // the guess histories restart
+    GADE_HIP(hipGetLastError());
     GADE_HIP(hipStreamSynchronize(m->stream));
     return GADE_OK;
 }
@@ -1616,9 +1811,17 @@ int gade_cn_run(gade_model_t *m, double *psi, double E0, double omega, double cy
         time = time + m->dt;
         times[step] = time;
     }
-    if (m->device_loop)
-        return cn_run_device(m, (double2 *)psi, times.data(), nsteps, E0, omega, cycles, tol, maxit, iters, tol_out,
-                             usec);
+    if (m->device_loop || m->B > 1) {
+        std::vector<double> params((size_t)4 * m->B);
+        for (int b = 0; b < m->B; b++) {
+            params[(size_t)4 * b] = E0;
+            params[(size_t)4 * b + 1] = omega;
+            params[(size_t)4 * b + 2] = cycles;
+            params[(size_t)4 * b + 3] = 0.0;
+        }
+        return cn_run_device(m, (double2 *)psi, times.data(), nsteps, params.data(), nullptr, tol, maxit, iters,
+                             tol_out, usec);
+    }
     hipEvent_t ea, eb;
     GADE_HIP(hipEventCreate(&ea));
     GADE_HIP(hipEventCreate(&eb));
@@ -1649,6 +1852,24 @@ int gade_cn_run(gade_model_t *m, double *psi, double E0, double omega, double cy
     return GADE_OK;
 }
 
+int gade_model_batch(const gade_model_t *m) { return m ? m->B : 0; }
+
+int gade_cn_run_batch(gade_model_t *m, double *psi, const double *params, const int *nsteps_b, double t0, int nsteps,
+                      double tol, int maxit, int *iters, double *tol_out) {
+    if (!m || !psi || !params || nsteps < 0) return GADE_EINVAL;
+    if (nsteps_b)
+        for (int b = 0; b < m->B; b++)
+            if (nsteps_b[b] < 0) return GADE_EINVAL;
+    std::vector<double> times(nsteps);
+    double time = t0;
+    for (int step = 0; step < nsteps; step++) {
+        time = time + m->dt;  // tools/dirac_solver.c:509
+        times[step] = time;
+    }
+    return cn_run_device(m, (double2 *)psi, times.data(), nsteps, params, nsteps_b, tol, maxit, iters, tol_out,
+                         nullptr);
+}
+
 int gade_snorm(gade_model_t *m, const double *x, double *norm, double *pop_host) {
     if (!m || !x) return GADE_EINVAL;
     double *d_pop = nullptr;
@@ -1676,6 +1897,10 @@ int gade_snorm(gade_model_t *m, const double *x, double *norm, double *pop_host)
 int gade_assembled_setup(gade_model_t *m) {
     if (!m) return GADE_EINVAL;
     if (m->asm_ready) return GADE_OK;
+    if (m->B != 1) {
+        gade::set_error("the assembled formulation serves one instance");
+        return GADE_EINVAL;
+    }
     m->nnz_full = (long long)m->hnnz * m->nnzb;
     if (m->nnz_full > 2147483647LL) {
         gade::set_error("assembled Hfull exceeds int32 indexing (csr_index_t, tools/types.h:37)");
@@ -1862,6 +2087,10 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
             return GADE_EINVAL;
         }
         const int kind = value == 0 ? (m->dia_ok ? 2 : 1) : value;
+        if (kind == 1 && m->B > 1) {
+            gade::set_error("batched models need the banded radial kernel");
+            return GADE_EINVAL;
+        }
         if (kind != m->radial_kind) {
             m->radial_kind = kind;
             GADE_TRY(launch_gsums(m, kind != 2));  // coefficients of the current f(t) in the new layout
@@ -2044,16 +2273,22 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
     return GADE_OK;
 }
 
-int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream, const int *gate) {
+int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream, const int *gate,
+                        int gstride, int nblk1) {
     if (!lu) {
         gade::set_error("preconditioner not set up");
+        return GADE_EINVAL;
+    }
+    if (lu->ch != 16 && gstride != 0) {
+        gade::set_error("gated (batched) preconditioner solves need 16-row levels (radial half-bandwidth < 16)");
         return GADE_EINVAL;
     }
     if (lu->ch == 16) {
         // one block per wave, four lanes per row
         const size_t shq = sizeof(double2) * ((size_t)lu->nch * 16 + 16);
         hipLaunchKernelGGL(k_psolve_q<4>, dim3(lu->norder), dim3(64), shq, stream, b, x, lu->d_order, lu->d_blk_q, lu->ilv,
-                           lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch, gate);
+                           lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch, gate, gstride,
+                           nblk1);
         GADE_HIP(hipGetLastError());
         return GADE_OK;
     }

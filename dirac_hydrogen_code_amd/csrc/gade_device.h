@@ -43,25 +43,34 @@ __device__ __forceinline__ double2 ldnt(const double2 *p) {
 
 // ---------------------------------------------------------------------------
 // gated launches: a kernel given a non-null 'gate' returns at once when the
-// device word it points at is non-zero.  The device-driven CN stepping uses
-// this to enqueue a step's iterations ahead of knowing how many it needs.
+// device word it points at is non-zero.  With several instances (a batch of
+// pulses sharing the couplings) each instance has its own words, 'gstride'
+// ints apart, and a workgroup checks the word of the instance it serves.
+// The device-driven CN stepping uses this to enqueue a step's iterations
+// ahead of knowing how many each instance needs.
 // ---------------------------------------------------------------------------
-#define GADE_GATED(gate)                          \
-    do {                                          \
-        if ((gate) != nullptr && *(gate) != 0) return; \
+__device__ __forceinline__ bool gated(const int *gate, int gstride, int b) {
+    return gate != nullptr && gate[(size_t)b * gstride] != 0;
+}
+#define GADE_GATED_B(gate, gstride, b)             \
+    do {                                           \
+        if (gated((gate), (gstride), (b))) return; \
     } while (0)
+#define GADE_GATED(gate) GADE_GATED_B(gate, 0, 0)
 
-// control words of the device-driven solve (int, in d_ctl)
+// per-instance control words of the device-driven solve (int, CT_STRIDE apart)
 enum {
-    CT_STATUS = 0,   // 0 iterating, 1 converged, 2 needs the host loop, 3 maxit reached
-    CT_ITER = 1,     // completed iterations of the current solve
-    CT_STALLED = 2,  // a step ended unfinished: every later gated launch is void
-    CT_SKIP = 3,     // CT_STATUS != 0 or CT_STALLED: the iteration gate
+    CT_STATUS = 0,  // 0 iterating, 1 converged, 2 needs the host (restart, zero rhs, non-finite), 3 maxit reached
+    CT_ITER = 1,    // completed iterations of the current solve
+    CT_SKIP = 2,    // iteration gate: set once the solve is decided, the instance is idle or the run stalled
+    CT_PSKIP = 3,   // step-prologue gate: the instance is idle this step or the run stalled
+    CT_HSEL = 4,    // gate of work the host enqueues for selected instances (0 = selected)
     CT_MAXIT = 5,
-    CT_STALL_STEP = 6,
-    CT_HIST = 7,     // previous solutions held in the history ring
-    CT_COUNT = 8
+    CT_HIST = 6,    // previous solutions held in the guess history
+    CT_STRIDE = 8
 };
+// run-wide words (int, in the global control array)
+enum { GC_STALLED = 0, GC_STALL_STEP = 1, GC_COUNT = 4 };
 
 constexpr int kMaxGuessOrder = 3;  // polynomial extrapolation order of the initial guess
 constexpr int kHistSlots = kMaxGuessOrder;
@@ -70,10 +79,15 @@ constexpr int kHistSlots = kMaxGuessOrder;
 // x0 = sum_{k=0..p} c_k psi_{n-k}, c_k = (-1)^k binom(p+1, k+1) (Lagrange at
 // t_{n+1} through equally spaced t_n..t_{n-p}), with p = min(order, history).
 // psi holds psi_n on entry and x0 on exit; psi_n enters the history ring.
-[[maybe_unused]] static __global__ __launch_bounds__(256) void k_extrap(long long N, double2 *__restrict__ psi, double2 *__restrict__ hist,
-                                                int order, int *ctl, const int *__restrict__ gate) {
-    GADE_GATED(gate);
-    const int have = ctl[CT_HIST];
+// blockIdx.y = instance; hist is [slot][total vector].
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_extrap(long long N1, long long Ntot,
+                                                                       double2 *__restrict__ psi,
+                                                                       double2 *__restrict__ hist, int order,
+                                                                       const int *__restrict__ ctl, const int *gate,
+                                                                       int gstride) {
+    const int b = blockIdx.y;
+    GADE_GATED_B(gate, gstride, b);
+    const int have = ctl[(size_t)b * CT_STRIDE + CT_HIST];
     const int p = order < have ? order : have;
     // ring slot of psi_{n-k} before the push: (have - k) mod kHistSlots; psi_n goes to slot have mod kHistSlots
     const int slot0 = have % kHistSlots;
@@ -86,30 +100,33 @@ constexpr int kHistSlots = kMaxGuessOrder;
             bnm = bnm * (p - k) / (k + 2);
         }
     }
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)b * N1 + j;
         const double2 xn = psi[i];
         double2 x = cscale(xn, c[0]);
 #pragma unroll
         for (int k = 1; k <= kMaxGuessOrder; k++) {
             if (k > p) break;
             const int sl = ((slot0 - k) % kHistSlots + kHistSlots) % kHistSlots;
-            x = cadd(x, cscale(hist[(size_t)sl * N + i], c[k]));
+            x = cadd(x, cscale(hist[(size_t)sl * Ntot + i], c[k]));
         }
-        hist[(size_t)slot0 * N + i] = xn;
+        hist[(size_t)slot0 * Ntot + i] = xn;
         psi[i] = x;
     }
 }
 
-// history count update after k_extrap (one thread, ordered after it on the stream)
-[[maybe_unused]] static __global__ void k_hist_count(int *ctl, const int *__restrict__ gate) {
-    GADE_GATED(gate);
-    if (threadIdx.x == 0) ctl[CT_HIST] = ctl[CT_HIST] + 1;
+// history count update after k_extrap (one thread per instance, ordered after it on the stream)
+[[maybe_unused]] static __global__ void k_hist_count(int *ctl, const int *gate, int gstride) {
+    const int b = blockIdx.x;
+    GADE_GATED_B(gate, gstride, b);
+    if (threadIdx.x == 0) ctl[(size_t)b * CT_STRIDE + CT_HIST] += 1;
 }
 
 // ---------------------------------------------------------------------------
-// BiCGSTAB vector kernels with deterministic last-block reductions
+// BiCGSTAB vector kernels with deterministic two-kernel reductions, one
+// instance per blockIdx.y (k_bicg) or blockIdx.x (k_bicg_fin)
 // ---------------------------------------------------------------------------
-// scalar slots in d_sc
+// scalar slots in d_sc (doubles, SC_STRIDE per instance)
 enum {
     SC_RHO = 0,        // 2
     SC_ALPHA = 2,      // 2
@@ -120,7 +137,8 @@ enum {
     SC_RHONEXT = 9,    // 2
     SC_TOL = 11,       // requested relative tolerance (device-driven solve)
     SC_TOL2 = 12,      // tol^2 |rhs|^2
-    SC_COUNT = 16
+    SC_COUNT = 16,
+    SC_STRIDE = 16
 };
 constexpr double kRestartEps2 = 1e-24;  // restart when |rho| < 1e-24 |r0|^2 (tools/gpu_bicgstab.c:195)
 enum RedOp { OP_INIT = 0, OP_RESTART, OP_P, OP_ALPHA, OP_S, OP_W, OP_XR };
@@ -155,28 +173,23 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh) {
     __syncthreads();
 }
 
+// One BiCGSTAB vector step over the N1 entries of instance blockIdx.y (vectors
+// are instance-major, N1 apart); partial sums to part[instance][block][4].
 template <int OP>
-__global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2 *__restrict__ b, double2 *__restrict__ r,
+__global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const double2 *__restrict__ b, double2 *__restrict__ r,
                                                        double2 *__restrict__ r0, double2 *__restrict__ p,
                                                        const double2 *__restrict__ v, double2 *__restrict__ s,
                                                        const double2 *__restrict__ t, const double2 *__restrict__ yv,
                                                        const double2 *__restrict__ zv, double2 *__restrict__ x,
-                                                       const double *__restrict__ sc, double *__restrict__ part,
-                                                       const int *gate = nullptr, int *ctl = nullptr) {
-    GADE_GATED(gate);
+                                                       const double *__restrict__ sc0, double *__restrict__ part0,
+                                                       const int *gate = nullptr, int gstride = 0) {
+    const int inst = blockIdx.y;
+    GADE_GATED_B(gate, gstride, inst);
     __shared__ double sh[4 * 16];
+    const double *sc = sc0 + (size_t)inst * SC_STRIDE;
+    double *part = part0 + ((size_t)inst * gridDim.x) * 4;
+    const long long o = (long long)inst * N1;
     double acc[4] = {0, 0, 0, 0};
-    if (OP == OP_P && ctl != nullptr) {
-        // top of an iteration of the device-driven solve: a restart is left to the host loop
-        const double rn = hypot(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
-        if (rn < kRestartEps2 * sc[SC_R0SQ]) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
-                ctl[CT_STATUS] = 2;
-                ctl[CT_SKIP] = 1;
-            }
-            return;
-        }
-    }
     double2 beta = make_double2(0, 0), w = make_double2(0, 0), alpha = make_double2(0, 0);
     if (OP == OP_P) {
         // beta = (rho_new / rho_old) * (alpha / w)
@@ -188,7 +201,8 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2
     }
     if (OP == OP_S || OP == OP_XR) alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
     if (OP == OP_XR) w = make_double2(sc[SC_W], sc[SC_W + 1]);
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = o + j;
         if (OP == OP_INIT || OP == OP_RESTART) {
             double2 ri = csub(b[i], r[i]);
             r[i] = ri;
@@ -231,12 +245,17 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N, const double2
         for (int i = 0; i < 4; i++) part[(size_t)blockIdx.x * 4 + i] = acc[i];
 }
 
-// Sums the partials in a fixed order and updates the scalars (one block).
+// Sums the partials of instance blockIdx.x in a fixed order and updates its
+// scalars; with ctl, also takes the host loop's decisions for it.
 template <int OP>
-__global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const double *__restrict__ part, double *sc,
-                                                           const int *gate = nullptr, int *ctl = nullptr) {
-    GADE_GATED(gate);
+__global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const double *__restrict__ part0, double *sc0,
+                                                           const int *gate = nullptr, int gstride = 0,
+                                                           int *ctl0 = nullptr) {
+    const int inst = blockIdx.x;
+    GADE_GATED_B(gate, gstride, inst);
     __shared__ double sh[4 * 16];
+    const double *part = part0 + (size_t)inst * nparts * 4;
+    double *sc = sc0 + (size_t)inst * SC_STRIDE;
     double tot[4] = {0, 0, 0, 0};
     for (int bidx = threadIdx.x; bidx < nparts; bidx += blockDim.x)
         for (int i = 0; i < 4; i++) tot[i] += part[(size_t)bidx * 4 + i];
@@ -273,18 +292,20 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
         sc[SC_RHONEXT] = tot[1];
         sc[SC_RHONEXT + 1] = tot[2];
     }
-    if (ctl == nullptr) return;
+    if (ctl0 == nullptr) return;
+    int *ctl = ctl0 + (size_t)inst * CT_STRIDE;
     // device-driven solve: the same decisions as the host loop (break-before-increment)
+    int st = -1;
     if (OP == OP_INIT) {
         const double tol = sc[SC_TOL];
         const double tol2 = tol * tol * tot[1];
         sc[SC_TOL2] = tol2;
-        const int st = (tot[1] == 0) ? 2 : (tot[0] < tol2 ? 1 : 0);
-        ctl[CT_STATUS] = st;
+        st = (tot[1] == 0) ? 2 : (tot[0] < tol2 ? 1 : 0);
         ctl[CT_ITER] = 0;
-        ctl[CT_SKIP] = st != 0;
+    } else if (OP == OP_RESTART) {
+        st = 0;
     } else if (OP == OP_XR) {
-        int st = 0;
+        st = 0;
         if (!isfinite(tot[0])) {
             st = 2;
         } else if (tot[0] < sc[SC_TOL2]) {
@@ -293,25 +314,69 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
             const int it = ctl[CT_ITER] + 1;
             ctl[CT_ITER] = it;
             if (it >= ctl[CT_MAXIT]) st = 3;
+            // the top of the next iteration restarts when |rho| < eps^2 |r0|^2: left to the host
+            else if (hypot(tot[1], tot[2]) < kRestartEps2 * sc[SC_R0SQ]) st = 2;
         }
+    }
+    if (st >= 0) {
         ctl[CT_STATUS] = st;
         ctl[CT_SKIP] = st != 0;
     }
 }
 
-// End of a device-driven CN step: record iterations and tolerance of a
-// finished solve, or mark the run stalled at this step so the host continues it.
-[[maybe_unused]] static __global__ void k_step_end(int *ctl, const double *__restrict__ sc, int step, int *iters, double *tols) {
-    if (threadIdx.x != 0 || ctl[CT_STALLED] != 0) return;
+// Start of a device-driven CN step for instance blockIdx.x: idle when the step
+// is past its pulse or the run has stalled.
+[[maybe_unused]] static __global__ void k_step_begin(int *ctl0, const int *__restrict__ gctl, int step,
+                                                     const int *__restrict__ nsteps) {
+    if (threadIdx.x != 0) return;
+    const int b = blockIdx.x;
+    int *ctl = ctl0 + (size_t)b * CT_STRIDE;
+    const int idle = gctl[GC_STALLED] != 0 || (nsteps != nullptr && step >= nsteps[b]);
+    ctl[CT_PSKIP] = idle;
+    if (idle) ctl[CT_SKIP] = 1;
+}
+
+// End of a device-driven CN step: record iterations and tolerance of each
+// finished solve, or mark the run stalled at this step (any instance unfinished)
+// so the host continues it.
+[[maybe_unused]] static __global__ void k_step_end(int *ctl0, int *gctl, const double *__restrict__ sc0, int step,
+                                                   int nsteps_cap, int *iters, double *tols) {
+    if (threadIdx.x != 0) return;
+    const int b = blockIdx.x;
+    const int *ctl = ctl0 + (size_t)b * CT_STRIDE;
+    const double *sc = sc0 + (size_t)b * SC_STRIDE;
+    if (ctl[CT_PSKIP] != 0 || gctl[GC_STALLED] != 0) return;
     const int st = ctl[CT_STATUS];
     if (st == 1 || st == 3) {
-        iters[step] = ctl[CT_ITER];
-        tols[step] = sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]);
+        iters[(size_t)b * nsteps_cap + step] = ctl[CT_ITER];
+        tols[(size_t)b * nsteps_cap + step] = sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]);
     } else {
-        ctl[CT_STALLED] = 1;
-        ctl[CT_SKIP] = 1;
-        ctl[CT_STALL_STEP] = step;
+        gctl[GC_STALLED] = 1;  // every unfinished instance writes the same values
+        gctl[GC_STALL_STEP] = step;
     }
+}
+
+// End of the extra iterations the host enqueues for a stalled step: record every decided solve.
+[[maybe_unused]] static __global__ void k_step_end_resume(int *ctl0, int *gctl, const double *__restrict__ sc0,
+                                                          int step, int nsteps_cap, int *iters, double *tols) {
+    if (threadIdx.x != 0) return;
+    const int b = blockIdx.x;
+    const int *ctl = ctl0 + (size_t)b * CT_STRIDE;
+    const double *sc = sc0 + (size_t)b * SC_STRIDE;
+    const int st = ctl[CT_STATUS];
+    if (st == 1 || st == 3) {
+        iters[(size_t)b * nsteps_cap + step] = ctl[CT_ITER];
+        tols[(size_t)b * nsteps_cap + step] = sc[SC_RHSSQ] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]) : 0.0;
+    }
+    (void)gctl;
+}
+
+// After k_step_end: once the run has stalled, every instance idles.
+[[maybe_unused]] static __global__ void k_stall_spread(int *ctl0, const int *__restrict__ gctl) {
+    if (threadIdx.x != 0 || gctl[GC_STALLED] == 0) return;
+    int *ctl = ctl0 + (size_t)blockIdx.x * CT_STRIDE;
+    ctl[CT_SKIP] = 1;
+    ctl[CT_PSKIP] = 1;
 }
 
 // y = alpha A x + beta y for a CSR matrix: one wave per row, streaming loads.
@@ -462,13 +527,13 @@ __global__ __launch_bounds__(64) void k_psolve_q(const double2 *__restrict__ b, 
                                                  int ilv, const double2 *__restrict__ Linv,
                                                  const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
                                                  const double2 *__restrict__ Uoff, int n, int band, int nch,
-                                                 const int *__restrict__ gate = nullptr) {
-    GADE_GATED(gate);
+                                                 const int *gate = nullptr, int gstride = 0, int nblk1 = 1) {
     constexpr int CH = 16;
     extern __shared__ double2 smem[];
     const int lane = threadIdx.x, i = lane >> 2, k = lane & 3;
     const int r = order[blockIdx.x];
     if (r < 0) return;  // padding slot (whole workgroup)
+    GADE_GATED_B(gate, gstride, r / nblk1);
     const int q = blk_q[r];
     double2 *yv = smem;              // [nch*CH]: b, then y, then x (interleaved order)
     double2 *acc = smem + nch * CH;  // [CH]

@@ -61,22 +61,25 @@ namespace gade {
 int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
                   const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out);
 int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream,
-                        const int *gate = nullptr);
+                        const int *gate = nullptr, int gstride = 0, int nblk1 = 1);
 void blocklu_free(gade_blocklu *lu);
 int hip_alloc(void **p, size_t bytes);
 }  // namespace gade
 
 struct gade_model {
-    // sizes
+    // sizes.  A model holds B instances (pulses) that share the couplings: rows are
+    // instance-major (row b*nblk1 + r), nblk = B*nblk1 rows, N = B*N1 entries.
     int nblk = 0, n = 0, lmax = 0, nnzb = 0, ellw = 0, hnnz = 0, nq = 0;
     long long N = 0;
+    int B = 1, nblk1 = 0;
+    long long N1 = 0;
     double h = 1, dt = 0;
     hipStream_t stream = nullptr;
 
     // host-side structure kept for setup
     std::vector<int> rad_Ap, rad_Ai, hall_Ap, hall_Ai, kap_blk, kap_q;  // kap_q: kappa of each distinct index
     std::vector<int> blk_q;   // distinct-kappa index of each block
-    std::vector<double> ft;   // last f(t), 12 doubles
+    std::vector<double> ft;   // last f(t) of every instance, [B][12] doubles
 
     // radial pattern (ELL, transposed: [slot][row])
     int *d_ell_col = nullptr, *d_ell_src = nullptr, *d_tmap = nullptr, *d_ell_pos = nullptr;
@@ -103,10 +106,12 @@ struct gade_model {
     int nh = 0, nhp = 0, nd = 0, nseg = 0, ndtiles = 0;
     int *d_dia_src = nullptr;      // [4][nd][nhp] radial CSR index or -1
     double2 *d_gs = nullptr, *d_gts = nullptr;  // [lmax][3][4][4][nd][nhp] DIA-ordered couplings
-    double2 *d_GD = nullptr;       // [lmax][4][4][nd][nhp][2] kappa-folded G-sum pairs
+    double2 *d_GD = nullptr;       // [B][lmax][4][4][nd][nhp][2] kappa-folded G-sum pairs
     double2 *d_DD = nullptr;       // [3][nq][4][nd][nhp] diagonal operators
     int *d_dtiles = nullptr, *d_dtile_q = nullptr;
     double *d_dtile_k = nullptr;
+    int *d_dtile_b = nullptr;      // instance of each tile (tiles never mix instances)
+    size_t gd_stride = 0;          // GD entries per instance
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     double2 *d_ypart = nullptr;    // [lmax][nblk][n] per-multipole partial outputs of the radial phase
     // preconditioner
@@ -115,11 +120,11 @@ struct gade_model {
     // BiCGSTAB workspace (persists across calls, tools/gpu_bicgstab.c:150-167)
     double2 *r = nullptr, *r0 = nullptr, *v = nullptr, *p = nullptr, *y = nullptr, *z = nullptr,
             *s = nullptr, *t = nullptr, *tmp = nullptr;
-    double *d_sc = nullptr;        // device scalars
-    double *d_part = nullptr;      // reduction partials
+    double *d_sc = nullptr;        // device scalars [B][SC_STRIDE]
+    double *d_part = nullptr;      // reduction partials [B][red_blocks][4]
     unsigned *d_cnt = nullptr;     // last-block ticket
     double *h_sc = nullptr;        // pinned host mirror
-    int red_blocks = 0;
+    int red_blocks = 0;            // reduction blocks per instance
     // assembled reference formulation
     bool asm_ready = false;
     long long nnz_full = 0;
@@ -132,19 +137,23 @@ struct gade_model {
     double2 *d_evec = nullptr;
     double *d_eval = nullptr;
     // device-driven CN stepping: control words, per-step outputs, launch gate
-    int *d_ctl = nullptr;          // CT_* words
+    int *d_ctl = nullptr;          // [B][CT_STRIDE] per-instance words
     int *h_ctl = nullptr;          // pinned mirror
-    int *d_iters = nullptr;        // [steps_cap]
-    double *d_tols = nullptr;      // [steps_cap]
+    int *d_gctl = nullptr;         // [GC_COUNT] run-wide words
+    int *h_gctl = nullptr;         // pinned mirror
+    int *d_nsteps = nullptr;       // [B] steps of each instance's pulse
+    int *d_iters = nullptr;        // [B][steps_cap]
+    double *d_tols = nullptr;      // [B][steps_cap]
     int steps_cap = 0;
     const int *gate = nullptr;     // gate passed to every model launch (null: ungated)
+    int gstride = 0;               // ints between the gate words of consecutive instances
     bool ctl_on = false;           // reductions update d_ctl
     bool device_loop = true;       // CN steps driven on the device (false: host loop per iteration)
     int window = 8;                // steps enqueued per host synchronization
     int iter_batch = 0;            // iterations enqueued per step (0: adaptive)
     int last_k = 2;                // adaptive iterations per step
     int guess_order = 0;           // initial guess: 0 psi_n (the reference), p: extrapolation of order p
-    double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions
+    double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions (per instance)
     // profiling: event pairs are recorded without synchronizing and resolved
     // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families
     int profile = 0;

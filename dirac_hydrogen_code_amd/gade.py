@@ -24,7 +24,8 @@ EXPORTS = (
     "gade_cn_step", "gade_cn_run", "gade_pulse", "gade_ik", "gade_imu", "gade_snorm", "gade_assembled_setup",
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
     "gade_set_param", "gade_get_param", "gade_eig_setup", "gade_eig_values", "gade_populations",
-    "gade_extrapolate",
+    "gade_extrapolate", "gade_model_create_batch", "gade_model_load_batch", "gade_model_batch", "gade_cn_run_batch",
+    "gade_set_time_batch", "gade_pulse_cep",
 )
 
 _lib = None
@@ -45,7 +46,9 @@ def lib():
         L.gade_last_error.restype = C.c_char_p
         sig = {
             "gade_malloc": [C.POINTER(vp), sz], "gade_free": [vp], "gade_memcpy_h2d": [vp, vp, sz],
-            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_extrapolate": [vp, vp, i32], "gade_eig_values": [vp, vp, vp, vp],
+            "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_extrapolate": [vp, vp, i32],
+            "gade_model_load_batch": [C.c_char_p, i32, i32, f64, f64, vp, vp],
+            "gade_cn_run_batch": [vp, vp, vp, vp, f64, i32, f64, i32, vp, vp], "gade_set_time_batch": [vp, vp], "gade_eig_values": [vp, vp, vp, vp],
             "gade_populations": [vp, vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), vp], "gade_device_sync": [],
             "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
             "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -68,6 +71,8 @@ def lib():
             fn.restype = C.c_int
         L.gade_pulse.argtypes = [f64, f64, f64, f64, vp]
         L.gade_pulse.restype = None
+        L.gade_pulse_cep.argtypes = [f64, f64, f64, f64, f64, vp]
+        L.gade_pulse_cep.restype = None
         L.gade_ik.argtypes = [i32]
         L.gade_ik.restype = i32
         L.gade_imu.argtypes = [i32]
@@ -79,6 +84,13 @@ def lib():
 def check(rc):
     if rc != 0:
         raise GadeError(f"libgade error {rc}: {lib().gade_last_error().decode()}")
+
+
+def pulse_cep(E0, omega, cycles, cep, t):
+    """f_alpha(t) with carrier phase -pi/2 + cep as complex128[6]."""
+    out = np.zeros(6, np.complex128)
+    lib().gade_pulse_cep(E0, omega, cycles, cep, t, out.ctypes.data)
+    return out
 
 
 def pulse(E0, omega, cycles, t):
@@ -138,11 +150,12 @@ class DeviceArray:
 class Propagator:
     """Device model of one coupling directory: fused A-apply, block-LU, BiCGSTAB, CN."""
 
-    def __init__(self, directory, lmax, dt, h=1.0):
+    def __init__(self, directory, lmax, dt, h=1.0, batch=1):
         self.directory = directory
         p = C.c_void_p()
-        check(lib().gade_model_load(directory.encode(), lmax, h, dt, None, C.byref(p)))
+        check(lib().gade_model_load_batch(directory.encode(), lmax, batch, h, dt, None, C.byref(p)))
         self.ptr = p.value
+        self.batch = batch
         nb, bd, hn, nz, nk = (C.c_int() for _ in range(5))
         nf = C.c_longlong()
         check(lib().gade_model_info(self.ptr, C.byref(nb), C.byref(bd), C.byref(hn), C.byref(nz), C.byref(nf),
@@ -163,6 +176,21 @@ class Propagator:
             self.close()
         except Exception:
             pass
+
+    def cn_run_batch(self, psi: DeviceArray, params, nsteps, nsteps_b=None, t0=0.0, tol=1e-12, maxit=500):
+        """Batched pulse scan: params [batch][4] = (E0, omega, cycles, cep); psi holds batch*n entries.
+        Returns iters, tols of shape [batch][nsteps] (entries past an instance's own count stay 0)."""
+        params = np.ascontiguousarray(params, np.float64).reshape(self.batch, 4)
+        iters = np.zeros((self.batch, nsteps), np.int32)
+        tols = np.zeros((self.batch, nsteps), np.float64)
+        nb = None if nsteps_b is None else np.ascontiguousarray(nsteps_b, np.int32)
+        check(lib().gade_cn_run_batch(self.ptr, psi.ptr, params.ctypes.data, None if nb is None else nb.ctypes.data,
+                                      t0, nsteps, tol, maxit, iters.ctypes.data, tols.ctypes.data))
+        return iters, tols
+
+    def set_time_batch(self, ft):
+        ft = np.ascontiguousarray(ft, np.complex128).reshape(self.batch, 6)
+        check(lib().gade_set_time_batch(self.ptr, ft.ctypes.data))
 
     def set_time(self, ft):
         ft = np.ascontiguousarray(ft, np.complex128)

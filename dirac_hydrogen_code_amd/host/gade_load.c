@@ -18,6 +18,11 @@ static int same_pattern(const sparse_csr_t *a, const sparse_csr_t *b)
 
 int gade_model_load(const char *dir, int lmax, double h, double dt, void *stream, gade_model_t **out)
 {
+    return gade_model_load_batch(dir, lmax, 1, h, dt, stream, out);
+}
+
+int gade_model_load_batch(const char *dir, int lmax, int batch, double h, double dt, void *stream, gade_model_t **out)
+{
     char fname[4096];
     int rc = GADE_EIO, ng = 24 * lmax;
     if (!dir || lmax <= 0 || !out) return GADE_EINVAL;
@@ -88,7 +93,7 @@ int gade_model_load(const char *dir, int lmax, double h, double dt, void *stream
         in.g = gx;
         in.h0 = hx;
         in.s = sx;
-        rc = gade_model_create(&in, h, dt, stream, out);
+        rc = gade_model_create_batch(&in, batch, h, dt, stream, out);
     }
 done:
     csr_free(&Hall);

@@ -53,6 +53,26 @@ void beoyndDipolePulse_axialPart(beyondDipolePulse_t *pulse, double t, cdouble_t
 int gade_ik(int i) { return ik(i); }
 double gade_imu(int i) { return imu(i); }
 
+void gade_pulse_cep(double E0, double omega, double cycles, double cep, double t, double *ft)
+{
+    /* the reference's pulse with carrier phase -pi/2 + cep (cep = 0 is tools/solver_common.c:46-65) */
+    const double T = cycles * 2 * M_PI / omega, phi = -M_PI / 2 + cep;
+    if (cep == 0.0) {
+        gade_pulse(E0, omega, cycles, t, ft);
+        return;
+    }
+    const double f[6] = {-E0 / (4 * omega) * cos(phi + t * (2 * M_PI / T + omega)),
+                         -E0 / (4 * omega) * sin(phi + t * (2 * M_PI / T + omega)),
+                         -E0 / (4 * omega) * cos(-phi + t * (2 * M_PI / T - omega)),
+                         E0 / (4 * omega) * sin(-phi + t * (2 * M_PI / T - omega)),
+                         E0 / (2 * omega) * cos(phi + t * omega),
+                         E0 / (2 * omega) * sin(phi + t * omega)};
+    for (int a = 0; a < 6; a++) {
+        ft[2 * a] = f[a];
+        ft[2 * a + 1] = 0.0;
+    }
+}
+
 void gade_pulse(double E0, double omega, double cycles, double t, double *ft)
 {
     beyondDipolePulse_t p;

@@ -74,6 +74,12 @@ int gade_device_sync(void);
  * per-kappa diagonal operators S(k) +- i h dt/2 H0(k) (tools/solver.c:409-458).
  * stream: a hipStream_t or NULL. */
 int gade_model_create(const gade_inputs_t *in, double h, double dt, void *stream, gade_model_t **out);
+/* A batch of B instances (pulse scans) sharing the couplings, S and H0: vectors are [B][nblk*blkdim]
+ * (instance-major); each instance has its own f(t), BiCGSTAB scalars and convergence state.  B = 1 is
+ * gade_model_create.  Batched models need the banded radial pattern. */
+int gade_model_create_batch(const gade_inputs_t *in, int batch, double h, double dt, void *stream, gade_model_t **out);
+int gade_model_load_batch(const char *dir, int lmax, int batch, double h, double dt, void *stream, gade_model_t **out);
+int gade_model_batch(const gade_model_t *m);
 /* Same, reading H.csr, H0l*, H1l*, g*, h0*, s* from a directory
  * (tools/dirac_solver.c:223-342). */
 int gade_model_load(const char *dir, int lmax, double h, double dt, void *stream, gade_model_t **out);
@@ -85,6 +91,8 @@ int gade_model_info(const gade_model_t *m, int *nblk, int *blkdim, int *hall_nnz
 /* Per-step G-sums (tools/solver.c:236-287; tools/gpu_solver.cu:50-82) for
  * time coefficients ft[6] (complex, interleaved: 12 doubles). */
 int gade_set_time(gade_model_t *m, const double *ft);
+/* f(t) per instance of a batched model: ft [batch][12] */
+int gade_set_time_batch(gade_model_t *m, const double *ft);
 
 /* Fused operator, no assembly (tools/solver.c:327-395 evaluated matrix-free):
  *   y = (S + cH * Hfull(t)) x + beta * y     with cH in {+1, -1, 0}.
@@ -119,9 +127,19 @@ int gade_cn_step(gade_model_t *m, double *psi_dev, const double *ft, double tol,
 int gade_cn_run(gade_model_t *m, double *psi_dev, double E0, double omega, double cycles, double t0,
                 int nsteps, double tol, int maxit, int *iters, double *tol_out, double *usec);
 
+/* Batched pulse scan: nsteps CN steps of every instance of a batched model from t0 (one dt for all).
+ * params [batch][4] = (E0, omega, cycles, cep) of each instance's pulse; nsteps_b [batch] steps of each
+ * instance (NULL: all run nsteps; an instance idles after its own count, later entries untouched).
+ * psi_dev [batch][N1] holds x0 and the results; iters / tol_out [batch][nsteps] (NULL allowed).
+ * All instances advance in one set of launches per step; each converges on its own. */
+int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, const int *nsteps_b, double t0,
+                      int nsteps, double tol, int maxit, int *iters, double *tol_out);
+
 /* Pulse coefficients f_alpha(t), alpha = 0..5 (tools/solver_common.c:46-65),
  * as 12 doubles (complex with zero imaginary part). */
 void gade_pulse(double E0, double omega, double cycles, double t, double *ft);
+/* the same with carrier phase -pi/2 + cep (the reference fixes cep = 0, tools/solver_common.c:55) */
+void gade_pulse_cep(double E0, double omega, double cycles, double cep, double t, double *ft);
 int gade_ik(int i);       /* tools/solver_common.c:30-37 */
 double gade_imu(int i);   /* tools/solver_common.c:39-44 */
 
