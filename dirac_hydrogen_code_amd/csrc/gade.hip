@@ -430,12 +430,16 @@ __device__ __forceinline__ int dia_swz(int p) { return p ^ (((p >> 3) ^ (p >> 4)
 // GD[l][v][qd][d][ip] = (G_m1, G_m2) of the kappa-folded pair of v, from the
 // DIA-ordered couplings gs[l][j][nn][qd][d][ip] (g of alpha = 2j + (l even))
 // and gts (conjugated transposes).  One thread per (l, qd, d, ip).
+// Coefficients: by value (cg, instances b0 + blockIdx.y), or from the device window table
+// cf_dev[slot][B] at slot gctl[GC_WSLOT] (the form a replayed step graph uses).
 __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ gs, const double2 *__restrict__ gts,
                                                    CoefGroup cg, int b0, size_t gd_stride, int lmax, int nd, int nhp,
-                                                   double2 *__restrict__ GD0, const int *gate, int gstride) {
+                                                   double2 *__restrict__ GD0, const int *gate, int gstride,
+                                                   const Coef6 *__restrict__ cf_dev, const int *__restrict__ gctl,
+                                                   int B) {
     const int b = b0 + blockIdx.y;
     GADE_GATED_B(gate, gstride, b);
-    const Coef6 &cf = cg.c[blockIdx.y];
+    const Coef6 cf = cf_dev ? cf_dev[(size_t)gctl[GC_WSLOT] * B + b] : cg.c[blockIdx.y];
     double2 *__restrict__ GD = GD0 + (size_t)b * gd_stride;
     const int S = 4 * nd * nhp;  // entries per (l, j, nn) plane
     const long long total = (long long)lmax * S;
@@ -830,6 +834,11 @@ void free_all(gade_model *m) {
     if (m->h_sc) hipHostFree(m->h_sc);
     if (m->h_ctl) hipHostFree(m->h_ctl);
     if (m->h_gctl) hipHostFree(m->h_gctl);
+    if (m->h_cfwin) hipHostFree(m->h_cfwin);
+    if (m->d_cfwin) hipFree(m->d_cfwin);
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
     for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval,
                     (void *)m->d_hist, (void *)m->d_gctl, (void *)m->d_nsteps, (void *)m->d_dtile_b})
         if (p) hipFree(p);
@@ -1000,7 +1009,8 @@ int launch_gsums(gade_model *m, bool ell) {
             CoefGroup cg{};
             for (int k = 0; k < nb; k++) cg.c[k] = coef_of(m, b0 + k);
             hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), nb), dim3(256), 0, m->stream, m->d_gs,
-                               m->d_gts, cg, b0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride);
+                               m->d_gts, cg, b0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride,
+                               (const Coef6 *)nullptr, (const int *)nullptr, m->B);
         }
     }
     GADE_HIP(hipGetLastError());
@@ -1589,6 +1599,81 @@ int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K,
     return enqueue_step_end(m, step);
 }
 
+// the G-sums of the current window slot from the device coefficient table
+int launch_gsums_window(gade_model *m) {
+    Prof pr(m, "gsums", GADE_PROF_GSUMS);
+    long long total = (long long)m->lmax * 4 * m->nd * m->nhp;
+    CoefGroup cg{};
+    hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), m->B), dim3(256), 0, m->stream, m->d_gs,
+                       m->d_gts, cg, 0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride,
+                       (const Coef6 *)m->d_cfwin, m->d_gctl, m->B);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// One CN step whose per-step inputs are on the device (the window's coefficient table and the run-wide
+// step words), so the same launches serve every step: the body of the step graph.
+int enqueue_step_window(gade_model *m, double2 *psi, int K, bool per_instance_steps) {
+    GateScope gs(m);
+    hipLaunchKernelGGL(k_step_begin, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, -1,
+                       per_instance_steps ? m->d_nsteps : nullptr);
+    GADE_HIP(hipGetLastError());
+    m->gate = m->d_ctl + CT_PSKIP;
+    GADE_TRY(launch_gsums_window(m));
+    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
+    GADE_TRY(launch_guess(m, psi));                    // x0 (psi_n unless extrapolating)
+    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0
+    GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
+    GADE_TRY(enqueue_iterations(m, psi, K));
+    hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, -1,
+                       m->steps_cap, m->d_iters, m->d_tols);
+    hipLaunchKernelGGL(k_stall_spread, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl);
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, m->stream, m->d_gctl);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// the step graph for this K (captured again when anything it bakes in has changed)
+int step_graph(gade_model *m, double2 *psi, int K, bool per_instance_steps) {
+    const void *key[6] = {psi, (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
+                          (const void *)(intptr_t)(per_instance_steps ? 1 : 0), m->d_iters,
+                          (const void *)(intptr_t)m->steps_cap};
+    if (m->graph_exec && !memcmp(key, m->graph_key, sizeof(key))) return GADE_OK;
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->graph_exec = nullptr;
+    m->graph = nullptr;
+    if (!m->cap_stream) GADE_HIP(hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking));
+    if (m->guess_order > 0 && !m->d_hist)  // no allocation while capturing
+        GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+    hipStream_t saved = m->stream;
+    m->stream = m->cap_stream;
+    GADE_HIP(hipStreamBeginCapture(m->cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step_window(m, psi, K, per_instance_steps);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(m->cap_stream, &g);
+    m->stream = saved;
+    GADE_TRY(rc);
+    GADE_HIP(e);
+    m->graph = g;
+    GADE_HIP(hipGraphInstantiate(&m->graph_exec, m->graph, nullptr, nullptr, 0));
+    memcpy(m->graph_key, key, sizeof(key));
+    return GADE_OK;
+}
+
+int ensure_cfwin(gade_model *m, int slots) {
+    if (slots <= m->cfwin_cap) return GADE_OK;
+    if (m->d_cfwin) (void)hipFree(m->d_cfwin);
+    if (m->h_cfwin) (void)hipHostFree(m->h_cfwin);
+    m->d_cfwin = m->h_cfwin = nullptr;
+    m->cfwin_cap = 0;
+    const size_t bytes = sizeof(Coef6) * (size_t)slots * m->B;
+    GADE_TRY(hip_alloc((void **)&m->d_cfwin, bytes));
+    GADE_HIP(hipHostMalloc((void **)&m->h_cfwin, bytes, hipHostMallocDefault));
+    m->cfwin_cap = slots;
+    return GADE_OK;
+}
+
 // The run stalled at 'step': finish it for every instance from the device state.
 // it_h/tol_h: [B][steps_cap] host copies of the per-step outputs.
 int finish_stalled_step(gade_model *m, double2 *psi, int step, int K, std::vector<int> &it_h,
@@ -1696,14 +1781,37 @@ int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, 
             for (auto &e : ev) GADE_HIP(hipEventCreate(&e));
             GADE_HIP(hipEventRecord(ev[0], m->stream));
         }
-        for (int w = 0; w < W; w++) {
+        // f(t) of every step of the window, [W][B][12]
+        ft.resize((size_t)12 * B * W);
+        for (int w = 0; w < W; w++)
             for (int b = 0; b < B; b++) {
                 const double *pp = params + (size_t)4 * b;
-                gade_pulse_cep(pp[0], pp[1], pp[2], pp[3], times[s + w], ft.data() + (size_t)12 * b);
+                gade_pulse_cep(pp[0], pp[1], pp[2], pp[3], times[s + w], ft.data() + (size_t)12 * (w * B + b));
             }
-            int rc = enqueue_step(m, psi, ft.data(), s + w, K, nsteps_b != nullptr);
+        // DIA: the window's pulse coefficients go to a device table that the steps index by the
+        // run-wide step words, so every step is the same launches (one graph); ELL: per-step launches
+        const bool table = m->radial_kind == 2;
+        if (table) {
+            GADE_TRY(ensure_cfwin(m, W));
+            Coef6 *tab = (Coef6 *)m->h_cfwin;
+            for (int w = 0; w < W; w++) {
+                std::copy(ft.data() + (size_t)12 * B * w, ft.data() + (size_t)12 * B * (w + 1), m->ft.begin());
+                for (int b = 0; b < B; b++) tab[(size_t)w * B + b] = coef_of(m, b);
+            }
+            GADE_HIP(hipMemcpyAsync(m->d_cfwin, m->h_cfwin, sizeof(Coef6) * (size_t)W * B, hipMemcpyHostToDevice,
+                                    m->stream));
+            hipLaunchKernelGGL(k_window_begin, dim3(1), dim3(64), 0, m->stream, m->d_gctl, s);
+            GADE_HIP(hipGetLastError());
+        }
+        const bool graph = table && m->use_graph && m->profile == 0;
+        if (graph) GADE_TRY(step_graph(m, psi, K, nsteps_b != nullptr));
+        for (int w = 0; w < W; w++) {
+            int rc = graph   ? (hipGraphLaunch(m->graph_exec, m->stream) == hipSuccess ? GADE_OK : GADE_EHIP)
+                     : table ? enqueue_step_window(m, psi, K, nsteps_b != nullptr)
+                             : enqueue_step(m, psi, ft.data() + (size_t)12 * B * w, s + w, K, nsteps_b != nullptr);
             if (rc == GADE_OK && usec) rc = hipEventRecord(ev[w + 1], m->stream) == hipSuccess ? GADE_OK : GADE_EHIP;
             if (rc) {
+                if (graph && rc == GADE_EHIP) gade::set_error("step graph launch failed");
                 free_events();
                 return rc;
             }
@@ -2048,6 +2156,10 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->device_loop = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "graph")) {
+        m->use_graph = value != 0;
+        return GADE_OK;
+    }
     if (!strcmp(name, "guess_order")) {
         if (value < 0 || value > kMaxGuessOrder) {
             gade::set_error("guess_order must be 0..3");
@@ -2117,6 +2229,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "guess_order")) {
         *value = m->guess_order;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "graph")) {
+        *value = m->use_graph;
         return GADE_OK;
     }
     if (!strcmp(name, "window")) {

@@ -70,7 +70,7 @@ enum {
     CT_STRIDE = 8
 };
 // run-wide words (int, in the global control array)
-enum { GC_STALLED = 0, GC_STALL_STEP = 1, GC_COUNT = 4 };
+enum { GC_STALLED = 0, GC_STALL_STEP = 1, GC_STEP = 2, GC_WSLOT = 3, GC_COUNT = 4 };
 
 constexpr int kMaxGuessOrder = 3;  // polynomial extrapolation order of the initial guess
 constexpr int kHistSlots = kMaxGuessOrder;
@@ -326,9 +326,11 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
 
 // Start of a device-driven CN step for instance blockIdx.x: idle when the step
 // is past its pulse or the run has stalled.
+// (step < 0: the step index is the run-wide GC_STEP word, as in a replayed step graph)
 [[maybe_unused]] static __global__ void k_step_begin(int *ctl0, const int *__restrict__ gctl, int step,
                                                      const int *__restrict__ nsteps) {
     if (threadIdx.x != 0) return;
+    if (step < 0) step = gctl[GC_STEP];
     const int b = blockIdx.x;
     int *ctl = ctl0 + (size_t)b * CT_STRIDE;
     const int idle = gctl[GC_STALLED] != 0 || (nsteps != nullptr && step >= nsteps[b]);
@@ -342,6 +344,7 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
 [[maybe_unused]] static __global__ void k_step_end(int *ctl0, int *gctl, const double *__restrict__ sc0, int step,
                                                    int nsteps_cap, int *iters, double *tols) {
     if (threadIdx.x != 0) return;
+    if (step < 0) step = gctl[GC_STEP];
     const int b = blockIdx.x;
     const int *ctl = ctl0 + (size_t)b * CT_STRIDE;
     const double *sc = sc0 + (size_t)b * SC_STRIDE;
@@ -369,6 +372,20 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
         tols[(size_t)b * nsteps_cap + step] = sc[SC_RHSSQ] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]) : 0.0;
     }
     (void)gctl;
+}
+
+// Next step of a window (after k_stall_spread): advances the run-wide step words.
+[[maybe_unused]] static __global__ void k_step_advance(int *gctl) {
+    if (threadIdx.x == 0) {
+        gctl[GC_STEP] += 1;
+        gctl[GC_WSLOT] += 1;
+    }
+}
+[[maybe_unused]] static __global__ void k_window_begin(int *gctl, int step) {
+    if (threadIdx.x == 0) {
+        gctl[GC_STEP] = step;
+        gctl[GC_WSLOT] = 0;
+    }
 }
 
 // After k_step_end: once the run has stalled, every instance idles.

@@ -153,6 +153,14 @@ struct gade_model {
     int iter_batch = 0;            // iterations enqueued per step (0: adaptive)
     int last_k = 2;                // adaptive iterations per step
     int guess_order = 0;           // initial guess: 0 psi_n (the reference), p: extrapolation of order p
+    // per-step pulse coefficients of a window on the device ([slot][B] x 6 complex) and the step graph
+    double2 *d_cfwin = nullptr, *h_cfwin = nullptr;
+    int cfwin_cap = 0;             // slots
+    bool use_graph = true;         // replay one captured hipGraph per CN step of a window
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    const void *graph_key[6] = {};  // what the captured step depends on
     double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions (per instance)
     // profiling: event pairs are recorded without synchronizing and resolved
     // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families
