@@ -512,7 +512,9 @@ __global__ void k_build_DD(const double2 *__restrict__ D, const int *__restrict_
     }
 }
 
-template <int BW>
+// DUAL: a second output y2/ypart2 = (D_sel2 + G-part) x from the same G sweeps (beta must be 0); with
+// cH = -1, sel = 1, sel2 = 0 this is the CN pair (S - H) x, (S + H) x, each bit-identical to its own launch.
+template <int BW, bool DUAL>
 __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x, const double2 *__restrict__ V,
                                                    const double2 *__restrict__ GD, const double2 *__restrict__ DD,
                                                    const int *__restrict__ tiles, const int *__restrict__ tile_q,
@@ -520,7 +522,8 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
                                                    int nhp, int nseg, int lgrid, int nq, int sel, double cH,
                                                    double beta, double2 *__restrict__ y,
                                                    double2 *__restrict__ ypart, const int *gate, int gstride,
-                                                   const int *__restrict__ tile_b, size_t gd_stride) {
+                                                   const int *__restrict__ tile_b, size_t gd_stride,
+                                                   double2 *__restrict__ y2, double2 *__restrict__ ypart2, int sel2) {
     constexpr int R = kDiaR, P = kDiaP, ND = 2 * BW + 1, W = kDiaSeg + 2 * BW;
     __shared__ double2 sm[R * W];
     const int lane = threadIdx.x;
@@ -624,6 +627,24 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
         }
     };
 
+    // acc -> output (per-multipole partial, or y itself when there is a single multipole grid)
+#define DIA_EMIT(YO, YPO)                                                     \
+    _Pragma("unroll") for (int i = 0; i < R; i++) {                           \
+        if (rows[i] < 0) continue;                                            \
+        _Pragma("unroll") for (int j = 0; j < P; j++) {                       \
+            const int ip = i0 + P * lane + j;                                 \
+            if (ip >= nh) continue;                                           \
+            const size_t o = (size_t)rows[i] * n + h * nh + ip;               \
+            if (YPO) {                                                        \
+                (YPO)[(size_t)l * plane + o] = acc[i][j];                     \
+            } else {                                                          \
+                double2 r_ = acc[i][j];                                       \
+                if (beta != 0.0) r_ = cadd(r_, cscale((YO)[o], beta));        \
+                (YO)[o] = r_;                                                 \
+            }                                                                 \
+        }                                                                     \
+    }
+
     if (cH != 0.0) {
         for (int v = 0; v < 4; v++) {
             for (int hp = 0; hp < 2; hp++) {
@@ -631,12 +652,16 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
                 sweep(GD + ((size_t)(l * 4 + v) * S + (size_t)(2 * h + hp) * ND * nhp) * 2, true);
             }
         }
-        if (cH != 1.0) {
+    }
+    // the second output's unscaled G-part: final for l > 0, parked in its own output slot for l = 0
+    if constexpr (DUAL) {
+        DIA_EMIT(y2, ypart2)
+    }
+    if (cH != 0.0 && cH != 1.0) {
 #pragma unroll
-            for (int i = 0; i < R; i++)
+        for (int i = 0; i < R; i++)
 #pragma unroll
-                for (int j = 0; j < P; j++) acc[i][j] = cscale(acc[i][j], cH);
-        }
+            for (int j = 0; j < P; j++) acc[i][j] = cscale(acc[i][j], cH);
     }
     if (l == 0) {
         for (int hp = 0; hp < 2; hp++) {
@@ -644,23 +669,29 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
             sweep(DD + ((size_t)(sel * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
         }
     }
+    DIA_EMIT(y, ypart)
+    if constexpr (DUAL) {
+        if (l == 0) {
+            // reload this lane's parked values (written by this thread above), then add D_sel2 x
 #pragma unroll
-    for (int i = 0; i < R; i++) {
-        if (rows[i] < 0) continue;
+            for (int i = 0; i < R; i++)
 #pragma unroll
-        for (int j = 0; j < P; j++) {
-            const int ip = i0 + P * lane + j;
-            if (ip >= nh) continue;
-            const size_t o = (size_t)rows[i] * n + h * nh + ip;
-            if (ypart) {
-                ypart[(size_t)l * plane + o] = acc[i][j];
-            } else {
-                double2 r = acc[i][j];
-                if (beta != 0.0) r = cadd(r, cscale(y[o], beta));
-                y[o] = r;
+                for (int j = 0; j < P; j++) {
+                    const int ip = i0 + P * lane + j;
+                    acc[i][j] = make_double2(0, 0);
+                    if (rows[i] >= 0 && ip < nh) {
+                        const size_t o = (size_t)rows[i] * n + h * nh + ip;
+                        acc[i][j] = ypart2 ? ypart2[o] : y2[o];
+                    }
+                }
+            for (int hp = 0; hp < 2; hp++) {
+                stage(x, hp);
+                sweep(DD + ((size_t)(sel2 * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
             }
+            DIA_EMIT(y2, ypart2)
         }
     }
+#undef DIA_EMIT
 }
 
 // per-block S-norm contributions pop[r] = x_r^H S(k_r) x_r
@@ -840,7 +871,8 @@ void free_all(gade_model *m) {
     if (m->graph) (void)hipGraphDestroy(m->graph);
     if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
     for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval,
-                    (void *)m->d_hist, (void *)m->d_gctl, (void *)m->d_nsteps, (void *)m->d_dtile_b})
+                    (void *)m->d_hist, (void *)m->d_gctl, (void *)m->d_nsteps, (void *)m->d_dtile_b,
+                    (void *)m->d_ypart2})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -856,23 +888,36 @@ int launch_angular(gade_model *m, const double2 *x) {
     return GADE_OK;
 }
 
-int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta) {
+// y2 (banded kernel only): also y2 = (D_sel2 + G-part) x from the same pass, beta 0
+int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta,
+                  double2 *y2 = nullptr, int sel2 = 0) {
     const bool inter = cH != 0.0;
     if (m->radial_kind == 2) {
         const int lgrid = inter ? m->lmax : 1;
         dim3 grid((unsigned)m->ndtiles * 2 * m->nseg * lgrid);
-        {
+        double2 *yp = inter && lgrid > 1 ? m->d_ypart : nullptr;
+        double2 *yp2 = y2 && yp ? m->d_ypart2 : nullptr;  // allocated with the model when lmax > 1
+        if (y2) {
+            Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
+            hipLaunchKernelGGL((k_radial_dia<kDiaBW, true>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
+                               m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp,
+                               m->nseg, lgrid, m->nq, sel, cH, 0.0, y, yp, m->gate, m->gstride, m->d_dtile_b,
+                               m->gd_stride, y2, yp2, sel2);
+        } else {
             Prof pr(m, "radial", GADE_PROF_RADIAL);
-            hipLaunchKernelGGL(k_radial_dia<kDiaBW>, grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD, m->d_DD,
-                               m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, m->nseg, lgrid,
-                               m->nq, sel, cH, beta, y, inter ? m->d_ypart : nullptr, m->gate, m->gstride,
-                               m->d_dtile_b, m->gd_stride);
-            GADE_HIP(hipGetLastError());
+            hipLaunchKernelGGL((k_radial_dia<kDiaBW, false>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
+                               m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp,
+                               m->nseg, lgrid, m->nq, sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b,
+                               m->gd_stride, (double2 *)nullptr, (double2 *)nullptr, 0);
         }
-        if (inter) {
+        GADE_HIP(hipGetLastError());
+        if (yp) {
             Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
             hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
-                               m->d_ypart, lgrid, m->N, beta, y, m->gate, m->gstride, m->N1);
+                               m->d_ypart, lgrid, m->N, y2 ? 0.0 : beta, y, m->gate, m->gstride, m->N1);
+            if (y2)
+                hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
+                                   m->d_ypart2, lgrid, m->N, 0.0, y2, m->gate, m->gstride, m->N1);
             GADE_HIP(hipGetLastError());
         }
         return GADE_OK;
@@ -1254,7 +1299,8 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
     if ((rc = hip_alloc((void **)&m->d_GE, sizeof(double2) * (size_t)lmax * 8 * ellw * n)) ||
         (rc = hip_alloc((void **)&m->d_D, sizeof(double2) * (size_t)3 * m->nq * ellw * n)) ||
         (rc = hip_alloc((void **)&m->d_V, sizeof(double2) * (size_t)lmax * 4 * m->N)) ||
-        (rc = hip_alloc((void **)&m->d_ypart, sizeof(double2) * (size_t)lmax * m->N)))
+        (rc = hip_alloc((void **)&m->d_ypart, sizeof(double2) * (size_t)lmax * m->N)) ||
+        (lmax > 1 && (rc = hip_alloc((void **)&m->d_ypart2, sizeof(double2) * (size_t)lmax * m->N))))
         return fail(rc);
     {
         int *d_kq = nullptr;
@@ -1351,6 +1397,20 @@ static int apply_async(gade_model *m, const double2 *x, double2 *y, double cH, d
         GADE_TRY(launch_angular(m, x));
     }
     return launch_radial(m, x, y, sel, cH, beta);
+}
+
+// the CN pair from one angular pass: ym = (S - H) x (the rhs), yp = (S + H) x (A x), each equal bit for
+// bit to its own apply_async; two applies when the radial phase is not banded
+static int apply_pair_async(gade_model *m, const double2 *x, double2 *ym, double2 *yp) {
+    if (m->radial_kind != 2) {
+        GADE_TRY(apply_async(m, x, ym, -1.0, 0.0));
+        return apply_async(m, x, yp, 1.0, 0.0);
+    }
+    {
+        Prof pr(m, "angular", GADE_PROF_ANGULAR);
+        GADE_TRY(launch_angular(m, x));
+    }
+    return launch_radial(m, x, ym, 1, -1.0, 0.0, yp, 0);
 }
 
 int gade_apply(gade_model_t *m, const double *x, double *y, double cH, double beta) {
@@ -1574,6 +1634,15 @@ int enqueue_iterations(gade_model *m, double2 *psi, int K) {
     return GADE_OK;
 }
 
+// rhs = (S - H) psi into tmp (tools/dirac_solver.c:531-536), x0 into psi (psi_n unless extrapolating),
+// A x0 into r; with x0 = psi_n the rhs and A x0 come from one paired apply
+int enqueue_rhs_ax0(gade_model *m, double2 *psi) {
+    if (m->guess_order == 0) return apply_pair_async(m, psi, m->tmp, m->r);
+    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));
+    GADE_TRY(launch_guess(m, psi));
+    return apply_async(m, psi, m->r, 1.0, 0.0);
+}
+
 int enqueue_step_end(gade_model *m, int step) {
     hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, step,
                        m->steps_cap, m->d_iters, m->d_tols);
@@ -1591,9 +1660,7 @@ int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K,
     GADE_HIP(hipGetLastError());
     m->gate = m->d_ctl + CT_PSKIP;
     GADE_TRY(launch_gsums(m, m->radial_kind != 2));
-    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
-    GADE_TRY(launch_guess(m, psi));                    // x0 (psi_n unless extrapolating)
-    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0
+    GADE_TRY(enqueue_rhs_ax0(m, psi));
     GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
     GADE_TRY(enqueue_iterations(m, psi, K));
     return enqueue_step_end(m, step);
@@ -1620,9 +1687,7 @@ int enqueue_step_window(gade_model *m, double2 *psi, int K, bool per_instance_st
     GADE_HIP(hipGetLastError());
     m->gate = m->d_ctl + CT_PSKIP;
     GADE_TRY(launch_gsums_window(m));
-    GADE_TRY(apply_async(m, psi, m->tmp, -1.0, 0.0));  // rhs = (S - H) psi (tools/dirac_solver.c:531-536)
-    GADE_TRY(launch_guess(m, psi));                    // x0 (psi_n unless extrapolating)
-    GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));     // A x0
+    GADE_TRY(enqueue_rhs_ax0(m, psi));
     GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
     GADE_TRY(enqueue_iterations(m, psi, K));
     hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, -1,

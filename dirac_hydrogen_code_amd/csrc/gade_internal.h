@@ -114,6 +114,7 @@ struct gade_model {
     size_t gd_stride = 0;          // GD entries per instance
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     double2 *d_ypart = nullptr;    // [lmax][nblk][n] per-multipole partial outputs of the radial phase
+    double2 *d_ypart2 = nullptr;   // the same for the second output of a paired apply (lmax > 1)
     // preconditioner
     gade_blocklu *pre = nullptr;
     int *d_flag = nullptr;

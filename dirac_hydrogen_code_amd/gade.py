@@ -100,7 +100,8 @@ def pulse(E0, omega, cycles, t):
     return out
 
 
-PROF = {"gsums": 1, "angular": 2, "radial": 4, "psolve": 8, "blas1": 16, "assemble": 32, "csr_spmv": 64, "radial_sum": 128}
+PROF = {"gsums": 1, "angular": 2, "radial": 4, "psolve": 8, "blas1": 16, "assemble": 32, "csr_spmv": 64, "radial_sum": 128,
+        "radial_pair": 4}
 PROF_ALL = 0x7fffffff
 
 

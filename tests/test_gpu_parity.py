@@ -210,11 +210,12 @@ def test_radial_kernels_agree_and_switch(datadir):
     assert relmax(res[("dia", 1.0)][0], om.apply(x, "S") + om.apply(x, "H")) <= 1e-13
 
 
-@pytest.mark.parametrize("window,batch", [(8, 0), (3, 1), (1, 0)])
-def test_device_loop_matches_host_loop(datadir, window, batch):
-    """The device-driven stepping (decisions on the device, one host round trip per window) gives the
-    host loop's trajectory bit for bit, including steps it has to hand back to the host (iter_batch 1)."""
-    cfg = "nondipole_small"
+@pytest.mark.parametrize("cfg,window,batch", [("nondipole_small", 8, 0), ("nondipole_small", 3, 1),
+                                              ("nondipole_small", 1, 0), ("dipole", 8, 0), ("tiny", 4, 1)])
+def test_device_loop_matches_host_loop(datadir, cfg, window, batch):
+    """The device-driven stepping (decisions on the device, one host round trip per window, the rhs and
+    A x0 from one paired apply) gives the host loop's trajectory (separate applies) bit for bit, including
+    steps it has to hand back to the host (iter_batch 1); dipole: a single multipole grid."""
     d, dt, om, gp = pair(datadir, cfg)
     psi0, _ = gade.read_psi0(d, gp.n)
     say(f"device loop window={window} batch={batch}")
