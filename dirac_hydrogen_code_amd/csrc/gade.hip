@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1604,8 +1605,18 @@ struct GateScope {
     }
 };
 
+// the step graph holds device addresses of the model's buffers: drop it before any of them is replaced
+void drop_step_graph(gade_model *m) {
+    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    m->graph_exec = nullptr;
+    m->graph = nullptr;
+    memset(m->graph_key, 0, sizeof(m->graph_key));
+}
+
 int ensure_step_buffers(gade_model *m, int nsteps) {
     if (nsteps <= m->steps_cap) return GADE_OK;
+    drop_step_graph(m);  // it writes the per-step outputs being replaced
     if (m->d_iters) (void)hipFree(m->d_iters);
     if (m->d_tols) (void)hipFree(m->d_tols);
     m->d_iters = nullptr;
@@ -1700,17 +1711,17 @@ int enqueue_step_window(gade_model *m, double2 *psi, int K, bool per_instance_st
 
 // the step graph for this K (captured again when anything it bakes in has changed)
 int step_graph(gade_model *m, double2 *psi, int K, bool per_instance_steps) {
-    const void *key[6] = {psi, (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
-                          (const void *)(intptr_t)(per_instance_steps ? 1 : 0), m->d_iters,
-                          (const void *)(intptr_t)m->steps_cap};
-    if (m->graph_exec && !memcmp(key, m->graph_key, sizeof(key))) return GADE_OK;
-    if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
-    m->graph_exec = nullptr;
-    m->graph = nullptr;
-    if (!m->cap_stream) GADE_HIP(hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking));
     if (m->guess_order > 0 && !m->d_hist)  // no allocation while capturing
         GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+    // everything the captured launches bake in that can change between runs (buffers that are replaced
+    // on growth also drop the graph themselves)
+    const void *key[8] = {psi,      (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
+                          (const void *)(intptr_t)(per_instance_steps ? 1 : 0), m->d_iters,
+                          (const void *)(intptr_t)m->steps_cap, m->d_cfwin, m->d_hist};
+    static_assert(sizeof(key) == sizeof(m->graph_key), "graph key size");
+    if (m->graph_exec && !memcmp(key, m->graph_key, sizeof(key))) return GADE_OK;
+    drop_step_graph(m);
+    if (!m->cap_stream) GADE_HIP(hipStreamCreateWithFlags(&m->cap_stream, hipStreamNonBlocking));
     hipStream_t saved = m->stream;
     m->stream = m->cap_stream;
     GADE_HIP(hipStreamBeginCapture(m->cap_stream, hipStreamCaptureModeThreadLocal));
@@ -1728,6 +1739,7 @@ int step_graph(gade_model *m, double2 *psi, int K, bool per_instance_steps) {
 
 int ensure_cfwin(gade_model *m, int slots) {
     if (slots <= m->cfwin_cap) return GADE_OK;
+    drop_step_graph(m);  // it reads the table being replaced
     if (m->d_cfwin) (void)hipFree(m->d_cfwin);
     if (m->h_cfwin) (void)hipHostFree(m->h_cfwin);
     m->d_cfwin = m->h_cfwin = nullptr;
@@ -2041,6 +2053,146 @@ int gade_cn_run_batch(gade_model_t *m, double *psi, const double *params, const 
     }
     return cn_run_device(m, (double2 *)psi, times.data(), nsteps, params, nsteps_b, tol, maxit, iters, tol_out,
                          nullptr);
+}
+
+int gade_autotune(gade_model_t *m, const double *psi, const double *params, double t0, int nsteps, double tol,
+                  int maxit, int reps, double *report, int *ncand, int *chosen) {
+    if (!m || !psi || !params || nsteps <= 0 || reps <= 0) return GADE_EINVAL;
+    GADE_TRY(gade_precond_setup(m));
+    std::vector<double> times(nsteps);
+    double time = t0;
+    for (int step = 0; step < nsteps; step++) {
+        time = time + m->dt;
+        times[step] = time;
+    }
+    struct Cand {
+        int window, iter_batch, graph;
+        double ms;
+    };
+    std::vector<Cand> cands;
+    for (int g : {1, 0}) {
+        if (g && m->radial_kind != 2) continue;
+        for (int w : {2, 4, 8, 16, 32})
+            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, 0.0});
+    }
+    const size_t N = (size_t)m->N, B = (size_t)m->B;
+    // the solver state a run starts from (BiCGSTAB workspace, which carries over between solves, scalars,
+    // control words and guess histories): saved once, restored before every probe and on return, so that
+    // tuning leaves the next run exactly as it would have been
+    double2 *ws[] = {m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t};
+    constexpr int NWS = 8;
+    const bool hist = m->d_hist != nullptr;
+    const size_t vec_bytes = sizeof(double2) * N;
+    const size_t state_bytes = vec_bytes * (NWS + (hist ? kHistSlots : 0)) + sizeof(double) * B * SC_STRIDE +
+                               sizeof(int) * B * CT_STRIDE;
+    double2 *work = nullptr;
+    char *state = nullptr;
+    GADE_TRY(hip_alloc((void **)&work, vec_bytes));
+    if (int rc = hip_alloc((void **)&state, state_bytes)) {
+        (void)hipFree(work);
+        return rc;
+    }
+    auto copy_state = [&](bool save) -> int {
+        char *q = state;
+        auto mv = [&](void *dev, size_t bytes) {
+            hipError_t e = save ? hipMemcpyAsync(q, dev, bytes, hipMemcpyDeviceToDevice, m->stream)
+                                : hipMemcpyAsync(dev, q, bytes, hipMemcpyDeviceToDevice, m->stream);
+            q += bytes;
+            return e;
+        };
+        hipError_t e = hipSuccess;
+        for (int k = 0; k < NWS && e == hipSuccess; k++) e = mv(ws[k], vec_bytes);
+        if (hist && e == hipSuccess) e = mv(m->d_hist, vec_bytes * kHistSlots);
+        if (e == hipSuccess) e = mv(m->d_sc, sizeof(double) * B * SC_STRIDE);
+        if (e == hipSuccess) e = mv(m->d_ctl, sizeof(int) * B * CT_STRIDE);
+        if (e != hipSuccess) {
+            gade::set_error(std::string("gade_autotune: state copy: ") + hipGetErrorString(e));
+            return GADE_EHIP;
+        }
+        return GADE_OK;
+    };
+    const int sv_window = m->window, sv_batch = m->iter_batch, sv_profile = m->profile, sv_k = m->last_k;
+    const bool sv_graph = m->use_graph;
+    auto restore = [&]() {
+        m->window = sv_window;
+        m->iter_batch = sv_batch;
+        m->use_graph = sv_graph;
+        m->profile = sv_profile;
+        m->last_k = sv_k;
+        const int rc = copy_state(false);
+        (void)hipStreamSynchronize(m->stream);
+        (void)hipFree(work);
+        (void)hipFree(state);
+        return rc;
+    };
+    if (int rc = copy_state(true)) {
+        (void)hipFree(work);
+        (void)hipFree(state);
+        return rc;
+    }
+    m->profile = 0;
+    std::vector<double2> ref(N), cur(N);
+    std::vector<int> ref_it(B * nsteps), it(B * nsteps);
+    std::vector<double> ref_tol(B * nsteps), tl(B * nsteps);
+    for (size_t c = 0; c < cands.size(); c++) {
+        m->window = cands[c].window;
+        m->iter_batch = cands[c].iter_batch;
+        m->use_graph = cands[c].graph != 0;
+        double best = 1e300;
+        for (int r = 0; r < reps; r++) {
+            int rc = GADE_OK;
+            if (hipMemcpyAsync(work, psi, sizeof(double2) * N, hipMemcpyDeviceToDevice, m->stream) != hipSuccess)
+                rc = GADE_EHIP;
+            if (rc == GADE_OK) rc = copy_state(false);
+            m->last_k = sv_k;
+            const auto t_a = std::chrono::steady_clock::now();
+            if (rc == GADE_OK)
+                rc = cn_run_device(m, work, times.data(), nsteps, params, nullptr, tol, maxit, it.data(), tl.data(),
+                                   nullptr);
+            if (rc == GADE_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = GADE_EHIP;
+            const auto t_b = std::chrono::steady_clock::now();
+            if (rc != GADE_OK) {
+                restore();
+                return rc;
+            }
+            best = std::min(best, std::chrono::duration<double, std::milli>(t_b - t_a).count());
+        }
+        cands[c].ms = best;
+        if (hipMemcpy(cur.data(), work, sizeof(double2) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+            restore();
+            gade::set_error("gade_autotune: copy of the probe result failed");
+            return GADE_EHIP;
+        }
+        if (c == 0) {
+            ref.swap(cur);
+            ref_it = it;
+            ref_tol = tl;
+        } else if (memcmp(ref.data(), cur.data(), sizeof(double2) * N) || ref_it != it ||
+                   memcmp(ref_tol.data(), tl.data(), sizeof(double) * tl.size())) {
+            restore();
+            gade::set_error("gade_autotune: candidate window " + std::to_string(cands[c].window) + " iter_batch " +
+                            std::to_string(cands[c].iter_batch) + " graph " + std::to_string(cands[c].graph) +
+                            " changed the trajectory");
+            return GADE_EINVAL;
+        }
+    }
+    size_t pick = 0;
+    for (size_t c = 1; c < cands.size(); c++)
+        if (cands[c].ms < cands[pick].ms) pick = c;
+    GADE_TRY(restore());
+    m->window = cands[pick].window;
+    m->iter_batch = cands[pick].iter_batch;
+    m->use_graph = cands[pick].graph != 0;
+    if (report)
+        for (size_t c = 0; c < cands.size() && c < GADE_TUNE_MAX; c++) {
+            report[4 * c] = cands[c].window;
+            report[4 * c + 1] = cands[c].iter_batch;
+            report[4 * c + 2] = cands[c].graph;
+            report[4 * c + 3] = cands[c].ms;
+        }
+    if (ncand) *ncand = (int)std::min<size_t>(cands.size(), GADE_TUNE_MAX);
+    if (chosen) *chosen = (int)pick;
+    return GADE_OK;
 }
 
 int gade_snorm(gade_model_t *m, const double *x, double *norm, double *pop_host) {

@@ -161,7 +161,7 @@ struct gade_model {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    const void *graph_key[6] = {};  // what the captured step depends on
+    const void *graph_key[8] = {};  // what the captured step depends on
     double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions (per instance)
     // profiling: event pairs are recorded without synchronizing and resolved
     // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families

@@ -25,8 +25,10 @@ EXPORTS = (
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
     "gade_set_param", "gade_get_param", "gade_eig_setup", "gade_eig_values", "gade_populations",
     "gade_extrapolate", "gade_model_create_batch", "gade_model_load_batch", "gade_model_batch", "gade_cn_run_batch",
-    "gade_set_time_batch", "gade_pulse_cep",
+    "gade_set_time_batch", "gade_pulse_cep", "gade_autotune",
 )
+
+GADE_TUNE_MAX = 64
 
 _lib = None
 
@@ -48,7 +50,8 @@ def lib():
             "gade_malloc": [C.POINTER(vp), sz], "gade_free": [vp], "gade_memcpy_h2d": [vp, vp, sz],
             "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_extrapolate": [vp, vp, i32],
             "gade_model_load_batch": [C.c_char_p, i32, i32, f64, f64, vp, vp],
-            "gade_cn_run_batch": [vp, vp, vp, vp, f64, i32, f64, i32, vp, vp], "gade_set_time_batch": [vp, vp], "gade_eig_values": [vp, vp, vp, vp],
+            "gade_cn_run_batch": [vp, vp, vp, vp, f64, i32, f64, i32, vp, vp],
+            "gade_autotune": [vp, vp, vp, f64, i32, f64, i32, i32, vp, C.POINTER(i32), C.POINTER(i32)], "gade_set_time_batch": [vp, vp], "gade_eig_values": [vp, vp, vp, vp],
             "gade_populations": [vp, vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), vp], "gade_device_sync": [],
             "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
             "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -188,6 +191,18 @@ class Propagator:
         check(lib().gade_cn_run_batch(self.ptr, psi.ptr, params.ctypes.data, None if nb is None else nb.ctypes.data,
                                       t0, nsteps, tol, maxit, iters.ctypes.data, tols.ctypes.data))
         return iters, tols
+
+    def autotune(self, psi: DeviceArray, params, nsteps, t0=0.0, tol=1e-12, maxit=500, reps=2):
+        """Time every stepping candidate (window, iter_batch, graph) on nsteps probe steps from psi (unchanged),
+        keep the fastest; every candidate must reproduce the same trajectory bit for bit.
+        Returns (chosen, [dict(window, iter_batch, graph, ms)])."""
+        params = np.ascontiguousarray(params, np.float64).reshape(self.batch, 4)
+        rep = np.zeros((GADE_TUNE_MAX, 4), np.float64)
+        nc, ch = C.c_int(), C.c_int()
+        check(lib().gade_autotune(self.ptr, psi.ptr, params.ctypes.data, t0, nsteps, tol, maxit, reps,
+                                  rep.ctypes.data, C.byref(nc), C.byref(ch)))
+        rows = [dict(window=int(r[0]), iter_batch=int(r[1]), graph=int(r[2]), ms=float(r[3])) for r in rep[:nc.value]]
+        return ch.value, rows
 
     def set_time_batch(self, ft):
         ft = np.ascontiguousarray(ft, np.complex128).reshape(self.batch, 6)

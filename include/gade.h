@@ -135,6 +135,19 @@ int gade_cn_run(gade_model_t *m, double *psi_dev, double E0, double omega, doubl
 int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, const int *nsteps_b, double t0,
                       int nsteps, double tol, int maxit, int *iters, double *tol_out);
 
+/* Autotuner of the device-driven stepping (no reference counterpart): times nsteps probe steps from
+ * psi_dev (left unchanged) with the pulses params [batch][4] (as gade_cn_run_batch, t0 the start time)
+ * for every candidate (window in {2,4,8,16,32}, iter_batch in {0,1,2,4}, graph in {0,1}, the graph
+ * only with the banded radial kernel), best of reps runs each, and keeps the fastest.  Every candidate
+ * must give the same trajectory, iterations and tolerances bit for bit, else GADE_EINVAL with the
+ * message naming it.  report [GADE_TUNE_MAX][4] (NULL allowed) receives (window, iter_batch, graph, ms)
+ * per candidate and *ncand their count; the chosen index goes to *chosen.  Every probe starts from the
+ * model's current solver state (BiCGSTAB workspace, scalars, guess histories), which is restored on
+ * return: a run after tuning equals the same run without it. */
+#define GADE_TUNE_MAX 64
+int gade_autotune(gade_model_t *m, const double *psi_dev, const double *params, double t0, int nsteps,
+                  double tol, int maxit, int reps, double *report, int *ncand, int *chosen);
+
 /* Pulse coefficients f_alpha(t), alpha = 0..5 (tools/solver_common.c:46-65),
  * as 12 doubles (complex with zero imaginary part). */
 void gade_pulse(double E0, double omega, double cycles, double t, double *ft);

@@ -81,3 +81,61 @@ def test_graph_batched_masks(datadir):
         out[graph] = (x.get(), it, tols)
     assert np.array_equal(bits(out[1][0]), bits(out[0][0]))
     assert np.array_equal(out[1][1], out[0][1]) and np.array_equal(out[1][2], out[0][2])
+
+
+@pytest.mark.parametrize("cfg,batch", [("nondipole_small", 1), ("dipole", 3)])
+def test_autotune_keeps_trajectory(datadir, cfg, batch):
+    """The autotuner times every (window, iter_batch, graph) candidate on probe steps, requires them to agree
+    bit for bit (it fails otherwise), keeps the fastest, and restores the solver state: the run after tuning
+    equals the same run without it."""
+    d = datadir(cfg)
+    dt = dt_of(cfg)
+    say(f"autotune {cfg} batch={batch}")
+    params = [(10.0, 50.0, 1.0, 0.3 * b) for b in range(batch)]
+    out = {}
+    for tune in (1, 0):
+        gp = gade.Propagator(d, LMAX[cfg], dt, batch=batch)
+        psi0, _ = gade.read_psi0(d, gp.n)
+        x = gade.DeviceArray.from_host(np.concatenate([psi0] * batch))
+        it1, tol1 = gp.cn_run_batch(x, params, 5, tol=1e-12)
+        if tune:
+            chosen, rows = gp.autotune(x, params, 6, t0=5 * dt, reps=1)
+            assert len(rows) == (40 if gp.get_param("radial") == 2 else 20)
+            assert all(r["ms"] > 0 for r in rows)
+            c = rows[chosen]
+            assert (gp.get_param("window"), gp.get_param("iter_batch"), gp.get_param("graph")) == (
+                c["window"], c["iter_batch"], c["graph"])
+            say(f"  chose {c} of {len(rows)}")
+        it2, tol2 = gp.cn_run_batch(x, params, 10, t0=5 * dt, tol=1e-12)
+        out[tune] = (x.get(), it1, it2, tol2)
+    assert np.array_equal(bits(out[1][0]), bits(out[0][0]))
+    for k in (1, 2, 3):
+        assert np.array_equal(out[1][k], out[0][k])
+
+
+def test_graph_recaptured_when_window_table_grows(datadir):
+    """A larger window replaces the device table of pulse coefficients; the cached step graph (same K, same
+    psi) must be captured again against the new table: results equal the one-by-one launches."""
+    cfg = "dipole"
+    d = datadir(cfg)
+    dt = dt_of(cfg)
+    say("graph window growth")
+    params = [(10.0, 50.0, 1.0, 0.0), (30.0, 40.0, 1.0, 1.0)]
+    out = {}
+    for graph in (1, 0):
+        gp = gade.Propagator(d, LMAX[cfg], dt, batch=2)
+        gp.set_param("graph", graph)
+        gp.set_param("iter_batch", 3)  # the same K in every window
+        psi0, _ = gade.read_psi0(d, gp.n)
+        x = gade.DeviceArray.from_host(np.concatenate([psi0, psi0]))
+        res = []
+        t0 = 0.0
+        for w in (2, 5, 9, 16):
+            gp.set_param("window", w)
+            it, tols = gp.cn_run_batch(x, params, w, t0=t0, tol=1e-12)
+            t0 += w * dt
+            res.append((x.get(), it, tols))
+        out[graph] = res
+    for a, b in zip(out[1], out[0]):
+        assert np.array_equal(bits(a[0]), bits(b[0]))
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
