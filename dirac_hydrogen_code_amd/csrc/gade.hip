@@ -158,16 +158,18 @@ __global__ void k_build_D(const double2 *__restrict__ s, const double2 *__restri
 constexpr int kAngThreads = 256;
 constexpr int kAngLB = 4;
 
-template <int LB>
-__global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restrict__ x, const int *__restrict__ hAp,
+template <int LB, class T = double2>
+__global__ __launch_bounds__(kAngThreads) void k_angular(const T *__restrict__ x, const int *__restrict__ hAp,
                                                          const int *__restrict__ hAi,
-                                                         const double2 *__restrict__ coef,
+                                                         const T *__restrict__ coef,
                                                          const int *__restrict__ kap, int nblk, int n, int lmax,
-                                                         int maxrow, double2 *__restrict__ V,
+                                                         int maxrow, T *__restrict__ V,
                                                          const int *gate, int gstride, int nblk1) {
     const int r = blockIdx.x;
     GADE_GATED_B(gate, gstride, r / nblk1);
-    extern __shared__ double2 shc[];  // [maxrow][LB][2] coefficients, then [maxrow] (col, kappa) packed
+    using Rt = typename cx<T>::R;
+    extern __shared__ __align__(16) char ang_raw[];
+    T *shc = reinterpret_cast<T *>(ang_raw);  // [maxrow][LB][2] coefficients, then [maxrow] (col, kappa) packed
     const int a = blockIdx.y * blockDim.x + threadIdx.x;
     const int l0 = blockIdx.z * LB;
     const int nlb = min(LB, lmax - l0);
@@ -176,20 +178,20 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restri
     int2 *shk = (int2 *)(shc + (size_t)maxrow * LB * 2);
     for (int e = threadIdx.x; e < len * LB * 2; e += blockDim.x) {
         const int p = e / (LB * 2), rem = e % (LB * 2), l = rem / 2, w = rem % 2;
-        shc[e] = (l < nlb) ? coef[((size_t)(pb + p) * lmax + l0 + l) * 2 + w] : make_double2(0, 0);
+        shc[e] = (l < nlb) ? coef[((size_t)(pb + p) * lmax + l0 + l) * 2 + w] : czero<T>();
     }
     for (int e = threadIdx.x; e < len; e += blockDim.x) {
         const int c = hAi[pb + e];
         shk[e] = make_int2(c, kap[c]);
     }
     __syncthreads();
-    double2 z0[LB], z2[LB], w0[LB], w1[LB];
+    T z0[LB], z2[LB], w0[LB], w1[LB];
 #pragma unroll
-    for (int l = 0; l < LB; l++) z0[l] = z2[l] = w0[l] = w1[l] = make_double2(0, 0);
+    for (int l = 0; l < LB; l++) z0[l] = z2[l] = w0[l] = w1[l] = czero<T>();
     const int ac = a < n ? a : n - 1;
     constexpr int U = 4;
     for (int p0 = 0; p0 < len; p0 += U) {
-        double2 xv[U];
+        T xv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int p = min(p0 + u, len - 1);
@@ -199,11 +201,11 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const double2 *__restri
         for (int u = 0; u < U; u++) {
             const int p = p0 + u;
             if (p >= len) break;
-            const double2 kx = cscale(xv[u], (double)shk[p].y);
-            const double2 *cp = shc + (size_t)p * LB * 2;
+            const T kx = cscale(xv[u], (Rt)shk[p].y);
+            const T *cp = shc + (size_t)p * LB * 2;
 #pragma unroll
             for (int l = 0; l < LB; l++) {
-                const double2 h0 = cp[2 * l], h1 = cp[2 * l + 1];
+                const T h0 = cp[2 * l], h1 = cp[2 * l + 1];
                 if (cnz(h0)) {
                     z0[l] = cfma(h0, xv[u], z0[l]);
                     z2[l] = cfma(h0, kx, z2[l]);
@@ -389,15 +391,17 @@ __global__ __launch_bounds__(64) void k_radial(const double2 *__restrict__ x, co
 }
 
 // y = sum_l ypart[l] + beta y  (fixed summation order); blockIdx.y = instance of N1 entries
-__global__ __launch_bounds__(256) void k_radial_sum(const double2 *__restrict__ ypart, int lgrid, long long N,
-                                                    double beta, double2 *__restrict__ y, const int *gate,
+template <class T = double2>
+__global__ __launch_bounds__(256) void k_radial_sum(const T *__restrict__ ypart, int lgrid, long long N,
+                                                    double beta, T *__restrict__ y, const int *gate,
                                                     int gstride, long long N1) {
     GADE_GATED_B(gate, gstride, blockIdx.y);
+    using Rt = typename cx<T>::R;
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
         const long long i = (long long)blockIdx.y * N1 + j;
-        double2 s = ypart[i];
+        T s = ypart[i];
         for (int l = 1; l < lgrid; l++) s = cadd(s, ypart[(size_t)l * N + i]);
-        if (beta != 0.0) s = cadd(s, cscale(y[i], beta));
+        if (beta != 0.0) s = cadd(s, cscale(y[i], (Rt)beta));
         y[i] = s;
     }
 }
@@ -515,18 +519,19 @@ __global__ void k_build_DD(const double2 *__restrict__ D, const int *__restrict_
 
 // DUAL: a second output y2/ypart2 = (D_sel2 + G-part) x from the same G sweeps (beta must be 0); with
 // cH = -1, sel = 1, sel2 = 0 this is the CN pair (S - H) x, (S + H) x, each bit-identical to its own launch.
-template <int BW, bool DUAL>
-__global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x, const double2 *__restrict__ V,
-                                                   const double2 *__restrict__ GD, const double2 *__restrict__ DD,
+template <int BW, bool DUAL, class T = double2>
+__global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, const T *__restrict__ V,
+                                                   const T *__restrict__ GD, const T *__restrict__ DD,
                                                    const int *__restrict__ tiles, const int *__restrict__ tile_q,
                                                    const double *__restrict__ tile_k, int nblk, int n, int nh,
                                                    int nhp, int nseg, int lgrid, int nq, int sel, double cH,
-                                                   double beta, double2 *__restrict__ y,
-                                                   double2 *__restrict__ ypart, const int *gate, int gstride,
+                                                   double beta, T *__restrict__ y,
+                                                   T *__restrict__ ypart, const int *gate, int gstride,
                                                    const int *__restrict__ tile_b, size_t gd_stride,
-                                                   double2 *__restrict__ y2, double2 *__restrict__ ypart2, int sel2) {
+                                                   T *__restrict__ y2, T *__restrict__ ypart2, int sel2) {
     constexpr int R = kDiaR, P = kDiaP, ND = 2 * BW + 1, W = kDiaSeg + 2 * BW;
-    __shared__ double2 sm[R * W];
+    using Rt = typename cx<T>::R;
+    __shared__ T sm[R * W];
     const int lane = threadIdx.x;
     int rest = blockIdx.x;
     const int l = rest % lgrid;
@@ -538,7 +543,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
     const int inst = tile_b[tile];
     GADE_GATED_B(gate, gstride, inst);
     GD += (size_t)inst * gd_stride;  // this instance's G-sums
-    const double kap = tile_k[tile];
+    const Rt kap = (Rt)tile_k[tile];
     const int q = tile_q[tile];
     const int i0 = seg * kDiaSeg;
     int rows[R];
@@ -547,25 +552,25 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
     const size_t plane = (size_t)nblk * n;
     const int S = 4 * ND * nhp;
 
-    double2 acc[R][P];
+    T acc[R][P];
 #pragma unroll
     for (int i = 0; i < R; i++)
 #pragma unroll
-        for (int j = 0; j < P; j++) acc[i][j] = make_double2(0, 0);
+        for (int j = 0; j < P; j++) acc[i][j] = czero<T>();
 
     // window of half hp of 'src' for every row of the tile: position p <-> half index i0 - BW + p.
     // All loads of the window are issued before the first LDS write.
-    auto stage = [&](const double2 *src, int hp) {
+    auto stage = [&](const T *src, int hp) {
         constexpr int NC = (W + 63) / 64;
-        double2 t[R][NC];
+        T t[R][NC];
 #pragma unroll
         for (int i = 0; i < R; i++) {
-            const double2 *rp = src + (size_t)(rows[i] >= 0 ? rows[i] : 0) * n + hp * nh;
+            const T *rp = src + (size_t)(rows[i] >= 0 ? rows[i] : 0) * n + hp * nh;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 const int p = c * 64 + lane;
                 const int b = i0 - BW + p;
-                t[i][c] = make_double2(0, 0);
+                t[i][c] = czero<T>();
                 if (rows[i] >= 0 && p < W && b >= 0 && b < nh) t[i][c] = rp[b];
             }
         }
@@ -581,39 +586,39 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
     };
     // acc[i][j] += sum_d M_d[i0 + P*lane + j] * window[P*lane + j + d]; with two coefficient
     // streams M = C1 + kap*C2 (pairs), else M = C1.  Coefficients are fetched PF diagonals ahead.
-    auto sweep = [&](const double2 *C, bool pair) {
+    auto sweep = [&](const T *C, bool pair) {
         constexpr int PF = 3;
         const int stride = pair ? 2 : 1;
-        const double2 *cp = C + (size_t)(i0 + P * lane) * stride;
-        double2 w[R][P];
+        const T *cp = C + (size_t)(i0 + P * lane) * stride;
+        T w[R][P];
 #pragma unroll
         for (int i = 0; i < R; i++)
 #pragma unroll
             for (int j = 0; j < P; j++) w[i][j] = sm[i * W + dia_swz(P * lane + j)];
-        double2 c1[PF][P], c2[PF][P];
+        T c1[PF][P], c2[PF][P];
 #pragma unroll
         for (int d = 0; d < PF - 1; d++)
 #pragma unroll
             for (int j = 0; j < P; j++) {
-                const double2 *np = cp + (size_t)d * nhp * stride;
+                const T *np = cp + (size_t)d * nhp * stride;
                 c1[d][j] = np[j * stride];
-                c2[d][j] = pair ? np[j * stride + 1] : make_double2(0, 0);
+                c2[d][j] = pair ? np[j * stride + 1] : czero<T>();
             }
 #pragma unroll
         for (int d = 0; d < ND; d++) {
             if (d + PF - 1 < ND) {
-                const double2 *np = cp + (size_t)(d + PF - 1) * nhp * stride;
+                const T *np = cp + (size_t)(d + PF - 1) * nhp * stride;
 #pragma unroll
                 for (int j = 0; j < P; j++) {
                     c1[(d + PF - 1) % PF][j] = np[j * stride];
-                    c2[(d + PF - 1) % PF][j] = pair ? np[j * stride + 1] : make_double2(0, 0);
+                    c2[(d + PF - 1) % PF][j] = pair ? np[j * stride + 1] : czero<T>();
                 }
             }
-            double2 M[P];
+            T M[P];
 #pragma unroll
             for (int j = 0; j < P; j++) {
-                const double2 a = c1[d % PF][j], g = c2[d % PF][j];
-                M[j] = pair ? make_double2(fma(kap, g.x, a.x), fma(kap, g.y, a.y)) : a;
+                const T a = c1[d % PF][j], g = c2[d % PF][j];
+                M[j] = pair ? cx<T>::mk(fma(kap, g.x, a.x), fma(kap, g.y, a.y)) : a;
             }
 #pragma unroll
             for (int i = 0; i < R; i++) {
@@ -639,8 +644,8 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
             if (YPO) {                                                        \
                 (YPO)[(size_t)l * plane + o] = acc[i][j];                     \
             } else {                                                          \
-                double2 r_ = acc[i][j];                                       \
-                if (beta != 0.0) r_ = cadd(r_, cscale((YO)[o], beta));        \
+                T r_ = acc[i][j];                                       \
+                if (beta != 0.0) r_ = cadd(r_, cscale((YO)[o], (Rt)beta));        \
                 (YO)[o] = r_;                                                 \
             }                                                                 \
         }                                                                     \
@@ -662,7 +667,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
 #pragma unroll
         for (int i = 0; i < R; i++)
 #pragma unroll
-            for (int j = 0; j < P; j++) acc[i][j] = cscale(acc[i][j], cH);
+            for (int j = 0; j < P; j++) acc[i][j] = cscale(acc[i][j], (Rt)cH);
     }
     if (l == 0) {
         for (int hp = 0; hp < 2; hp++) {
@@ -679,7 +684,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const double2 *__restrict__ x
 #pragma unroll
                 for (int j = 0; j < P; j++) {
                     const int ip = i0 + P * lane + j;
-                    acc[i][j] = make_double2(0, 0);
+                    acc[i][j] = czero<T>();
                     if (rows[i] >= 0 && ip < nh) {
                         const size_t o = (size_t)rows[i] * n + h * nh + ip;
                         acc[i][j] = ypart2 ? ypart2[o] : y2[o];
@@ -873,7 +878,11 @@ void free_all(gade_model *m) {
     if (m->cap_stream) (void)hipStreamDestroy(m->cap_stream);
     for (void *p : {(void *)m->d_ctl, (void *)m->d_iters, (void *)m->d_tols, (void *)m->d_evec, (void *)m->d_eval,
                     (void *)m->d_hist, (void *)m->d_gctl, (void *)m->d_nsteps, (void *)m->d_dtile_b,
-                    (void *)m->d_ypart2})
+                    (void *)m->d_ypart2, (void *)m->d_V32, (void *)m->d_ypart32, (void *)m->d_GD32,
+                    (void *)m->d_DD32, (void *)m->d_coef32, (void *)m->lu32[0], (void *)m->lu32[1],
+                    (void *)m->lu32[2], (void *)m->lu32[3], (void *)m->mp_r, (void *)m->mp_r0, (void *)m->mp_p,
+                    (void *)m->mp_v, (void *)m->mp_y, (void *)m->mp_z, (void *)m->mp_s, (void *)m->mp_t,
+                    (void *)m->mp_d, (void *)m->d_mp_ax, (void *)m->d_sc32})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -883,7 +892,7 @@ void free_all(gade_model *m) {
 int launch_angular(gade_model *m, const double2 *x) {
     dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
     size_t shm = sizeof(double2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
-    hipLaunchKernelGGL(k_angular<kAngLB>, grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef,
+    hipLaunchKernelGGL((k_angular<kAngLB, double2>), grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef,
                        m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V, m->gate, m->gstride, m->nblk1);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
@@ -900,13 +909,13 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
         double2 *yp2 = y2 && yp ? m->d_ypart2 : nullptr;  // allocated with the model when lmax > 1
         if (y2) {
             Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
-            hipLaunchKernelGGL((k_radial_dia<kDiaBW, true>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
+            hipLaunchKernelGGL((k_radial_dia<kDiaBW, true, double2>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
                                m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp,
                                m->nseg, lgrid, m->nq, sel, cH, 0.0, y, yp, m->gate, m->gstride, m->d_dtile_b,
                                m->gd_stride, y2, yp2, sel2);
         } else {
             Prof pr(m, "radial", GADE_PROF_RADIAL);
-            hipLaunchKernelGGL((k_radial_dia<kDiaBW, false>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
+            hipLaunchKernelGGL((k_radial_dia<kDiaBW, false, double2>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
                                m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp,
                                m->nseg, lgrid, m->nq, sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b,
                                m->gd_stride, (double2 *)nullptr, (double2 *)nullptr, 0);
@@ -914,10 +923,10 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
         GADE_HIP(hipGetLastError());
         if (yp) {
             Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
-            hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
+            hipLaunchKernelGGL(k_radial_sum<double2>, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
                                m->d_ypart, lgrid, m->N, y2 ? 0.0 : beta, y, m->gate, m->gstride, m->N1);
             if (y2)
-                hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
+                hipLaunchKernelGGL(k_radial_sum<double2>, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
                                    m->d_ypart2, lgrid, m->N, 0.0, y2, m->gate, m->gstride, m->N1);
             GADE_HIP(hipGetLastError());
         }
@@ -942,7 +951,7 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
     GADE_HIP(hipGetLastError());
     if (inter) {
         Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
-        hipLaunchKernelGGL(k_radial_sum, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream, m->d_ypart,
+        hipLaunchKernelGGL(k_radial_sum<double2>, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream, m->d_ypart,
                            lgrid, m->N, beta, y, m->gate, m->gstride, m->N1);
         GADE_HIP(hipGetLastError());
     }
@@ -1645,6 +1654,150 @@ int enqueue_iterations(gade_model *m, double2 *psi, int K) {
     return GADE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// mixed precision: FP32 copies of the operator and preconditioner, FP32 inner BiCGSTAB, FP64 defect
+// correction (residual, update, tolerance test and every accumulator stay in FP64)
+// ---------------------------------------------------------------------------
+int mp_setup(gade_model *m) {
+    if (m->mp_ready) return GADE_OK;
+    if (m->radial_kind != 2 || !m->pre || m->pre->ch != 16) {
+        gade::set_error("mixed precision needs the banded radial kernel and 16-row preconditioner levels");
+        return GADE_EINVAL;
+    }
+    const size_t N = (size_t)m->N, L = (size_t)m->lmax;
+    const size_t S = (size_t)4 * m->nd * m->nhp;
+    const size_t nDD = 3 * (size_t)m->nq * S, nco = (size_t)m->hnnz * L * 2;
+    const size_t nlu = (size_t)m->nq * m->pre->nch * 256;
+    GADE_TRY(hip_alloc((void **)&m->d_V32, sizeof(float2) * L * 4 * N));
+    GADE_TRY(hip_alloc((void **)&m->d_ypart32, sizeof(float2) * L * N));
+    GADE_TRY(hip_alloc((void **)&m->d_GD32, sizeof(float2) * m->gd_stride * m->B));
+    GADE_TRY(hip_alloc((void **)&m->d_DD32, sizeof(float2) * nDD));
+    GADE_TRY(hip_alloc((void **)&m->d_coef32, sizeof(float2) * nco));
+    for (auto &f : m->lu32) GADE_TRY(hip_alloc((void **)&f, sizeof(float2) * nlu));
+    for (float2 **v : {&m->mp_r, &m->mp_r0, &m->mp_p, &m->mp_v, &m->mp_y, &m->mp_z, &m->mp_s, &m->mp_t, &m->mp_d})
+        GADE_TRY(hip_alloc((void **)v, sizeof(float2) * N));
+    GADE_TRY(hip_alloc((void **)&m->d_mp_ax, sizeof(double2) * N));
+    GADE_TRY(hip_alloc((void **)&m->d_sc32, sizeof(double) * (size_t)m->B * SC_STRIDE));
+    auto conv = [&](const double2 *src, float2 *dst, size_t n) {
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)n, 256, 8192), 1), dim3(256), 0, m->stream,
+                           (long long)n, (size_t)0, src, dst, (const int *)nullptr, 0);
+    };
+    conv(m->d_DD, m->d_DD32, nDD);
+    conv(m->d_coef, m->d_coef32, nco);
+    conv(m->pre->d_Linv, m->lu32[0], nlu);
+    conv(m->pre->d_Uinv, m->lu32[1], nlu);
+    conv(m->pre->d_Loff, m->lu32[2], nlu);
+    conv(m->pre->d_Uoff, m->lu32[3], nlu);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    m->mp_ready = true;
+    return GADE_OK;
+}
+
+// y = (S + H) x in FP32 (the FP32 G-sums of the current step)
+int apply32_async(gade_model *m, const float2 *x, float2 *y) {
+    {
+        Prof pr(m, "angular_f32", GADE_PROF_ANGULAR);
+        dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
+        size_t shm = sizeof(float2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
+        hipLaunchKernelGGL((k_angular<kAngLB, float2>), grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp,
+                           m->d_hAi, m->d_coef32, m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V32, m->gate,
+                           m->gstride, m->nblk1);
+        GADE_HIP(hipGetLastError());
+    }
+    const int lgrid = m->lmax;
+    float2 *yp = lgrid > 1 ? m->d_ypart32 : nullptr;
+    {
+        Prof pr(m, "radial_f32", GADE_PROF_RADIAL);
+        dim3 grid((unsigned)m->ndtiles * 2 * m->nseg * lgrid);
+        hipLaunchKernelGGL((k_radial_dia<kDiaBW, false, float2>), grid, dim3(64), 0, m->stream, x, m->d_V32,
+                           m->d_GD32, m->d_DD32, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh,
+                           m->nhp, m->nseg, lgrid, m->nq, 0, 1.0, 0.0, y, yp, m->gate, m->gstride, m->d_dtile_b,
+                           m->gd_stride, (float2 *)nullptr, (float2 *)nullptr, 0);
+        GADE_HIP(hipGetLastError());
+    }
+    if (yp) {
+        Prof pr(m, "radial_sum_f32", GADE_PROF_RADIAL_SUM);
+        hipLaunchKernelGGL(k_radial_sum<float2>, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
+                           yp, lgrid, m->N, 0.0, y, m->gate, m->gstride, m->N1);
+        GADE_HIP(hipGetLastError());
+    }
+    return GADE_OK;
+}
+
+int psolve32_async(gade_model *m, const float2 *b, float2 *x) {
+    Prof pr(m, "psolve_f32", GADE_PROF_PSOLVE);
+    const gade_blocklu *lu = m->pre;
+    const size_t shq = sizeof(float2) * ((size_t)lu->nch * 16 + 16);
+    hipLaunchKernelGGL((k_psolve_q<4, float2>), dim3(lu->norder), dim3(64), shq, m->stream, b, x, lu->d_order,
+                       lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n, lu->band,
+                       lu->nch, m->gate, m->gstride, m->nblk1);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+template <int OP>
+int bicg32_op(gade_model *m) {
+    Prof pr(m, "blas1_f32", GADE_PROF_BLAS1);
+    hipLaunchKernelGGL((k_bicg<OP, float2>), dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream, m->N1,
+                       (const float2 *)nullptr, m->mp_r, m->mp_r0, m->mp_p, m->mp_v, m->mp_s, m->mp_t, m->mp_y,
+                       m->mp_z, m->mp_d, m->d_sc32, m->d_part, m->gate, m->gstride);
+    if (OP != OP_P && OP != OP_S)
+        hipLaunchKernelGGL((k_bicg_fin<OP, true>), dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks,
+                           m->d_part, m->d_sc32, m->gate, m->gstride, m->d_ctl);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// J defect-correction sweeps of every instance still solving (gated per instance)
+int enqueue_sweeps(gade_model *m, double2 *psi, int J) {
+    const dim3 red(m->red_blocks, m->B);
+    for (int j = 0; j < J; j++) {
+        m->gate = m->d_ctl + CT_SKIP;
+        hipLaunchKernelGGL(k_mp_start, red, dim3(kRedThreads), 0, m->stream, m->N1, (const double2 *)m->r, m->mp_r,
+                           m->mp_r0, m->mp_d, m->mp_p, m->mp_v, m->d_part, m->gate, m->gstride);
+        hipLaunchKernelGGL(k_mp_start_fin, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part,
+                           m->d_sc32, m->d_ctl, m->mp_inner_tol);
+        GADE_HIP(hipGetLastError());
+        m->gate = m->d_ctl + CT_ISKIP;
+        for (int k = 0; k < m->mp_inner_iters; k++) {
+            GADE_TRY(bicg32_op<OP_P>(m));
+            GADE_TRY(psolve32_async(m, m->mp_p, m->mp_y));
+            GADE_TRY(apply32_async(m, m->mp_y, m->mp_v));
+            GADE_TRY(bicg32_op<OP_ALPHA>(m));
+            GADE_TRY(bicg32_op<OP_S>(m));
+            GADE_TRY(psolve32_async(m, m->mp_s, m->mp_z));
+            GADE_TRY(apply32_async(m, m->mp_z, m->mp_t));
+            GADE_TRY(bicg32_op<OP_W>(m));
+            GADE_TRY(bicg32_op<OP_XR>(m));
+        }
+        m->gate = m->d_ctl + CT_SKIP;
+        hipLaunchKernelGGL(k_mp_update, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream, m->N1, psi,
+                           (const float2 *)m->mp_d, m->gate, m->gstride);
+        GADE_TRY(apply_async(m, psi, m->d_mp_ax, 1.0, 0.0));
+        hipLaunchKernelGGL(k_mp_resid, red, dim3(kRedThreads), 0, m->stream, m->N1, (const double2 *)m->tmp,
+                           (const double2 *)m->d_mp_ax, m->r, m->d_part, m->gate, m->gstride);
+        hipLaunchKernelGGL(k_mp_resid_fin, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part,
+                           m->d_sc, m->d_ctl, m->gate, m->gstride);
+        GADE_HIP(hipGetLastError());
+    }
+    return GADE_OK;
+}
+
+// the FP32 G-sums of the step (after the FP64 ones), per instance under the step-prologue gate
+int mp_convert_gsums(gade_model *m) {
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)m->gd_stride, 256, 4096), m->B), dim3(256), 0, m->stream,
+                       (long long)m->gd_stride, m->gd_stride, (const double2 *)m->d_GD, m->d_GD32, m->gate,
+                       m->gstride);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// the solve part of a device-driven step after OP_INIT: FP64 iterations or mixed sweeps
+int enqueue_solve(gade_model *m, double2 *psi, int K) {
+    return m->precision == 1 ? enqueue_sweeps(m, psi, K) : enqueue_iterations(m, psi, K);
+}
+
 // rhs = (S - H) psi into tmp (tools/dirac_solver.c:531-536), x0 into psi (psi_n unless extrapolating),
 // A x0 into r; with x0 = psi_n the rhs and A x0 come from one paired apply
 int enqueue_rhs_ax0(gade_model *m, double2 *psi) {
@@ -1671,9 +1824,10 @@ int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K,
     GADE_HIP(hipGetLastError());
     m->gate = m->d_ctl + CT_PSKIP;
     GADE_TRY(launch_gsums(m, m->radial_kind != 2));
+    if (m->precision == 1) GADE_TRY(mp_convert_gsums(m));
     GADE_TRY(enqueue_rhs_ax0(m, psi));
     GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
-    GADE_TRY(enqueue_iterations(m, psi, K));
+    GADE_TRY(enqueue_solve(m, psi, K));
     return enqueue_step_end(m, step);
 }
 
@@ -1698,9 +1852,10 @@ int enqueue_step_window(gade_model *m, double2 *psi, int K, bool per_instance_st
     GADE_HIP(hipGetLastError());
     m->gate = m->d_ctl + CT_PSKIP;
     GADE_TRY(launch_gsums_window(m));
+    if (m->precision == 1) GADE_TRY(mp_convert_gsums(m));
     GADE_TRY(enqueue_rhs_ax0(m, psi));
     GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
-    GADE_TRY(enqueue_iterations(m, psi, K));
+    GADE_TRY(enqueue_solve(m, psi, K));
     hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, -1,
                        m->steps_cap, m->d_iters, m->d_tols);
     hipLaunchKernelGGL(k_stall_spread, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl);
@@ -1715,9 +1870,13 @@ int step_graph(gade_model *m, double2 *psi, int K, bool per_instance_steps) {
         GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
     // everything the captured launches bake in that can change between runs (buffers that are replaced
     // on growth also drop the graph themselves)
-    const void *key[8] = {psi,      (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
-                          (const void *)(intptr_t)(per_instance_steps ? 1 : 0), m->d_iters,
-                          (const void *)(intptr_t)m->steps_cap, m->d_cfwin, m->d_hist};
+    intptr_t tol_bits = 0;
+    memcpy(&tol_bits, &m->mp_inner_tol, sizeof(double) < sizeof(intptr_t) ? sizeof(double) : sizeof(intptr_t));
+    const void *key[11] = {psi,      (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
+                           (const void *)(intptr_t)(per_instance_steps ? 1 : 0), m->d_iters,
+                           (const void *)(intptr_t)m->steps_cap, m->d_cfwin, m->d_hist,
+                           (const void *)(intptr_t)m->precision, (const void *)(intptr_t)m->mp_inner_iters,
+                           (const void *)tol_bits};
     static_assert(sizeof(key) == sizeof(m->graph_key), "graph key size");
     if (m->graph_exec && !memcmp(key, m->graph_key, sizeof(key))) return GADE_OK;
     drop_step_graph(m);
@@ -1778,6 +1937,9 @@ int finish_stalled_step(gade_model *m, double2 *psi, int step, int K, std::vecto
                 } else if (!std::isfinite(sc[SC_RNORM2])) {
                     gade::set_error("BiCGSTAB: non-finite residual");
                     return GADE_ENOTCONV;
+                } else if (m->precision == 1) {
+                    c[CT_STATUS] = 0;  // no restart in the defect correction: another sweep
+                    more = true;
                 } else {
                     // restart: r = rhs - A x, r0 = r, rho = |r|^2 (tools/gpu_bicgstab.c:222-241)
                     c[CT_HSEL] = 0;
@@ -1806,7 +1968,7 @@ int finish_stalled_step(gade_model *m, double2 *psi, int step, int K, std::vecto
                 GADE_TRY(apply_async(m, psi, m->r, 1.0, 0.0));
                 GADE_TRY(bicg_op<OP_RESTART>(m, m->tmp));
             }
-            GADE_TRY(enqueue_iterations(m, psi, K));
+            GADE_TRY(enqueue_solve(m, psi, K));
         }
         // record every decided solve of this step (zero-rhs instances with 0 iterations, tolerance 0)
         hipLaunchKernelGGL(k_step_end_resume, dim3(B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, step,
@@ -1833,6 +1995,7 @@ int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, 
     if (nsteps == 0) return GADE_OK;
     const int B = m->B;
     GADE_TRY(gade_precond_setup(m));
+    if (m->precision == 1) GADE_TRY(mp_setup(m));
     GADE_TRY(ensure_step_buffers(m, nsteps));
     const size_t cap = (size_t)m->steps_cap;
     std::vector<int> it_h((size_t)B * cap, 0);
@@ -1851,7 +2014,8 @@ int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, 
     int s = 0;
     while (s < nsteps) {
         const int W = std::min(m->window, nsteps - s);
-        const int K = m->iter_batch > 0 ? m->iter_batch : m->last_k;
+        // iterations (FP64) or sweeps (mixed) enqueued per step; more are handed to the host
+        const int K = m->iter_batch > 0 ? m->iter_batch : (m->precision == 1 ? 2 : m->last_k);
         if (usec) {
             free_events();
             ev.resize(W + 1);
@@ -1974,6 +2138,10 @@ int gade_extrapolate(gade_model_t *m, double *psi, int order) {
 
 int gade_cn_step(gade_model_t *m, double *psi, const double *ft, double tol, int maxit, int *iters, double *tol_error) {
     if (!m || !psi || !ft) return GADE_EINVAL;
+    if (m->precision != 0) {
+        gade::set_error("gade_cn_step is the FP64 host-driven step; mixed precision runs through gade_cn_run");
+        return GADE_EINVAL;
+    }
     GADE_TRY(gade_set_time(m, ft));
     // rhs = (S - Hfull) psi  (tools/dirac_solver.c:531-536)
     GADE_TRY(apply_async(m, (const double2 *)psi, m->tmp, -1.0, 0.0));
@@ -1996,7 +2164,7 @@ int gade_cn_run(gade_model_t *m, double *psi, double E0, double omega, double cy
         time = time + m->dt;
         times[step] = time;
     }
-    if (m->device_loop || m->B > 1) {
+    if (m->device_loop || m->B > 1 || m->precision == 1) {
         std::vector<double> params((size_t)4 * m->B);
         for (int b = 0; b < m->B; b++) {
             params[(size_t)4 * b] = E0;
@@ -2377,6 +2545,30 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->use_graph = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "precision")) {
+        if (value != 0 && value != 1) {
+            gade::set_error("precision must be 0 (FP64) or 1 (mixed)");
+            return GADE_EINVAL;
+        }
+        m->precision = value;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "mp_inner_iters")) {
+        if (value < 1 || value > 64) {
+            gade::set_error("mp_inner_iters must be 1..64");
+            return GADE_EINVAL;
+        }
+        m->mp_inner_iters = value;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "mp_inner_tol_exp")) {
+        if (value < 1 || value > 12) {
+            gade::set_error("mp_inner_tol_exp must be 1..12 (inner tolerance 10^-value)");
+            return GADE_EINVAL;
+        }
+        m->mp_inner_tol = std::pow(10.0, -value);
+        return GADE_OK;
+    }
     if (!strcmp(name, "guess_order")) {
         if (value < 0 || value > kMaxGuessOrder) {
             gade::set_error("guess_order must be 0..3");
@@ -2450,6 +2642,18 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "graph")) {
         *value = m->use_graph;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "precision")) {
+        *value = m->precision;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "mp_inner_iters")) {
+        *value = m->mp_inner_iters;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "mp_inner_tol_exp")) {
+        *value = (int)std::lround(-std::log10(m->mp_inner_tol));
         return GADE_OK;
     }
     if (!strcmp(name, "window")) {

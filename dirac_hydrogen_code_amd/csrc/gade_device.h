@@ -34,6 +34,47 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
     return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 __device__ __forceinline__ bool cnz(double2 a) { return a.x != 0.0 || a.y != 0.0; }
+// the same for single precision (the inner solve of the mixed-precision form)
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 acc) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+    return acc;
+}
+__device__ __forceinline__ float2 cfmaconj(float2 a, float2 b, float2 acc) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(-a.y, b.x, acc.y);
+    return acc;
+}
+__device__ __forceinline__ bool cnz(float2 a) { return a.x != 0.0f || a.y != 0.0f; }
+
+// complex type traits: real type and constructor of double2 / float2
+template <class T> struct cx;
+template <> struct cx<double2> {
+    using R = double;
+    static __device__ __forceinline__ double2 mk(double re, double im) { return make_double2(re, im); }
+};
+template <> struct cx<float2> {
+    using R = float;
+    static __device__ __forceinline__ float2 mk(float re, float im) { return make_float2(re, im); }
+};
+template <class T> __device__ __forceinline__ T czero() { return cx<T>::mk(0, 0); }
+__device__ __forceinline__ float2 to_f32(double2 a) { return make_float2((float)a.x, (float)a.y); }
+__device__ __forceinline__ double2 to_f64(float2 a) { return make_double2((double)a.x, (double)a.y); }
+__device__ __forceinline__ double2 to_f64(double2 a) { return a; }
+template <class T> __device__ __forceinline__ T from_f64(double2 a);
+template <> __device__ __forceinline__ double2 from_f64<double2>(double2 a) { return a; }
+template <> __device__ __forceinline__ float2 from_f64<float2>(double2 a) { return to_f32(a); }
+
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 // streaming (read-once) load of a complex value
 __device__ __forceinline__ double2 ldnt(const double2 *p) {
@@ -67,6 +108,7 @@ enum {
     CT_HSEL = 4,    // gate of work the host enqueues for selected instances (0 = selected)
     CT_MAXIT = 5,
     CT_HIST = 6,    // previous solutions held in the guess history
+    CT_ISKIP = 7,   // inner-solve gate of the mixed-precision form (set once the inner solve stops)
     CT_STRIDE = 8
 };
 // run-wide words (int, in the global control array)
@@ -175,12 +217,12 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh) {
 
 // One BiCGSTAB vector step over the N1 entries of instance blockIdx.y (vectors
 // are instance-major, N1 apart); partial sums to part[instance][block][4].
-template <int OP>
-__global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const double2 *__restrict__ b, double2 *__restrict__ r,
-                                                       double2 *__restrict__ r0, double2 *__restrict__ p,
-                                                       const double2 *__restrict__ v, double2 *__restrict__ s,
-                                                       const double2 *__restrict__ t, const double2 *__restrict__ yv,
-                                                       const double2 *__restrict__ zv, double2 *__restrict__ x,
+template <int OP, class T = double2>
+__global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const T *__restrict__ b, T *__restrict__ r,
+                                                       T *__restrict__ r0, T *__restrict__ p,
+                                                       const T *__restrict__ v, T *__restrict__ s,
+                                                       const T *__restrict__ t, const T *__restrict__ yv,
+                                                       const T *__restrict__ zv, T *__restrict__ x,
                                                        const double *__restrict__ sc0, double *__restrict__ part0,
                                                        const int *gate = nullptr, int gstride = 0) {
     const int inst = blockIdx.y;
@@ -201,40 +243,44 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const double
     }
     if (OP == OP_S || OP == OP_XR) alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
     if (OP == OP_XR) w = make_double2(sc[SC_W], sc[SC_W + 1]);
+    // the vector updates run in T, the dot products accumulate in double
+    const T betaT = from_f64<T>(beta), wT = from_f64<T>(w), alphaT = from_f64<T>(alpha);
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
         const long long i = o + j;
         if (OP == OP_INIT || OP == OP_RESTART) {
-            double2 ri = csub(b[i], r[i]);
-            r[i] = ri;
-            r0[i] = ri;
+            const T riT = csub(b[i], r[i]);
+            r[i] = riT;
+            r0[i] = riT;
+            const double2 ri = to_f64(riT);
             acc[0] = fma(ri.x, ri.x, fma(ri.y, ri.y, acc[0]));
             if (OP == OP_INIT) {
-                double2 bi = b[i];
+                const double2 bi = to_f64(b[i]);
                 acc[1] = fma(bi.x, bi.x, fma(bi.y, bi.y, acc[1]));
             }
         } else if (OP == OP_P) {
-            double2 pi = csub(p[i], cmul(w, v[i]));
-            p[i] = cadd(r[i], cmul(beta, pi));
+            const T pi = csub(p[i], cmul(wT, v[i]));
+            p[i] = cadd(r[i], cmul(betaT, pi));
         } else if (OP == OP_ALPHA) {
-            double2 d = cfmaconj(r0[i], v[i], make_double2(acc[0], acc[1]));
+            double2 d = cfmaconj(to_f64(r0[i]), to_f64(v[i]), make_double2(acc[0], acc[1]));
             acc[0] = d.x;
             acc[1] = d.y;
         } else if (OP == OP_S) {
-            s[i] = csub(r[i], cmul(alpha, v[i]));
+            s[i] = csub(r[i], cmul(alphaT, v[i]));
         } else if (OP == OP_W) {
-            double2 ti = t[i];
+            const double2 ti = to_f64(t[i]);
             acc[0] = fma(ti.x, ti.x, fma(ti.y, ti.y, acc[0]));
-            double2 d = cfmaconj(ti, s[i], make_double2(acc[1], acc[2]));
+            double2 d = cfmaconj(ti, to_f64(s[i]), make_double2(acc[1], acc[2]));
             acc[1] = d.x;
             acc[2] = d.y;
         } else if (OP == OP_XR) {
-            double2 xi = x[i];
-            xi = cadd(xi, cadd(cmul(alpha, yv[i]), cmul(w, zv[i])));
+            T xi = x[i];
+            xi = cadd(xi, cadd(cmul(alphaT, yv[i]), cmul(wT, zv[i])));
             x[i] = xi;
-            double2 ri = csub(s[i], cmul(w, t[i]));
-            r[i] = ri;
+            const T riT = csub(s[i], cmul(wT, t[i]));
+            r[i] = riT;
+            const double2 ri = to_f64(riT);
             acc[0] = fma(ri.x, ri.x, fma(ri.y, ri.y, acc[0]));
-            double2 d = cfmaconj(r0[i], ri, make_double2(acc[1], acc[2]));
+            double2 d = cfmaconj(to_f64(r0[i]), ri, make_double2(acc[1], acc[2]));
             acc[1] = d.x;
             acc[2] = d.y;
         }
@@ -247,7 +293,9 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const double
 
 // Sums the partials of instance blockIdx.x in a fixed order and updates its
 // scalars; with ctl, also takes the host loop's decisions for it.
-template <int OP>
+// INNER (the FP32 inner solve of the mixed-precision form): the only decision is when the inner
+// solve stops (CT_ISKIP), and CT_ITER counts every inner iteration of the step.
+template <int OP, bool INNER = false>
 __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const double *__restrict__ part0, double *sc0,
                                                            const int *gate = nullptr, int gstride = 0,
                                                            int *ctl0 = nullptr) {
@@ -294,6 +342,16 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
     }
     if (ctl0 == nullptr) return;
     int *ctl = ctl0 + (size_t)inst * CT_STRIDE;
+    if constexpr (INNER) {
+        if (OP == OP_XR) {
+            const int it = ctl[CT_ITER] + 1;
+            ctl[CT_ITER] = it;
+            if (!isfinite(tot[0]) || tot[0] < sc[SC_TOL2] || it >= ctl[CT_MAXIT] ||
+                hypot(tot[1], tot[2]) < kRestartEps2 * sc[SC_R0SQ])
+                ctl[CT_ISKIP] = 1;
+        }
+        return;
+    }
     // device-driven solve: the same decisions as the host loop (break-before-increment)
     int st = -1;
     if (OP == OP_INIT) {
@@ -322,6 +380,129 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg_fin(int nparts, const doub
         ctl[CT_STATUS] = st;
         ctl[CT_SKIP] = st != 0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: FP64 defect correction around an FP32 inner BiCGSTAB.
+// Per sweep: d = A^{-1} r approximately in FP32 (from d = 0), x += d in FP64,
+// r = b - A x in FP64, and the FP64 residual decides convergence.
+// ---------------------------------------------------------------------------
+// inner start of instance blockIdx.y: ri = r0i = fp32(r), d = p = v = 0; partial |ri|^2
+[[maybe_unused]] static __global__ __launch_bounds__(kRedThreads) void k_mp_start(
+    long long N1, const double2 *__restrict__ r, float2 *__restrict__ ri, float2 *__restrict__ r0i,
+    float2 *__restrict__ d, float2 *__restrict__ p, float2 *__restrict__ v, double *__restrict__ part0,
+    const int *gate, int gstride) {
+    const int inst = blockIdx.y;
+    GADE_GATED_B(gate, gstride, inst);
+    __shared__ double sh[4 * 16];
+    double *part = part0 + ((size_t)inst * gridDim.x) * 4;
+    double acc[4] = {0, 0, 0, 0};
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)inst * N1 + j;
+        const float2 rv = to_f32(r[i]);
+        ri[i] = rv;
+        r0i[i] = rv;
+        d[i] = make_float2(0.f, 0.f);
+        p[i] = make_float2(0.f, 0.f);  // a fresh Krylov start for every sweep
+        v[i] = make_float2(0.f, 0.f);
+        const double2 vd = to_f64(rv);
+        acc[0] = fma(vd.x, vd.x, fma(vd.y, vd.y, acc[0]));
+    }
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[(size_t)blockIdx.x * 4 + k] = acc[k];
+}
+
+// inner scalars of instance blockIdx.x (as OP_INIT, with the inner tolerance), and its inner gate:
+// closed when the outer solve is decided or the rounded residual is zero
+[[maybe_unused]] static __global__ __launch_bounds__(kRedThreads) void k_mp_start_fin(
+    int nparts, const double *__restrict__ part0, double *sci0, int *ctl0, double tol_in) {
+    const int inst = blockIdx.x;
+    __shared__ double sh[4 * 16];
+    const double *part = part0 + (size_t)inst * nparts * 4;
+    double tot[4] = {0, 0, 0, 0};
+    for (int bidx = threadIdx.x; bidx < nparts; bidx += blockDim.x) tot[0] += part[(size_t)bidx * 4];
+    block_sum<4>(tot, sh);
+    if (threadIdx.x != 0) return;
+    int *ctl = ctl0 + (size_t)inst * CT_STRIDE;
+    if (ctl[CT_SKIP] != 0) {
+        ctl[CT_ISKIP] = 1;
+        return;
+    }
+    double *sc = sci0 + (size_t)inst * SC_STRIDE;
+    sc[SC_R0SQ] = tot[0];
+    sc[SC_RHSSQ] = tot[0];
+    sc[SC_RNORM2] = tot[0];
+    sc[SC_RHONEXT] = tot[0];
+    sc[SC_RHONEXT + 1] = 0;
+    sc[SC_RHO] = 1; sc[SC_RHO + 1] = 0;
+    sc[SC_ALPHA] = 1; sc[SC_ALPHA + 1] = 0;
+    sc[SC_W] = 1; sc[SC_W + 1] = 0;
+    sc[SC_TOL2] = tol_in * tol_in * tot[0];
+    ctl[CT_ISKIP] = tot[0] > 0 ? 0 : 1;
+}
+
+// x += d (FP64 update with the FP32 correction), instance blockIdx.y
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_mp_update(long long N1, double2 *__restrict__ x,
+                                                                          const float2 *__restrict__ d,
+                                                                          const int *gate, int gstride) {
+    const int inst = blockIdx.y;
+    GADE_GATED_B(gate, gstride, inst);
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)inst * N1 + j;
+        x[i] = cadd(x[i], to_f64(d[i]));
+    }
+}
+
+// r = b - A x (FP64) and partial |r|^2, instance blockIdx.y
+[[maybe_unused]] static __global__ __launch_bounds__(kRedThreads) void k_mp_resid(
+    long long N1, const double2 *__restrict__ b, const double2 *__restrict__ ax, double2 *__restrict__ r,
+    double *__restrict__ part0, const int *gate, int gstride) {
+    const int inst = blockIdx.y;
+    GADE_GATED_B(gate, gstride, inst);
+    __shared__ double sh[4 * 16];
+    double *part = part0 + ((size_t)inst * gridDim.x) * 4;
+    double acc[4] = {0, 0, 0, 0};
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)inst * N1 + j;
+        const double2 v = csub(b[i], ax[i]);
+        r[i] = v;
+        acc[0] = fma(v.x, v.x, fma(v.y, v.y, acc[0]));
+    }
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; k++) part[(size_t)blockIdx.x * 4 + k] = acc[k];
+}
+
+// outer decision of instance blockIdx.x after a sweep: converged (FP64 tolerance), non-finite (host),
+// iteration budget spent, or another sweep
+[[maybe_unused]] static __global__ __launch_bounds__(kRedThreads) void k_mp_resid_fin(
+    int nparts, const double *__restrict__ part0, double *sc0, int *ctl0, const int *gate, int gstride) {
+    const int inst = blockIdx.x;
+    GADE_GATED_B(gate, gstride, inst);
+    __shared__ double sh[4 * 16];
+    const double *part = part0 + (size_t)inst * nparts * 4;
+    double tot[4] = {0, 0, 0, 0};
+    for (int bidx = threadIdx.x; bidx < nparts; bidx += blockDim.x) tot[0] += part[(size_t)bidx * 4];
+    block_sum<4>(tot, sh);
+    if (threadIdx.x != 0) return;
+    double *sc = sc0 + (size_t)inst * SC_STRIDE;
+    int *ctl = ctl0 + (size_t)inst * CT_STRIDE;
+    sc[SC_RNORM2] = tot[0];
+    const int st = !isfinite(tot[0]) ? 2 : (tot[0] < sc[SC_TOL2] ? 1 : (ctl[CT_ITER] >= ctl[CT_MAXIT] ? 3 : 0));
+    ctl[CT_STATUS] = st;
+    ctl[CT_SKIP] = st != 0;
+}
+
+// y = fp32(x) over n entries, blockIdx.y = instance of 'stride' entries (gated per instance)
+[[maybe_unused]] static __global__ __launch_bounds__(256) void k_to_f32(long long n, size_t stride,
+                                                                       const double2 *__restrict__ x,
+                                                                       float2 *__restrict__ y, const int *gate,
+                                                                       int gstride) {
+    GADE_GATED_B(gate, gstride, blockIdx.y);
+    const size_t o = (size_t)blockIdx.y * stride;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        y[o + i] = to_f32(x[o + i]);
 }
 
 // Start of a device-driven CN step for instance blockIdx.x: idle when the step
@@ -524,7 +705,14 @@ __device__ __forceinline__ double dpp_quad_swap(double v, int ctrl_is_xor2) {
     }
     return __hiloint2double(h2, l2);
 }
-__device__ __forceinline__ double2 quad_sum(double2 v) {
+__device__ __forceinline__ float dpp_quad_swap(float v, int ctrl_is_xor2) {
+    const int i = __float_as_int(v);
+    const int o = ctrl_is_xor2 ? __builtin_amdgcn_update_dpp(0, i, 0x4E, 0xF, 0xF, false)
+                               : __builtin_amdgcn_update_dpp(0, i, 0xB1, 0xF, 0xF, false);
+    return __int_as_float(o);
+}
+template <class T>
+__device__ __forceinline__ T quad_sum(T v) {
     v.x += dpp_quad_swap(v.x, 0);
     v.y += dpp_quad_swap(v.y, 0);
     v.x += dpp_quad_swap(v.x, 1);
@@ -538,30 +726,31 @@ __device__ __forceinline__ double2 quad_sum(double2 v) {
 // b is staged in LDS; the factors of level ch+PF are loaded as soon as the
 // register slot of level ch is free, so PF levels of loads are in flight.
 // Requires CH = 16 and band <= 16 (coupling rows padded to 16, zero beyond band).
-template <int PF>
-__global__ __launch_bounds__(64) void k_psolve_q(const double2 *__restrict__ b, double2 *__restrict__ xo,
+template <int PF, class T = double2>
+__global__ __launch_bounds__(64) void k_psolve_q(const T *__restrict__ b, T *__restrict__ xo,
                                                  const int *__restrict__ order, const int *__restrict__ blk_q,
-                                                 int ilv, const double2 *__restrict__ Linv,
-                                                 const double2 *__restrict__ Uinv, const double2 *__restrict__ Loff,
-                                                 const double2 *__restrict__ Uoff, int n, int band, int nch,
+                                                 int ilv, const T *__restrict__ Linv,
+                                                 const T *__restrict__ Uinv, const T *__restrict__ Loff,
+                                                 const T *__restrict__ Uoff, int n, int band, int nch,
                                                  const int *gate = nullptr, int gstride = 0, int nblk1 = 1) {
     constexpr int CH = 16;
-    extern __shared__ double2 smem[];
+    extern __shared__ __align__(16) char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
     const int lane = threadIdx.x, i = lane >> 2, k = lane & 3;
     const int r = order[blockIdx.x];
     if (r < 0) return;  // padding slot (whole workgroup)
     GADE_GATED_B(gate, gstride, r / nblk1);
     const int q = blk_q[r];
-    double2 *yv = smem;              // [nch*CH]: b, then y, then x (interleaved order)
-    double2 *acc = smem + nch * CH;  // [CH]
-    const double2 *bb = b + (size_t)r * n;
+    T *yv = smem;              // [nch*CH]: b, then y, then x (interleaved order)
+    T *acc = smem + nch * CH;  // [CH]
+    const T *bb = b + (size_t)r * n;
     const int ntot = nch * CH;
     for (int c0 = 0; c0 < ntot; c0 += 64 * 8) {
-        double2 t[8];
+        T t[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int c = c0 + 64 * u + lane;
-            t[u] = (c < n) ? bb[natural_index(c, n, ilv)] : make_double2(0, 0);
+            t[u] = (c < n) ? bb[natural_index(c, n, ilv)] : czero<T>();
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -571,11 +760,11 @@ __global__ __launch_bounds__(64) void k_psolve_q(const double2 *__restrict__ b, 
     }
     __syncthreads();
     const size_t cs = (size_t)CH * CH;
-    double2 fc[PF][4], fv[PF][4];
+    T fc[PF][4], fv[PF][4];
     // factors of level ch into slot s: coupling rows 4k..4k+3 and inverse columns 4k..4k+3 of row i
-    auto load = [&](const double2 *Off, const double2 *Inv, int ch, double2 (&c)[4], double2 (&v)[4]) {
-        const double2 *po = Off + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
-        const double2 *pi = Inv + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
+    auto load = [&](const T *Off, const T *Inv, int ch, T (&c)[4], T (&v)[4]) {
+        const T *po = Off + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
+        const T *pi = Inv + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             c[u] = po[(size_t)u * CH];
@@ -583,18 +772,18 @@ __global__ __launch_bounds__(64) void k_psolve_q(const double2 *__restrict__ b, 
         }
     };
     // one level: val = b_i - sum_t C[t][i] y[cbase + t];  y_i = sum_j Inv[j][i] val_j
-    auto level = [&](int ch, int cbase, bool couple, const double2 (&c)[4], const double2 (&v)[4]) {
+    auto level = [&](int ch, int cbase, bool couple, const T (&c)[4], const T (&v)[4]) {
         const int gi = ch * CH + i;
-        double2 s = make_double2(0, 0);
+        T s = czero<T>();
         if (couple) {
 #pragma unroll
             for (int u = 0; u < 4; u++) s = cfma(c[u], yv[cbase + 4 * k + u], s);
             s = quad_sum(s);
         }
-        const double2 val = csub(yv[gi], s);
+        const T val = csub(yv[gi], s);
         if (k == 0) acc[i] = val;
         __syncthreads();
-        double2 y = make_double2(0, 0);
+        T y = czero<T>();
 #pragma unroll
         for (int u = 0; u < 4; u++) y = cfma(v[u], acc[4 * k + u], y);
         y = quad_sum(y);

@@ -161,8 +161,19 @@ struct gade_model {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    const void *graph_key[8] = {};  // what the captured step depends on
+    const void *graph_key[11] = {};  // what the captured step depends on
     double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions (per instance)
+    // mixed precision: FP32 inner BiCGSTAB with the FP32 operator and preconditioner, FP64 defect correction
+    int precision = 0;             // 0: FP64 (the reference), 1: mixed
+    int mp_inner_iters = 2;        // inner iterations per sweep
+    double mp_inner_tol = 1e-6;    // inner relative tolerance
+    bool mp_ready = false;
+    float2 *d_V32 = nullptr, *d_ypart32 = nullptr, *d_GD32 = nullptr, *d_DD32 = nullptr, *d_coef32 = nullptr;
+    float2 *lu32[4] = {};          // Linv, Uinv, Loff, Uoff
+    float2 *mp_r = nullptr, *mp_r0 = nullptr, *mp_p = nullptr, *mp_v = nullptr, *mp_y = nullptr, *mp_z = nullptr,
+           *mp_s = nullptr, *mp_t = nullptr, *mp_d = nullptr;
+    double2 *d_mp_ax = nullptr;    // A x of the outer residual
+    double *d_sc32 = nullptr;      // inner scalars [B][SC_STRIDE]
     // profiling: event pairs are recorded without synchronizing and resolved
     // by gade_profile_get; 'profile' is a mask of GADE_PROF_* families
     int profile = 0;
