@@ -85,9 +85,6 @@ __global__ __launch_bounds__(256) void k_copy(const double2 *__restrict__ src, d
 // ---------------------------------------------------------------------------
 // kernels: per-step G-sums and stationary diagonal operators
 // ---------------------------------------------------------------------------
-struct Coef6 {
-    double2 c[6];
-};
 // the pulse coefficients of up to kCoefGroup instances, passed by value to one launch
 constexpr int kCoefGroup = 32;
 struct CoefGroup {
@@ -872,6 +869,8 @@ void free_all(gade_model *m) {
     if (m->h_ctl) hipHostFree(m->h_ctl);
     if (m->h_gctl) hipHostFree(m->h_gctl);
     if (m->h_cfwin) hipHostFree(m->h_cfwin);
+    gade::lg_free(m->lg);
+    m->lg = nullptr;
     if (m->d_cfwin) hipFree(m->d_cfwin);
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
@@ -1592,6 +1591,8 @@ __global__ void k_ctl_set(int *ctl0, int *gctl, double *sc0, double tol, int max
 
 // x0 by extrapolation of each instance's history (order m->guess_order); no-op for order 0
 int launch_guess(gade_model *m, double2 *psi) {
+    if (m->guess_order == GADE_GUESS_LEARNED || (m->lg_features && m->guess_order > 0))
+        return gade::lg_step_prologue(m, psi);
     if (m->guess_order <= 0) return GADE_OK;
     if (!m->d_hist) GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
     hipLaunchKernelGGL(k_extrap, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream, m->N1, m->N, psi,
@@ -1614,6 +1615,8 @@ struct GateScope {
     }
 };
 
+}  // namespace
+namespace gade {
 // the step graph holds device addresses of the model's buffers: drop it before any of them is replaced
 void drop_step_graph(gade_model *m) {
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
@@ -1622,6 +1625,9 @@ void drop_step_graph(gade_model *m) {
     m->graph = nullptr;
     memset(m->graph_key, 0, sizeof(m->graph_key));
 }
+}  // namespace gade
+using gade::drop_step_graph;
+namespace {
 
 int ensure_step_buffers(gade_model *m, int nsteps) {
     if (nsteps <= m->steps_cap) return GADE_OK;
@@ -1868,15 +1874,16 @@ int enqueue_step_window(gade_model *m, double2 *psi, int K, bool per_instance_st
 int step_graph(gade_model *m, double2 *psi, int K, bool per_instance_steps) {
     if (m->guess_order > 0 && !m->d_hist)  // no allocation while capturing
         GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+    if (m->guess_order == GADE_GUESS_LEARNED || m->lg_features) GADE_TRY(gade::lg_prepare(m));
     // everything the captured launches bake in that can change between runs (buffers that are replaced
     // on growth also drop the graph themselves)
     intptr_t tol_bits = 0;
     memcpy(&tol_bits, &m->mp_inner_tol, sizeof(double) < sizeof(intptr_t) ? sizeof(double) : sizeof(intptr_t));
-    const void *key[11] = {psi,      (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
+    const void *key[12] = {psi,      (const void *)(intptr_t)K, (const void *)(intptr_t)m->guess_order,
                            (const void *)(intptr_t)(per_instance_steps ? 1 : 0), m->d_iters,
                            (const void *)(intptr_t)m->steps_cap, m->d_cfwin, m->d_hist,
                            (const void *)(intptr_t)m->precision, (const void *)(intptr_t)m->mp_inner_iters,
-                           (const void *)tol_bits};
+                           (const void *)tol_bits, (const void *)(intptr_t)m->lg_features};
     static_assert(sizeof(key) == sizeof(m->graph_key), "graph key size");
     if (m->graph_exec && !memcmp(key, m->graph_key, sizeof(key))) return GADE_OK;
     drop_step_graph(m);
@@ -2121,6 +2128,7 @@ int gade_reset_workspace(gade_model_t *m) {
     for (auto w : ws) GADE_HIP(hipMemsetAsync(w, 0, sizeof(double2) * m->N, m->stream));
     hipLaunchKernelGGL(k_hist_reset, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl);  // the guess histories restart
     GADE_HIP(hipGetLastError());
+    if (m->lg) GADE_TRY(gade::lg_reset(m));
     GADE_HIP(hipStreamSynchronize(m->stream));
     return GADE_OK;
 }
@@ -2138,8 +2146,9 @@ int gade_extrapolate(gade_model_t *m, double *psi, int order) {
 
 int gade_cn_step(gade_model_t *m, double *psi, const double *ft, double tol, int maxit, int *iters, double *tol_error) {
     if (!m || !psi || !ft) return GADE_EINVAL;
-    if (m->precision != 0) {
-        gade::set_error("gade_cn_step is the FP64 host-driven step; mixed precision runs through gade_cn_run");
+    if (m->precision != 0 || m->guess_order == GADE_GUESS_LEARNED || m->lg_features) {
+        gade::set_error("gade_cn_step is the FP64 host-driven step; mixed precision and the learned guess run "
+                        "through gade_cn_run / gade_cn_run_batch");
         return GADE_EINVAL;
     }
     GADE_TRY(gade_set_time(m, ft));
@@ -2570,11 +2579,15 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         return GADE_OK;
     }
     if (!strcmp(name, "guess_order")) {
-        if (value < 0 || value > kMaxGuessOrder) {
-            gade::set_error("guess_order must be 0..3");
+        if (value < 0 || (value > kMaxGuessOrder && value != GADE_GUESS_LEARNED)) {
+            gade::set_error("guess_order must be 0..3 (extrapolation) or 4 (learned)");
             return GADE_EINVAL;
         }
         m->guess_order = value;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "lg_features")) {
+        m->lg_features = value != 0;
         return GADE_OK;
     }
     if (!strcmp(name, "window") || !strcmp(name, "iter_batch")) {
@@ -2646,6 +2659,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "precision")) {
         *value = m->precision;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "lg_features")) {
+        *value = m->lg_features;
         return GADE_OK;
     }
     if (!strcmp(name, "mp_inner_iters")) {

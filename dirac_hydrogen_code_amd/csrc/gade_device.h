@@ -99,6 +99,11 @@ __device__ __forceinline__ bool gated(const int *gate, int gstride, int b) {
     } while (0)
 #define GADE_GATED(gate) GADE_GATED_B(gate, 0, 0)
 
+// pulse coefficients c_alpha = SoL * (i h dt / 2) * f_alpha of one instance (alpha = 0..5)
+struct Coef6 {
+    double2 c[6];
+};
+
 // per-instance control words of the device-driven solve (int, CT_STRIDE apart)
 enum {
     CT_STATUS = 0,  // 0 iterating, 1 converged, 2 needs the host (restart, zero rhs, non-finite), 3 maxit reached
@@ -115,7 +120,7 @@ enum {
 enum { GC_STALLED = 0, GC_STALL_STEP = 1, GC_STEP = 2, GC_WSLOT = 3, GC_COUNT = 4 };
 
 constexpr int kMaxGuessOrder = 3;  // polynomial extrapolation order of the initial guess
-constexpr int kHistSlots = kMaxGuessOrder;
+constexpr int kHistSlots = 6;  // previous solutions kept (the learned guess mixes psi_n..psi_{n-5})
 
 // Initial guess by polynomial extrapolation of the step history:
 // x0 = sum_{k=0..p} c_k psi_{n-k}, c_k = (-1)^k binom(p+1, k+1) (Lagrange at

@@ -56,6 +56,8 @@ struct gade_blocklu {
     int norder = 0;
 };
 
+struct gade_lg;  // learned initial guess (learned.hip)
+
 namespace gade {
 // Pq: values of each distinct block in the CSR order of (rAp, rAi), [nq][nnz]
 int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
@@ -65,6 +67,10 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
 void blocklu_free(gade_blocklu *lu);
 int hip_alloc(void **p, size_t bytes);
 }  // namespace gade
+
+namespace gade {
+void lg_free(gade_lg *lg);
+}
 
 struct gade_model {
     // sizes.  A model holds B instances (pulses) that share the couplings: rows are
@@ -161,8 +167,10 @@ struct gade_model {
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    const void *graph_key[11] = {};  // what the captured step depends on
-    double2 *d_hist = nullptr;     // [3][N] history ring of previous solutions (per instance)
+    const void *graph_key[12] = {};  // what the captured step depends on
+    double2 *d_hist = nullptr;     // [kHistSlots][N] history ring of previous solutions (per instance)
+    gade_lg *lg = nullptr;         // learned initial guess
+    int lg_features = 0;           // compute the learned guess's inputs every step (training data)
     // mixed precision: FP32 inner BiCGSTAB with the FP32 operator and preconditioner, FP64 defect correction
     int precision = 0;             // 0: FP64 (the reference), 1: mixed
     int mp_inner_iters = 2;        // inner iterations per sweep
@@ -184,3 +192,10 @@ struct gade_model {
     int prof_floor_ns = 0;                     // timed launches shorter than this are not counted
     std::map<std::string, long long> prof_seen;
 };
+
+namespace gade {
+void drop_step_graph(gade_model *m);
+int lg_prepare(gade_model *m);
+int lg_reset(gade_model *m);  // residual history of the learned guess's inputs
+int lg_step_prologue(gade_model *m, double2 *psi);
+}  // namespace gade

@@ -1,0 +1,1011 @@
+// Learned initial guess for the CN solve (no reference counterpart).
+//
+// A small MLP (3 hidden layers of 256, GELU, BF16 weights, FP32 accumulation on
+// MFMA) predicts the initial guess of every step from f_alpha(t_{n+1}), the last
+// six log residual norms, the sizes of the backward differences of the solution
+// history and its last two higher differences in a POD basis:
+//
+//   x0 = x2 + s * sum_{k=0..5} w_k D_k / |D_k| + s * U c
+//
+// with D_k = nabla^k psi_n (backward differences of psi_n..psi_{n-5}),
+// x2 = psi_n + D_1 + D_2 the order-2 polynomial extrapolation, s = |D_3| (the
+// size of its leading error term), U the npod POD vectors of extrapolation
+// defects (top left singular vectors of solver snapshots) and (w, c) the
+// network's outputs (6 real mixing weights, npod complex POD coefficients).  A
+// zero output is the extrapolation itself, which is also the fallback while the
+// history is short or an output is not finite.  The converged solve at the
+// unchanged tolerance judges every guess: only the iteration count can change.
+//
+// Training minimises the relative squared error |psi_{n+1} - x0|^2 / |e2|^2
+// (e2 = psi_{n+1} - x2), which is a quadratic form in the outputs; each training
+// sample therefore holds the small projected quantities of that form (G, a, P, q,
+// beta) next to its inputs, and no full vector is touched while training.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "gade_device.h"
+#include "gade_internal.h"
+
+using gade::hip_alloc;
+using gade::set_error;
+
+namespace {
+
+constexpr int kIn = GADE_LG_IN, kHid = GADE_LG_HIDDEN, kOut = GADE_LG_OUT, kPod = GADE_LG_POD;
+constexpr int kSample = GADE_LG_SAMPLE;
+constexpr int kNV = 7;  // vectors of the difference stage: D_0..D_5, e2 (record)
+constexpr int kNG = kNV * (kNV + 1) / 2;  // Gram entries Re(V_i^H V_j), i <= j
+// sample layout (floats)
+constexpr int S_FEAT = 0, S_G = kIn, S_A = S_G + 36, S_P = S_A + 6, S_Q = S_P + kPod * 6 * 2, S_BETA = S_Q + kPod * 2,
+              S_VALID = S_BETA + 1;
+static_assert(S_VALID < kSample, "sample layout");
+// feature layout
+constexpr int F_F = 0, F_LRES = 6, F_DN = 12, F_Z3 = 17, F_Z4 = F_Z3 + 2 * kPod, F_END = F_Z4 + 2 * kPod;
+static_assert(F_END <= kIn, "feature layout");
+// parameter layout (floats): W1 b1 W2 b2 W3 b3 W4 b4, W_l row-major [out][in]
+constexpr size_t P_W1 = 0, P_B1 = P_W1 + (size_t)kHid * kIn, P_W2 = P_B1 + kHid, P_B2 = P_W2 + (size_t)kHid * kHid,
+                 P_W3 = P_B2 + kHid, P_B3 = P_W3 + (size_t)kHid * kHid, P_W4 = P_B3 + kHid,
+                 P_B4 = P_W4 + (size_t)kOut * kHid, P_TOTAL = P_B4 + kOut;
+constexpr int kLayers = 4;
+const size_t kWoff[kLayers] = {P_W1, P_W2, P_W3, P_W4}, kBoff[kLayers] = {P_B1, P_B2, P_B3, P_B4};
+const int kLin[kLayers] = {kIn, kHid, kHid, kHid}, kLout[kLayers] = {kHid, kHid, kHid, kOut};
+
+__host__ __device__ inline int pad16(int x) { return (x + 15) & ~15; }
+inline unsigned grid_for_lg(long long n) {
+    long long g = (n + 255) / 256;
+    return (unsigned)(g < 1 ? 1 : (g > 2048 ? 2048 : g));
+}
+
+// ---------------------------------------------------------------------------
+// bf16 (round to nearest even) and GELU
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint16_t f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA GEMM: C(m,n) = sum_k A(m,k) B(k,n), FP32 accumulation, with
+// A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn] (bf16 bits).  One wave
+// per 16x16 tile of C (v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3],
+// B[4(l/16)..+3][l%16] and C[4(l/16)+i][l%16]).  M, N, K multiples of 16.
+// ---------------------------------------------------------------------------
+typedef short bf4 __attribute__((ext_vector_type(4)));
+typedef float fl4 __attribute__((ext_vector_type(4)));
+enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
+
+template <int EPI>
+__global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ a, long sam,
+                                                  long sak, const uint16_t *__restrict__ b, long sbk, long sbn,
+                                                  float *__restrict__ c, int ldc, const float *__restrict__ bias,
+                                                  uint16_t *__restrict__ act, int ldact) {
+    const int tiles_n = N / 16;
+    const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+    if (tm * 16 >= M) return;
+    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const uint16_t *ap = a + (long)(tm * 16 + r) * sam;
+    const uint16_t *bp = b + (long)(tn * 16 + r) * sbn;
+    fl4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf4 av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long kk = k0 + 4 * g + j;
+            av[j] = (short)ap[kk * sak];
+            bv[j] = (short)bp[kk * sbk];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = tm * 16 + 4 * g + i, n = tn * 16 + r;
+        float v = acc[i];
+        if (EPI != EPI_STORE) v += bias[n];
+        c[(long)m * ldc + n] = v;
+        if (EPI == EPI_BIAS_GELU) act[(long)m * ldact + n] = f2bf(gelu(v));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// difference stage: V_0 = psi_n, V_k = nabla^k psi_n (k = 1..5), V_6 = e2 (record:
+// psi_{n+1} - (psi_n + V_1 + V_2)), and the partial Gram sums Re(V_i^H V_j).
+// Inference (rec = 0, before psi_n enters the ring): psi_n = psi, psi_{n-k} in
+// ring slot (have - k) mod S.  Record (rec = 1, after): psi_n in slot (have - 1),
+// psi_{n-k} in slot (have - 1 - k), psi_{n+1} = psi.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lg_diff(long long N1, long long Ntot, const double2 *__restrict__ psi,
+                                                 const double2 *__restrict__ hist, const int *__restrict__ ctl0,
+                                                 int rec, double2 *__restrict__ V, double *__restrict__ part0,
+                                                 const int *gate, int gstride) {
+    const int b = blockIdx.y;
+    GADE_GATED_B(gate, gstride, b);
+    __shared__ double sh[kNG * 4];
+    const int have = ctl0[(size_t)b * CT_STRIDE + CT_HIST];
+    const double2 *h[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int back = rec ? k + 1 : k;
+        if (!rec && k == 0) {
+            h[k] = psi;
+        } else {
+            const int sl = (((have - back) % kHistSlots) + kHistSlots) % kHistSlots;
+            h[k] = hist + (size_t)sl * Ntot;
+        }
+    }
+    double acc[kNG];
+#pragma unroll
+    for (int t = 0; t < kNG; t++) acc[t] = 0;
+    const int nv = rec ? kNV : kNV - 1;
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)b * N1 + j;
+        double2 x[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) x[k] = h[k][i];
+        // backward differences nabla^k psi_n = sum_j (-1)^j binom(k, j) psi_{n-j}, by repeated differencing
+        double2 v[kNV];
+        v[0] = x[0];
+#pragma unroll
+        for (int k = 1; k < 6; k++) {
+#pragma unroll
+            for (int q = 5; q >= k; q--) x[q] = csub(x[q - 1], x[q]);
+            v[k] = x[k];
+        }
+        v[6] = rec ? csub(psi[i], cadd(v[0], cadd(v[1], v[2]))) : make_double2(0, 0);
+#pragma unroll
+        for (int k = 0; k < kNV; k++)
+            if (k < nv) V[(size_t)k * Ntot + i] = v[k];
+        int t = 0;
+#pragma unroll
+        for (int p = 0; p < kNV; p++)
+#pragma unroll
+            for (int q = p; q < kNV; q++, t++) acc[t] += v[p].x * v[q].x + v[p].y * v[q].y;
+    }
+    // block reduction of the kNG sums
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < kNG; t++) {
+        double s = wave_sum(acc[t]);
+        if (lane == 0) sh[wid * kNG + t] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNG) {
+        double s = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += sh[w * kNG + threadIdx.x];
+        part0[((size_t)b * gridDim.x + blockIdx.x) * kNG + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_lg_gram_fin(int nparts, const double *__restrict__ part0, double *__restrict__ gram,
+                              const int *gate, int gstride) {
+    const int b = blockIdx.x;
+    GADE_GATED_B(gate, gstride, b);
+    const int t = threadIdx.x;
+    if (t >= kNG) return;
+    double s = 0;
+    for (int p = 0; p < nparts; p++) s += part0[((size_t)b * nparts + p) * kNG + t];
+    gram[(size_t)b * kNG + t] = s;
+}
+
+__device__ __forceinline__ int gidx(int p, int q) {  // index of Re(V_p^H V_q), p <= q
+    return p * kNV - p * (p - 1) / 2 + (q - p);
+}
+
+// out[b][v][j] = U_j^H V_{v0+v} (instance b), one block per (j, b)
+__global__ __launch_bounds__(256) void k_lg_proj(long long N1, long long Ntot, int npod,
+                                                 const double2 *__restrict__ U, const double2 *__restrict__ V,
+                                                 int v0, int nv, double2 *__restrict__ out, const int *gate,
+                                                 int gstride) {
+    const int j = blockIdx.x, b = blockIdx.y;
+    GADE_GATED_B(gate, gstride, b);
+    __shared__ double sh[2 * kNV * 4];
+    double acc[2 * kNV];
+#pragma unroll
+    for (int t = 0; t < 2 * kNV; t++) acc[t] = 0;
+    const double2 *u = U + (size_t)j * N1;
+    for (long long k = threadIdx.x; k < N1; k += blockDim.x) {
+        const double2 uu = u[k];
+#pragma unroll
+        for (int v = 0; v < kNV; v++) {
+            if (v >= nv) break;
+            const double2 x = V[(size_t)(v0 + v) * Ntot + (size_t)b * N1 + k];
+            acc[2 * v] += uu.x * x.x + uu.y * x.y;
+            acc[2 * v + 1] += uu.x * x.y - uu.y * x.x;
+        }
+    }
+    block_sum<2 * kNV>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int v = 0; v < nv; v++) out[((size_t)b * kNV + v) * npod + j] = make_double2(acc[2 * v], acc[2 * v + 1]);
+}
+
+// log10 residual ring [b][8], newest first: push the previous solve's final relative residual
+__global__ void k_lg_lres_push(const double *__restrict__ sc0, double *__restrict__ lres, const int *gate,
+                               int gstride) {
+    const int b = blockIdx.x;
+    GADE_GATED_B(gate, gstride, b);
+    if (threadIdx.x != 0) return;
+    const double *sc = sc0 + (size_t)b * SC_STRIDE;
+    double *r = lres + (size_t)b * 8;
+    for (int k = 7; k > 0; k--) r[k] = r[k - 1];
+    const double t = sc[SC_RHSSQ] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]) : 0.0;
+    r[0] = log10(fmax(t, 1e-30));
+}
+
+// raw inputs of instance b and its standardized bf16 row.  f_alpha(t_{n+1}) comes from the window's
+// coefficient table (c_alpha = hd * i * f_alpha, f real).
+__global__ void k_lg_features(const Coef6 *__restrict__ cf_dev, const int *__restrict__ gctl, int B, double hd,
+                              const double *__restrict__ lres, const double *__restrict__ gram,
+                              const double2 *__restrict__ z, int npod, const float *__restrict__ mean,
+                              const float *__restrict__ stdv, float *__restrict__ feat, uint16_t *__restrict__ xin,
+                              const int *gate, int gstride) {
+    const int b = blockIdx.x;
+    GADE_GATED_B(gate, gstride, b);
+    const Coef6 *cf = cf_dev + (size_t)gctl[GC_WSLOT] * B + b;
+    const double *g = gram + (size_t)b * kNG;
+    const double n0 = sqrt(fmax(g[gidx(0, 0)], 1e-300));
+    const double s = sqrt(g[gidx(3, 3)]);
+    const double inv_s = s > 0 ? 1.0 / s : 0.0;
+    float *fr = feat + (size_t)b * kIn;
+    for (int t = threadIdx.x; t < kIn; t += blockDim.x) {
+        double v = 0;
+        if (t < F_LRES) v = cf->c[t - F_F].y / hd;
+        else if (t < F_DN) v = lres[(size_t)b * 8 + (t - F_LRES)];
+        else if (t < F_Z3) v = 0.5 * log10(fmax(g[gidx(t - F_DN + 1, t - F_DN + 1)], 1e-300)) - log10(n0);
+        else if (t < F_Z4) {
+            const int q = (t - F_Z3) >> 1;
+            const double2 zz = q < npod ? z[((size_t)b * kNV + 0) * npod + q] : make_double2(0, 0);
+            v = ((t - F_Z3) & 1 ? zz.y : zz.x) * inv_s;
+        } else if (t < F_END) {
+            const int q = (t - F_Z4) >> 1;
+            const double2 zz = q < npod ? z[((size_t)b * kNV + 1) * npod + q] : make_double2(0, 0);
+            v = ((t - F_Z4) & 1 ? zz.y : zz.x) * inv_s;
+        }
+        fr[t] = (float)v;
+        xin[(size_t)b * kIn + t] = t < F_END ? f2bf(((float)v - mean[t]) / stdv[t]) : (uint16_t)0;
+    }
+}
+
+// the guess of instance b into psi (psi_n enters the ring); fallback: order-2 extrapolation (or lower)
+__global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, double2 *__restrict__ psi,
+                                                  double2 *__restrict__ hist, const double2 *__restrict__ V,
+                                                  const double *__restrict__ gram, const float *__restrict__ Y,
+                                                  const double2 *__restrict__ U, int npod,
+                                                  const int *__restrict__ ctl0, const int *gate, int gstride) {
+    const int b = blockIdx.y;
+    GADE_GATED_B(gate, gstride, b);
+    __shared__ double2 cs[kPod];
+    __shared__ double ws[6];
+    __shared__ int ok_s;
+    const int have = ctl0[(size_t)b * CT_STRIDE + CT_HIST];
+    const double *g = gram + (size_t)b * kNG;
+    const double s = sqrt(g[gidx(3, 3)]);
+    const float *y = Y + (size_t)b * kOut;
+    if (threadIdx.x == 0) {
+        bool ok = have >= 5 && s > 0 && isfinite(s);
+        for (int k = 0; k < 6 + 2 * npod; k++) ok = ok && isfinite(y[k]);
+        ok_s = ok;
+    }
+    __syncthreads();
+    const bool ok = ok_s != 0;
+    if (ok) {
+        for (int k = threadIdx.x; k < 6; k += blockDim.x) {
+            const double nk = sqrt(g[gidx(k, k)]);
+            ws[k] = nk > 0 ? s * y[k] / nk : 0.0;
+        }
+        for (int q = threadIdx.x; q < npod; q += blockDim.x) cs[q] = make_double2(s * y[6 + 2 * q], s * y[7 + 2 * q]);
+    }
+    __syncthreads();
+    const int p = have < 2 ? have : 2;  // extrapolation order of the fallback
+    const int slot0 = have % kHistSlots;
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = (long long)b * N1 + j;
+        const double2 xn = psi[i];
+        double2 x;
+        if (ok) {
+            // x2 + sum_k w_k V_k + U c
+            x = cadd(xn, cadd(V[(size_t)1 * Ntot + i], V[(size_t)2 * Ntot + i]));
+            x = cadd(x, cscale(xn, ws[0]));
+#pragma unroll
+            for (int k = 1; k < 6; k++) x = cadd(x, cscale(V[(size_t)k * Ntot + i], ws[k]));
+            for (int q = 0; q < npod; q++) x = cfma(cs[q], U[(size_t)q * N1 + j], x);
+        } else {
+            // Lagrange extrapolation of order p through psi_n..psi_{n-p}
+            const double c2[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
+            x = cscale(xn, c2[p][0]);
+            for (int k = 1; k <= p; k++) {
+                const int sl = ((slot0 - k) % kHistSlots + kHistSlots) % kHistSlots;
+                x = cadd(x, cscale(hist[(size_t)sl * Ntot + i], c2[p][k]));
+            }
+        }
+        hist[(size_t)slot0 * Ntot + i] = xn;
+        psi[i] = x;
+    }
+}
+
+// training sample of instance b after its step (record mode of k_lg_diff / k_lg_proj): the projected
+// quadratic form of the normalized error, next to the inputs saved at the step's start
+__global__ void k_lg_record(const double *__restrict__ gram, const double2 *__restrict__ proj, int npod,
+                            const float *__restrict__ feat, const int *__restrict__ ctl0, float *__restrict__ out) {
+    const int b = blockIdx.x;
+    const double *g = gram + (size_t)b * kNG;
+    const int *ctl = ctl0 + (size_t)b * CT_STRIDE;
+    float *o = out + (size_t)b * kSample;
+    double nrm[kNV];
+    for (int k = 0; k < kNV; k++) nrm[k] = sqrt(fmax(g[gidx(k, k)], 0.0));
+    const double s = nrm[3];
+    const bool valid = ctl[CT_HIST] >= 6 && ctl[CT_PSKIP] == 0 && s > 0 && isfinite(s) &&
+                       (ctl[CT_STATUS] == 1 || ctl[CT_STATUS] == 3);
+    for (int t = threadIdx.x; t < kSample; t += blockDim.x) {
+        double v = 0;
+        if (!valid) {
+            v = 0;
+        } else if (t < S_G) {
+            v = feat[(size_t)b * kIn + t];
+        } else if (t < S_A) {
+            const int k = (t - S_G) / 6, l = (t - S_G) % 6;
+            v = g[k <= l ? gidx(k, l) : gidx(l, k)] / (nrm[k] * nrm[l]);
+        } else if (t < S_P) {
+            const int k = t - S_A;
+            v = g[gidx(k, 6)] / (nrm[k] * s);
+        } else if (t < S_Q) {
+            const int e = t - S_P, q = e / 12, k = (e % 12) >> 1;
+            const double2 pv = proj[((size_t)b * kNV + k) * npod + q];
+            v = ((e & 1) ? pv.y : pv.x) / nrm[k];
+        } else if (t < S_BETA) {
+            const int e = t - S_Q, q = e >> 1;
+            const double2 qv = proj[((size_t)b * kNV + 6) * npod + q];
+            v = ((e & 1) ? qv.y : qv.x) / s;
+        } else if (t == S_BETA) {
+            v = g[gidx(6, 6)] / (s * s);
+        }
+        if (t == S_VALID) v = valid ? 1.0 : 0.0;
+        o[t] = (float)v;
+    }
+}
+
+// normalized extrapolation defect e2 / |e2| of instance b into snapshot column col0 + b (valid instances)
+__global__ void k_lg_snap(long long N1, long long Ntot, const double2 *__restrict__ V,
+                          const double *__restrict__ gram, const int *__restrict__ ctl0, double2 *__restrict__ snaps,
+                          int col0, int ncols) {
+    const int b = blockIdx.y;
+    const int *ctl = ctl0 + (size_t)b * CT_STRIDE;
+    if (col0 + b >= ncols || ctl[CT_HIST] < 6 || ctl[CT_PSKIP] != 0) return;
+    const double ne = sqrt(gram[(size_t)b * kNG + gidx(6, 6)]);
+    if (!(ne > 0)) return;
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x)
+        snaps[(size_t)(col0 + b) * N1 + j] = cscale(V[(size_t)6 * Ntot + (size_t)b * N1 + j], 1.0 / ne);
+}
+
+// ---------------------------------------------------------------------------
+// training kernels
+// ---------------------------------------------------------------------------
+// X[m] = standardized bf16 inputs of sample idx[m]
+__global__ void k_lg_gather(const float *__restrict__ samples, const int *__restrict__ idx, const float *mean,
+                            const float *stdv, uint16_t *__restrict__ X) {
+    const int m = blockIdx.x;
+    const float *s = samples + (size_t)idx[m] * kSample;
+    for (int t = threadIdx.x; t < kIn; t += blockDim.x)
+        X[(size_t)m * kIn + t] = t < F_END ? f2bf((s[t] - mean[t]) / stdv[t]) : (uint16_t)0;
+}
+
+// X[m] = standardized bf16 inputs of raw row m (stride kIn)
+__global__ void k_lg_gather_rows(const float *__restrict__ x, const float *mean, const float *stdv,
+                                 uint16_t *__restrict__ X) {
+    const int m = blockIdx.x;
+    for (int t = threadIdx.x; t < kIn; t += blockDim.x)
+        X[(size_t)m * kIn + t] = t < F_END ? f2bf((x[(size_t)m * kIn + t] - mean[t]) / stdv[t]) : (uint16_t)0;
+}
+
+// per sample: loss L = |e_hat - D w - U c|^2 / beta (the quadratic form) and dL/dY / M
+__global__ __launch_bounds__(64) void k_lg_loss(const float *__restrict__ Y, const float *__restrict__ samples,
+                                                const int *__restrict__ idx, int M, float *__restrict__ dY,
+                                                float *__restrict__ loss) {
+    const int m = blockIdx.x, t = threadIdx.x;
+    const float *y = Y + (size_t)m * kOut;
+    const float *s = samples + (size_t)idx[m] * kSample;
+    __shared__ float w[6], pw[2 * kPod], part[64];
+    if (t < 6) w[t] = y[t];
+    __syncthreads();
+    const float beta = s[S_BETA];
+    const float ib = beta > 0 ? 1.0f / beta : 0.0f;
+    float contrib = 0;
+    // POD part, one coefficient per lane
+    {
+        const int q = t;
+        float pr = 0, pi = 0;
+        for (int k = 0; k < 6; k++) {
+            pr += s[S_P + (q * 6 + k) * 2] * w[k];
+            pi += s[S_P + (q * 6 + k) * 2 + 1] * w[k];
+        }
+        pw[2 * q] = pr;
+        pw[2 * q + 1] = pi;
+        const float cr = y[6 + 2 * q], ci = y[7 + 2 * q];
+        const float qr = s[S_Q + 2 * q], qi = s[S_Q + 2 * q + 1];
+        contrib += -2.f * (cr * qr + ci * qi) + cr * cr + ci * ci + 2.f * (cr * pr + ci * pi);
+        dY[(size_t)m * kOut + 6 + 2 * q] = 2.f * ib * (cr - qr + pr) / M;
+        dY[(size_t)m * kOut + 7 + 2 * q] = 2.f * ib * (ci - qi + pi) / M;
+    }
+    __syncthreads();
+    if (t < 6) {
+        float gw = 0;
+        for (int l = 0; l < 6; l++) gw += s[S_G + t * 6 + l] * w[l];
+        float pc = 0;
+        for (int q = 0; q < kPod; q++) pc += s[S_P + (q * 6 + t) * 2] * y[6 + 2 * q] + s[S_P + (q * 6 + t) * 2 + 1] * y[7 + 2 * q];
+        contrib += -2.f * w[t] * s[S_A + t] + w[t] * gw;
+        dY[(size_t)m * kOut + t] = 2.f * ib * (-s[S_A + t] + gw + pc) / M;
+    }
+    if (t < kOut - (6 + 2 * kPod)) dY[(size_t)m * kOut + 6 + 2 * kPod + t] = 0.f;  // padding outputs
+    part[t] = contrib;
+    __syncthreads();
+    if (t == 0) {
+        float tot = 0;
+        for (int k = 0; k < 64; k++) tot += part[k];
+        loss[m] = beta > 0 ? 1.0f + tot * ib : 0.f;
+    }
+}
+
+// mean of loss[0..M) into out[0]
+__global__ void k_lg_mean(const float *__restrict__ loss, int M, float *__restrict__ out) {
+    __shared__ float sh[256];
+    float s = 0;
+    for (int m = threadIdx.x; m < M; m += blockDim.x) s += loss[m];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0;
+        for (int k = 0; k < (int)blockDim.x; k++) t += sh[k];
+        out[0] = t / M;
+    }
+}
+
+// bf16 copy of an FP32 matrix
+__global__ void k_lg_tobf(long n, const float *__restrict__ x, uint16_t *__restrict__ y) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = f2bf(x[i]);
+}
+
+// dZ = dA * gelu'(Z), and its bf16 copy
+__global__ void k_lg_gelu_back(long n, const float *__restrict__ dA, const float *__restrict__ Z, float *__restrict__ dZ,
+                               uint16_t *__restrict__ G) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float v = dA[i] * gelu_d(Z[i]);
+        dZ[i] = v;
+        G[i] = f2bf(v);
+    }
+}
+
+// column sums db[n] = sum_m dZ[m][n]
+__global__ void k_lg_colsum(int M, int N, const float *__restrict__ dZ, float *__restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0;
+    for (int m = 0; m < M; m++) s += dZ[(size_t)m * N + n];
+    db[n] = s;
+}
+
+// Adam step on the FP32 parameters, refreshing the bf16 weights
+__global__ void k_lg_adam(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m1,
+                          float *__restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
+                          uint16_t *__restrict__ pbf) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float a = b1 * m1[i] + (1.f - b1) * gi;
+        const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = v;
+        const float pi = p[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+        p[i] = pi;
+        pbf[i] = f2bf(pi);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// model-side state
+// ---------------------------------------------------------------------------
+struct gade_lg {
+    int npod = 0;
+    double2 *U = nullptr;         // [npod][N1] POD basis (orthonormal columns)
+    double2 *V = nullptr;         // [7][N] difference stage vectors
+    double *part = nullptr;       // [B][nparts][kNG]
+    int nparts = 0;
+    double *gram = nullptr;       // [B][kNG]
+    double2 *proj = nullptr;      // [B][7][npod]
+    double *lres = nullptr;       // [B][8]
+    float *feat = nullptr;        // [B][kIn] raw inputs of the current step
+    float *mean = nullptr, *stdv = nullptr;  // [kIn]
+    int Mi = 0;                   // inference rows (B padded to 16)
+    uint16_t *xin = nullptr;      // [Mi][kIn]
+    float *zi[3] = {}, *yi = nullptr;  // inference pre-activations [Mi][kHid], outputs [Mi][kOut]
+    uint16_t *ai[3] = {};         // inference activations
+    // network
+    bool net = false;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;  // [P_TOTAL]
+    uint16_t *pbf = nullptr;      // bf16 copy (weights)
+    long adam_t = 0;
+    // training buffers (batch Mt)
+    int Mt = 0;
+    uint16_t *xt = nullptr, *at[3] = {}, *gt[4] = {};
+    float *zt[3] = {}, *yt = nullptr, *dy = nullptr, *da = nullptr, *dz = nullptr, *loss = nullptr;
+};
+
+namespace gade {
+
+void lg_free(gade_lg *lg) {
+    if (!lg) return;
+    std::vector<void *> ptrs = {lg->U, lg->V, lg->part, lg->gram, lg->proj, lg->lres, lg->feat, lg->mean, lg->stdv,
+                                lg->xin, lg->yi, lg->p, lg->g, lg->m1, lg->m2, lg->pbf, lg->xt, lg->yt, lg->dy,
+                                lg->da, lg->dz, lg->loss};
+    for (int k = 0; k < 3; k++) {
+        ptrs.push_back(lg->zi[k]);
+        ptrs.push_back(lg->ai[k]);
+        ptrs.push_back(lg->at[k]);
+        ptrs.push_back(lg->zt[k]);
+    }
+    for (int k = 0; k < 4; k++) ptrs.push_back(lg->gt[k]);
+    for (void *q : ptrs)
+        if (q) (void)hipFree(q);
+    delete lg;
+}
+
+}  // namespace gade
+
+namespace {
+
+int lg_ensure(gade_model *m) {
+    if (m->lg) return GADE_OK;
+    gade_lg *lg = new gade_lg();
+    m->lg = lg;
+    const size_t N = (size_t)m->N, B = (size_t)m->B;
+    lg->nparts = 64;
+    lg->Mi = pad16(m->B);
+    GADE_TRY(hip_alloc((void **)&lg->V, sizeof(double2) * kNV * N));
+    GADE_TRY(hip_alloc((void **)&lg->part, sizeof(double) * B * lg->nparts * kNG));
+    GADE_TRY(hip_alloc((void **)&lg->gram, sizeof(double) * B * kNG));
+    GADE_TRY(hip_alloc((void **)&lg->proj, sizeof(double2) * B * kNV * kPod));
+    GADE_TRY(hip_alloc((void **)&lg->lres, sizeof(double) * B * 8));
+    GADE_TRY(hip_alloc((void **)&lg->feat, sizeof(float) * B * kIn));
+    GADE_TRY(hip_alloc((void **)&lg->mean, sizeof(float) * kIn));
+    GADE_TRY(hip_alloc((void **)&lg->stdv, sizeof(float) * kIn));
+    GADE_TRY(hip_alloc((void **)&lg->xin, sizeof(uint16_t) * lg->Mi * kIn));
+    GADE_TRY(hip_alloc((void **)&lg->yi, sizeof(float) * lg->Mi * kOut));
+    for (int k = 0; k < 3; k++) {
+        GADE_TRY(hip_alloc((void **)&lg->zi[k], sizeof(float) * lg->Mi * kHid));
+        GADE_TRY(hip_alloc((void **)&lg->ai[k], sizeof(uint16_t) * lg->Mi * kHid));
+    }
+    GADE_HIP(hipMemset(lg->xin, 0, sizeof(uint16_t) * lg->Mi * kIn));
+    GADE_HIP(hipMemset(lg->lres, 0, sizeof(double) * B * 8));
+    GADE_HIP(hipMemset(lg->proj, 0, sizeof(double2) * B * kNV * kPod));
+    std::vector<float> one(kIn, 1.0f);
+    GADE_HIP(hipMemset(lg->mean, 0, sizeof(float) * kIn));
+    GADE_HIP(hipMemcpy(lg->stdv, one.data(), sizeof(float) * kIn, hipMemcpyHostToDevice));
+    if (!m->d_hist) GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+    return GADE_OK;
+}
+
+int lg_net_alloc(gade_lg *lg) {
+    if (lg->net) return GADE_OK;
+    for (float **q : {&lg->p, &lg->g, &lg->m1, &lg->m2}) GADE_TRY(hip_alloc((void **)q, sizeof(float) * P_TOTAL));
+    GADE_TRY(hip_alloc((void **)&lg->pbf, sizeof(uint16_t) * P_TOTAL));
+    GADE_HIP(hipMemset(lg->g, 0, sizeof(float) * P_TOTAL));
+    lg->net = true;
+    return GADE_OK;
+}
+
+int lg_train_alloc(gade_lg *lg, int Mt) {
+    if (lg->Mt >= Mt) return GADE_OK;
+    for (void *q : {(void *)lg->xt, (void *)lg->yt, (void *)lg->dy, (void *)lg->da, (void *)lg->dz, (void *)lg->loss})
+        if (q) (void)hipFree(q);
+    for (int k = 0; k < 3; k++) {
+        if (lg->at[k]) (void)hipFree(lg->at[k]);
+        if (lg->zt[k]) (void)hipFree(lg->zt[k]);
+    }
+    for (int k = 0; k < 4; k++)
+        if (lg->gt[k]) (void)hipFree(lg->gt[k]);
+    lg->Mt = Mt;
+    GADE_TRY(hip_alloc((void **)&lg->xt, sizeof(uint16_t) * Mt * kIn));
+    GADE_TRY(hip_alloc((void **)&lg->yt, sizeof(float) * Mt * kOut));
+    GADE_TRY(hip_alloc((void **)&lg->dy, sizeof(float) * Mt * kOut));
+    GADE_TRY(hip_alloc((void **)&lg->da, sizeof(float) * Mt * kHid));
+    GADE_TRY(hip_alloc((void **)&lg->dz, sizeof(float) * Mt * kHid));
+    GADE_TRY(hip_alloc((void **)&lg->loss, sizeof(float) * Mt));
+    for (int k = 0; k < 3; k++) {
+        GADE_TRY(hip_alloc((void **)&lg->at[k], sizeof(uint16_t) * Mt * kHid));
+        GADE_TRY(hip_alloc((void **)&lg->zt[k], sizeof(float) * Mt * kHid));
+    }
+    for (int k = 0; k < 4; k++) GADE_TRY(hip_alloc((void **)&lg->gt[k], sizeof(uint16_t) * Mt * kHid));
+    return GADE_OK;
+}
+
+// forward of M rows: X (bf16 [M][kIn]) -> Z_l, A_l, Y
+void mlp_forward(const gade_lg *lg, hipStream_t st, int M, const uint16_t *X, float *const Z[3], uint16_t *const A[3],
+                 float *Y) {
+    const uint16_t *in = X;
+    for (int l = 0; l < kLayers; l++) {
+        const int K = kLin[l], N = kLout[l];
+        const dim3 grid((M / 16) * (N / 16));
+        const uint16_t *W = lg->pbf + kWoff[l];
+        const float *bias = lg->p + kBoff[l];
+        if (l < 3)
+            hipLaunchKernelGGL(k_gemm_bf16<EPI_BIAS_GELU>, grid, dim3(64), 0, st, M, N, K, in, (long)K, 1L, W, 1L,
+                               (long)K, Z[l], N, bias, A[l], N);
+        else
+            hipLaunchKernelGGL(k_gemm_bf16<EPI_BIAS>, grid, dim3(64), 0, st, M, N, K, in, (long)K, 1L, W, 1L, (long)K,
+                               Y, N, bias, (uint16_t *)nullptr, 0);
+        if (l < 3) in = A[l];
+    }
+}
+
+// backward of M rows from dY (FP32 [M][kOut]): gradients of every parameter into lg->g
+void mlp_backward(gade_lg *lg, hipStream_t st, int M, const uint16_t *X) {
+    hipLaunchKernelGGL(k_lg_tobf, dim3(64), dim3(256), 0, st, (long)M * kOut, (const float *)lg->dy, lg->gt[3]);
+    const float *dZ = lg->dy;
+    for (int l = kLayers - 1; l >= 0; l--) {
+        const int K = kLin[l], N = kLout[l];
+        const uint16_t *Gl = lg->gt[l];
+        const uint16_t *Ain = l > 0 ? lg->at[l - 1] : X;
+        // dW[n][k] = sum_m dZ[m][n] A[m][k]
+        hipLaunchKernelGGL(k_gemm_bf16<EPI_STORE>, dim3((N / 16) * (K / 16)), dim3(64), 0, st, N, K, M, Gl, 1L,
+                           (long)N, Ain, (long)K, 1L, lg->g + kWoff[l], K, (const float *)nullptr,
+                           (uint16_t *)nullptr, 0);
+        hipLaunchKernelGGL(k_lg_colsum, dim3((N + 255) / 256), dim3(256), 0, st, M, N, dZ, lg->g + kBoff[l]);
+        if (l > 0) {
+            // dA[m][k] = sum_n dZ[m][n] W[n][k], then dZ_{l-1} = dA * gelu'(Z_{l-1})
+            hipLaunchKernelGGL(k_gemm_bf16<EPI_STORE>, dim3((M / 16) * (K / 16)), dim3(64), 0, st, M, K, N, Gl,
+                               (long)N, 1L, lg->pbf + kWoff[l], (long)K, 1L, lg->da, K, (const float *)nullptr,
+                               (uint16_t *)nullptr, 0);
+            hipLaunchKernelGGL(k_lg_gelu_back, dim3(64), dim3(256), 0, st, (long)M * K, (const float *)lg->da,
+                               (const float *)lg->zt[l - 1], lg->dz, lg->gt[l - 1]);
+            dZ = lg->dz;
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// hooks used by the CN step
+// ---------------------------------------------------------------------------
+namespace gade {
+
+// buffers of the learned guess (no allocation may happen while a step graph is captured)
+int lg_prepare(gade_model *m) { return lg_ensure(m); }
+
+int lg_reset(gade_model *m) {
+    if (!m->lg) return GADE_OK;
+    GADE_HIP(hipMemsetAsync(m->lg->lres, 0, sizeof(double) * m->B * 8, m->stream));
+    return GADE_OK;
+}
+
+// Step prologue of the learned guess (guess_order == GADE_GUESS_LEARNED) or of the training-data mode
+// (lg_features with an extrapolated guess): inputs of this step, then the guess into psi.
+int lg_step_prologue(gade_model *m, double2 *psi) {
+    GADE_TRY(lg_ensure(m));
+    gade_lg *lg = m->lg;
+    const bool learned = m->guess_order == GADE_GUESS_LEARNED;
+    if (!m->d_cfwin || m->radial_kind != 2) {
+        set_error("learned guess: runs in the device-driven stepping with the banded radial kernel");
+        return GADE_EINVAL;
+    }
+    if (learned && (!lg->net || lg->npod == 0)) {
+        set_error("learned guess: no trained network (gade_lg_train) or POD basis (gade_lg_pod_build)");
+        return GADE_EINVAL;
+    }
+    const int B = m->B;
+    const long long N1 = m->N1, N = m->N;
+    const dim3 gd(lg->nparts, B);
+    hipLaunchKernelGGL(k_lg_lres_push, dim3(B), dim3(64), 0, m->stream, (const double *)m->d_sc, lg->lres, m->gate,
+                       m->gstride);
+    hipLaunchKernelGGL(k_lg_diff, gd, dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
+                       (const double2 *)m->d_hist, (const int *)m->d_ctl, 0, lg->V, lg->part, m->gate, m->gstride);
+    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(64), 0, m->stream, lg->nparts, (const double *)lg->part,
+                       lg->gram, m->gate, m->gstride);
+    if (lg->npod > 0)
+        hipLaunchKernelGGL(k_lg_proj, dim3(lg->npod, B), dim3(256), 0, m->stream, N1, N, lg->npod,
+                           (const double2 *)lg->U, (const double2 *)lg->V, 3, 2, lg->proj, m->gate, m->gstride);
+    const double hd = kSoL * m->h * m->dt / 2;
+    hipLaunchKernelGGL(k_lg_features, dim3(B), dim3(64), 0, m->stream, (const Coef6 *)m->d_cfwin,
+                       (const int *)m->d_gctl, B, hd, (const double *)lg->lres, (const double *)lg->gram,
+                       (const double2 *)lg->proj, lg->npod, (const float *)lg->mean, (const float *)lg->stdv,
+                       lg->feat, lg->xin, m->gate, m->gstride);
+    GADE_HIP(hipGetLastError());
+    if (learned) {
+        mlp_forward(lg, m->stream, lg->Mi, lg->xin, lg->zi, lg->ai, lg->yi);
+        hipLaunchKernelGGL(k_lg_guess, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
+                           (const double2 *)lg->V, (const double *)lg->gram, (const float *)lg->yi,
+                           (const double2 *)lg->U, lg->npod, (const int *)m->d_ctl, m->gate, m->gstride);
+    } else {
+        hipLaunchKernelGGL(k_extrap, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
+                           m->guess_order, (const int *)m->d_ctl, m->gate, m->gstride);
+    }
+    hipLaunchKernelGGL(k_hist_count, dim3(B), dim3(64), 0, m->stream, m->d_ctl, m->gate, m->gstride);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+}  // namespace gade
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int gade_lg_record(gade_model_t *m, const double *psi, float *samples, double *snaps, int snap_col, int snap_cols) {
+    if (!m || !psi) return GADE_EINVAL;
+    GADE_TRY(lg_ensure(m));
+    gade_lg *lg = m->lg;
+    const int B = m->B;
+    const long long N1 = m->N1, N = m->N;
+    hipLaunchKernelGGL(k_lg_diff, dim3(lg->nparts, B), dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
+                       (const double2 *)m->d_hist, (const int *)m->d_ctl, 1, lg->V, lg->part, (const int *)nullptr,
+                       0);
+    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(64), 0, m->stream, lg->nparts, (const double *)lg->part,
+                       lg->gram, (const int *)nullptr, 0);
+    if (snaps)
+        hipLaunchKernelGGL(k_lg_snap, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N,
+                           (const double2 *)lg->V, (const double *)lg->gram, (const int *)m->d_ctl,
+                           (double2 *)snaps, snap_col, snap_cols);
+    if (samples) {
+        if (lg->npod == 0) {
+            set_error("gade_lg_record: samples need the POD basis (gade_lg_pod_build)");
+            return GADE_EINVAL;
+        }
+        hipLaunchKernelGGL(k_lg_proj, dim3(lg->npod, B), dim3(256), 0, m->stream, N1, N, lg->npod,
+                           (const double2 *)lg->U, (const double2 *)lg->V, 0, kNV, lg->proj, (const int *)nullptr, 0);
+        hipLaunchKernelGGL(k_lg_record, dim3(B), dim3(256), 0, m->stream, (const double *)lg->gram,
+                           (const double2 *)lg->proj, lg->npod, (const float *)lg->feat, (const int *)m->d_ctl,
+                           samples);
+    }
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod, double *sv) {
+    if (!m || !snaps || npod <= 0 || npod > kPod || nsnap < npod) return GADE_EINVAL;
+    GADE_TRY(lg_ensure(m));
+    gade_lg *lg = m->lg;
+    const int n = (int)m->N1;
+    if (nsnap > n) return GADE_EINVAL;
+    gade::drop_step_graph(m);  // the graph reads the basis being replaced
+    double2 *A = nullptr, *tau = nullptr, *R = nullptr, *Sv = nullptr, *Uq = nullptr, *C = nullptr;
+    double *S = nullptr, *E = nullptr;
+    int *info = nullptr;
+    auto cleanup = [&]() {
+        for (void *q : {(void *)A, (void *)tau, (void *)R, (void *)Sv, (void *)Uq, (void *)C, (void *)S, (void *)E,
+                        (void *)info})
+            if (q) (void)hipFree(q);
+    };
+    int rc = GADE_OK;
+    const size_t ns = (size_t)nsnap;
+    if ((rc = hip_alloc((void **)&A, sizeof(double2) * n * ns)) || (rc = hip_alloc((void **)&tau, sizeof(double2) * ns)) ||
+        (rc = hip_alloc((void **)&R, sizeof(double2) * ns * ns)) || (rc = hip_alloc((void **)&Sv, sizeof(double2) * ns * ns)) ||
+        (rc = hip_alloc((void **)&Uq, sizeof(double2) * ns * ns)) || (rc = hip_alloc((void **)&C, sizeof(double2) * n * npod)) ||
+        (rc = hip_alloc((void **)&S, sizeof(double) * ns)) || (rc = hip_alloc((void **)&E, sizeof(double) * ns)) ||
+        (rc = hip_alloc((void **)&info, sizeof(int)))) {
+        cleanup();
+        return rc;
+    }
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) {
+        cleanup();
+        set_error("gade_lg_pod_build: rocblas_create_handle failed");
+        return GADE_EHIP;
+    }
+    rocblas_set_stream(h, m->stream);
+    // snapshots [col][n] are the column-major n x nsnap matrix X; X = Q R; R = Ur S Vr^H; U = Q Ur[:, :npod]
+    bool ok = hipMemcpyAsync(A, snaps, sizeof(double2) * n * ns, hipMemcpyDeviceToDevice, m->stream) == hipSuccess;
+    ok = ok && rocsolver_zgeqrf(h, n, nsnap, (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau) ==
+                   rocblas_status_success;
+    // R: upper triangle of the first nsnap rows
+    std::vector<double2> Ah;
+    if (ok) {
+        Ah.resize((size_t)n * ns);
+        ok = hipMemcpyAsync(Ah.data(), A, sizeof(double2) * n * ns, hipMemcpyDeviceToHost, m->stream) == hipSuccess &&
+             hipStreamSynchronize(m->stream) == hipSuccess;
+    }
+    std::vector<double2> Rh(ns * ns, make_double2(0, 0));
+    if (ok) {
+        for (size_t c = 0; c < ns; c++)
+            for (size_t r = 0; r <= c; r++) Rh[c * ns + r] = Ah[c * n + r];
+        ok = hipMemcpy(R, Rh.data(), sizeof(double2) * ns * ns, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    ok = ok && rocsolver_zgesvd(h, rocblas_svect_all, rocblas_svect_none, nsnap, nsnap, (rocblas_double_complex *)R,
+                                nsnap, S, (rocblas_double_complex *)Uq, nsnap, (rocblas_double_complex *)Sv, nsnap, E,
+                                rocblas_outofplace, info) == rocblas_status_success;
+    // C = [Ur[:, :npod]; 0] (n x npod), then U = Q C
+    std::vector<double2> Uh, Ch;
+    if (ok) {
+        Uh.resize(ns * ns);
+        ok = hipMemcpyAsync(Uh.data(), Uq, sizeof(double2) * ns * ns, hipMemcpyDeviceToHost, m->stream) == hipSuccess &&
+             hipStreamSynchronize(m->stream) == hipSuccess;
+    }
+    if (ok) {
+        Ch.assign((size_t)n * npod, make_double2(0, 0));
+        for (int c = 0; c < npod; c++)
+            for (size_t r = 0; r < ns; r++) Ch[(size_t)c * n + r] = Uh[(size_t)c * ns + r];
+        ok = hipMemcpy(C, Ch.data(), sizeof(double2) * n * npod, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    ok = ok && rocsolver_zunmqr(h, rocblas_side_left, rocblas_operation_none, n, npod, nsnap,
+                                (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau,
+                                (rocblas_double_complex *)C, n) == rocblas_status_success;
+    int info_h = 0;
+    if (ok)
+        ok = hipMemcpy(&info_h, info, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && info_h == 0 &&
+             hipStreamSynchronize(m->stream) == hipSuccess;
+    rocblas_destroy_handle(h);
+    if (!ok) {
+        cleanup();
+        set_error("gade_lg_pod_build: rocSOLVER QR / SVD failed (info " + std::to_string(info_h) + ")");
+        return GADE_EHIP;
+    }
+    if (sv) {
+        std::vector<double> Sh(ns);
+        (void)hipMemcpy(Sh.data(), S, sizeof(double) * ns, hipMemcpyDeviceToHost);
+        memcpy(sv, Sh.data(), sizeof(double) * ns);
+    }
+    if (lg->U) (void)hipFree(lg->U);
+    lg->U = C;
+    C = nullptr;
+    lg->npod = npod;
+    cleanup();
+    return GADE_OK;
+}
+
+int gade_lg_basis(gade_model_t *m, double *U) {
+    if (!m || !U || !m->lg || !m->lg->U) return GADE_EINVAL;
+    GADE_HIP(hipMemcpy(U, m->lg->U, sizeof(double2) * m->N1 * m->lg->npod, hipMemcpyDeviceToHost));
+    return GADE_OK;
+}
+
+int gade_lg_set_norm(gade_model_t *m, const float *mean, const float *stdv) {
+    if (!m || !mean || !stdv) return GADE_EINVAL;
+    GADE_TRY(lg_ensure(m));
+    for (int t = 0; t < kIn; t++)
+        if (!(stdv[t] > 0)) {
+            set_error("gade_lg_set_norm: every standard deviation must be positive");
+            return GADE_EINVAL;
+        }
+    GADE_HIP(hipMemcpy(m->lg->mean, mean, sizeof(float) * kIn, hipMemcpyHostToDevice));
+    GADE_HIP(hipMemcpy(m->lg->stdv, stdv, sizeof(float) * kIn, hipMemcpyHostToDevice));
+    return GADE_OK;
+}
+
+int gade_lg_init(gade_model_t *m, unsigned seed) {
+    if (!m) return GADE_EINVAL;
+    GADE_TRY(lg_ensure(m));
+    GADE_TRY(lg_net_alloc(m->lg));
+    // He-normal hidden layers, zero output layer and biases: the untrained network is the extrapolation
+    std::vector<float> p(P_TOTAL, 0.0f);
+    std::mt19937 rng(seed);
+    for (int l = 0; l < 3; l++) {
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / kLin[l]));
+        for (size_t i = 0; i < (size_t)kLout[l] * kLin[l]; i++) p[kWoff[l] + i] = nd(rng);
+    }
+    return gade_lg_set_params(m, p.data());
+}
+
+int gade_lg_set_params(gade_model_t *m, const float *params) {
+    if (!m || !params) return GADE_EINVAL;
+    GADE_TRY(lg_ensure(m));
+    gade_lg *lg = m->lg;
+    GADE_TRY(lg_net_alloc(lg));
+    GADE_HIP(hipMemcpy(lg->p, params, sizeof(float) * P_TOTAL, hipMemcpyHostToDevice));
+    GADE_HIP(hipMemset(lg->m1, 0, sizeof(float) * P_TOTAL));
+    GADE_HIP(hipMemset(lg->m2, 0, sizeof(float) * P_TOTAL));
+    lg->adam_t = 0;
+    hipLaunchKernelGGL(k_lg_tobf, dim3(256), dim3(256), 0, m->stream, (long)P_TOTAL, (const float *)lg->p, lg->pbf);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_lg_get_params(gade_model_t *m, float *params) {
+    if (!m || !params || !m->lg || !m->lg->net) return GADE_EINVAL;
+    GADE_HIP(hipMemcpy(params, m->lg->p, sizeof(float) * P_TOTAL, hipMemcpyDeviceToHost));
+    return GADE_OK;
+}
+
+// forward of M rows of raw inputs (device FP32 [M][GADE_LG_IN], M a multiple of 16) into Y [M][GADE_LG_OUT]
+int gade_lg_forward(gade_model_t *m, const float *x, int M, float *y) {
+    if (!m || !x || !y || M <= 0 || M % 16) return GADE_EINVAL;
+    if (!m->lg || !m->lg->net) return GADE_EINVAL;
+    gade_lg *lg = m->lg;
+    GADE_TRY(lg_train_alloc(lg, M));
+    hipLaunchKernelGGL(k_lg_gather_rows, dim3(M), dim3(64), 0, m->stream, x, (const float *)lg->mean,
+                       (const float *)lg->stdv, lg->xt);
+    mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
+    hipError_t e = hipMemcpyAsync(y, lg->yt, sizeof(float) * M * kOut, hipMemcpyDeviceToDevice, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e != hipSuccess) {
+        set_error(std::string("gade_lg_forward: ") + hipGetErrorString(e));
+        return GADE_EHIP;
+    }
+    return GADE_OK;
+}
+
+// loss and gradient (no update) of the batch idx [M] of samples: loss_out = mean loss, grad (device FP32
+// [GADE_LG_PARAMS], may be null) = its gradient
+int gade_lg_grad(gade_model_t *m, const float *samples, const int *idx, int M, float *loss_out, float *grad) {
+    if (!m || !samples || !idx || M <= 0 || M % 16) return GADE_EINVAL;
+    if (!m->lg || !m->lg->net) return GADE_EINVAL;
+    gade_lg *lg = m->lg;
+    GADE_TRY(lg_train_alloc(lg, M));
+    float *d_mean = nullptr;
+    GADE_TRY(hip_alloc((void **)&d_mean, sizeof(float)));
+    hipLaunchKernelGGL(k_lg_gather, dim3(M), dim3(64), 0, m->stream, samples, idx, (const float *)lg->mean,
+                       (const float *)lg->stdv, lg->xt);
+    mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
+    hipLaunchKernelGGL(k_lg_loss, dim3(M), dim3(64), 0, m->stream, (const float *)lg->yt, samples, idx, M, lg->dy,
+                       lg->loss);
+    hipLaunchKernelGGL(k_lg_mean, dim3(1), dim3(256), 0, m->stream, (const float *)lg->loss, M, d_mean);
+    mlp_backward(lg, m->stream, M, lg->xt);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && grad)
+        e = hipMemcpyAsync(grad, lg->g, sizeof(float) * P_TOTAL, hipMemcpyDeviceToDevice, m->stream);
+    if (e == hipSuccess && loss_out)
+        e = hipMemcpyAsync(loss_out, d_mean, sizeof(float), hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(d_mean);
+    if (e != hipSuccess) {
+        set_error(std::string("gade_lg_grad: ") + hipGetErrorString(e));
+        return GADE_EHIP;
+    }
+    return GADE_OK;
+}
+
+// nsteps Adam steps; step k uses the samples idx[k*M .. (k+1)*M) (device ints); loss_hist [nsteps] (host,
+// may be null) receives each step's mean loss
+int gade_lg_train(gade_model_t *m, const float *samples, const int *idx, int M, int nsteps, float lr,
+                  float *loss_hist) {
+    if (!m || !samples || !idx || M <= 0 || M % 16 || nsteps <= 0 || !(lr > 0)) return GADE_EINVAL;
+    if (!m->lg || !m->lg->net) {
+        set_error("gade_lg_train: initialise the network first (gade_lg_init)");
+        return GADE_EINVAL;
+    }
+    gade_lg *lg = m->lg;
+    GADE_TRY(lg_train_alloc(lg, M));
+    gade::drop_step_graph(m);
+    float *d_hist = nullptr;
+    GADE_TRY(hip_alloc((void **)&d_hist, sizeof(float) * nsteps));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int k = 0; k < nsteps; k++) {
+        const int *ik = idx + (size_t)k * M;
+        hipLaunchKernelGGL(k_lg_gather, dim3(M), dim3(64), 0, m->stream, samples, ik, (const float *)lg->mean,
+                           (const float *)lg->stdv, lg->xt);
+        mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
+        hipLaunchKernelGGL(k_lg_loss, dim3(M), dim3(64), 0, m->stream, (const float *)lg->yt, samples, ik, M, lg->dy,
+                           lg->loss);
+        hipLaunchKernelGGL(k_lg_mean, dim3(1), dim3(256), 0, m->stream, (const float *)lg->loss, M, d_hist + k);
+        mlp_backward(lg, m->stream, M, lg->xt);
+        lg->adam_t++;
+        const float c1 = 1.f - std::pow(b1, (float)lg->adam_t), c2 = 1.f - std::pow(b2, (float)lg->adam_t);
+        hipLaunchKernelGGL(k_lg_adam, dim3(256), dim3(256), 0, m->stream, (long)P_TOTAL, lg->p, (const float *)lg->g,
+                           lg->m1, lg->m2, lr, b1, b2, eps, c1, c2, lg->pbf);
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && loss_hist)
+        e = hipMemcpyAsync(loss_hist, d_hist, sizeof(float) * nsteps, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(d_hist);
+    if (e != hipSuccess) {
+        set_error(std::string("gade_lg_train: ") + hipGetErrorString(e));
+        return GADE_EHIP;
+    }
+    return GADE_OK;
+}
+
+}  // extern "C"

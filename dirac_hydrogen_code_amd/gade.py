@@ -25,8 +25,16 @@ EXPORTS = (
     "gade_assembled_timedep", "gade_assembled_spmv", "gade_assembled_get", "gade_profile", "gade_profile_get",
     "gade_set_param", "gade_get_param", "gade_eig_setup", "gade_eig_values", "gade_populations",
     "gade_extrapolate", "gade_model_create_batch", "gade_model_load_batch", "gade_model_batch", "gade_cn_run_batch",
-    "gade_set_time_batch", "gade_pulse_cep", "gade_autotune",
+    "gade_set_time_batch", "gade_pulse_cep", "gade_autotune", "gade_lg_record", "gade_lg_pod_build",
+    "gade_lg_basis", "gade_lg_set_norm", "gade_lg_init", "gade_lg_set_params", "gade_lg_get_params",
+    "gade_lg_forward", "gade_lg_grad", "gade_lg_train",
 )
+
+# learned initial guess (include/gade.h)
+GUESS_LEARNED = 4
+LG_IN, LG_HIDDEN, LG_OUT, LG_POD, LG_SAMPLE, LG_PARAMS = 288, 256, 144, 64, 1232, 242576
+LG_FEATURES = 17 + 4 * LG_POD                               # used inputs (the rest is zero padding)
+LG_SAMPLE_VALID = LG_IN + 36 + 6 + 12 * LG_POD + 2 * LG_POD + 1  # index of a sample's valid flag
 
 GADE_TUNE_MAX = 64
 
@@ -51,7 +59,12 @@ def lib():
             "gade_memcpy_d2h": [vp, vp, sz], "gade_memset": [vp, i32, sz], "gade_memcpy_d2d": [vp, vp, sz], "gade_eig_setup": [vp], "gade_extrapolate": [vp, vp, i32],
             "gade_model_load_batch": [C.c_char_p, i32, i32, f64, f64, vp, vp],
             "gade_cn_run_batch": [vp, vp, vp, vp, f64, i32, f64, i32, vp, vp],
-            "gade_autotune": [vp, vp, vp, f64, i32, f64, i32, i32, vp, C.POINTER(i32), C.POINTER(i32)], "gade_set_time_batch": [vp, vp], "gade_eig_values": [vp, vp, vp, vp],
+            "gade_autotune": [vp, vp, vp, f64, i32, f64, i32, i32, vp, C.POINTER(i32), C.POINTER(i32)],
+            "gade_lg_record": [vp, vp, vp, vp, i32, i32], "gade_lg_pod_build": [vp, vp, i32, i32, vp],
+            "gade_lg_basis": [vp, vp], "gade_lg_set_norm": [vp, vp, vp], "gade_lg_init": [vp, C.c_uint],
+            "gade_lg_set_params": [vp, vp], "gade_lg_get_params": [vp, vp], "gade_lg_forward": [vp, vp, i32, vp],
+            "gade_lg_grad": [vp, vp, vp, i32, C.POINTER(C.c_float), vp],
+            "gade_lg_train": [vp, vp, vp, i32, i32, C.c_float, vp], "gade_set_time_batch": [vp, vp], "gade_eig_values": [vp, vp, vp, vp],
             "gade_populations": [vp, vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), vp], "gade_device_sync": [],
             "gade_model_load": [C.c_char_p, i32, f64, f64, vp, C.POINTER(vp)], "gade_model_destroy": [vp],
             "gade_model_info": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
@@ -203,6 +216,53 @@ class Propagator:
                                   rep.ctypes.data, C.byref(nc), C.byref(ch)))
         rows = [dict(window=int(r[0]), iter_batch=int(r[1]), graph=int(r[2]), ms=float(r[3])) for r in rep[:nc.value]]
         return ch.value, rows
+
+    # ---- learned initial guess ----
+    def lg_record(self, psi: DeviceArray, samples=None, snaps=None, snap_col=0, snap_cols=0):
+        check(lib().gade_lg_record(self.ptr, psi.ptr, samples.ptr if samples is not None else None,
+                                   snaps.ptr if snaps is not None else None, snap_col, snap_cols))
+
+    def lg_pod_build(self, snaps: DeviceArray, nsnap, npod=LG_POD):
+        sv = np.zeros(nsnap, np.float64)
+        check(lib().gade_lg_pod_build(self.ptr, snaps.ptr, nsnap, npod, sv.ctypes.data))
+        return sv
+
+    def lg_basis(self, npod=LG_POD):
+        U = np.zeros((npod, self.nblk * self.blkdim), np.complex128)
+        check(lib().gade_lg_basis(self.ptr, U.ctypes.data))
+        return U
+
+    def lg_set_norm(self, mean, std):
+        mean = np.ascontiguousarray(mean, np.float32)
+        std = np.ascontiguousarray(std, np.float32)
+        check(lib().gade_lg_set_norm(self.ptr, mean.ctypes.data, std.ctypes.data))
+
+    def lg_init(self, seed=0):
+        check(lib().gade_lg_init(self.ptr, seed))
+
+    def lg_set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == LG_PARAMS
+        check(lib().gade_lg_set_params(self.ptr, p.ctypes.data))
+
+    def lg_get_params(self):
+        p = np.zeros(LG_PARAMS, np.float32)
+        check(lib().gade_lg_get_params(self.ptr, p.ctypes.data))
+        return p
+
+    def lg_forward(self, x: DeviceArray, M, y: DeviceArray):
+        check(lib().gade_lg_forward(self.ptr, x.ptr, M, y.ptr))
+
+    def lg_grad(self, samples: DeviceArray, idx: DeviceArray, M, grad: DeviceArray = None):
+        loss = C.c_float()
+        check(lib().gade_lg_grad(self.ptr, samples.ptr, idx.ptr, M, C.byref(loss),
+                                 grad.ptr if grad is not None else None))
+        return loss.value
+
+    def lg_train(self, samples: DeviceArray, idx: DeviceArray, M, nsteps, lr):
+        hist = np.zeros(nsteps, np.float32)
+        check(lib().gade_lg_train(self.ptr, samples.ptr, idx.ptr, M, nsteps, lr, hist.ctypes.data))
+        return hist
 
     def set_time_batch(self, ft):
         ft = np.ascontiguousarray(ft, np.complex128).reshape(self.batch, 6)

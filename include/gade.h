@@ -145,6 +145,48 @@ int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, co
  * model's current solver state (BiCGSTAB workspace, scalars, guess histories), which is restored on
  * return: a run after tuning equals the same run without it. */
 #define GADE_TUNE_MAX 64
+
+/* ---- learned initial guess (no reference counterpart) ----------------------------------------------
+ * guess_order GADE_GUESS_LEARNED: x0 = x2 + s sum_k w_k D_k/|D_k| + s U c per step, with D_k the backward
+ * differences of psi_n..psi_{n-5}, x2 = psi_n + D_1 + D_2 (order-2 extrapolation, also the fallback while
+ * the history is short), s = |D_3|, U the POD basis and (w, c) the outputs of an MLP (GADE_LG_IN inputs,
+ * 3 hidden layers of GADE_LG_HIDDEN with GELU, GADE_LG_OUT outputs: 6 weights then GADE_LG_POD complex
+ * coefficients), BF16 weights with FP32 accumulation on MFMA.  Runs in the device-driven stepping with the
+ * banded radial kernel.  Inputs (raw, standardized by gade_lg_set_norm): f_alpha(t_{n+1}) (6), log10 of
+ * the last 6 final relative residuals, log10 |D_k|/|psi_n| (k = 1..5), U^H D_3 / s and U^H D_4 / s
+ * (re, im interleaved), zero padding.  Training sample (GADE_LG_SAMPLE floats): inputs, then
+ * G = Re(Dh^H Dh) (6x6), a = Re(Dh^H eh) (6), P = U^H Dh (npod x 6 complex), q = U^H eh (npod complex),
+ * beta = |eh|^2, valid, with Dh_k = D_k/|D_k| and eh = (psi_{n+1} - x2)/s; its loss is
+ * |eh - Dh w - U c|^2 / beta, the guess error relative to the extrapolation's. */
+#define GADE_GUESS_LEARNED 4
+#define GADE_LG_IN 288
+#define GADE_LG_HIDDEN 256
+#define GADE_LG_OUT 144
+#define GADE_LG_POD 64
+#define GADE_LG_SAMPLE 1232
+#define GADE_LG_PARAMS 242576
+
+/* After a step run with set_param("lg_features", 1) and guess_order 2: write each instance's training
+ * sample (samples [batch][GADE_LG_SAMPLE], device, may be NULL; needs the POD basis) and its normalized
+ * extrapolation defect as snapshot column snap_col + b of snaps [snap_cols][N1] (device complex, may be
+ * NULL).  psi_dev holds psi_{n+1}.  Instances that did not step, or whose history is shorter than 6,
+ * give valid = 0 and no snapshot. */
+int gade_lg_record(gade_model_t *m, const double *psi_dev, float *samples_dev, double *snaps_dev, int snap_col,
+                   int snap_cols);
+/* POD basis: the top npod (<= GADE_LG_POD) left singular vectors of the nsnap snapshots (device complex
+ * [nsnap][N1]), by Householder QR and an SVD of R on the device; singular values to sv (host, nsnap). */
+int gade_lg_pod_build(gade_model_t *m, const double *snaps_dev, int nsnap, int npod, double *sv);
+int gade_lg_basis(gade_model_t *m, double *U_host);   /* [npod][N1] complex */
+int gade_lg_set_norm(gade_model_t *m, const float *mean, const float *stdv);  /* [GADE_LG_IN] each */
+int gade_lg_init(gade_model_t *m, unsigned seed);   /* He-normal hidden layers, zero output layer */
+int gade_lg_set_params(gade_model_t *m, const float *params);  /* host FP32 [GADE_LG_PARAMS] */
+int gade_lg_get_params(gade_model_t *m, float *params);
+/* W1 [256][288] b1 W2 [256][256] b2 W3 b3 W4 [144][256] b4, row-major [out][in] */
+int gade_lg_forward(gade_model_t *m, const float *x_dev, int M, float *y_dev);
+int gade_lg_grad(gade_model_t *m, const float *samples_dev, const int *idx_dev, int M, float *loss,
+                 float *grad_dev);
+int gade_lg_train(gade_model_t *m, const float *samples_dev, const int *idx_dev, int M, int nsteps, float lr,
+                  float *loss_hist);
 int gade_autotune(gade_model_t *m, const double *psi_dev, const double *params, double t0, int nsteps,
                   double tol, int maxit, int reps, double *report, int *ncand, int *chosen);
 
