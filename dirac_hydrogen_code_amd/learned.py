@@ -37,8 +37,9 @@ def scan_pulses(seed, B, dt):
 
 
 def step_scan(gp, psi0, params, nsteps_b, nsteps, after_step=None, tol=1e-12):
-    """Steps the batch one step per call from psi0 (every instance), calling after_step(n, x) after step n.
-    Step times accumulate as in a single run.  Returns x and the per-step iterations [nsteps][batch]."""
+    """Steps the batch one step per call from psi0 (every instance), calling after_step(n, x) after step n
+    (a true return stops the scan).  Step times accumulate as in a single run.  Returns x and the per-step
+    iterations [steps][batch]."""
     B = gp.batch
     x = gade.DeviceArray.from_host(np.tile(psi0, B))
     gp.reset_workspace()
@@ -51,8 +52,8 @@ def step_scan(gp, psi0, params, nsteps_b, nsteps, after_step=None, tol=1e-12):
         it, _ = gp.cn_run_batch(x, params, 1, nsteps_b=nb, t0=t, tol=tol)
         its.append(it[:, 0])
         t = t + gp.dt
-        if after_step is not None:
-            after_step(n, x)
+        if after_step is not None and after_step(n, x):
+            break
     return x, np.array(its)
 
 
@@ -83,6 +84,7 @@ class Trainer:
             if n >= snap_first and (n - snap_first) % snap_every == 0 and cols[0] < self.nsnap:
                 gade.check(L.gade_lg_record(gp.ptr, x.ptr, None, snaps.ptr, cols[0], self.nsnap))
                 cols[0] += B
+            return cols[0] >= self.nsnap
 
         last = snap_first + snap_every * ((self.nsnap + B - 1) // B)
         step_scan(gp, self.psi0, self.params, self.nsteps_b, last + 1, rec)
@@ -103,13 +105,12 @@ class Trainer:
         count = [0]
 
         def rec(n, x):
-            if count[0] >= self.nsamples:
-                return
             ptr = C.c_void_p(buf.ptr + n * B * gade.LG_SAMPLE * 4)
             gade.check(L.gade_lg_record(gp.ptr, x.ptr, ptr, None, 0, 0))
             slab = np.zeros(B * gade.LG_SAMPLE, np.float32)
             gade.check(L.gade_memcpy_d2h(slab.ctypes.data, ptr, slab.nbytes))
             count[0] += int(slab.reshape(B, gade.LG_SAMPLE)[:, gade.LG_SAMPLE_VALID].sum())
+            return count[0] >= self.nsamples
 
         step_scan(gp, self.psi0, self.params, self.nsteps_b, max_steps, rec)
         host = buf.get().reshape(max_steps * B, gade.LG_SAMPLE)
