@@ -2248,7 +2248,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
     };
     std::vector<Cand> cands;
     for (int g : {1, 0}) {
-        if (g && m->radial_kind != 2) continue;
+        if (g && (m->radial_kind != 2 || !m->use_graph)) continue;  // graph 0 on the model excludes graphs
         for (int w : {2, 4, 8, 16, 32})
             for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, 0.0});
     }

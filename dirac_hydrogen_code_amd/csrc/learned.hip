@@ -204,31 +204,60 @@ __device__ __forceinline__ int gidx(int p, int q) {  // index of Re(V_p^H V_q), 
     return p * kNV - p * (p - 1) / 2 + (q - p);
 }
 
-// out[b][v][j] = U_j^H V_{v0+v} (instance b), one block per (j, b)
-__global__ __launch_bounds__(256) void k_lg_proj(long long N1, long long Ntot, int npod,
-                                                 const double2 *__restrict__ U, const double2 *__restrict__ V,
-                                                 int v0, int nv, double2 *__restrict__ out, const int *gate,
-                                                 int gstride) {
-    const int j = blockIdx.x, b = blockIdx.y;
+// partial[b][c][v][j] = sum over chunk c of conj(UT[k][j]) V_{v0+v}[b][k]: one block per (chunk, b), lanes
+// (j = t % 64, s = t / 64), UT the k-major copy of the basis (coalesced over j), V shared by the 64 lanes
+constexpr int kProjChunk = 256;
+__global__ __launch_bounds__(256) void k_lg_proj_part(long long N1, long long Ntot, int npod,
+                                                      const double2 *__restrict__ UT, const double2 *__restrict__ V,
+                                                      int v0, int nv, double2 *__restrict__ part, const int *gate,
+                                                      int gstride) {
+    const int c = blockIdx.x, b = blockIdx.y, nch = gridDim.x;
     GADE_GATED_B(gate, gstride, b);
-    __shared__ double sh[2 * kNV * 4];
-    double acc[2 * kNV];
+    __shared__ double2 sh[4][kNV][64];
+    const int j = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    double2 acc[kNV];
 #pragma unroll
-    for (int t = 0; t < 2 * kNV; t++) acc[t] = 0;
-    const double2 *u = U + (size_t)j * N1;
-    for (long long k = threadIdx.x; k < N1; k += blockDim.x) {
-        const double2 uu = u[k];
+    for (int v = 0; v < kNV; v++) acc[v] = make_double2(0, 0);
+    const long long k1 = (long long)(c + 1) * kProjChunk < N1 ? (long long)(c + 1) * kProjChunk : N1;
+    if (j < npod)
+        for (long long k = (long long)c * kProjChunk + sl; k < k1; k += 4) {
+            const double2 u = UT[(size_t)k * npod + j];
 #pragma unroll
-        for (int v = 0; v < kNV; v++) {
-            if (v >= nv) break;
-            const double2 x = V[(size_t)(v0 + v) * Ntot + (size_t)b * N1 + k];
-            acc[2 * v] += uu.x * x.x + uu.y * x.y;
-            acc[2 * v + 1] += uu.x * x.y - uu.y * x.x;
+            for (int v = 0; v < kNV; v++) {
+                if (v >= nv) break;
+                acc[v] = cfmaconj(u, V[(size_t)(v0 + v) * Ntot + (size_t)b * N1 + k], acc[v]);
+            }
         }
+#pragma unroll
+    for (int v = 0; v < kNV; v++) sh[sl][v][j] = acc[v];
+    __syncthreads();
+    if (sl == 0 && j < npod)
+        for (int v = 0; v < nv; v++)
+            part[(((size_t)b * nch + c) * kNV + v) * 64 + j] =
+                cadd(cadd(sh[0][v][j], sh[1][v][j]), cadd(sh[2][v][j], sh[3][v][j]));
+}
+
+// out[b][v][j] = sum over chunks of the partials (fixed order)
+__global__ __launch_bounds__(256) void k_lg_proj_fin(int nch, int npod, int nv, const double2 *__restrict__ part,
+                                                     double2 *__restrict__ out, const int *gate, int gstride) {
+    const int b = blockIdx.x;
+    GADE_GATED_B(gate, gstride, b);
+    for (int t = threadIdx.x; t < nv * 64; t += blockDim.x) {
+        const int v = t / 64, j = t % 64;
+        if (j >= npod) continue;
+        double2 s = make_double2(0, 0);
+        for (int c = 0; c < nch; c++) s = cadd(s, part[(((size_t)b * nch + c) * kNV + v) * 64 + j]);
+        out[((size_t)b * kNV + v) * npod + j] = s;
     }
-    block_sum<2 * kNV>(acc, sh);
-    if (threadIdx.x == 0)
-        for (int v = 0; v < nv; v++) out[((size_t)b * kNV + v) * npod + j] = make_double2(acc[2 * v], acc[2 * v + 1]);
+}
+
+// UT[k][j] = U[j][k]
+__global__ void k_lg_transpose(long long N1, int npod, const double2 *__restrict__ U, double2 *__restrict__ UT) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N1 * npod; i += (long long)gridDim.x * blockDim.x) {
+        const long long k = i / npod;
+        const int j = (int)(i % npod);
+        UT[i] = U[(size_t)j * N1 + k];
+    }
 }
 
 // log10 residual ring [b][8], newest first: push the previous solve's final relative residual
@@ -398,7 +427,7 @@ __global__ void k_lg_gather(const float *__restrict__ samples, const int *__rest
     const int m = blockIdx.x;
     const float *s = samples + (size_t)idx[m] * kSample;
     for (int t = threadIdx.x; t < kIn; t += blockDim.x)
-        X[(size_t)m * kIn + t] = t < F_END ? f2bf((s[t] - mean[t]) / stdv[t]) : (uint16_t)0;
+        X[(size_t)m * kIn + t] = t < F_END ? f2bf((s[S_FEAT + t] - mean[t]) / stdv[t]) : (uint16_t)0;
 }
 
 // X[m] = standardized bf16 inputs of raw row m (stride kIn)
@@ -486,13 +515,17 @@ __global__ void k_lg_gelu_back(long n, const float *__restrict__ dA, const float
     }
 }
 
-// column sums db[n] = sum_m dZ[m][n]
-__global__ void k_lg_colsum(int M, int N, const float *__restrict__ dZ, float *__restrict__ db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// column sums db[n] = sum_m dZ[m][n]: 64 columns per block, 4 row groups
+__global__ __launch_bounds__(256) void k_lg_colsum(int M, int N, const float *__restrict__ dZ, float *__restrict__ db) {
+    __shared__ float sh[4][64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
     float s = 0;
-    for (int m = 0; m < M; m++) s += dZ[(size_t)m * N + n];
-    db[n] = s;
+    if (n < N)
+        for (int m = g; m < M; m += 4) s += dZ[(size_t)m * N + n];
+    sh[g][c] = s;
+    __syncthreads();
+    if (g == 0 && n < N) db[n] = (sh[0][c] + sh[1][c]) + (sh[2][c] + sh[3][c]);
 }
 
 // Adam step on the FP32 parameters, refreshing the bf16 weights
@@ -519,6 +552,9 @@ __global__ void k_lg_adam(long n, float *__restrict__ p, const float *__restrict
 struct gade_lg {
     int npod = 0;
     double2 *U = nullptr;         // [npod][N1] POD basis (orthonormal columns)
+    double2 *UT = nullptr;        // [N1][npod] the same, k-major (projections)
+    double2 *ppart = nullptr;     // [B][nchunk][7][64] projection partials
+    int nchunk = 0;
     double2 *V = nullptr;         // [7][N] difference stage vectors
     double *part = nullptr;       // [B][nparts][kNG]
     int nparts = 0;
@@ -546,7 +582,7 @@ namespace gade {
 
 void lg_free(gade_lg *lg) {
     if (!lg) return;
-    std::vector<void *> ptrs = {lg->U, lg->V, lg->part, lg->gram, lg->proj, lg->lres, lg->feat, lg->mean, lg->stdv,
+    std::vector<void *> ptrs = {lg->U, lg->UT, lg->ppart, lg->V, lg->part, lg->gram, lg->proj, lg->lres, lg->feat, lg->mean, lg->stdv,
                                 lg->xin, lg->yi, lg->p, lg->g, lg->m1, lg->m2, lg->pbf, lg->xt, lg->yt, lg->dy,
                                 lg->da, lg->dz, lg->loss};
     for (int k = 0; k < 3; k++) {
@@ -570,12 +606,14 @@ int lg_ensure(gade_model *m) {
     gade_lg *lg = new gade_lg();
     m->lg = lg;
     const size_t N = (size_t)m->N, B = (size_t)m->B;
-    lg->nparts = 64;
+    lg->nparts = (int)std::min<long long>(256, std::max<long long>(1, (m->N1 + 1023) / 1024));
     lg->Mi = pad16(m->B);
     GADE_TRY(hip_alloc((void **)&lg->V, sizeof(double2) * kNV * N));
     GADE_TRY(hip_alloc((void **)&lg->part, sizeof(double) * B * lg->nparts * kNG));
     GADE_TRY(hip_alloc((void **)&lg->gram, sizeof(double) * B * kNG));
     GADE_TRY(hip_alloc((void **)&lg->proj, sizeof(double2) * B * kNV * kPod));
+    lg->nchunk = (int)((m->N1 + kProjChunk - 1) / kProjChunk);
+    GADE_TRY(hip_alloc((void **)&lg->ppart, sizeof(double2) * B * lg->nchunk * kNV * 64));
     GADE_TRY(hip_alloc((void **)&lg->lres, sizeof(double) * B * 8));
     GADE_TRY(hip_alloc((void **)&lg->feat, sizeof(float) * B * kIn));
     GADE_TRY(hip_alloc((void **)&lg->mean, sizeof(float) * kIn));
@@ -630,6 +668,15 @@ int lg_train_alloc(gade_lg *lg, int Mt) {
     return GADE_OK;
 }
 
+// out[b][v][j] = U_j^H V_{v0+v} for v < nv
+void lg_project(gade_model *m, int v0, int nv, const int *gate, int gstride) {
+    gade_lg *lg = m->lg;
+    hipLaunchKernelGGL(k_lg_proj_part, dim3(lg->nchunk, m->B), dim3(256), 0, m->stream, m->N1, m->N, lg->npod,
+                       (const double2 *)lg->UT, (const double2 *)lg->V, v0, nv, lg->ppart, gate, gstride);
+    hipLaunchKernelGGL(k_lg_proj_fin, dim3(m->B), dim3(256), 0, m->stream, lg->nchunk, lg->npod, nv,
+                       (const double2 *)lg->ppart, lg->proj, gate, gstride);
+}
+
 // forward of M rows: X (bf16 [M][kIn]) -> Z_l, A_l, Y
 void mlp_forward(const gade_lg *lg, hipStream_t st, int M, const uint16_t *X, float *const Z[3], uint16_t *const A[3],
                  float *Y) {
@@ -661,7 +708,7 @@ void mlp_backward(gade_lg *lg, hipStream_t st, int M, const uint16_t *X) {
         hipLaunchKernelGGL(k_gemm_bf16<EPI_STORE>, dim3((N / 16) * (K / 16)), dim3(64), 0, st, N, K, M, Gl, 1L,
                            (long)N, Ain, (long)K, 1L, lg->g + kWoff[l], K, (const float *)nullptr,
                            (uint16_t *)nullptr, 0);
-        hipLaunchKernelGGL(k_lg_colsum, dim3((N + 255) / 256), dim3(256), 0, st, M, N, dZ, lg->g + kBoff[l]);
+        hipLaunchKernelGGL(k_lg_colsum, dim3((N + 63) / 64), dim3(256), 0, st, M, N, dZ, lg->g + kBoff[l]);
         if (l > 0) {
             // dA[m][k] = sum_n dZ[m][n] W[n][k], then dZ_{l-1} = dA * gelu'(Z_{l-1})
             hipLaunchKernelGGL(k_gemm_bf16<EPI_STORE>, dim3((M / 16) * (K / 16)), dim3(64), 0, st, M, K, N, Gl,
@@ -713,9 +760,7 @@ int lg_step_prologue(gade_model *m, double2 *psi) {
                        (const double2 *)m->d_hist, (const int *)m->d_ctl, 0, lg->V, lg->part, m->gate, m->gstride);
     hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(64), 0, m->stream, lg->nparts, (const double *)lg->part,
                        lg->gram, m->gate, m->gstride);
-    if (lg->npod > 0)
-        hipLaunchKernelGGL(k_lg_proj, dim3(lg->npod, B), dim3(256), 0, m->stream, N1, N, lg->npod,
-                           (const double2 *)lg->U, (const double2 *)lg->V, 3, 2, lg->proj, m->gate, m->gstride);
+    if (lg->npod > 0) lg_project(m, 3, 2, m->gate, m->gstride);
     const double hd = kSoL * m->h * m->dt / 2;
     hipLaunchKernelGGL(k_lg_features, dim3(B), dim3(64), 0, m->stream, (const Coef6 *)m->d_cfwin,
                        (const int *)m->d_gctl, B, hd, (const double *)lg->lres, (const double *)lg->gram,
@@ -763,8 +808,7 @@ int gade_lg_record(gade_model_t *m, const double *psi, float *samples, double *s
             set_error("gade_lg_record: samples need the POD basis (gade_lg_pod_build)");
             return GADE_EINVAL;
         }
-        hipLaunchKernelGGL(k_lg_proj, dim3(lg->npod, B), dim3(256), 0, m->stream, N1, N, lg->npod,
-                           (const double2 *)lg->U, (const double2 *)lg->V, 0, kNV, lg->proj, (const int *)nullptr, 0);
+        lg_project(m, 0, kNV, (const int *)nullptr, 0);
         hipLaunchKernelGGL(k_lg_record, dim3(B), dim3(256), 0, m->stream, (const double *)lg->gram,
                            (const double2 *)lg->proj, lg->npod, (const float *)lg->feat, (const int *)m->d_ctl,
                            samples);
@@ -862,6 +906,11 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
     C = nullptr;
     lg->npod = npod;
     cleanup();
+    if (!lg->UT) GADE_TRY(hip_alloc((void **)&lg->UT, sizeof(double2) * (size_t)n * kPod));
+    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for_lg((long long)n * npod)), dim3(256), 0, m->stream, (long long)n,
+                       npod, (const double2 *)lg->U, lg->UT);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
     return GADE_OK;
 }
 

@@ -138,7 +138,8 @@ int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, co
 /* Autotuner of the device-driven stepping (no reference counterpart): times nsteps probe steps from
  * psi_dev (left unchanged) with the pulses params [batch][4] (as gade_cn_run_batch, t0 the start time)
  * for every candidate (window in {2,4,8,16,32}, iter_batch in {0,1,2,4}, graph in {0,1}, the graph
- * only with the banded radial kernel), best of reps runs each, and keeps the fastest.  Every candidate
+ * only with the banded radial kernel and when the model's "graph" parameter is 1), best of reps runs
+ * each, and keeps the fastest.  Every candidate
  * must give the same trajectory, iterations and tolerances bit for bit, else GADE_EINVAL with the
  * message naming it.  report [GADE_TUNE_MAX][4] (NULL allowed) receives (window, iter_batch, graph, ms)
  * per candidate and *ncand their count; the chosen index goes to *chosen.  Every probe starts from the
