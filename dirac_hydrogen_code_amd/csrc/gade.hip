@@ -420,14 +420,20 @@ __global__ __launch_bounds__(256) void k_radial_sum(const T *__restrict__ ypart,
 // XCD streams a single multipole's coefficients from its own L2).  The
 // per-multipole results go to ypart and are summed in a fixed order.
 // ---------------------------------------------------------------------------
-constexpr int kDiaR = 8;
-constexpr int kDiaP = 2;
-constexpr int kDiaSeg = 64 * kDiaP;
+constexpr int kDiaR = 8;        // default rows per tile
+constexpr int kDiaP = 2;        // default outputs per lane
+constexpr int kDiaSegMax = 256; // longest output run of a wave (64 lanes x 4 outputs); nhp is a multiple of it
 constexpr int kDiaBW = 7;
 
 // LDS slot of window position p: swaps neighbours on alternate 8-slot rows so
 // that lanes reading positions 2*lane+t hit distinct banks
-__device__ __forceinline__ int dia_swz(int p) { return p ^ (((p >> 3) ^ (p >> 4)) & 1); }
+// (P = 2); for P = 4 (lanes reading positions 4*lane+t) the low two bits are XORed instead.  Both are
+// conflict-free for the sliding reads and for the stride-1 staging writes (64 banks, 16 lanes per pass).
+template <int P>
+__device__ __forceinline__ int dia_swz(int p) {
+    if constexpr (P == 4) return p ^ (((p >> 2) ^ (p >> 4)) & 3);
+    return p ^ (((p >> 3) ^ (p >> 4)) & 1);
+}
 
 // GD[l][v][qd][d][ip] = (G_m1, G_m2) of the kappa-folded pair of v, from the
 // DIA-ordered couplings gs[l][j][nn][qd][d][ip] (g of alpha = 2j + (l even))
@@ -516,7 +522,7 @@ __global__ void k_build_DD(const double2 *__restrict__ D, const int *__restrict_
 
 // DUAL: a second output y2/ypart2 = (D_sel2 + G-part) x from the same G sweeps (beta must be 0); with
 // cH = -1, sel = 1, sel2 = 0 this is the CN pair (S - H) x, (S + H) x, each bit-identical to its own launch.
-template <int BW, bool DUAL, class T = double2>
+template <int BW, bool DUAL, class T, int R, int P>
 __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, const T *__restrict__ V,
                                                    const T *__restrict__ GD, const T *__restrict__ DD,
                                                    const int *__restrict__ tiles, const int *__restrict__ tile_q,
@@ -526,7 +532,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
                                                    T *__restrict__ ypart, const int *gate, int gstride,
                                                    const int *__restrict__ tile_b, size_t gd_stride,
                                                    T *__restrict__ y2, T *__restrict__ ypart2, int sel2) {
-    constexpr int R = kDiaR, P = kDiaP, ND = 2 * BW + 1, W = kDiaSeg + 2 * BW;
+    constexpr int SEG = 64 * P, ND = 2 * BW + 1, W = SEG + 2 * BW;
     using Rt = typename cx<T>::R;
     __shared__ T sm[R * W];
     const int lane = threadIdx.x;
@@ -542,7 +548,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
     GD += (size_t)inst * gd_stride;  // this instance's G-sums
     const Rt kap = (Rt)tile_k[tile];
     const int q = tile_q[tile];
-    const int i0 = seg * kDiaSeg;
+    const int i0 = seg * SEG;
     int rows[R];
 #pragma unroll
     for (int i = 0; i < R; i++) rows[i] = tiles[tile * R + i];
@@ -577,21 +583,21 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
 #pragma unroll
             for (int c = 0; c < NC; c++) {
                 const int p = c * 64 + lane;
-                if (p < W) sm[i * W + dia_swz(p)] = t[i][c];
+                if (p < W) sm[i * W + dia_swz<P>(p)] = t[i][c];
             }
         __syncthreads();
     };
     // acc[i][j] += sum_d M_d[i0 + P*lane + j] * window[P*lane + j + d]; with two coefficient
     // streams M = C1 + kap*C2 (pairs), else M = C1.  Coefficients are fetched PF diagonals ahead.
     auto sweep = [&](const T *C, bool pair) {
-        constexpr int PF = 3;
+        constexpr int PF = P >= 4 ? 2 : 3;  // coefficient diagonals in flight
         const int stride = pair ? 2 : 1;
         const T *cp = C + (size_t)(i0 + P * lane) * stride;
         T w[R][P];
 #pragma unroll
         for (int i = 0; i < R; i++)
 #pragma unroll
-            for (int j = 0; j < P; j++) w[i][j] = sm[i * W + dia_swz(P * lane + j)];
+            for (int j = 0; j < P; j++) w[i][j] = sm[i * W + dia_swz<P>(P * lane + j)];
         T c1[PF][P], c2[PF][P];
 #pragma unroll
         for (int d = 0; d < PF - 1; d++)
@@ -624,7 +630,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
                 if (d + 1 < ND) {
 #pragma unroll
                     for (int j = 0; j + 1 < P; j++) w[i][j] = w[i][j + 1];
-                    w[i][P - 1] = sm[i * W + dia_swz(P * lane + P + d)];
+                    w[i][P - 1] = sm[i * W + dia_swz<P>(P * lane + P + d)];
                 }
             }
         }
@@ -897,29 +903,48 @@ int launch_angular(gade_model *m, const double2 *x) {
     return GADE_OK;
 }
 
+// the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
+// each output in the same order, so all give the same results bit for bit
+template <bool DUAL, class T>
+int launch_dia(gade_model *m, int lgrid, const T *x, const T *V, const T *GD, const T *DD, int sel, double cH,
+               double beta, T *y, T *yp, T *y2, T *yp2, int sel2) {
+    const int P = m->dia_p;
+    const int nseg = (m->nh + 64 * P - 1) / (64 * P);
+    const dim3 grid((unsigned)m->ndtiles * 2 * nseg * lgrid);
+#define GADE_DIA(R_, P_)                                                                                        \
+    hipLaunchKernelGGL((k_radial_dia<kDiaBW, DUAL, T, R_, P_>), grid, dim3(64), 0, m->stream, x, V, GD, DD,    \
+                       m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, nseg, lgrid, m->nq, \
+                       sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b, m->gd_stride, y2, yp2, sel2)
+    switch (m->dia_r * 10 + P) {
+        case 82: GADE_DIA(8, 2); break;
+        case 42: GADE_DIA(4, 2); break;
+        case 44: GADE_DIA(4, 4); break;
+        default:
+            gade::set_error("unsupported banded tile shape (rows, outputs per lane): (8,2), (4,2) or (4,4)");
+            return GADE_EINVAL;
+    }
+#undef GADE_DIA
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
 // y2 (banded kernel only): also y2 = (D_sel2 + G-part) x from the same pass, beta 0
 int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta,
                   double2 *y2 = nullptr, int sel2 = 0) {
     const bool inter = cH != 0.0;
     if (m->radial_kind == 2) {
         const int lgrid = inter ? m->lmax : 1;
-        dim3 grid((unsigned)m->ndtiles * 2 * m->nseg * lgrid);
         double2 *yp = inter && lgrid > 1 ? m->d_ypart : nullptr;
         double2 *yp2 = y2 && yp ? m->d_ypart2 : nullptr;  // allocated with the model when lmax > 1
         if (y2) {
             Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
-            hipLaunchKernelGGL((k_radial_dia<kDiaBW, true, double2>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
-                               m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp,
-                               m->nseg, lgrid, m->nq, sel, cH, 0.0, y, yp, m->gate, m->gstride, m->d_dtile_b,
-                               m->gd_stride, y2, yp2, sel2);
+            GADE_TRY((launch_dia<true, double2>(m, lgrid, x, m->d_V, m->d_GD, m->d_DD, sel, cH, 0.0, y, yp, y2, yp2,
+                                                sel2)));
         } else {
             Prof pr(m, "radial", GADE_PROF_RADIAL);
-            hipLaunchKernelGGL((k_radial_dia<kDiaBW, false, double2>), grid, dim3(64), 0, m->stream, x, m->d_V, m->d_GD,
-                               m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp,
-                               m->nseg, lgrid, m->nq, sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b,
-                               m->gd_stride, (double2 *)nullptr, (double2 *)nullptr, 0);
+            GADE_TRY((launch_dia<false, double2>(m, lgrid, x, m->d_V, m->d_GD, m->d_DD, sel, cH, beta, y, yp,
+                                                 (double2 *)nullptr, (double2 *)nullptr, 0)));
         }
-        GADE_HIP(hipGetLastError());
         if (yp) {
             Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
             hipLaunchKernelGGL(k_radial_sum<double2>, dim3(grid_for(m->N1, 256, 2048), m->B), dim3(256), 0, m->stream,
@@ -1006,8 +1031,8 @@ int setup_dia(gade_model *m) {
             if (std::abs(m->rad_Ai[k] % nh - a % nh) > kDiaBW) return GADE_OK;
     m->nh = nh;
     m->nd = 2 * kDiaBW + 1;
-    m->nseg = (nh + kDiaSeg - 1) / kDiaSeg;
-    m->nhp = m->nseg * kDiaSeg;
+    m->nseg = (nh + 64 * kDiaP - 1) / (64 * kDiaP);
+    m->nhp = (nh + kDiaSegMax - 1) / kDiaSegMax * kDiaSegMax;
     const int S = 4 * m->nd * m->nhp;
     std::vector<int> dia_src(S, -1);
     for (int qd = 0; qd < 4; qd++)
@@ -1026,6 +1051,7 @@ int setup_dia(gade_model *m) {
     GADE_TRY(hip_alloc((void **)&m->d_gts, sizeof(double2) * (size_t)lmax * 12 * S));
     m->gd_stride = (size_t)lmax * 4 * S * 2;
     GADE_TRY(hip_alloc((void **)&m->d_GD, sizeof(double2) * m->gd_stride * m->B));
+    GADE_HIP(hipMemset(m->d_GD, 0, sizeof(double2) * m->gd_stride * m->B));  // f = 0 until the first set_time
     GADE_TRY(hip_alloc((void **)&m->d_DD, sizeof(double2) * (size_t)3 * m->nq * S));
     const long long tg = (long long)lmax * 12 * S, td = (long long)3 * m->nq * S;
     hipLaunchKernelGGL(k_build_gdia, dim3(grid_for(tg, 256)), dim3(256), 0, m->stream, m->d_g, m->d_dia_src, m->d_tmap,
@@ -1311,6 +1337,8 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
         (rc = hip_alloc((void **)&m->d_ypart, sizeof(double2) * (size_t)lmax * m->N)) ||
         (lmax > 1 && (rc = hip_alloc((void **)&m->d_ypart2, sizeof(double2) * (size_t)lmax * m->N))))
         return fail(rc);
+    if (hipMemset(m->d_GE, 0, sizeof(double2) * (size_t)lmax * 8 * ellw * n) != hipSuccess)  // f = 0 until set_time
+        return fail(GADE_EHIP);
     {
         int *d_kq = nullptr;
         if ((rc = upload(&d_kq, m->kap_q.data(), m->kap_q.size()))) return fail(rc);
@@ -1715,12 +1743,9 @@ int apply32_async(gade_model *m, const float2 *x, float2 *y) {
     float2 *yp = lgrid > 1 ? m->d_ypart32 : nullptr;
     {
         Prof pr(m, "radial_f32", GADE_PROF_RADIAL);
-        dim3 grid((unsigned)m->ndtiles * 2 * m->nseg * lgrid);
-        hipLaunchKernelGGL((k_radial_dia<kDiaBW, false, float2>), grid, dim3(64), 0, m->stream, x, m->d_V32,
-                           m->d_GD32, m->d_DD32, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh,
-                           m->nhp, m->nseg, lgrid, m->nq, 0, 1.0, 0.0, y, yp, m->gate, m->gstride, m->d_dtile_b,
-                           m->gd_stride, (float2 *)nullptr, (float2 *)nullptr, 0);
-        GADE_HIP(hipGetLastError());
+        GADE_TRY((launch_dia<false, float2>(m, lgrid, x, (const float2 *)m->d_V32, (const float2 *)m->d_GD32,
+                                            (const float2 *)m->d_DD32, 0, 1.0, 0.0, y, yp, (float2 *)nullptr,
+                                            (float2 *)nullptr, 0)));
     }
     if (yp) {
         Prof pr(m, "radial_sum_f32", GADE_PROF_RADIAL_SUM);
@@ -2243,14 +2268,19 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         times[step] = time;
     }
     struct Cand {
-        int window, iter_batch, graph;
+        int window, iter_batch, graph, dia_r, dia_p;
         double ms;
     };
+    // stage 1: banded tile shapes with the model's stepping; stage 2: stepping with the best shape
     std::vector<Cand> cands;
+    if (m->radial_kind == 2)
+        for (auto rp : {std::pair<int, int>{8, 2}, {4, 2}, {4, 4}})
+            cands.push_back({m->window, m->iter_batch, m->use_graph ? 1 : 0, rp.first, rp.second, 0.0});
+    const size_t nshape = cands.size();
     for (int g : {1, 0}) {
         if (g && (m->radial_kind != 2 || !m->use_graph)) continue;  // graph 0 on the model excludes graphs
         for (int w : {2, 4, 8, 16, 32})
-            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, 0.0});
+            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, 0.0});
     }
     const size_t N = (size_t)m->N, B = (size_t)m->B;
     // the solver state a run starts from (BiCGSTAB workspace, which carries over between solves, scalars,
@@ -2289,13 +2319,24 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         return GADE_OK;
     };
     const int sv_window = m->window, sv_batch = m->iter_batch, sv_profile = m->profile, sv_k = m->last_k;
+    const int sv_r = m->dia_r, sv_p = m->dia_p;
     const bool sv_graph = m->use_graph;
+    auto set_shape = [&](int r, int pp) -> int {
+        if (!m->dia_ok || (r == m->dia_r && pp == m->dia_p)) return GADE_OK;
+        drop_step_graph(m);
+        if (r != m->dia_r)
+            GADE_TRY(kappa_tiles(m, r, &m->d_dtiles, &m->d_dtile_q, &m->d_dtile_k, &m->ndtiles, &m->d_dtile_b));
+        m->dia_r = r;
+        m->dia_p = pp;
+        return GADE_OK;
+    };
     auto restore = [&]() {
         m->window = sv_window;
         m->iter_batch = sv_batch;
         m->use_graph = sv_graph;
         m->profile = sv_profile;
         m->last_k = sv_k;
+        (void)set_shape(sv_r, sv_p);
         const int rc = copy_state(false);
         (void)hipStreamSynchronize(m->stream);
         (void)hipFree(work);
@@ -2311,7 +2352,23 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
     std::vector<double2> ref(N), cur(N);
     std::vector<int> ref_it(B * nsteps), it(B * nsteps);
     std::vector<double> ref_tol(B * nsteps), tl(B * nsteps);
+    size_t best_shape = 0;
     for (size_t c = 0; c < cands.size(); c++) {
+        if (c == nshape && nshape > 0) {
+            for (size_t k = 1; k < nshape; k++)
+                if (cands[k].ms < cands[best_shape].ms) best_shape = k;
+            if (int rc = set_shape(cands[best_shape].dia_r, cands[best_shape].dia_p)) {
+                restore();
+                return rc;
+            }
+        }
+        if (cands[c].dia_r < 0) {
+            cands[c].dia_r = m->dia_r;
+            cands[c].dia_p = m->dia_p;
+        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p)) {
+            restore();
+            return rc;
+        }
         m->window = cands[c].window;
         m->iter_batch = cands[c].iter_batch;
         m->use_graph = cands[c].graph != 0;
@@ -2349,23 +2406,28 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             restore();
             gade::set_error("gade_autotune: candidate window " + std::to_string(cands[c].window) + " iter_batch " +
                             std::to_string(cands[c].iter_batch) + " graph " + std::to_string(cands[c].graph) +
+                            " tile " + std::to_string(cands[c].dia_r) + "x" + std::to_string(cands[c].dia_p) +
                             " changed the trajectory");
             return GADE_EINVAL;
         }
     }
-    size_t pick = 0;
-    for (size_t c = 1; c < cands.size(); c++)
+    size_t pick = nshape < cands.size() ? nshape : 0;  // the fastest stepping (with the best shape)
+    for (size_t c = pick + 1; c < cands.size(); c++)
         if (cands[c].ms < cands[pick].ms) pick = c;
     GADE_TRY(restore());
     m->window = cands[pick].window;
     m->iter_batch = cands[pick].iter_batch;
     m->use_graph = cands[pick].graph != 0;
+    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p));
     if (report)
         for (size_t c = 0; c < cands.size() && c < GADE_TUNE_MAX; c++) {
-            report[4 * c] = cands[c].window;
-            report[4 * c + 1] = cands[c].iter_batch;
-            report[4 * c + 2] = cands[c].graph;
-            report[4 * c + 3] = cands[c].ms;
+            double *r = report + (size_t)GADE_TUNE_COLS * c;
+            r[0] = cands[c].window;
+            r[1] = cands[c].iter_batch;
+            r[2] = cands[c].graph;
+            r[3] = cands[c].dia_r;
+            r[4] = cands[c].dia_p;
+            r[5] = cands[c].ms;
         }
     if (ncand) *ncand = (int)std::min<size_t>(cands.size(), GADE_TUNE_MAX);
     if (chosen) *chosen = (int)pick;
@@ -2554,6 +2616,24 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->use_graph = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "dia_rows") || !strcmp(name, "dia_outputs")) {
+        const bool rows = name[4] == 'r';
+        const int r = rows ? value : m->dia_r, pp = rows ? m->dia_p : value;
+        if (!((r == 8 && pp == 2) || (r == 4 && pp == 2) || (r == 4 && pp == 4))) {
+            gade::set_error("banded tile shape (dia_rows, dia_outputs) must be (8,2), (4,2) or (4,4)");
+            return GADE_EINVAL;
+        }
+        if (!m->dia_ok) {
+            gade::set_error("the radial pattern is not banded");
+            return GADE_EINVAL;
+        }
+        drop_step_graph(m);
+        if (rows && r != m->dia_r)
+            GADE_TRY(kappa_tiles(m, r, &m->d_dtiles, &m->d_dtile_q, &m->d_dtile_k, &m->ndtiles, &m->d_dtile_b));
+        m->dia_r = r;
+        m->dia_p = pp;
+        return GADE_OK;
+    }
     if (!strcmp(name, "precision")) {
         if (value != 0 && value != 1) {
             gade::set_error("precision must be 0 (FP64) or 1 (mixed)");
@@ -2659,6 +2739,14 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "precision")) {
         *value = m->precision;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "dia_rows")) {
+        *value = m->dia_r;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "dia_outputs")) {
+        *value = m->dia_p;
         return GADE_OK;
     }
     if (!strcmp(name, "lg_features")) {

@@ -37,6 +37,7 @@ LG_FEATURES = 17 + 4 * LG_POD                               # used inputs (the r
 LG_SAMPLE_VALID = LG_IN + 36 + 6 + 12 * LG_POD + 2 * LG_POD + 1  # index of a sample's valid flag
 
 GADE_TUNE_MAX = 64
+GADE_TUNE_COLS = 6
 
 _lib = None
 
@@ -210,11 +211,12 @@ class Propagator:
         keep the fastest; every candidate must reproduce the same trajectory bit for bit.
         Returns (chosen, [dict(window, iter_batch, graph, ms)])."""
         params = np.ascontiguousarray(params, np.float64).reshape(self.batch, 4)
-        rep = np.zeros((GADE_TUNE_MAX, 4), np.float64)
+        rep = np.zeros((GADE_TUNE_MAX, GADE_TUNE_COLS), np.float64)
         nc, ch = C.c_int(), C.c_int()
         check(lib().gade_autotune(self.ptr, psi.ptr, params.ctypes.data, t0, nsteps, tol, maxit, reps,
                                   rep.ctypes.data, C.byref(nc), C.byref(ch)))
-        rows = [dict(window=int(r[0]), iter_batch=int(r[1]), graph=int(r[2]), ms=float(r[3])) for r in rep[:nc.value]]
+        rows = [dict(window=int(r[0]), iter_batch=int(r[1]), graph=int(r[2]), dia_rows=int(r[3]),
+                     dia_outputs=int(r[4]), ms=float(r[5])) for r in rep[:nc.value]]
         return ch.value, rows
 
     # ---- learned initial guess ----
