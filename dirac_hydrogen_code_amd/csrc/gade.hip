@@ -153,7 +153,7 @@ __global__ void k_build_D(const double2 *__restrict__ s, const double2 *__restri
 // x loads of four block columns are in flight together.
 // ---------------------------------------------------------------------------
 constexpr int kAngThreads = 256;
-constexpr int kAngLB = 4;
+constexpr int kAngLB = 4;  // default multipoles per block
 
 template <int LB, class T = double2>
 __global__ __launch_bounds__(kAngThreads) void k_angular(const T *__restrict__ x, const int *__restrict__ hAp,
@@ -894,14 +894,35 @@ void free_all(gade_model *m) {
     m->event_pool.clear();
 }
 
-int launch_angular(gade_model *m, const double2 *x) {
-    dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
-    size_t shm = sizeof(double2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
-    hipLaunchKernelGGL((k_angular<kAngLB, double2>), grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, m->d_coef,
-                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V, m->gate, m->gstride, m->nblk1);
+// the angular kernel stages a row's coefficients for its multipole group in LDS
+bool ang_lds_ok(const gade_model *m, int lb) {
+    return (size_t)m->maxrow * (lb * 2 * sizeof(double2) + sizeof(int2)) <= 60 * 1024;
+}
+
+// the angular kernel with the model's multipole group size (2, 4 or 8 multipoles per block; each V entry is
+// summed over the row's couplings in the same order for every group size)
+template <class T>
+int launch_angular_t(gade_model *m, const T *x, const T *coef, T *V) {
+    const int LB = m->ang_lb;
+    dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + LB - 1) / LB);
+    size_t shm = sizeof(T) * (size_t)m->maxrow * LB * 2 + sizeof(int2) * (size_t)m->maxrow;
+#define GADE_ANG(LB_)                                                                                            \
+    hipLaunchKernelGGL((k_angular<LB_, T>), grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, coef, \
+                       m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, V, m->gate, m->gstride, m->nblk1)
+    switch (LB) {
+        case 2: GADE_ANG(2); break;
+        case 4: GADE_ANG(4); break;
+        case 8: GADE_ANG(8); break;
+        default:
+            gade::set_error("angular multipole group must be 2, 4 or 8");
+            return GADE_EINVAL;
+    }
+#undef GADE_ANG
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
+
+int launch_angular(gade_model *m, const double2 *x) { return launch_angular_t<double2>(m, x, m->d_coef, m->d_V); }
 
 // the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
 // each output in the same order, so all give the same results bit for bit
@@ -1241,7 +1262,7 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
         return fail(GADE_EINVAL);
     }
     for (int r = 0; r < m->nblk; r++) m->maxrow = std::max(m->maxrow, m->hall_Ap[r + 1] - m->hall_Ap[r]);
-    if ((size_t)m->maxrow * (kAngLB * 2 * sizeof(double2) + sizeof(int2)) > 60 * 1024) {
+    if (!ang_lds_ok(m, kAngLB)) {
         gade::set_error("Hall rows too long for the angular kernel's LDS staging");
         return fail(GADE_EINVAL);
     }
@@ -1732,12 +1753,7 @@ int mp_setup(gade_model *m) {
 int apply32_async(gade_model *m, const float2 *x, float2 *y) {
     {
         Prof pr(m, "angular_f32", GADE_PROF_ANGULAR);
-        dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + kAngLB - 1) / kAngLB);
-        size_t shm = sizeof(float2) * (size_t)m->maxrow * kAngLB * 2 + sizeof(int2) * (size_t)m->maxrow;
-        hipLaunchKernelGGL((k_angular<kAngLB, float2>), grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp,
-                           m->d_hAi, m->d_coef32, m->d_kap, m->nblk, m->n, m->lmax, m->maxrow, m->d_V32, m->gate,
-                           m->gstride, m->nblk1);
-        GADE_HIP(hipGetLastError());
+        GADE_TRY(launch_angular_t<float2>(m, x, (const float2 *)m->d_coef32, m->d_V32));
     }
     const int lgrid = m->lmax;
     float2 *yp = lgrid > 1 ? m->d_ypart32 : nullptr;
@@ -2268,19 +2284,21 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         times[step] = time;
     }
     struct Cand {
-        int window, iter_batch, graph, dia_r, dia_p;
+        int window, iter_batch, graph, dia_r, dia_p, ang_lb;
         double ms;
     };
-    // stage 1: banded tile shapes with the model's stepping; stage 2: stepping with the best shape
+    // stage 1: kernel shapes (banded tile shape x angular multipole group) with the model's stepping;
+    // stage 2: stepping with the best shapes
     std::vector<Cand> cands;
     if (m->radial_kind == 2)
         for (auto rp : {std::pair<int, int>{8, 2}, {4, 2}, {4, 4}})
-            cands.push_back({m->window, m->iter_batch, m->use_graph ? 1 : 0, rp.first, rp.second, 0.0});
+            for (int lb : {4, 2, 8})
+                if (ang_lds_ok(m, lb)) cands.push_back({m->window, m->iter_batch, m->use_graph ? 1 : 0, rp.first, rp.second, lb, 0.0});
     const size_t nshape = cands.size();
     for (int g : {1, 0}) {
         if (g && (m->radial_kind != 2 || !m->use_graph)) continue;  // graph 0 on the model excludes graphs
         for (int w : {2, 4, 8, 16, 32})
-            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, 0.0});
+            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, -1, 0.0});
     }
     const size_t N = (size_t)m->N, B = (size_t)m->B;
     // the solver state a run starts from (BiCGSTAB workspace, which carries over between solves, scalars,
@@ -2319,9 +2337,13 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         return GADE_OK;
     };
     const int sv_window = m->window, sv_batch = m->iter_batch, sv_profile = m->profile, sv_k = m->last_k;
-    const int sv_r = m->dia_r, sv_p = m->dia_p;
+    const int sv_r = m->dia_r, sv_p = m->dia_p, sv_lb = m->ang_lb;
     const bool sv_graph = m->use_graph;
-    auto set_shape = [&](int r, int pp) -> int {
+    auto set_shape = [&](int r, int pp, int lb) -> int {
+        if (lb != m->ang_lb) {
+            drop_step_graph(m);
+            m->ang_lb = lb;
+        }
         if (!m->dia_ok || (r == m->dia_r && pp == m->dia_p)) return GADE_OK;
         drop_step_graph(m);
         if (r != m->dia_r)
@@ -2336,7 +2358,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         m->use_graph = sv_graph;
         m->profile = sv_profile;
         m->last_k = sv_k;
-        (void)set_shape(sv_r, sv_p);
+        (void)set_shape(sv_r, sv_p, sv_lb);
         const int rc = copy_state(false);
         (void)hipStreamSynchronize(m->stream);
         (void)hipFree(work);
@@ -2357,7 +2379,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         if (c == nshape && nshape > 0) {
             for (size_t k = 1; k < nshape; k++)
                 if (cands[k].ms < cands[best_shape].ms) best_shape = k;
-            if (int rc = set_shape(cands[best_shape].dia_r, cands[best_shape].dia_p)) {
+            if (int rc = set_shape(cands[best_shape].dia_r, cands[best_shape].dia_p, cands[best_shape].ang_lb)) {
                 restore();
                 return rc;
             }
@@ -2365,7 +2387,8 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         if (cands[c].dia_r < 0) {
             cands[c].dia_r = m->dia_r;
             cands[c].dia_p = m->dia_p;
-        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p)) {
+            cands[c].ang_lb = m->ang_lb;
+        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p, cands[c].ang_lb)) {
             restore();
             return rc;
         }
@@ -2407,6 +2430,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             gade::set_error("gade_autotune: candidate window " + std::to_string(cands[c].window) + " iter_batch " +
                             std::to_string(cands[c].iter_batch) + " graph " + std::to_string(cands[c].graph) +
                             " tile " + std::to_string(cands[c].dia_r) + "x" + std::to_string(cands[c].dia_p) +
+                            " angular group " + std::to_string(cands[c].ang_lb) +
                             " changed the trajectory");
             return GADE_EINVAL;
         }
@@ -2418,7 +2442,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
     m->window = cands[pick].window;
     m->iter_batch = cands[pick].iter_batch;
     m->use_graph = cands[pick].graph != 0;
-    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p));
+    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p, cands[pick].ang_lb));
     if (report)
         for (size_t c = 0; c < cands.size() && c < GADE_TUNE_MAX; c++) {
             double *r = report + (size_t)GADE_TUNE_COLS * c;
@@ -2427,7 +2451,8 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             r[2] = cands[c].graph;
             r[3] = cands[c].dia_r;
             r[4] = cands[c].dia_p;
-            r[5] = cands[c].ms;
+            r[5] = cands[c].ang_lb;
+            r[6] = cands[c].ms;
         }
     if (ncand) *ncand = (int)std::min<size_t>(cands.size(), GADE_TUNE_MAX);
     if (chosen) *chosen = (int)pick;
@@ -2616,6 +2641,15 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->use_graph = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "ang_lb")) {
+        if ((value != 2 && value != 4 && value != 8) || !ang_lds_ok(m, value)) {
+            gade::set_error("ang_lb must be 2, 4 or 8 (and the Hall rows short enough for its LDS staging)");
+            return GADE_EINVAL;
+        }
+        drop_step_graph(m);
+        m->ang_lb = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "dia_rows") || !strcmp(name, "dia_outputs")) {
         const bool rows = name[4] == 'r';
         const int r = rows ? value : m->dia_r, pp = rows ? m->dia_p : value;
@@ -2743,6 +2777,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "dia_rows")) {
         *value = m->dia_r;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "ang_lb")) {
+        *value = m->ang_lb;
         return GADE_OK;
     }
     if (!strcmp(name, "dia_outputs")) {

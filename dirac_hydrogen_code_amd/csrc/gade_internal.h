@@ -118,6 +118,7 @@ struct gade_model {
     double *d_dtile_k = nullptr;
     int *d_dtile_b = nullptr;      // instance of each tile (tiles never mix instances)
     int dia_r = 8, dia_p = 2;      // banded tile shape: rows per tile, outputs per lane
+    int ang_lb = 4;                // multipoles per angular block
     size_t gd_stride = 0;          // GD entries per instance
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     double2 *d_ypart = nullptr;    // [lmax][nblk][n] per-multipole partial outputs of the radial phase

@@ -37,7 +37,7 @@ LG_FEATURES = 17 + 4 * LG_POD                               # used inputs (the r
 LG_SAMPLE_VALID = LG_IN + 36 + 6 + 12 * LG_POD + 2 * LG_POD + 1  # index of a sample's valid flag
 
 GADE_TUNE_MAX = 64
-GADE_TUNE_COLS = 6
+GADE_TUNE_COLS = 7
 
 _lib = None
 
@@ -216,7 +216,7 @@ class Propagator:
         check(lib().gade_autotune(self.ptr, psi.ptr, params.ctypes.data, t0, nsteps, tol, maxit, reps,
                                   rep.ctypes.data, C.byref(nc), C.byref(ch)))
         rows = [dict(window=int(r[0]), iter_batch=int(r[1]), graph=int(r[2]), dia_rows=int(r[3]),
-                     dia_outputs=int(r[4]), ms=float(r[5])) for r in rep[:nc.value]]
+                     dia_outputs=int(r[4]), ang_lb=int(r[5]), ms=float(r[6])) for r in rep[:nc.value]]
         return ch.value, rows
 
     # ---- learned initial guess ----
