@@ -448,6 +448,13 @@ __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ g
     const int b = b0 + blockIdx.y;
     GADE_GATED_B(gate, gstride, b);
     const Coef6 cf = cf_dev ? cf_dev[(size_t)gctl[GC_WSLOT] * B + b] : cg.c[blockIdx.y];
+    // coefficients of the alpha of each multipole parity, held with static indices (no private-memory copy)
+    double2 c_even[3], c_odd[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        c_even[j] = cf.c[2 * j + 1];
+        c_odd[j] = cf.c[2 * j];
+    }
     double2 *__restrict__ GD = GD0 + (size_t)b * gd_stride;
     const int S = 4 * nd * nhp;  // entries per (l, j, nn) plane
     const long long total = (long long)lmax * S;
@@ -460,7 +467,7 @@ __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ g
         for (int m = 0; m < 8; m++) G[m] = make_double2(0, 0);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const double2 c = cf.c[2 * j + ((l % 2 == 0) ? 1 : 0)];
+            const double2 c = (l % 2 == 0) ? c_even[j] : c_odd[j];
 #pragma unroll
             for (int nn = 0; nn < 4; nn++) {
                 const size_t o = ((size_t)(l * 3 + j) * 4 + nn) * S + e;
@@ -1491,7 +1498,7 @@ int gade_precond_setup(gade_model_t *m) {
             for (int k = m->rad_Ap[a]; k < m->rad_Ap[a + 1]; k++)
                 Pq[(size_t)q * m->nnzb + k] = Dh[((size_t)q * m->ellw + (k - m->rad_Ap[a])) * n + a];
     return gade::blocklu_build(m->nblk, n, m->rad_Ap.data(), m->rad_Ai.data(), nq, m->blk_q.data(), Pq, m->stream,
-                               &m->pre);
+                               &m->pre, m->psolve_ch);
 }
 
 }  // extern "C"
@@ -1714,15 +1721,30 @@ int enqueue_iterations(gade_model *m, double2 *psi, int K) {
 // correction (residual, update, tolerance test and every accumulator stay in FP64)
 // ---------------------------------------------------------------------------
 int mp_setup(gade_model *m) {
-    if (m->mp_ready) return GADE_OK;
-    if (m->radial_kind != 2 || !m->pre || m->pre->ch != 16) {
-        gade::set_error("mixed precision needs the banded radial kernel and 16-row preconditioner levels");
+    if (m->mp_ready && m->lu32[0]) return GADE_OK;
+    if (m->radial_kind != 2 || !m->pre || !((m->pre->ch == 16 || m->pre->ch == 32) && m->pre->band <= 16)) {
+        gade::set_error("mixed precision needs the banded radial kernel and 16- or 32-row preconditioner levels");
         return GADE_EINVAL;
+    }
+    const size_t nlu = (size_t)m->nq * m->pre->nch * m->pre->ch * m->pre->ch;
+    auto conv = [&](const double2 *src, float2 *dst, size_t n) {
+        hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)n, 256, 8192), 1), dim3(256), 0, m->stream,
+                           (long long)n, (size_t)0, src, dst, (const int *)nullptr, 0);
+    };
+    if (m->mp_ready) {
+        // the preconditioner was rebuilt: its FP32 copies only
+        for (auto &f : m->lu32) GADE_TRY(hip_alloc((void **)&f, sizeof(float2) * nlu));
+        conv(m->pre->d_Linv, m->lu32[0], nlu);
+        conv(m->pre->d_Uinv, m->lu32[1], nlu);
+        conv(m->pre->d_Loff, m->lu32[2], nlu);
+        conv(m->pre->d_Uoff, m->lu32[3], nlu);
+        GADE_HIP(hipGetLastError());
+        GADE_HIP(hipStreamSynchronize(m->stream));
+        return GADE_OK;
     }
     const size_t N = (size_t)m->N, L = (size_t)m->lmax;
     const size_t S = (size_t)4 * m->nd * m->nhp;
     const size_t nDD = 3 * (size_t)m->nq * S, nco = (size_t)m->hnnz * L * 2;
-    const size_t nlu = (size_t)m->nq * m->pre->nch * 256;
     GADE_TRY(hip_alloc((void **)&m->d_V32, sizeof(float2) * L * 4 * N));
     GADE_TRY(hip_alloc((void **)&m->d_ypart32, sizeof(float2) * L * N));
     GADE_TRY(hip_alloc((void **)&m->d_GD32, sizeof(float2) * m->gd_stride * m->B));
@@ -1733,10 +1755,6 @@ int mp_setup(gade_model *m) {
         GADE_TRY(hip_alloc((void **)v, sizeof(float2) * N));
     GADE_TRY(hip_alloc((void **)&m->d_mp_ax, sizeof(double2) * N));
     GADE_TRY(hip_alloc((void **)&m->d_sc32, sizeof(double) * (size_t)m->B * SC_STRIDE));
-    auto conv = [&](const double2 *src, float2 *dst, size_t n) {
-        hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)n, 256, 8192), 1), dim3(256), 0, m->stream,
-                           (long long)n, (size_t)0, src, dst, (const int *)nullptr, 0);
-    };
     conv(m->d_DD, m->d_DD32, nDD);
     conv(m->d_coef, m->d_coef32, nco);
     conv(m->pre->d_Linv, m->lu32[0], nlu);
@@ -1775,10 +1793,15 @@ int apply32_async(gade_model *m, const float2 *x, float2 *y) {
 int psolve32_async(gade_model *m, const float2 *b, float2 *x) {
     Prof pr(m, "psolve_f32", GADE_PROF_PSOLVE);
     const gade_blocklu *lu = m->pre;
-    const size_t shq = sizeof(float2) * ((size_t)lu->nch * 16 + 16);
-    hipLaunchKernelGGL((k_psolve_q<4, float2>), dim3(lu->norder), dim3(64), shq, m->stream, b, x, lu->d_order,
-                       lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n, lu->band,
-                       lu->nch, m->gate, m->gstride, m->nblk1);
+    const size_t shq = sizeof(float2) * ((size_t)lu->nch * lu->ch + lu->ch);
+    if (lu->ch == 16)
+        hipLaunchKernelGGL((k_psolve_q<4, float2, 16>), dim3(lu->norder), dim3(64), shq, m->stream, b, x,
+                           lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
+                           lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
+    else
+        hipLaunchKernelGGL((k_psolve_q<2, float2, 32>), dim3(lu->norder), dim3(128), shq, m->stream, b, x,
+                           lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
+                           lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -2641,6 +2664,24 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->use_graph = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "psolve_chunk")) {
+        if (value != 16 && value != 32) {
+            gade::set_error("psolve_chunk must be 16 or 32");
+            return GADE_EINVAL;
+        }
+        if (value != m->psolve_ch) {
+            GADE_HIP(hipStreamSynchronize(m->stream));
+            drop_step_graph(m);
+            gade::blocklu_free(m->pre);  // rebuilt with the new level height on next use
+            m->pre = nullptr;
+            for (auto &f : m->lu32) {
+                if (f) (void)hipFree(f);
+                f = nullptr;
+            }
+            m->psolve_ch = value;
+        }
+        return GADE_OK;
+    }
     if (!strcmp(name, "ang_lb")) {
         if ((value != 2 && value != 4 && value != 8) || !ang_lds_ok(m, value)) {
             gade::set_error("ang_lb must be 2, 4 or 8 (and the Hall rows short enough for its LDS staging)");
@@ -2783,6 +2824,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
         *value = m->ang_lb;
         return GADE_OK;
     }
+    if (!strcmp(name, "psolve_chunk")) {
+        *value = m->pre ? m->pre->ch : m->psolve_ch;
+        return GADE_OK;
+    }
     if (!strcmp(name, "dia_outputs")) {
         *value = m->dia_p;
         return GADE_OK;
@@ -2851,7 +2896,7 @@ void blocklu_free(gade_blocklu *lu) {
 }
 
 int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
-                  const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out) {
+                  const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out, int min_ch) {
     const int nnzb = rAp[n];
     // interleave the two radial halves so that banded quadrants form one band
     std::vector<int> ilv(n);
@@ -2875,6 +2920,7 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
     lu->ilv = inter;
     lu->band = std::max(band, 1);
     lu->ch = lu->band < 16 ? 16 : (lu->band < 32 ? 32 : 64);
+    if (min_ch > lu->ch) lu->ch = min_ch >= 64 ? 64 : (min_ch >= 32 ? 32 : 16);
     const int CH = lu->ch;
     lu->nch = (n + CH - 1) / CH;
     const int w = 2 * lu->band + 1;
@@ -2959,16 +3005,22 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
         gade::set_error("preconditioner not set up");
         return GADE_EINVAL;
     }
-    if (lu->ch != 16 && gstride != 0) {
-        gade::set_error("gated (batched) preconditioner solves need 16-row levels (radial half-bandwidth < 16)");
+    const bool quad = (lu->ch == 16 || lu->ch == 32) && lu->band <= 16;
+    if (!quad && gstride != 0) {
+        gade::set_error("gated (batched) preconditioner solves need 16- or 32-row levels and a band of at most 16");
         return GADE_EINVAL;
     }
-    if (lu->ch == 16) {
-        // one block per wave, four lanes per row
-        const size_t shq = sizeof(double2) * ((size_t)lu->nch * 16 + 16);
-        hipLaunchKernelGGL(k_psolve_q<4>, dim3(lu->norder), dim3(64), shq, stream, b, x, lu->d_order, lu->d_blk_q, lu->ilv,
-                           lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n, lu->band, lu->nch, gate, gstride,
-                           nblk1);
+    if (quad) {
+        // one block per workgroup, four lanes per row
+        const size_t shq = sizeof(double2) * ((size_t)lu->nch * lu->ch + lu->ch);
+        if (lu->ch == 16)
+            hipLaunchKernelGGL((k_psolve_q<4, double2, 16>), dim3(lu->norder), dim3(64), shq, stream, b, x,
+                               lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff,
+                               lu->n, lu->band, lu->nch, gate, gstride, nblk1);
+        else
+            hipLaunchKernelGGL((k_psolve_q<2, double2, 32>), dim3(lu->norder), dim3(128), shq, stream, b, x,
+                               lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff,
+                               lu->n, lu->band, lu->nch, gate, gstride, nblk1);
         GADE_HIP(hipGetLastError());
         return GADE_OK;
     }

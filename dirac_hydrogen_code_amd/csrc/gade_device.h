@@ -725,20 +725,19 @@ __device__ __forceinline__ T quad_sum(T v) {
     return v;
 }
 
-// x = P^{-1} b with 16-row levels and one block per wave: lane = 4*row + k,
-// and each of the four lanes of a row takes a quarter of the band coupling
-// and of the level inverse (terms 4k..4k+3), summed over the quad with DPP.
-// b is staged in LDS; the factors of level ch+PF are loaded as soon as the
-// register slot of level ch is free, so PF levels of loads are in flight.
-// Requires CH = 16 and band <= 16 (coupling rows padded to 16, zero beyond band).
-template <int PF, class T = double2>
-__global__ __launch_bounds__(64) void k_psolve_q(const T *__restrict__ b, T *__restrict__ xo,
-                                                 const int *__restrict__ order, const int *__restrict__ blk_q,
-                                                 int ilv, const T *__restrict__ Linv,
-                                                 const T *__restrict__ Uinv, const T *__restrict__ Loff,
-                                                 const T *__restrict__ Uoff, int n, int band, int nch,
-                                                 const int *gate = nullptr, int gstride = 0, int nblk1 = 1) {
-    constexpr int CH = 16;
+// x = P^{-1} b with CH-row levels (CH = 16 or 32) and one block per workgroup of 4*CH lanes: lane = 4*row
+// + k, and each of the four lanes of a row takes a quarter of the band coupling (terms 4k..4k+3) and of the
+// level inverse (terms (CH/4)k ..), summed over the quad with DPP.  b is staged in LDS; the factors of level
+// ch+PF are loaded as soon as the register slot of level ch is free, so PF levels of loads are in flight.
+// Requires band <= 16 (coupling rows padded to CH, zero beyond band).
+template <int PF, class T = double2, int CH = 16>
+__global__ __launch_bounds__(4 * CH) void k_psolve_q(const T *__restrict__ b, T *__restrict__ xo,
+                                                     const int *__restrict__ order, const int *__restrict__ blk_q,
+                                                     int ilv, const T *__restrict__ Linv, const T *__restrict__ Uinv,
+                                                     const T *__restrict__ Loff, const T *__restrict__ Uoff, int n,
+                                                     int band, int nch, const int *gate = nullptr, int gstride = 0,
+                                                     int nblk1 = 1) {
+    constexpr int NT = 4 * CH, QI = CH / 4;  // lanes; inverse terms per lane
     extern __shared__ __align__(16) char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
     const int lane = threadIdx.x, i = lane >> 2, k = lane & 3;
@@ -750,34 +749,33 @@ __global__ __launch_bounds__(64) void k_psolve_q(const T *__restrict__ b, T *__r
     T *acc = smem + nch * CH;  // [CH]
     const T *bb = b + (size_t)r * n;
     const int ntot = nch * CH;
-    for (int c0 = 0; c0 < ntot; c0 += 64 * 8) {
+    for (int c0 = 0; c0 < ntot; c0 += NT * 8) {
         T t[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int c = c0 + 64 * u + lane;
+            const int c = c0 + NT * u + lane;
             t[u] = (c < n) ? bb[natural_index(c, n, ilv)] : czero<T>();
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int c = c0 + 64 * u + lane;
+            const int c = c0 + NT * u + lane;
             if (c < ntot) yv[c] = t[u];
         }
     }
     __syncthreads();
     const size_t cs = (size_t)CH * CH;
-    T fc[PF][4], fv[PF][4];
-    // factors of level ch into slot s: coupling rows 4k..4k+3 and inverse columns 4k..4k+3 of row i
-    auto load = [&](const T *Off, const T *Inv, int ch, T (&c)[4], T (&v)[4]) {
+    T fc[PF][4], fv[PF][QI];
+    // factors of level ch into slot s: coupling rows 4k..4k+3 and inverse columns QI*k.. of row i
+    auto load = [&](const T *Off, const T *Inv, int ch, T (&c)[4], T (&v)[QI]) {
         const T *po = Off + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
-        const T *pi = Inv + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
+        const T *pi = Inv + ((size_t)q * nch + ch) * cs + (size_t)(QI * k) * CH + i;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            c[u] = po[(size_t)u * CH];
-            v[u] = pi[(size_t)u * CH];
-        }
+        for (int u = 0; u < 4; u++) c[u] = po[(size_t)u * CH];
+#pragma unroll
+        for (int u = 0; u < QI; u++) v[u] = pi[(size_t)u * CH];
     };
     // one level: val = b_i - sum_t C[t][i] y[cbase + t];  y_i = sum_j Inv[j][i] val_j
-    auto level = [&](int ch, int cbase, bool couple, const T (&c)[4], const T (&v)[4]) {
+    auto level = [&](int ch, int cbase, bool couple, const T (&c)[4], const T (&v)[QI]) {
         const int gi = ch * CH + i;
         T s = czero<T>();
         if (couple) {
@@ -790,7 +788,7 @@ __global__ __launch_bounds__(64) void k_psolve_q(const T *__restrict__ b, T *__r
         __syncthreads();
         T y = czero<T>();
 #pragma unroll
-        for (int u = 0; u < 4; u++) y = cfma(v[u], acc[4 * k + u], y);
+        for (int u = 0; u < QI; u++) y = cfma(v[u], acc[QI * k + u], y);
         y = quad_sum(y);
         if (k == 0) yv[gi] = y;
         __syncthreads();
@@ -821,7 +819,7 @@ __global__ __launch_bounds__(64) void k_psolve_q(const T *__restrict__ b, T *__r
             if (ch - PF >= 0) load(Uoff, Uinv, ch - PF, fc[u], fv[u]);
         }
     }
-    for (int c = lane; c < n; c += 64) xo[(size_t)r * n + natural_index(c, n, ilv)] = yv[c];
+    for (int c = lane; c < n; c += NT) xo[(size_t)r * n + natural_index(c, n, ilv)] = yv[c];
 }
 
 // x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =

@@ -60,8 +60,9 @@ struct gade_lg;  // learned initial guess (learned.hip)
 
 namespace gade {
 // Pq: values of each distinct block in the CSR order of (rAp, rAi), [nq][nnz]
+// min_ch: smallest level height (16 or 32 rows); taller levels when the band needs them
 int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
-                  const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out);
+                  const std::vector<double2> &Pq, hipStream_t stream, gade_blocklu **out, int min_ch = 16);
 int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hipStream_t stream,
                         const int *gate = nullptr, int gstride = 0, int nblk1 = 1);
 void blocklu_free(gade_blocklu *lu);
@@ -119,6 +120,7 @@ struct gade_model {
     int *d_dtile_b = nullptr;      // instance of each tile (tiles never mix instances)
     int dia_r = 8, dia_p = 2;      // banded tile shape: rows per tile, outputs per lane
     int ang_lb = 4;                // multipoles per angular block
+    int psolve_ch = 16;            // preconditioner level height (16 or 32 rows)
     size_t gd_stride = 0;          // GD entries per instance
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]
     double2 *d_ypart = nullptr;    // [lmax][nblk][n] per-multipole partial outputs of the radial phase

@@ -62,10 +62,13 @@ def test_fused_apply_matches_oracle(datadir, cfg, kind):
         assert relmax(dy.get(), S + H + 0.5 * y0) <= 1e-13
 
 
+@pytest.mark.parametrize("chunk", [16, 32])
 @pytest.mark.parametrize("cfg", CFGS)
-def test_psolve_matches_oracle(datadir, cfg):
+def test_psolve_matches_oracle(datadir, cfg, chunk):
+    """The level-scheduled block-LU solve (16- or 32-row levels) against the oracle's banded solve."""
     d, dt, om, gp = pair(datadir, cfg)
-    say(f"psolve {cfg}")
+    gp.set_param("psolve_chunk", chunk)
+    say(f"psolve {cfg} chunk {chunk}")
     rng = np.random.default_rng(1)
     b = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
     want = om.psolve(b)
@@ -245,6 +248,58 @@ def test_device_loop_zero_rhs(datadir):
         it, tols, _ = gp.cn_run(dpsi, 10.0, 50.0, 1.0, 3, t0=0.0, tol=1e-12, timing=False)
         assert np.all(it == 0) and np.all(tols == 0)
         assert not np.any(dpsi.get())
+
+
+def test_psolve_chunk_trajectory(datadir):
+    """32-row preconditioner levels: the device-driven trajectory (FP64 and mixed, batched) matches the 16-row
+    one to solver tolerance (the level inverses round differently) and meets the tolerance every step."""
+    cfg = "dipole"
+    d = datadir(cfg)
+    dt = dt_of(cfg)
+    say("psolve chunk trajectory")
+    params = [(10.0, 50.0, 1.0, 0.0), (25.0, 40.0, 1.0, 1.3)]
+    out = {}
+    for chunk in (16, 32):
+        for prec in (0, 1):
+            gp = gade.Propagator(d, LMAX[cfg], dt, batch=2)
+            gp.set_param("psolve_chunk", chunk)
+            gp.set_param("precision", prec)
+            psi0, _ = gade.read_psi0(d, gp.n)
+            x = gade.DeviceArray.from_host(np.tile(psi0, 2))
+            it, tols = gp.cn_run_batch(x, params, 30, tol=1e-12)
+            assert gp.get_param("psolve_chunk") == chunk
+            assert np.all(tols <= 1e-12)
+            out[(chunk, prec)] = x.get()
+    for prec in (0, 1):
+        a, b = out[(32, prec)], out[(16, prec)]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-10, prec
+
+
+def test_iteration_budget_reached(datadir):
+    """An unreachable tolerance with a small iteration budget: every solve stops at maxit (break-before-
+    increment count = maxit, tolerance above the request) and the device-driven loop equals the host loop bit for
+    bit; the mixed form stops at the same inner budget."""
+    cfg = "nondipole_small"
+    d, dt, om, gp = pair(datadir, cfg)
+    psi0, _ = gade.read_psi0(d, gp.n)
+    say("iteration budget")
+    out = {}
+    for dev in (0, 1):
+        gp.set_param("device_loop", dev)
+        gp.reset_workspace()
+        x = gade.DeviceArray.from_host(psi0)
+        it, tols, _ = gp.cn_run(x, 10.0, 50.0, 1.0, 5, tol=1e-18, maxit=2, timing=False)
+        out[dev] = (x.get(), it, tols)
+        assert np.all(it == 2) and np.all(tols > 1e-18), (dev, it, tols)
+    assert np.array_equal(out[0][0].view(np.float64), out[1][0].view(np.float64))
+    assert np.array_equal(out[0][2], out[1][2])
+    gp.set_param("device_loop", 1)
+    gp.set_param("precision", 1)
+    gp.reset_workspace()
+    x = gade.DeviceArray.from_host(psi0)
+    it, tols, _ = gp.cn_run(x, 10.0, 50.0, 1.0, 5, tol=1e-18, maxit=3, timing=False)
+    assert np.all(it == 3) and np.all(tols > 1e-18), (it, tols)
+    gp.set_param("precision", 0)
 
 
 @pytest.mark.parametrize("cfg", ["tiny", "nondipole_small"])
