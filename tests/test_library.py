@@ -54,6 +54,20 @@ def test_index_maps_and_pulse_match_oracle():
         assert np.array_equal(a.view(np.float64), b.view(np.float64))
 
 
+def test_index_maps_and_pulse_match_reference_golden_vectors():
+    """The library's host code against the reference-written fixtures (tests/golden), bit for bit."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_pulse_ik.npz"),
+                allow_pickle=False)
+    L = gade.lib()
+    for i, k, mu in zip(g["index"], g["ik"], g["imu"]):
+        assert L.gade_ik(int(i)) == int(k) and L.gade_imu(int(i)) == mu
+    for (E0, w, cyc, t), f in zip(g["pulse_args"], g["pulse_f"]):
+        assert np.array_equal(gade.pulse(E0, w, cyc, t).view(np.float64), f.view(np.float64))
+        # the carrier-phase form with phase 0 is the reference pulse
+        assert np.array_equal(gade.pulse_cep(E0, w, cyc, 0.0, t).view(np.float64), f.view(np.float64))
+
+
 class SparseCSR(C.Structure):
     """include/csr.h sparse_csr_t"""
     _fields_ = [(n, C.c_int) for n in ("nrows", "ncols", "nnz", "blk_dim", "blk_nnz", "npart", "row_beg",

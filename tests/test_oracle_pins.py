@@ -48,6 +48,22 @@ def test_ik_imu_bitexact_vs_reference():
         assert a == b and math.copysign(1, a) == math.copysign(1, b)
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_pulse_ik.npz")
+
+
+def test_ik_imu_pulse_golden_vectors():
+    """The oracle against fixtures written by the reference's own compiled tools/solver_common.c
+    (tests/golden/make_golden.py): bit for bit, without the reference sources at hand."""
+    g = np.load(GOLDEN, allow_pickle=False)
+    L = O.lib()
+    for i, k, mu in zip(g["index"], g["ik"], g["imu"]):
+        assert L.or_ik(int(i)) == int(k)
+        a = L.or_imu(int(i))
+        assert a == mu and math.copysign(1, a) == math.copysign(1, mu)
+    for (E0, w, cyc, t), f in zip(g["pulse_args"], g["pulse_f"]):
+        assert np.array_equal(O.pulse(E0, w, cyc, t).view(np.float64), f.view(np.float64))
+
+
 # ---------------------------------------------------------------- A1: pulse
 def test_pulse_closed_forms():
     for E0, w, cyc in ((10, 50, 1), (3.3, 25.0, 15), (100, 75.0, 4)):
