@@ -2190,6 +2190,8 @@ int gade_reset_workspace(gade_model_t *m) {
     if (!m) return GADE_EINVAL;
     double2 *ws[] = {m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t};
     for (auto w : ws) GADE_HIP(hipMemsetAsync(w, 0, sizeof(double2) * m->N, m->stream));
+    // scalars too (every solve re-initialises its own; the learned guess's inputs read the last residual)
+    GADE_HIP(hipMemsetAsync(m->d_sc, 0, sizeof(double) * m->B * SC_STRIDE, m->stream));
     hipLaunchKernelGGL(k_hist_reset, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl);  // the guess histories restart
     GADE_HIP(hipGetLastError());
     if (m->lg) GADE_TRY(gade::lg_reset(m));
@@ -2331,8 +2333,11 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
     constexpr int NWS = 8;
     const bool hist = m->d_hist != nullptr;
     const size_t vec_bytes = sizeof(double2) * N;
+    void *lg_ptr = nullptr;
+    size_t lg_bytes = 0;
+    gade::lg_state(m, &lg_ptr, &lg_bytes);
     const size_t state_bytes = vec_bytes * (NWS + (hist ? kHistSlots : 0)) + sizeof(double) * B * SC_STRIDE +
-                               sizeof(int) * B * CT_STRIDE;
+                               sizeof(int) * B * CT_STRIDE + lg_bytes;
     double2 *work = nullptr;
     char *state = nullptr;
     GADE_TRY(hip_alloc((void **)&work, vec_bytes));
@@ -2353,6 +2358,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         if (hist && e == hipSuccess) e = mv(m->d_hist, vec_bytes * kHistSlots);
         if (e == hipSuccess) e = mv(m->d_sc, sizeof(double) * B * SC_STRIDE);
         if (e == hipSuccess) e = mv(m->d_ctl, sizeof(int) * B * CT_STRIDE);
+        if (e == hipSuccess && lg_ptr) e = mv(lg_ptr, lg_bytes);
         if (e != hipSuccess) {
             gade::set_error(std::string("gade_autotune: state copy: ") + hipGetErrorString(e));
             return GADE_EHIP;

@@ -201,5 +201,6 @@ namespace gade {
 void drop_step_graph(gade_model *m);
 int lg_prepare(gade_model *m);
 int lg_reset(gade_model *m);  // residual history of the learned guess's inputs
+void lg_state(gade_model *m, void **ptr, size_t *bytes);
 int lg_step_prologue(gade_model *m, double2 *psi);
 }  // namespace gade

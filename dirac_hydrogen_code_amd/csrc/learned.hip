@@ -731,6 +731,12 @@ namespace gade {
 // buffers of the learned guess (no allocation may happen while a step graph is captured)
 int lg_prepare(gade_model *m) { return lg_ensure(m); }
 
+// device state of the learned guess that carries from step to step (the residual history of its inputs)
+void lg_state(gade_model *m, void **ptr, size_t *bytes) {
+    *ptr = m->lg ? (void *)m->lg->lres : nullptr;
+    *bytes = m->lg ? sizeof(double) * m->B * 8 : 0;
+}
+
 int lg_reset(gade_model *m) {
     if (!m->lg) return GADE_OK;
     GADE_HIP(hipMemsetAsync(m->lg->lres, 0, sizeof(double) * m->B * 8, m->stream));

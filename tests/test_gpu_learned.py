@@ -136,4 +136,17 @@ def test_learned_guess_pipeline_and_held_out_scan(datadir):
     for name in ("extrapolation", "learned"):
         rel = np.linalg.norm(res[name][0] - ref) / np.linalg.norm(ref)
         assert rel <= 1e-9, (name, rel)
+    # with the learned guess active the autotuner still leaves the trajectory unchanged (its probes restore the
+    # guess's input history with the rest of the solver state); two runs from a reset are identical
+    runs = []
+    for tune in (1, 0):
+        x = gade.DeviceArray.from_host(np.tile(psi0, B))
+        gp.reset_workspace()
+        gp.cn_run_batch(x, params, 8, tol=1e-12)
+        if tune:
+            gp.autotune(x, params, 4, t0=8 * dt, reps=1)
+        it, tols = gp.cn_run_batch(x, params, 12, t0=8 * dt, tol=1e-12)
+        runs.append((x.get(), it, tols))
+    assert np.array_equal(runs[0][0].view(np.float64), runs[1][0].view(np.float64))
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
     gp.set_param("guess_order", 0)
