@@ -2068,6 +2068,7 @@ int cn_run_device(gade_model *m, double2 *psi, const double *times, int nsteps, 
     GADE_TRY(gade_precond_setup(m));
     if (m->precision == 1) GADE_TRY(mp_setup(m));
     GADE_TRY(ensure_step_buffers(m, nsteps));
+    if (m->guess_order == GADE_GUESS_LEARNED || m->lg_features) GADE_TRY(gade::lg_prepare(m));
     const size_t cap = (size_t)m->steps_cap;
     std::vector<int> it_h((size_t)B * cap, 0);
     std::vector<double> tol_h((size_t)B * cap, 0.0);
@@ -2751,6 +2752,14 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->lg_features = value != 0;
         return GADE_OK;
     }
+    if (!strcmp(name, "lg_zmax")) {
+        if (value < 0 || value > 1000) {
+            gade::set_error("lg_zmax must be 0 (no range gate) .. 1000");
+            return GADE_EINVAL;
+        }
+        m->lg_zmax = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "window") || !strcmp(name, "iter_batch")) {
         if (value < (name[0] == 'w' ? 1 : 0) || value > 4096) {
             gade::set_error(std::string(name) + " out of range");
@@ -2840,6 +2849,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "lg_features")) {
         *value = m->lg_features;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "lg_zmax")) {
+        *value = m->lg_zmax;
         return GADE_OK;
     }
     if (!strcmp(name, "mp_inner_iters")) {

@@ -175,6 +175,7 @@ struct gade_model {
     double2 *d_hist = nullptr;     // [kHistSlots][N] history ring of previous solutions (per instance)
     gade_lg *lg = nullptr;         // learned initial guess
     int lg_features = 0;           // compute the learned guess's inputs every step (training data)
+    int lg_zmax = 8;               // learned guess: inputs beyond this many standard deviations take the fallback
     // mixed precision: FP32 inner BiCGSTAB with the FP32 operator and preconditioner, FP64 defect correction
     int precision = 0;             // 0: FP64 (the reference), 1: mixed
     int mp_inner_iters = 2;        // inner iterations per sweep

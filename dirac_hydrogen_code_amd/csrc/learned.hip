@@ -80,27 +80,30 @@ __device__ __forceinline__ float gelu_d(float z) {
 
 // ---------------------------------------------------------------------------
 // BF16 MFMA GEMM: C(m,n) = sum_k A(m,k) B(k,n), FP32 accumulation, with
-// A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn] (bf16 bits).  One wave
+// A(m,k) = a[m*sam + k*sak], B(k,n) = b[k*sbk + n*sbn] (bf16 bits).  One block
 // per 16x16 tile of C (v_mfma_f32_16x16x16_bf16: lane l holds A[l%16][4(l/16)..+3],
-// B[4(l/16)..+3][l%16] and C[4(l/16)+i][l%16]).  M, N, K multiples of 16.
+// B[4(l/16)..+3][l%16] and C[4(l/16)+i][l%16]).  The block's KS waves take the
+// 16-wide k steps in turn (wave w: steps w, w + KS, ...) and their accumulators
+// are added in wave order; KS = 1 for the training sizes, KS = 4 for the few
+// rows of inference (short per-wave chains).  M, N, K multiples of 16.
 // ---------------------------------------------------------------------------
 typedef short bf4 __attribute__((ext_vector_type(4)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
 enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_GELU = 2 };
 
-template <int EPI>
-__global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ a, long sam,
-                                                  long sak, const uint16_t *__restrict__ b, long sbk, long sbn,
-                                                  float *__restrict__ c, int ldc, const float *__restrict__ bias,
-                                                  uint16_t *__restrict__ act, int ldact) {
+template <int EPI, int KS = 1>
+__global__ __launch_bounds__(64 * KS) void k_gemm_bf16(int M, int N, int K, const uint16_t *__restrict__ a, long sam,
+                                                       long sak, const uint16_t *__restrict__ b, long sbk, long sbn,
+                                                       float *__restrict__ c, int ldc, const float *__restrict__ bias,
+                                                       uint16_t *__restrict__ act, int ldact) {
     const int tiles_n = N / 16;
     const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
     if (tm * 16 >= M) return;
-    const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
     const uint16_t *ap = a + (long)(tm * 16 + r) * sam;
     const uint16_t *bp = b + (long)(tn * 16 + r) * sbn;
     fl4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = 16 * w; k0 < K; k0 += 16 * KS) {
         bf4 av, bv;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -109,6 +112,14 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(int M, int N, int K, const uin
             bv[j] = (short)bp[kk * sbk];
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    if constexpr (KS > 1) {
+        __shared__ fl4 red[KS][64];
+        red[w][lane] = acc;
+        __syncthreads();
+        if (w != 0) return;
+#pragma unroll
+        for (int q = 1; q < KS; q++) acc += red[q][lane];
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -165,9 +176,10 @@ __global__ __launch_bounds__(256) void k_lg_diff(long long N1, long long Ntot, c
             v[k] = x[k];
         }
         v[6] = rec ? csub(psi[i], cadd(v[0], cadd(v[1], v[2]))) : make_double2(0, 0);
+        // D_0 = psi_n is kept only when recording (its projection is a sample input); inference reads psi
 #pragma unroll
         for (int k = 0; k < kNV; k++)
-            if (k < nv) V[(size_t)k * Ntot + i] = v[k];
+            if (k < nv && (rec || k > 0)) V[(size_t)k * Ntot + i] = v[k];
         int t = 0;
 #pragma unroll
         for (int p = 0; p < kNV; p++)
@@ -189,15 +201,26 @@ __global__ __launch_bounds__(256) void k_lg_diff(long long N1, long long Ntot, c
     }
 }
 
-__global__ void k_lg_gram_fin(int nparts, const double *__restrict__ part0, double *__restrict__ gram,
-                              const int *gate, int gstride) {
+// gram[b][e] = sum of the block partials: eight interleaved groups of partials, then the groups in order
+// (lanes e = t % 32, group t / 32; 256 threads)
+__global__ __launch_bounds__(256) void k_lg_gram_fin(int nparts, const double *__restrict__ part0,
+                                                     double *__restrict__ gram, const int *gate, int gstride) {
     const int b = blockIdx.x;
     GADE_GATED_B(gate, gstride, b);
-    const int t = threadIdx.x;
-    if (t >= kNG) return;
+    __shared__ double sh[8][32];
+    const int e = threadIdx.x & 31, g = threadIdx.x >> 5;
     double s = 0;
-    for (int p = 0; p < nparts; p++) s += part0[((size_t)b * nparts + p) * kNG + t];
-    gram[(size_t)b * kNG + t] = s;
+    if (e < kNG)
+#pragma unroll 4
+        for (int p = g; p < nparts; p += 8) s += part0[((size_t)b * nparts + p) * kNG + e];
+    sh[g][e] = s;
+    __syncthreads();
+    if (g == 0 && e < kNG) {
+        double t = sh[0][e];
+#pragma unroll
+        for (int q = 1; q < 8; q++) t += sh[q][e];
+        gram[(size_t)b * kNG + e] = t;
+    }
 }
 
 __device__ __forceinline__ int gidx(int p, int q) {  // index of Re(V_p^H V_q), p <= q
@@ -206,21 +229,28 @@ __device__ __forceinline__ int gidx(int p, int q) {  // index of Re(V_p^H V_q), 
 
 // partial[b][c][v][j] = sum over chunk c of conj(UT[k][j]) V_{v0+v}[b][k]: one block per (chunk, b), lanes
 // (j = t % 64, s = t / 64), UT the k-major copy of the basis (coalesced over j), V shared by the 64 lanes
-constexpr int kProjChunk = 256;
+// Grid (B, nchunk): the instances of one chunk are neighbouring blocks, so the chunk of the basis is read from
+// memory once and from cache for the other instances.
+constexpr int kProjChunk = 256;  // shortest chunk
+inline int proj_chunks(long long N1, int B) {  // chunks per instance: ~4k blocks in all, 64..1024 per instance
+    const long long want = std::max(64, std::min(1024, 4096 / std::max(1, B)));
+    return (int)std::max<long long>(1, std::min<long long>(want, (N1 + kProjChunk - 1) / kProjChunk));
+}
 __global__ __launch_bounds__(256) void k_lg_proj_part(long long N1, long long Ntot, int npod,
                                                       const double2 *__restrict__ UT, const double2 *__restrict__ V,
                                                       int v0, int nv, double2 *__restrict__ part, const int *gate,
-                                                      int gstride) {
-    const int c = blockIdx.x, b = blockIdx.y, nch = gridDim.x;
+                                                      int gstride, long long clen) {
+    const int b = blockIdx.x, c = blockIdx.y, nch = gridDim.y;
     GADE_GATED_B(gate, gstride, b);
     __shared__ double2 sh[4][kNV][64];
     const int j = threadIdx.x & 63, sl = threadIdx.x >> 6;
     double2 acc[kNV];
 #pragma unroll
     for (int v = 0; v < kNV; v++) acc[v] = make_double2(0, 0);
-    const long long k1 = (long long)(c + 1) * kProjChunk < N1 ? (long long)(c + 1) * kProjChunk : N1;
+    const long long k1 = (long long)(c + 1) * clen < N1 ? (long long)(c + 1) * clen : N1;
     if (j < npod)
-        for (long long k = (long long)c * kProjChunk + sl; k < k1; k += 4) {
+#pragma unroll 4
+        for (long long k = (long long)c * clen + sl; k < k1; k += 4) {
             const double2 u = UT[(size_t)k * npod + j];
 #pragma unroll
             for (int v = 0; v < kNV; v++) {
@@ -237,18 +267,35 @@ __global__ __launch_bounds__(256) void k_lg_proj_part(long long N1, long long Nt
                 cadd(cadd(sh[0][v][j], sh[1][v][j]), cadd(sh[2][v][j], sh[3][v][j]));
 }
 
-// out[b][v][j] = sum over chunks of the partials (fixed order)
+// out[b][v][j] = sum over chunks of the partials in a fixed order: sixteen interleaved chunk groups, then the
+// groups in order.  Grid (B, nv, 4): a block takes 16 of the 64 basis indices, lanes (j, group).
 __global__ __launch_bounds__(256) void k_lg_proj_fin(int nch, int npod, int nv, const double2 *__restrict__ part,
                                                      double2 *__restrict__ out, const int *gate, int gstride) {
-    const int b = blockIdx.x;
+    const int b = blockIdx.x, v = blockIdx.y;
     GADE_GATED_B(gate, gstride, b);
-    for (int t = threadIdx.x; t < nv * 64; t += blockDim.x) {
-        const int v = t / 64, j = t % 64;
-        if (j >= npod) continue;
-        double2 s = make_double2(0, 0);
-        for (int c = 0; c < nch; c++) s = cadd(s, part[(((size_t)b * nch + c) * kNV + v) * 64 + j]);
-        out[((size_t)b * kNV + v) * npod + j] = s;
+    __shared__ double2 sh[16][16];
+    const int jl = threadIdx.x & 15, g = threadIdx.x >> 4, j = blockIdx.z * 16 + jl;
+    double2 s = make_double2(0, 0);
+#pragma unroll 4
+    for (int c = g; c < nch; c += 16) s = cadd(s, part[(((size_t)b * nch + c) * kNV + v) * 64 + j]);
+    sh[g][jl] = s;
+    __syncthreads();
+    if (g == 0 && j < npod) {
+        double2 t = sh[0][jl];
+#pragma unroll
+        for (int q = 1; q < 16; q++) t = cadd(t, sh[q][jl]);
+        out[((size_t)b * kNV + v) * npod + j] = t;
     }
+}
+
+// squared norm of snapshot column c (one block per column)
+__global__ __launch_bounds__(256) void k_lg_colnorm(long long n, const double2 *__restrict__ X, double *__restrict__ out) {
+    __shared__ double sh[4 * 16];
+    const double2 *x = X + (size_t)blockIdx.x * n;
+    double acc[1] = {0};
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) acc[0] += x[i].x * x[i].x + x[i].y * x[i].y;
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
 }
 
 // UT[k][j] = U[j][k]
@@ -260,7 +307,10 @@ __global__ void k_lg_transpose(long long N1, int npod, const double2 *__restrict
     }
 }
 
-// log10 residual ring [b][8], newest first: push the previous solve's final relative residual
+// log10 residual ring [b][8], newest first: push the previous solve's final residual relative to its
+// requested bound tol |rhs| (0 where the solve ended just at the bound, below 0 the further it went under it):
+// the same scale at any tolerance, so a network trained on solves at one tolerance reads the same inputs at
+// another
 __global__ void k_lg_lres_push(const double *__restrict__ sc0, double *__restrict__ lres, const int *gate,
                                int gstride) {
     const int b = blockIdx.x;
@@ -269,19 +319,25 @@ __global__ void k_lg_lres_push(const double *__restrict__ sc0, double *__restric
     const double *sc = sc0 + (size_t)b * SC_STRIDE;
     double *r = lres + (size_t)b * 8;
     for (int k = 7; k > 0; k--) r[k] = r[k - 1];
-    const double t = sc[SC_RHSSQ] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_RHSSQ]) : 0.0;
+    const double t = sc[SC_TOL2] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_TOL2]) : 0.0;
     r[0] = log10(fmax(t, 1e-30));
 }
 
 // raw inputs of instance b and its standardized bf16 row.  f_alpha(t_{n+1}) comes from the window's
 // coefficient table (c_alpha = hd * i * f_alpha, f real).
-__global__ void k_lg_features(const Coef6 *__restrict__ cf_dev, const int *__restrict__ gctl, int B, double hd,
-                              const double *__restrict__ lres, const double *__restrict__ gram,
-                              const double2 *__restrict__ z, int npod, const float *__restrict__ mean,
-                              const float *__restrict__ stdv, float *__restrict__ feat, uint16_t *__restrict__ xin,
-                              const int *gate, int gstride) {
+// Range gate: oor[b] = 1 when a standardized input lies beyond *zmax standard deviations of the training
+// inputs (or is not finite), *zmax > 0; such a step takes the extrapolation instead of the network.
+// One wave per instance.
+__global__ __launch_bounds__(64) void k_lg_features(const Coef6 *__restrict__ cf_dev, const int *__restrict__ gctl,
+                                                    int B, double hd, const double *__restrict__ lres,
+                                                    const double *__restrict__ gram, const double2 *__restrict__ z,
+                                                    int npod, const float *__restrict__ mean,
+                                                    const float *__restrict__ stdv, float *__restrict__ feat,
+                                                    uint16_t *__restrict__ xin, const float *__restrict__ zmax,
+                                                    int *__restrict__ oor, const int *gate, int gstride) {
     const int b = blockIdx.x;
     GADE_GATED_B(gate, gstride, b);
+    float zabs = 0.f;
     const Coef6 *cf = cf_dev + (size_t)gctl[GC_WSLOT] * B + b;
     const double *g = gram + (size_t)b * kNG;
     const double n0 = sqrt(fmax(g[gidx(0, 0)], 1e-300));
@@ -303,16 +359,24 @@ __global__ void k_lg_features(const Coef6 *__restrict__ cf_dev, const int *__res
             v = ((t - F_Z4) & 1 ? zz.y : zz.x) * inv_s;
         }
         fr[t] = (float)v;
-        xin[(size_t)b * kIn + t] = t < F_END ? f2bf(((float)v - mean[t]) / stdv[t]) : (uint16_t)0;
+        const float zt = t < F_END ? ((float)v - mean[t]) / stdv[t] : 0.f;
+        zabs = (zt == zt) ? fmaxf(zabs, fabsf(zt)) : INFINITY;
+        xin[(size_t)b * kIn + t] = t < F_END ? f2bf(zt) : (uint16_t)0;
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) zabs = fmaxf(zabs, __shfl_xor(zabs, off));
+    if (threadIdx.x == 0) oor[b] = (*zmax > 0.f && !(zabs <= *zmax)) ? 1 : 0;
 }
 
 // the guess of instance b into psi (psi_n enters the ring); fallback: order-2 extrapolation (or lower)
+// (the network's term needs a full history, a non-zero s, finite outputs and inputs inside the range gate);
+// uses[b] counts the steps that took the network (0) and those the range gate sent to the extrapolation (1)
 __global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, double2 *__restrict__ psi,
                                                   double2 *__restrict__ hist, const double2 *__restrict__ V,
                                                   const double *__restrict__ gram, const float *__restrict__ Y,
                                                   const double2 *__restrict__ U, int npod,
-                                                  const int *__restrict__ ctl0, const int *gate, int gstride) {
+                                                  const int *__restrict__ ctl0, const int *__restrict__ oor,
+                                                  int *__restrict__ uses, const int *gate, int gstride) {
     const int b = blockIdx.y;
     GADE_GATED_B(gate, gstride, b);
     __shared__ double2 cs[kPod];
@@ -325,7 +389,8 @@ __global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, 
     if (threadIdx.x == 0) {
         bool ok = have >= 5 && s > 0 && isfinite(s);
         for (int k = 0; k < 6 + 2 * npod; k++) ok = ok && isfinite(y[k]);
-        ok_s = ok;
+        if (ok && blockIdx.x == 0) uses[2 * b + (oor[b] ? 1 : 0)] += 1;
+        ok_s = ok && oor[b] == 0;
     }
     __syncthreads();
     const bool ok = ok_s != 0;
@@ -349,6 +414,7 @@ __global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, 
             x = cadd(x, cscale(xn, ws[0]));
 #pragma unroll
             for (int k = 1; k < 6; k++) x = cadd(x, cscale(V[(size_t)k * Ntot + i], ws[k]));
+#pragma unroll 8
             for (int q = 0; q < npod; q++) x = cfma(cs[q], U[(size_t)q * N1 + j], x);
         } else {
             // Lagrange extrapolation of order p through psi_n..psi_{n-p}
@@ -365,7 +431,8 @@ __global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, 
 }
 
 // training sample of instance b after its step (record mode of k_lg_diff / k_lg_proj): the projected
-// quadratic form of the normalized error, next to the inputs saved at the step's start
+// quadratic form of the normalized error, next to the inputs saved at the step's start.  Valid: a full
+// history, an active instance and a converged solve (a solve that ran out of iterations is no target)
 __global__ void k_lg_record(const double *__restrict__ gram, const double2 *__restrict__ proj, int npod,
                             const float *__restrict__ feat, const int *__restrict__ ctl0, float *__restrict__ out) {
     const int b = blockIdx.x;
@@ -376,7 +443,7 @@ __global__ void k_lg_record(const double *__restrict__ gram, const double2 *__re
     for (int k = 0; k < kNV; k++) nrm[k] = sqrt(fmax(g[gidx(k, k)], 0.0));
     const double s = nrm[3];
     const bool valid = ctl[CT_HIST] >= 6 && ctl[CT_PSKIP] == 0 && s > 0 && isfinite(s) &&
-                       (ctl[CT_STATUS] == 1 || ctl[CT_STATUS] == 3);
+                       ctl[CT_STATUS] == 1;
     for (int t = threadIdx.x; t < kSample; t += blockDim.x) {
         double v = 0;
         if (!valid) {
@@ -563,6 +630,10 @@ struct gade_lg {
     double *lres = nullptr;       // [B][8]
     float *feat = nullptr;        // [B][kIn] raw inputs of the current step
     float *mean = nullptr, *stdv = nullptr;  // [kIn]
+    float *zmax = nullptr;        // range gate of the inputs (device scalar, standard deviations; 0: off)
+    float zmax_set = -1.f;        // the value last written there
+    int *oor = nullptr;           // [B] this step's inputs lie outside the range gate
+    int *uses = nullptr;          // [B][2] steps that took the network / that the range gate sent to the fallback
     int Mi = 0;                   // inference rows (B padded to 16)
     uint16_t *xin = nullptr;      // [Mi][kIn]
     float *zi[3] = {}, *yi = nullptr;  // inference pre-activations [Mi][kHid], outputs [Mi][kOut]
@@ -583,6 +654,7 @@ namespace gade {
 void lg_free(gade_lg *lg) {
     if (!lg) return;
     std::vector<void *> ptrs = {lg->U, lg->UT, lg->ppart, lg->V, lg->part, lg->gram, lg->proj, lg->lres, lg->feat, lg->mean, lg->stdv,
+                                lg->zmax, lg->oor, lg->uses,
                                 lg->xin, lg->yi, lg->p, lg->g, lg->m1, lg->m2, lg->pbf, lg->xt, lg->yt, lg->dy,
                                 lg->da, lg->dz, lg->loss};
     for (int k = 0; k < 3; k++) {
@@ -606,13 +678,13 @@ int lg_ensure(gade_model *m) {
     gade_lg *lg = new gade_lg();
     m->lg = lg;
     const size_t N = (size_t)m->N, B = (size_t)m->B;
-    lg->nparts = (int)std::min<long long>(256, std::max<long long>(1, (m->N1 + 1023) / 1024));
+    lg->nparts = (int)std::min<long long>(512, std::max<long long>(1, (m->N1 + 511) / 512));
     lg->Mi = pad16(m->B);
     GADE_TRY(hip_alloc((void **)&lg->V, sizeof(double2) * kNV * N));
     GADE_TRY(hip_alloc((void **)&lg->part, sizeof(double) * B * lg->nparts * kNG));
     GADE_TRY(hip_alloc((void **)&lg->gram, sizeof(double) * B * kNG));
     GADE_TRY(hip_alloc((void **)&lg->proj, sizeof(double2) * B * kNV * kPod));
-    lg->nchunk = (int)((m->N1 + kProjChunk - 1) / kProjChunk);
+    lg->nchunk = proj_chunks(m->N1, m->B);
     GADE_TRY(hip_alloc((void **)&lg->ppart, sizeof(double2) * B * lg->nchunk * kNV * 64));
     GADE_TRY(hip_alloc((void **)&lg->lres, sizeof(double) * B * 8));
     GADE_TRY(hip_alloc((void **)&lg->feat, sizeof(float) * B * kIn));
@@ -626,6 +698,13 @@ int lg_ensure(gade_model *m) {
     }
     GADE_HIP(hipMemset(lg->xin, 0, sizeof(uint16_t) * lg->Mi * kIn));
     GADE_HIP(hipMemset(lg->lres, 0, sizeof(double) * B * 8));
+    GADE_TRY(hip_alloc((void **)&lg->zmax, sizeof(float)));
+    GADE_TRY(hip_alloc((void **)&lg->oor, sizeof(int) * B));
+    GADE_TRY(hip_alloc((void **)&lg->uses, sizeof(int) * B * 2));
+    GADE_HIP(hipMemset(lg->oor, 0, sizeof(int) * B));
+    GADE_HIP(hipMemset(lg->zmax, 0, sizeof(float)));
+    lg->zmax_set = 0.f;
+    GADE_HIP(hipMemset(lg->uses, 0, sizeof(int) * B * 2));
     GADE_HIP(hipMemset(lg->proj, 0, sizeof(double2) * B * kNV * kPod));
     std::vector<float> one(kIn, 1.0f);
     GADE_HIP(hipMemset(lg->mean, 0, sizeof(float) * kIn));
@@ -671,13 +750,16 @@ int lg_train_alloc(gade_lg *lg, int Mt) {
 // out[b][v][j] = U_j^H V_{v0+v} for v < nv
 void lg_project(gade_model *m, int v0, int nv, const int *gate, int gstride) {
     gade_lg *lg = m->lg;
-    hipLaunchKernelGGL(k_lg_proj_part, dim3(lg->nchunk, m->B), dim3(256), 0, m->stream, m->N1, m->N, lg->npod,
-                       (const double2 *)lg->UT, (const double2 *)lg->V, v0, nv, lg->ppart, gate, gstride);
-    hipLaunchKernelGGL(k_lg_proj_fin, dim3(m->B), dim3(256), 0, m->stream, lg->nchunk, lg->npod, nv,
+    const long long clen = (m->N1 + lg->nchunk - 1) / lg->nchunk;
+    hipLaunchKernelGGL(k_lg_proj_part, dim3(m->B, lg->nchunk), dim3(256), 0, m->stream, m->N1, m->N, lg->npod,
+                       (const double2 *)lg->UT, (const double2 *)lg->V, v0, nv, lg->ppart, gate, gstride, clen);
+    hipLaunchKernelGGL(k_lg_proj_fin, dim3(m->B, nv, 4), dim3(256), 0, m->stream, lg->nchunk, lg->npod, nv,
                        (const double2 *)lg->ppart, lg->proj, gate, gstride);
 }
 
-// forward of M rows: X (bf16 [M][kIn]) -> Z_l, A_l, Y
+// forward of M rows: X (bf16 [M][kIn]) -> Z_l, A_l, Y; up to kSplitKRows rows (inference) each tile's k steps are
+// split over four waves
+constexpr int kSplitKRows = 64;
 void mlp_forward(const gade_lg *lg, hipStream_t st, int M, const uint16_t *X, float *const Z[3], uint16_t *const A[3],
                  float *Y) {
     const uint16_t *in = X;
@@ -686,12 +768,20 @@ void mlp_forward(const gade_lg *lg, hipStream_t st, int M, const uint16_t *X, fl
         const dim3 grid((M / 16) * (N / 16));
         const uint16_t *W = lg->pbf + kWoff[l];
         const float *bias = lg->p + kBoff[l];
-        if (l < 3)
+        if (M <= kSplitKRows) {
+            if (l < 3)
+                hipLaunchKernelGGL((k_gemm_bf16<EPI_BIAS_GELU, 4>), grid, dim3(256), 0, st, M, N, K, in, (long)K, 1L,
+                                   W, 1L, (long)K, Z[l], N, bias, A[l], N);
+            else
+                hipLaunchKernelGGL((k_gemm_bf16<EPI_BIAS, 4>), grid, dim3(256), 0, st, M, N, K, in, (long)K, 1L, W,
+                                   1L, (long)K, Y, N, bias, (uint16_t *)nullptr, 0);
+        } else if (l < 3) {
             hipLaunchKernelGGL(k_gemm_bf16<EPI_BIAS_GELU>, grid, dim3(64), 0, st, M, N, K, in, (long)K, 1L, W, 1L,
                                (long)K, Z[l], N, bias, A[l], N);
-        else
+        } else {
             hipLaunchKernelGGL(k_gemm_bf16<EPI_BIAS>, grid, dim3(64), 0, st, M, N, K, in, (long)K, 1L, W, 1L, (long)K,
                                Y, N, bias, (uint16_t *)nullptr, 0);
+        }
         if (l < 3) in = A[l];
     }
 }
@@ -729,7 +819,19 @@ void mlp_backward(gade_lg *lg, hipStream_t st, int M, const uint16_t *X) {
 namespace gade {
 
 // buffers of the learned guess (no allocation may happen while a step graph is captured)
-int lg_prepare(gade_model *m) { return lg_ensure(m); }
+// before a run's steps are enqueued (outside any capture): the range gate's bound into device memory, where
+// the steps read it; stream-ordered, after the launches of earlier runs that read the old bound
+int lg_prepare(gade_model *m) {
+    GADE_TRY(lg_ensure(m));
+    gade_lg *lg = m->lg;
+    const float z = (float)m->lg_zmax;
+    if (z != lg->zmax_set) {
+        lg->zmax_set = z;
+        GADE_HIP(hipMemcpyAsync(lg->zmax, &lg->zmax_set, sizeof(float), hipMemcpyHostToDevice, m->stream));
+        GADE_HIP(hipStreamSynchronize(m->stream));
+    }
+    return GADE_OK;
+}
 
 // device state of the learned guess that carries from step to step (the residual history of its inputs)
 void lg_state(gade_model *m, void **ptr, size_t *bytes) {
@@ -740,6 +842,7 @@ void lg_state(gade_model *m, void **ptr, size_t *bytes) {
 int lg_reset(gade_model *m) {
     if (!m->lg) return GADE_OK;
     GADE_HIP(hipMemsetAsync(m->lg->lres, 0, sizeof(double) * m->B * 8, m->stream));
+    GADE_HIP(hipMemsetAsync(m->lg->uses, 0, sizeof(int) * m->B * 2, m->stream));
     return GADE_OK;
 }
 
@@ -764,20 +867,21 @@ int lg_step_prologue(gade_model *m, double2 *psi) {
                        m->gstride);
     hipLaunchKernelGGL(k_lg_diff, gd, dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
                        (const double2 *)m->d_hist, (const int *)m->d_ctl, 0, lg->V, lg->part, m->gate, m->gstride);
-    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(64), 0, m->stream, lg->nparts, (const double *)lg->part,
+    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(256), 0, m->stream, lg->nparts, (const double *)lg->part,
                        lg->gram, m->gate, m->gstride);
     if (lg->npod > 0) lg_project(m, 3, 2, m->gate, m->gstride);
     const double hd = kSoL * m->h * m->dt / 2;
     hipLaunchKernelGGL(k_lg_features, dim3(B), dim3(64), 0, m->stream, (const Coef6 *)m->d_cfwin,
                        (const int *)m->d_gctl, B, hd, (const double *)lg->lres, (const double *)lg->gram,
                        (const double2 *)lg->proj, lg->npod, (const float *)lg->mean, (const float *)lg->stdv,
-                       lg->feat, lg->xin, m->gate, m->gstride);
+                       lg->feat, lg->xin, (const float *)lg->zmax, lg->oor, m->gate, m->gstride);
     GADE_HIP(hipGetLastError());
     if (learned) {
         mlp_forward(lg, m->stream, lg->Mi, lg->xin, lg->zi, lg->ai, lg->yi);
         hipLaunchKernelGGL(k_lg_guess, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
                            (const double2 *)lg->V, (const double *)lg->gram, (const float *)lg->yi,
-                           (const double2 *)lg->U, lg->npod, (const int *)m->d_ctl, m->gate, m->gstride);
+                           (const double2 *)lg->U, lg->npod, (const int *)m->d_ctl, (const int *)lg->oor, lg->uses,
+                           m->gate, m->gstride);
     } else {
         hipLaunchKernelGGL(k_extrap, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
                            m->guess_order, (const int *)m->d_ctl, m->gate, m->gstride);
@@ -803,7 +907,7 @@ int gade_lg_record(gade_model_t *m, const double *psi, float *samples, double *s
     hipLaunchKernelGGL(k_lg_diff, dim3(lg->nparts, B), dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
                        (const double2 *)m->d_hist, (const int *)m->d_ctl, 1, lg->V, lg->part, (const int *)nullptr,
                        0);
-    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(64), 0, m->stream, lg->nparts, (const double *)lg->part,
+    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(256), 0, m->stream, lg->nparts, (const double *)lg->part,
                        lg->gram, (const int *)nullptr, 0);
     if (snaps)
         hipLaunchKernelGGL(k_lg_snap, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N,
@@ -840,7 +944,32 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
             if (q) (void)hipFree(q);
     };
     int rc = GADE_OK;
-    const size_t ns = (size_t)nsnap;
+    // snapshot columns that are zero (unused slots) or not finite are left out: exact zero columns make R
+    // singular in a way the bidiagonal SVD does not converge on
+    std::vector<int> keep;
+    {
+        double *d_nrm = nullptr;
+        GADE_TRY(hip_alloc((void **)&d_nrm, sizeof(double) * nsnap));
+        hipLaunchKernelGGL(k_lg_colnorm, dim3(nsnap), dim3(256), 0, m->stream, (long long)n, (const double2 *)snaps,
+                           d_nrm);
+        std::vector<double> nrm(nsnap);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(nrm.data(), d_nrm, sizeof(double) * nsnap, hipMemcpyDeviceToHost, m->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+        (void)hipFree(d_nrm);
+        if (e != hipSuccess) {
+            set_error(std::string("gade_lg_pod_build: ") + hipGetErrorString(e));
+            return GADE_EHIP;
+        }
+        for (int c = 0; c < nsnap; c++)
+            if (nrm[c] > 0 && std::isfinite(nrm[c])) keep.push_back(c);
+    }
+    if ((int)keep.size() < npod) {
+        set_error("gade_lg_pod_build: " + std::to_string(keep.size()) + " non-zero snapshots, fewer than the basis");
+        return GADE_EINVAL;
+    }
+    const int nused = (int)keep.size();
+    const size_t ns = (size_t)nused;
     if ((rc = hip_alloc((void **)&A, sizeof(double2) * n * ns)) || (rc = hip_alloc((void **)&tau, sizeof(double2) * ns)) ||
         (rc = hip_alloc((void **)&R, sizeof(double2) * ns * ns)) || (rc = hip_alloc((void **)&Sv, sizeof(double2) * ns * ns)) ||
         (rc = hip_alloc((void **)&Uq, sizeof(double2) * ns * ns)) || (rc = hip_alloc((void **)&C, sizeof(double2) * n * npod)) ||
@@ -857,24 +986,26 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
     }
     rocblas_set_stream(h, m->stream);
     // snapshots [col][n] are the column-major n x nsnap matrix X; X = Q R; R = Ur S Vr^H; U = Q Ur[:, :npod]
-    bool ok = hipMemcpyAsync(A, snaps, sizeof(double2) * n * ns, hipMemcpyDeviceToDevice, m->stream) == hipSuccess;
-    ok = ok && rocsolver_zgeqrf(h, n, nsnap, (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau) ==
+    bool ok = true;
+    for (size_t c = 0; c < ns && ok; c++)
+        ok = hipMemcpyAsync(A + c * n, (const double2 *)snaps + (size_t)keep[c] * n, sizeof(double2) * n,
+                            hipMemcpyDeviceToDevice, m->stream) == hipSuccess;
+    ok = ok && rocsolver_zgeqrf(h, n, nused, (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau) ==
                    rocblas_status_success;
-    // R: upper triangle of the first nsnap rows
-    std::vector<double2> Ah;
-    if (ok) {
-        Ah.resize((size_t)n * ns);
-        ok = hipMemcpyAsync(Ah.data(), A, sizeof(double2) * n * ns, hipMemcpyDeviceToHost, m->stream) == hipSuccess &&
-             hipStreamSynchronize(m->stream) == hipSuccess;
-    }
+    // R: upper triangle of the first nused rows (only those rows come back: a strided copy)
     std::vector<double2> Rh(ns * ns, make_double2(0, 0));
     if (ok) {
+        ok = hipMemcpy2DAsync(Rh.data(), sizeof(double2) * ns, A, sizeof(double2) * n, sizeof(double2) * ns, ns,
+                              hipMemcpyDeviceToHost, m->stream) == hipSuccess &&
+             hipStreamSynchronize(m->stream) == hipSuccess;
+    }
+    if (ok) {
         for (size_t c = 0; c < ns; c++)
-            for (size_t r = 0; r <= c; r++) Rh[c * ns + r] = Ah[c * n + r];
+            for (size_t r = c + 1; r < ns; r++) Rh[c * ns + r] = make_double2(0, 0);  // below the diagonal: reflectors
         ok = hipMemcpy(R, Rh.data(), sizeof(double2) * ns * ns, hipMemcpyHostToDevice) == hipSuccess;
     }
-    ok = ok && rocsolver_zgesvd(h, rocblas_svect_all, rocblas_svect_none, nsnap, nsnap, (rocblas_double_complex *)R,
-                                nsnap, S, (rocblas_double_complex *)Uq, nsnap, (rocblas_double_complex *)Sv, nsnap, E,
+    ok = ok && rocsolver_zgesvd(h, rocblas_svect_all, rocblas_svect_none, nused, nused, (rocblas_double_complex *)R,
+                                nused, S, (rocblas_double_complex *)Uq, nused, (rocblas_double_complex *)Sv, nused, E,
                                 rocblas_outofplace, info) == rocblas_status_success;
     // C = [Ur[:, :npod]; 0] (n x npod), then U = Q C
     std::vector<double2> Uh, Ch;
@@ -889,7 +1020,7 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
             for (size_t r = 0; r < ns; r++) Ch[(size_t)c * n + r] = Uh[(size_t)c * ns + r];
         ok = hipMemcpy(C, Ch.data(), sizeof(double2) * n * npod, hipMemcpyHostToDevice) == hipSuccess;
     }
-    ok = ok && rocsolver_zunmqr(h, rocblas_side_left, rocblas_operation_none, n, npod, nsnap,
+    ok = ok && rocsolver_zunmqr(h, rocblas_side_left, rocblas_operation_none, n, npod, nused,
                                 (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau,
                                 (rocblas_double_complex *)C, n) == rocblas_status_success;
     int info_h = 0;
@@ -902,10 +1033,11 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
         set_error("gade_lg_pod_build: rocSOLVER QR / SVD failed (info " + std::to_string(info_h) + ")");
         return GADE_EHIP;
     }
-    if (sv) {
+    if (sv) {  // singular values of the used snapshots, then zeros for the columns left out
         std::vector<double> Sh(ns);
         (void)hipMemcpy(Sh.data(), S, sizeof(double) * ns, hipMemcpyDeviceToHost);
         memcpy(sv, Sh.data(), sizeof(double) * ns);
+        for (int c = nused; c < nsnap; c++) sv[c] = 0.0;
     }
     if (lg->U) (void)hipFree(lg->U);
     lg->U = C;
@@ -917,6 +1049,25 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
                        npod, (const double2 *)lg->U, lg->UT);
     GADE_HIP(hipGetLastError());
     GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
+
+int gade_lg_set_basis(gade_model_t *m, const double *U, int npod) {
+    if (!m || !U || npod <= 0 || npod > kPod) return GADE_EINVAL;
+    GADE_TRY(lg_ensure(m));
+    gade_lg *lg = m->lg;
+    gade::drop_step_graph(m);  // the graph reads the basis being replaced
+    const size_t n = (size_t)m->N1;
+    if (lg->U) (void)hipFree(lg->U);
+    lg->U = nullptr;
+    GADE_TRY(hip_alloc((void **)&lg->U, sizeof(double2) * n * npod));
+    if (!lg->UT) GADE_TRY(hip_alloc((void **)&lg->UT, sizeof(double2) * n * kPod));
+    GADE_HIP(hipMemcpy(lg->U, U, sizeof(double2) * n * npod, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for_lg((long long)n * npod)), dim3(256), 0, m->stream, (long long)n,
+                       npod, (const double2 *)lg->U, lg->UT);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    lg->npod = npod;
     return GADE_OK;
 }
 
@@ -975,6 +1126,17 @@ int gade_lg_get_params(gade_model_t *m, float *params) {
 }
 
 // forward of M rows of raw inputs (device FP32 [M][GADE_LG_IN], M a multiple of 16) into Y [M][GADE_LG_OUT]
+int gade_lg_uses(gade_model_t *m, int *counts) {
+    if (!m || !counts) return GADE_EINVAL;
+    if (!m->lg) {
+        for (int k = 0; k < 2 * m->B; k++) counts[k] = 0;
+        return GADE_OK;
+    }
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    GADE_HIP(hipMemcpy(counts, m->lg->uses, sizeof(int) * 2 * m->B, hipMemcpyDeviceToHost));
+    return GADE_OK;
+}
+
 int gade_lg_forward(gade_model_t *m, const float *x, int M, float *y) {
     if (!m || !x || !y || M <= 0 || M % 16) return GADE_EINVAL;
     if (!m->lg || !m->lg->net) return GADE_EINVAL;

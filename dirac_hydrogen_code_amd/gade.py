@@ -27,14 +27,15 @@ EXPORTS = (
     "gade_extrapolate", "gade_model_create_batch", "gade_model_load_batch", "gade_model_batch", "gade_cn_run_batch",
     "gade_set_time_batch", "gade_pulse_cep", "gade_autotune", "gade_lg_record", "gade_lg_pod_build",
     "gade_lg_basis", "gade_lg_set_norm", "gade_lg_init", "gade_lg_set_params", "gade_lg_get_params",
-    "gade_lg_forward", "gade_lg_grad", "gade_lg_train",
+    "gade_lg_forward", "gade_lg_grad", "gade_lg_train", "gade_lg_set_basis", "gade_lg_uses",
 )
 
 # learned initial guess (include/gade.h)
 GUESS_LEARNED = 4
 LG_IN, LG_HIDDEN, LG_OUT, LG_POD, LG_SAMPLE, LG_PARAMS = 288, 256, 144, 64, 1232, 242576
 LG_FEATURES = 17 + 4 * LG_POD                               # used inputs (the rest is zero padding)
-LG_SAMPLE_VALID = LG_IN + 36 + 6 + 12 * LG_POD + 2 * LG_POD + 1  # index of a sample's valid flag
+LG_SAMPLE_BETA = LG_IN + 36 + 6 + 12 * LG_POD + 2 * LG_POD  # index of |e2|^2 / s^2 (the loss's normalizer)
+LG_SAMPLE_VALID = LG_SAMPLE_BETA + 1                             # index of a sample's valid flag
 
 GADE_TUNE_MAX = 64
 GADE_TUNE_COLS = 7
@@ -62,8 +63,8 @@ def lib():
             "gade_cn_run_batch": [vp, vp, vp, vp, f64, i32, f64, i32, vp, vp],
             "gade_autotune": [vp, vp, vp, f64, i32, f64, i32, i32, vp, C.POINTER(i32), C.POINTER(i32)],
             "gade_lg_record": [vp, vp, vp, vp, i32, i32], "gade_lg_pod_build": [vp, vp, i32, i32, vp],
-            "gade_lg_basis": [vp, vp], "gade_lg_set_norm": [vp, vp, vp], "gade_lg_init": [vp, C.c_uint],
-            "gade_lg_set_params": [vp, vp], "gade_lg_get_params": [vp, vp], "gade_lg_forward": [vp, vp, i32, vp],
+            "gade_lg_basis": [vp, vp], "gade_lg_set_basis": [vp, vp, i32], "gade_lg_set_norm": [vp, vp, vp], "gade_lg_init": [vp, C.c_uint],
+            "gade_lg_set_params": [vp, vp], "gade_lg_get_params": [vp, vp], "gade_lg_forward": [vp, vp, i32, vp], "gade_lg_uses": [vp, vp],
             "gade_lg_grad": [vp, vp, vp, i32, C.POINTER(C.c_float), vp],
             "gade_lg_train": [vp, vp, vp, i32, i32, C.c_float, vp], "gade_set_time_batch": [vp, vp], "gade_eig_values": [vp, vp, vp, vp],
             "gade_populations": [vp, vp, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64), vp], "gade_device_sync": [],
@@ -234,6 +235,11 @@ class Propagator:
         check(lib().gade_lg_basis(self.ptr, U.ctypes.data))
         return U
 
+    def lg_set_basis(self, U):
+        U = np.ascontiguousarray(U, np.complex128)
+        assert U.shape[1] == self.nblk * self.blkdim
+        check(lib().gade_lg_set_basis(self.ptr, U.ctypes.data, U.shape[0]))
+
     def lg_set_norm(self, mean, std):
         mean = np.ascontiguousarray(mean, np.float32)
         std = np.ascontiguousarray(std, np.float32)
@@ -251,6 +257,13 @@ class Propagator:
         p = np.zeros(LG_PARAMS, np.float32)
         check(lib().gade_lg_get_params(self.ptr, p.ctypes.data))
         return p
+
+    def lg_uses(self):
+        """Per instance: steps since the last workspace reset that took the network, and those that the input
+        range gate (set_param "lg_zmax") sent to the extrapolation."""
+        c = np.zeros((self.batch, 2), np.int32)
+        check(lib().gade_lg_uses(self.ptr, c.ctypes.data))
+        return c
 
     def lg_forward(self, x: DeviceArray, M, y: DeviceArray):
         check(lib().gade_lg_forward(self.ptr, x.ptr, M, y.ptr))

@@ -60,9 +60,13 @@ def step_scan(gp, psi0, params, nsteps_b, nsteps, after_step=None, tol=1e-12):
 class Trainer:
     """Builds and trains the learned guess of a batched model ``gp`` (its couplings, dt and batch width)."""
 
-    def __init__(self, gp, psi0, seed=0x5EED, npod=gade.LG_POD, nsnap=400, nsamples=20000, log=None):
+    def __init__(self, gp, psi0, seed=0x5EED, npod=gade.LG_POD, nsnap=400, nsamples=20000, log=None, tol=1e-13):
+        """``tol``: tolerance of the solves that make the snapshots and samples.  It is tighter than the
+        production one, so that a step whose extrapolated guess already meets the production tolerance (it
+        ends at 0 iterations there) still gets a solution that differs from the guess: the sample keeps a
+        non-zero extrapolation error and the loss learns to leave such steps alone."""
         self.gp, self.psi0 = gp, np.asarray(psi0, np.complex128)
-        self.seed, self.npod, self.nsnap, self.nsamples = seed, npod, nsnap, nsamples
+        self.seed, self.npod, self.nsnap, self.nsamples, self.tol = seed, npod, nsnap, nsamples, tol
         self.params, self.nsteps_b = scan_pulses(seed, gp.batch, gp.dt)
         self.log = log or (lambda msg: None)
         self.N1 = gp.nblk * gp.blkdim
@@ -87,7 +91,7 @@ class Trainer:
             return cols[0] >= self.nsnap
 
         last = snap_first + snap_every * ((self.nsnap + B - 1) // B)
-        step_scan(gp, self.psi0, self.params, self.nsteps_b, last + 1, rec)
+        step_scan(gp, self.psi0, self.params, self.nsteps_b, last + 1, rec, tol=self.tol)
         sv = gp.lg_pod_build(snaps, self.nsnap, self.npod)
         self.log(f"POD: {self.nsnap} snapshots, singular values {sv[0]:.3e} .. {sv[self.npod - 1]:.3e} "
                  f"(basis of {self.npod})")
@@ -112,10 +116,13 @@ class Trainer:
             count[0] += int(slab.reshape(B, gade.LG_SAMPLE)[:, gade.LG_SAMPLE_VALID].sum())
             return count[0] >= self.nsamples
 
-        step_scan(gp, self.psi0, self.params, self.nsteps_b, max_steps, rec)
+        step_scan(gp, self.psi0, self.params, self.nsteps_b, max_steps, rec, tol=self.tol)
         host = buf.get().reshape(max_steps * B, gade.LG_SAMPLE)
         valid = np.nonzero(host[:, gade.LG_SAMPLE_VALID] > 0.5)[0]
-        self.log(f"samples: {len(valid)} valid of {max_steps * B} recorded")
+        nzero = int(np.count_nonzero(host[valid, gade.LG_SAMPLE_BETA] <= 0))
+        self.log(f"samples: {len(valid)} valid of {max_steps * B} recorded ({nzero} with a guess already at the "
+                 f"solution: no weight in the loss)")
+        self.nzero = nzero
         self.samples, self.samples_host, self.valid = buf, host, valid
         return buf, valid
 
@@ -128,7 +135,15 @@ class Trainer:
         std = np.where(used & (std > 1e-6 * (np.abs(mean) + 1)), std, 1.0)
         mean = np.where(used, mean, 0.0)
         self.gp.lg_set_norm(mean, std)
+        self.mean, self.std = mean, std
         return mean, std
+
+    def export_to(self, other):
+        """Installs the trained guess (inputs' standardization, network, POD basis) into another model of the
+        same couplings, e.g. a single-instance model for the run that uses it."""
+        other.lg_set_norm(self.mean, self.std)
+        other.lg_set_params(self.gp.lg_get_params())
+        other.lg_set_basis(self.gp.lg_basis(self.npod))
 
     def fit(self, epochs=30, M=256, lr=1e-3, lr_final=1e-4, holdout=0.1, init_seed=1):
         """Adam over the training rows (a held-out fraction is kept for validation); returns the per-step

@@ -158,7 +158,8 @@ int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, co
  * 3 hidden layers of GADE_LG_HIDDEN with GELU, GADE_LG_OUT outputs: 6 weights then GADE_LG_POD complex
  * coefficients), BF16 weights with FP32 accumulation on MFMA.  Runs in the device-driven stepping with the
  * banded radial kernel.  Inputs (raw, standardized by gade_lg_set_norm): f_alpha(t_{n+1}) (6), log10 of
- * the last 6 final relative residuals, log10 |D_k|/|psi_n| (k = 1..5), U^H D_3 / s and U^H D_4 / s
+ * the last 6 final residuals relative to their bounds tol |rhs|, log10 |D_k|/|psi_n| (k = 1..5), U^H D_3 / s
+ * and U^H D_4 / s
  * (re, im interleaved), zero padding.  Training sample (GADE_LG_SAMPLE floats): inputs, then
  * G = Re(Dh^H Dh) (6x6), a = Re(Dh^H eh) (6), P = U^H Dh (npod x 6 complex), q = U^H eh (npod complex),
  * beta = |eh|^2, valid, with Dh_k = D_k/|D_k| and eh = (psi_{n+1} - x2)/s; its loss is
@@ -182,6 +183,9 @@ int gade_lg_record(gade_model_t *m, const double *psi_dev, float *samples_dev, d
  * [nsnap][N1]), by Householder QR and an SVD of R on the device; singular values to sv (host, nsnap). */
 int gade_lg_pod_build(gade_model_t *m, const double *snaps_dev, int nsnap, int npod, double *sv);
 int gade_lg_basis(gade_model_t *m, double *U_host);   /* [npod][N1] complex */
+/* Installs a basis (host complex [npod][N1], orthonormal columns, e.g. from gade_lg_basis of another model of
+ * the same couplings): with gade_lg_set_params / gade_lg_set_norm it moves a trained guess between models. */
+int gade_lg_set_basis(gade_model_t *m, const double *U_host, int npod);
 int gade_lg_set_norm(gade_model_t *m, const float *mean, const float *stdv);  /* [GADE_LG_IN] each */
 int gade_lg_init(gade_model_t *m, unsigned seed);   /* He-normal hidden layers, zero output layer */
 int gade_lg_set_params(gade_model_t *m, const float *params);  /* host FP32 [GADE_LG_PARAMS] */
@@ -192,6 +196,12 @@ int gade_lg_grad(gade_model_t *m, const float *samples_dev, const int *idx_dev, 
                  float *grad_dev);
 int gade_lg_train(gade_model_t *m, const float *samples_dev, const int *idx_dev, int M, int nsteps, float lr,
                   float *loss_hist);
+/* Range gate of the learned guess: a step whose standardized inputs reach beyond set_param("lg_zmax", z)
+ * standard deviations of the training inputs (default 8; 0 turns the gate off) takes the order-2
+ * extrapolation instead of the network.  counts (host int [B][2]): per instance, the steps since the last
+ * workspace reset that took the network and those the gate sent to the extrapolation.  No counterpart in the
+ * reference (the learned guess is an addition). */
+int gade_lg_uses(gade_model_t *m, int *counts);
 int gade_autotune(gade_model_t *m, const double *psi_dev, const double *params, double t0, int nsteps,
                   double tol, int maxit, int reps, double *report, int *ncand, int *chosen);
 
@@ -260,7 +270,9 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
  *   "profile_floor_ns"  timed launches shorter than this (those a device-side
  *             gate ended at once) are left out of the family totals (default 0);
  *   "guess_order"  initial guess of each CN step: 0 psi_n (default, the
- *             reference), 1..3 polynomial extrapolation (gade_extrapolate).
+ *             reference), 1..3 polynomial extrapolation (gade_extrapolate),
+ *             GADE_GUESS_LEARNED the learned guess;
+ *   "lg_zmax"  range gate of the learned guess's inputs (gade_lg_uses).
  * Results do not depend on the choice. */
 int gade_set_param(gade_model_t *m, const char *name, int value);
 int gade_get_param(const gade_model_t *m, const char *name, int *value);

@@ -46,8 +46,11 @@ def net_model(datadir):
     return gade.Propagator(d, 1, dt_of("dipole"))
 
 
-def test_mlp_forward_matches_oracle(datadir):
-    say("mlp forward")
+@pytest.mark.parametrize("M", [48, 160])
+def test_mlp_forward_matches_oracle(datadir, M):
+    """Up to 64 rows (inference) every tile's k steps are split over four waves; more rows (training) use one
+    wave per tile: both within the bar."""
+    say(f"mlp forward M={M}")
     rng = np.random.default_rng(11)
     gp = net_model(datadir)
     p = random_params(rng)
@@ -56,7 +59,6 @@ def test_mlp_forward_matches_oracle(datadir):
     gp.lg_set_norm(mean, std)
     gp.lg_set_params(p)
     assert np.array_equal(gp.lg_get_params(), p)
-    M = 48
     x = (rng.standard_normal((M, LG.IN)) * 2).astype(np.float32)
     dx = gade.DeviceArray.from_host(x.ravel())
     dy = gade.DeviceArray(M * LG.OUT, np.float32)
@@ -150,3 +152,29 @@ def test_learned_guess_pipeline_and_held_out_scan(datadir):
     assert np.array_equal(runs[0][0].view(np.float64), runs[1][0].view(np.float64))
     assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
     gp.set_param("guess_order", 0)
+
+
+def test_pod_basis_leaves_out_empty_snapshots(datadir):
+    """Snapshot columns that are zero (slots of instances whose pulse ended) are left out of the POD: the basis
+    spans the top singular vectors of the non-zero columns (numpy SVD), orthonormal to 1e-12."""
+    d = datadir("dipole")
+    gp = gade.Propagator(d, 1, dt_of("dipole"))
+    n = gp.n
+    rng = np.random.default_rng(9)
+    ncols, npod = 120, 64
+    X = np.zeros((ncols, n), np.complex128)
+    used = rng.permutation(ncols)[:90]
+    # a decaying spectrum so the top-64 subspace is well separated
+    W = (rng.standard_normal((90, 90)) + 1j * rng.standard_normal((90, 90))) * (0.7 ** np.arange(90))[None, :]
+    Q, _ = np.linalg.qr(rng.standard_normal((n, 90)) + 1j * rng.standard_normal((n, 90)))
+    X[used] = (Q @ W).T
+    dX = gade.DeviceArray.from_host(X.ravel())
+    sv = gp.lg_pod_build(dX, ncols, npod)
+    U = gp.lg_basis(npod)
+    sref = np.linalg.svd(X[used].T, compute_uv=False)
+    assert np.allclose(sv[:90], sref, rtol=1e-10, atol=1e-12 * sref[0]) and np.all(sv[90:] == 0)
+    assert np.abs(U.conj() @ U.T - np.eye(npod)).max() <= 1e-12
+    Uref = np.linalg.svd(X[used].T, full_matrices=False)[0][:, :npod]
+    # same subspace: every principal angle between the two bases is zero (cosines 1)
+    cos = np.linalg.svd(Uref.conj().T @ U.T, compute_uv=False)
+    assert np.all(np.abs(cos - 1) <= 1e-8), cos.min()
