@@ -227,6 +227,93 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const T *__restrict__ x
     }
 }
 
+// Phase A from the coupling lists: per (row, multipole, part) only the columns with a non-zero H0l (part 0:
+// Z0, Z2) or H1l (part 1: W0, W1) coefficient, in increasing column order, padded to a multiple of
+// kAngListU entries with zero coefficients (on the list's last column).  The sums start at +0 and round to
+// nearest, so a padded term (+0 or -0 added) leaves them unchanged: every V entry is the same sum of the same
+// terms in the same order as in k_angular, bit for bit, without testing coefficients.  The list entries are
+// read at addresses uniform over the block.  A block's lists (its multipole group) are one contiguous stream;
+// the next group's x loads are issued before the current group is summed.  Lists are per instance-local row;
+// x is offset to the instance.
+constexpr int kAngListU = 4;
+template <class T>
+__device__ __forceinline__ void ang_list_group(const T (&xv)[kAngListU], const int2 *__restrict__ ck,
+                                               const T *__restrict__ cf, T &s0, T &s1) {
+    using Rt = typename cx<T>::R;
+#pragma unroll
+    for (int u = 0; u < kAngListU; u++) {
+        const T h = cf[u];
+        s0 = cfma(h, xv[u], s0);
+        s1 = cfma(h, cscale(xv[u], (Rt)ck[u].y), s1);
+    }
+}
+
+template <int LB, class T = double2>
+__global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restrict__ x, const int *__restrict__ aoff,
+                                                              const int2 *__restrict__ ack,
+                                                              const T *__restrict__ acf, int n, int lmax,
+                                                              int nblk, T *__restrict__ V, const int *gate,
+                                                              int gstride, int nblk1) {
+    // rows are the fastest block index: the blocks in flight together write different rows of the V planes
+    // (spread over the memory channels; one row's blocks together would hit the same offset of 16 planes)
+    const int r = blockIdx.x;
+    const int inst = r / nblk1;
+    GADE_GATED_B(gate, gstride, inst);
+    const int rl = r - inst * nblk1;
+    const int a = blockIdx.y * blockDim.x + threadIdx.x;
+    const int l0 = blockIdx.z * LB;
+    const int nlb = min(LB, lmax - l0);
+    const int *off = aoff + ((size_t)rl * lmax + l0) * 2;
+    const T *xi = x + (size_t)inst * nblk1 * n;
+    const int ac = a < n ? a : n - 1;
+    T z0[LB], z2[LB], w0[LB], w1[LB];
+#pragma unroll
+    for (int l = 0; l < LB; l++) z0[l] = z2[l] = w0[l] = w1[l] = czero<T>();
+    const int E0 = off[0], E1 = off[2 * nlb];
+    if (E1 > E0) {
+        T cur[kAngListU];
+#pragma unroll
+        for (int u = 0; u < kAngListU; u++) cur[u] = xi[(size_t)ack[E0 + u].x * n + ac];
+        int g = E0;
+#pragma unroll
+        for (int li = 0; li < 2 * LB; li++) {
+            if ((li >> 1) >= nlb) break;
+            T &s0 = (li & 1) ? w0[li >> 1] : z0[li >> 1];
+            T &s1 = (li & 1) ? w1[li >> 1] : z2[li >> 1];
+            const int e = off[li + 1];
+            // two groups per pass so the two register sets alternate (no copies that wait on the loads)
+            for (; g < e; g += 2 * kAngListU) {
+                T nxt[kAngListU];
+#pragma unroll
+                for (int u = 0; u < kAngListU; u++)
+                    nxt[u] = xi[(size_t)ack[min(g + kAngListU + u, E1 - 1)].x * n + ac];
+                ang_list_group(cur, ack + g, acf + g, s0, s1);
+                if (g + kAngListU >= e) {  // odd number of groups: the loaded group opens the next list
+#pragma unroll
+                    for (int u = 0; u < kAngListU; u++) cur[u] = nxt[u];
+                    g += kAngListU;
+                    break;
+                }
+#pragma unroll
+                for (int u = 0; u < kAngListU; u++)
+                    cur[u] = xi[(size_t)ack[min(g + 2 * kAngListU + u, E1 - 1)].x * n + ac];
+                ang_list_group(nxt, ack + g + kAngListU, acf + g + kAngListU, s0, s1);
+            }
+        }
+    }
+    if (a >= n) return;
+    const size_t plane = (size_t)nblk * n;
+#pragma unroll
+    for (int l = 0; l < LB; l++) {
+        if (l >= nlb) break;
+        size_t base = (size_t)((l0 + l) * 4) * plane + (size_t)r * n + a;
+        V[base] = z0[l];
+        V[base + plane] = z2[l];
+        V[base + 2 * plane] = w0[l];
+        V[base + 3 * plane] = w1[l];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // fused operator, phase B: radial SpMM on a kappa-grouped tile of row blocks
 //   y_r = D_sel(k) x_r + cH * sum_l [ (Gs0+k Gs1) Z0 + (Gs2+k Gs3) Z2 + (Gts0+k Gts2) W0 + (Gts1+k Gts3) W1 ]
@@ -894,7 +981,8 @@ void free_all(gade_model *m) {
                     (void *)m->d_DD32, (void *)m->d_coef32, (void *)m->lu32[0], (void *)m->lu32[1],
                     (void *)m->lu32[2], (void *)m->lu32[3], (void *)m->mp_r, (void *)m->mp_r0, (void *)m->mp_p,
                     (void *)m->mp_v, (void *)m->mp_y, (void *)m->mp_z, (void *)m->mp_s, (void *)m->mp_t,
-                    (void *)m->mp_d, (void *)m->d_mp_ax, (void *)m->d_sc32})
+                    (void *)m->mp_d, (void *)m->d_mp_ax, (void *)m->d_sc32, (void *)m->d_aoff, (void *)m->d_ack,
+                    (void *)m->d_acf, (void *)m->d_acf32})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -909,9 +997,25 @@ bool ang_lds_ok(const gade_model *m, int lb) {
 // the angular kernel with the model's multipole group size (2, 4 or 8 multipoles per block; each V entry is
 // summed over the row's couplings in the same order for every group size)
 template <class T>
-int launch_angular_t(gade_model *m, const T *x, const T *coef, T *V) {
+int launch_angular_t(gade_model *m, const T *x, const T *coef, const T *acf, T *V) {
     const int LB = m->ang_lb;
     dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + LB - 1) / LB);
+    if (m->ang_list) {
+#define GADE_ANGL(LB_)                                                                                        \
+    hipLaunchKernelGGL((k_angular_list<LB_, T>), grid, dim3(kAngThreads), 0, m->stream, x, m->d_aoff, m->d_ack, \
+                       acf, m->n, m->lmax, m->nblk, V, m->gate, m->gstride, m->nblk1)
+        switch (LB) {
+            case 2: GADE_ANGL(2); break;
+            case 4: GADE_ANGL(4); break;
+            case 8: GADE_ANGL(8); break;
+            default:
+                gade::set_error("angular multipole group must be 2, 4 or 8");
+                return GADE_EINVAL;
+        }
+#undef GADE_ANGL
+        GADE_HIP(hipGetLastError());
+        return GADE_OK;
+    }
     size_t shm = sizeof(T) * (size_t)m->maxrow * LB * 2 + sizeof(int2) * (size_t)m->maxrow;
 #define GADE_ANG(LB_)                                                                                            \
     hipLaunchKernelGGL((k_angular<LB_, T>), grid, dim3(kAngThreads), shm, m->stream, x, m->d_hAp, m->d_hAi, coef, \
@@ -929,7 +1033,9 @@ int launch_angular_t(gade_model *m, const T *x, const T *coef, T *V) {
     return GADE_OK;
 }
 
-int launch_angular(gade_model *m, const double2 *x) { return launch_angular_t<double2>(m, x, m->d_coef, m->d_V); }
+int launch_angular(gade_model *m, const double2 *x) {
+    return launch_angular_t<double2>(m, x, m->d_coef, m->d_acf, m->d_V);
+}
 
 // the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
 // each output in the same order, so all give the same results bit for bit
@@ -1305,6 +1411,36 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
                 cp[0] = v0.real(); cp[1] = v0.imag(); cp[2] = v1.real(); cp[3] = v1.imag();
             }
         }
+    // coupling lists of phase A (one instance's rows): per row, multipole and part, the columns whose
+    // coefficient is non-zero, in the Hall row's (increasing column) order
+    std::vector<int2> ack;
+    std::vector<double> acf;
+    m->h_aoff.assign((size_t)m->nblk * lmax * 2 + 1, 0);
+    for (int r = 0; r < m->nblk; r++)
+        for (int l = 0; l < lmax; l++)
+            for (int w = 0; w < 2; w++) {
+                m->h_aoff[((size_t)r * lmax + l) * 2 + w] = (int)ack.size();
+                const size_t first = ack.size();
+                for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++) {
+                    const double *cp = &coef[((size_t)p * lmax + l) * 4 + 2 * w];
+                    if (cp[0] != 0.0 || cp[1] != 0.0) {
+                        const int c = m->hall_Ai[p];
+                        ack.push_back(make_int2(c, m->kap_blk[c]));
+                        acf.push_back(cp[0]);
+                        acf.push_back(cp[1]);
+                    }
+                }
+                while ((ack.size() - first) % kAngListU) {  // zero terms on the last column
+                    ack.push_back(ack.back());
+                    acf.push_back(0.0);
+                    acf.push_back(0.0);
+                }
+            }
+    m->h_aoff.back() = (int)ack.size();
+    if (ack.empty()) {  // no coupling: one unread entry keeps the uploads non-empty
+        ack.push_back(make_int2(0, 0));
+        acf.assign(2, 0.0);
+    }
     // B instances: the Hall structure, kappa maps and angular coefficients repeat with row offsets
     m->N1 = (long long)m->nblk1 * n;
     if (B > 1) {
@@ -1357,6 +1493,9 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
         (rc = upload(&m->d_kap, m->kap_blk.data(), m->kap_blk.size())) ||
         (rc = upload(&m->d_blk_q, m->blk_q.data(), m->blk_q.size())) ||
         (rc = upload(&m->d_coef, (const double2 *)coef.data(), coef.size() / 2)) ||
+        (rc = upload(&m->d_aoff, m->h_aoff.data(), m->h_aoff.size())) ||
+        (rc = upload(&m->d_ack, ack.data(), ack.size())) ||
+        (rc = upload(&m->d_acf, (const double2 *)acf.data(), acf.size() / 2)) ||
         (rc = upload(&m->d_diag, diag.data(), diag.size())))
         return fail(rc);
     if ((rc = hip_alloc((void **)&m->d_GE, sizeof(double2) * (size_t)lmax * 8 * ellw * n)) ||
@@ -1750,6 +1889,8 @@ int mp_setup(gade_model *m) {
     GADE_TRY(hip_alloc((void **)&m->d_GD32, sizeof(float2) * m->gd_stride * m->B));
     GADE_TRY(hip_alloc((void **)&m->d_DD32, sizeof(float2) * nDD));
     GADE_TRY(hip_alloc((void **)&m->d_coef32, sizeof(float2) * nco));
+    const size_t nacf = (size_t)m->h_aoff.back();
+    GADE_TRY(hip_alloc((void **)&m->d_acf32, sizeof(float2) * std::max<size_t>(nacf, 1)));
     for (auto &f : m->lu32) GADE_TRY(hip_alloc((void **)&f, sizeof(float2) * nlu));
     for (float2 **v : {&m->mp_r, &m->mp_r0, &m->mp_p, &m->mp_v, &m->mp_y, &m->mp_z, &m->mp_s, &m->mp_t, &m->mp_d})
         GADE_TRY(hip_alloc((void **)v, sizeof(float2) * N));
@@ -1757,6 +1898,7 @@ int mp_setup(gade_model *m) {
     GADE_TRY(hip_alloc((void **)&m->d_sc32, sizeof(double) * (size_t)m->B * SC_STRIDE));
     conv(m->d_DD, m->d_DD32, nDD);
     conv(m->d_coef, m->d_coef32, nco);
+    if (nacf) conv(m->d_acf, m->d_acf32, nacf);
     conv(m->pre->d_Linv, m->lu32[0], nlu);
     conv(m->pre->d_Uinv, m->lu32[1], nlu);
     conv(m->pre->d_Loff, m->lu32[2], nlu);
@@ -1771,7 +1913,7 @@ int mp_setup(gade_model *m) {
 int apply32_async(gade_model *m, const float2 *x, float2 *y) {
     {
         Prof pr(m, "angular_f32", GADE_PROF_ANGULAR);
-        GADE_TRY(launch_angular_t<float2>(m, x, (const float2 *)m->d_coef32, m->d_V32));
+        GADE_TRY(launch_angular_t<float2>(m, x, (const float2 *)m->d_coef32, (const float2 *)m->d_acf32, m->d_V32));
     }
     const int lgrid = m->lmax;
     float2 *yp = lgrid > 1 ? m->d_ypart32 : nullptr;
@@ -2689,6 +2831,15 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         }
         return GADE_OK;
     }
+    if (!strcmp(name, "ang_list")) {
+        if (value != 0 && value != 1) {
+            gade::set_error("ang_list must be 0 (Hall rows) or 1 (coupling lists)");
+            return GADE_EINVAL;
+        }
+        if (m->ang_list != value) drop_step_graph(m);
+        m->ang_list = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "ang_lb")) {
         if ((value != 2 && value != 4 && value != 8) || !ang_lds_ok(m, value)) {
             gade::set_error("ang_lb must be 2, 4 or 8 (and the Hall rows short enough for its LDS staging)");
@@ -2833,6 +2984,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "dia_rows")) {
         *value = m->dia_r;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "ang_list")) {
+        *value = m->ang_list;
         return GADE_OK;
     }
     if (!strcmp(name, "ang_lb")) {

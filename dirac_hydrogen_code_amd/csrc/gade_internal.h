@@ -101,6 +101,14 @@ struct gade_model {
     // Hall and angular coefficients
     int *d_hAp = nullptr, *d_hAi = nullptr, *d_kap = nullptr, *d_blk_q = nullptr;
     double2 *d_coef = nullptr;     // [hnnz][lmax][2]
+    // the same couplings as lists (one instance's rows): per (row, multipole, part 0 H0l / 1 H1l) the
+    // non-zero coefficients in increasing column order; entries (column, kappa) and values
+    std::vector<int> h_aoff;       // [nblk1 * lmax * 2 + 1]
+    int *d_aoff = nullptr;
+    int2 *d_ack = nullptr;
+    double2 *d_acf = nullptr;
+    float2 *d_acf32 = nullptr;     // FP32 copy (mixed precision)
+    int ang_list = 1;              // phase A from the coupling lists (1) or from the Hall rows (0)
     int *d_diag = nullptr;         // [hnnz] 1 when orig_row == orig_col
     // kappa-grouped row tiles for the radial phase
     int tile_r = 2, ntiles = 0;

@@ -134,8 +134,25 @@ def test_learned_guess_pipeline_and_held_out_scan(datadir):
         res[name] = (x.get(), it, tols)
         assert np.all(tols <= 1e-12), name
         say(f"  held-out {name}: iterations/step {it.mean():.3f}")
+        if go == gade.GUESS_LEARNED:
+            uses = gp.lg_uses()
+            say(f"  held-out learned: network on {uses[:, 0].sum()} steps, range gate fallback on {uses[:, 1].sum()}")
+            # every step with a full history is counted once; the held-out pulses come from the training
+            # distribution, so the network takes most of them
+            assert np.all(uses.sum(1) <= K) and np.all(uses.sum(1) >= K - 8)
+            assert uses[:, 0].sum() >= 0.5 * uses.sum()
+    # a tight range gate sends more steps to the extrapolation; the converged result is unchanged
+    gp.set_param("lg_zmax", 1)
+    x = gade.DeviceArray.from_host(np.tile(psi0, B))
+    gp.reset_workspace()
+    it, tols = gp.cn_run_batch(x, params, K, tol=1e-12)
+    gated = gp.lg_uses()
+    say(f"  held-out learned, gate at 1 sigma: network on {gated[:, 0].sum()} steps, fallback on {gated[:, 1].sum()}")
+    assert gated[:, 1].sum() > uses[:, 1].sum() and np.all(tols <= 1e-12)
+    res["learned_gated"] = (x.get(), it, tols)
+    gp.set_param("lg_zmax", 8)
     ref = res["plain"][0]
-    for name in ("extrapolation", "learned"):
+    for name in ("extrapolation", "learned", "learned_gated"):
         rel = np.linalg.norm(res[name][0] - ref) / np.linalg.norm(ref)
         assert rel <= 1e-9, (name, rel)
     # with the learned guess active the autotuner still leaves the trajectory unchanged (its probes restore the

@@ -372,21 +372,24 @@ def test_extrapolated_guess_same_solution_fewer_iterations(datadir):
 
 @pytest.mark.parametrize("cfg,batch", [("nondipole_small", 1), ("dipole", 3)])
 def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
-    """Every banded tile shape (rows per tile, outputs per lane) and angular multipole group sums each output
-    in the same order: applies (both signs, FP64) and whole device-driven trajectories (paired apply, FP32
-    inner solves of the mixed form) are bit-identical across shapes."""
+    """Every banded tile shape (rows per tile, outputs per lane), angular multipole group and angular form
+    (coupling lists or Hall rows) sums each output in the same order: applies (both signs, FP64) and whole
+    device-driven trajectories (paired apply, FP32 inner solves of the mixed form) are bit-identical across
+    shapes."""
     d = datadir(cfg)
     dt = dt_of(cfg)
     say(f"tile shapes {cfg} batch={batch}")
     rng = np.random.default_rng(3)
     out = {}
-    for shape in ((8, 2, 4), (4, 2, 2), (4, 4, 8)):
+    shapes = ((8, 2, 4, 1), (8, 2, 4, 0), (4, 2, 2, 1), (4, 2, 2, 0), (4, 4, 8, 1), (4, 4, 8, 0))
+    for shape in shapes:
         gp = gade.Propagator(d, LMAX[cfg], dt, batch=batch)
         if gp.get_param("radial") != 2:
             pytest.skip("radial pattern not banded")
         gp.set_param("dia_rows", shape[0])
         gp.set_param("dia_outputs", shape[1])
         gp.set_param("ang_lb", shape[2])
+        gp.set_param("ang_list", shape[3])
         gp.set_time_batch(np.array([gade.pulse(10.0 + 5 * b, 50.0, 1.0, 0.013) for b in range(batch)]))
         n = gp.n * batch
         x = rng.standard_normal(n) + 1j * rng.standard_normal(n) if not out else out["x"]
@@ -407,8 +410,8 @@ def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
             it, tols = gp.cn_run_batch(p, params, 8, tol=1e-12)
             traj.append((p.get(), it, tols))
         out[shape] = (ys, traj)
-    ref = out[(8, 2, 4)]
-    for shape in ((4, 2, 2), (4, 4, 8)):
+    ref = out[shapes[0]]
+    for shape in shapes[1:]:
         for a, b in zip(out[shape][0], ref[0]):
             assert np.array_equal(a.view(np.float64), b.view(np.float64)), shape
         for a, b in zip(out[shape][1], ref[1]):
