@@ -961,7 +961,7 @@ void free_all(gade_model *m) {
     void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
                     m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
                     m->d_V, m->d_ypart, m->d_flag, m->d_dia_src, m->d_gs, m->d_gts, m->d_GD, m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k,
-                    m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_cnt,
+                    m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_part2, m->d_cnt,
                     m->d_fAp, m->d_fAi, m->d_fAx, m->d_SAp, m->d_SAi, m->d_SAx};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1536,6 +1536,7 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
     const size_t nsc = (size_t)B * SC_STRIDE, nct = (size_t)B * CT_STRIDE;
     if ((rc = hip_alloc((void **)&m->d_sc, sizeof(double) * nsc)) ||
         (rc = hip_alloc((void **)&m->d_part, sizeof(double) * 4 * (size_t)B * m->red_blocks)) ||
+        (rc = hip_alloc((void **)&m->d_part2, sizeof(double) * 4 * (size_t)B * m->red_blocks)) ||
         (rc = hip_alloc((void **)&m->d_cnt, sizeof(unsigned) * 4)) ||
         (rc = hip_alloc((void **)&m->d_flag, sizeof(int) * 4)) ||
         (rc = hip_alloc((void **)&m->d_ctl, sizeof(int) * nct)) ||
@@ -1661,6 +1662,53 @@ int bicg_op(gade_model *m, const double2 *b, double2 *x = nullptr) {
     if (OP != OP_P && OP != OP_S)
         hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part,
                            m->d_sc, m->gate, m->gstride, ctl);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// out = A x followed by the reduction OP (OP_ALPHA: out = v; OP_W: out = t), the radial phase's partial sum
+// and the reduction in one kernel (partials into 'part'); the same results as apply_async then bicg_op<OP>,
+// bit for bit.  Needs the banded radial phase with multipole partials (fused_sums()).  fin: finish the
+// reduction in its own k_bicg_fin (else the next vector step folds it in).
+bool fused_sums(const gade_model *m) { return m->radial_kind == 2 && m->lmax >= 2; }
+
+template <int OP>
+int apply_dot(gade_model *m, const double2 *x, double2 *out, double *part, bool fin) {
+    {
+        Prof pr(m, "angular", GADE_PROF_ANGULAR);
+        GADE_TRY(launch_angular(m, x));
+    }
+    {
+        Prof pr(m, "radial", GADE_PROF_RADIAL);
+        GADE_TRY((launch_dia<false, double2>(m, m->lmax, x, m->d_V, m->d_GD, m->d_DD, 0, 1.0, 0.0, out, m->d_ypart,
+                                             (double2 *)nullptr, (double2 *)nullptr, 0)));
+    }
+    Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
+    hipLaunchKernelGGL(k_sum_bicg<OP>, dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream, m->N1, m->N,
+                       m->lmax, (const double2 *)m->d_ypart, out, (const double2 *)m->r0, (const double2 *)m->s,
+                       part, m->gate, m->gstride);
+    if (fin) {
+        int *ctl = m->ctl_on ? m->d_ctl : nullptr;
+        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, part,
+                           m->d_sc, m->gate, m->gstride, ctl);
+    }
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
+// OP_S / OP_XR with the previous reduction (alpha / w) finished inside (its partials in fold_part), then for
+// OP_XR its own reduction and finish
+template <int OP>
+int bicg_op_fold(gade_model *m, const double *fold_part, double2 *x = nullptr) {
+    Prof pr(m, "blas1", GADE_PROF_BLAS1);
+    hipLaunchKernelGGL((k_bicg<OP, double2, true>), dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream,
+                       m->N1, (const double2 *)m->tmp, m->r, m->r0, m->p, m->v, m->s, m->t, m->y, m->z, x, m->d_sc,
+                       m->d_part, m->gate, m->gstride, fold_part, m->red_blocks);
+    if (OP == OP_XR) {
+        int *ctl = m->ctl_on ? m->d_ctl : nullptr;
+        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part,
+                           m->d_sc, m->gate, m->gstride, ctl);
+    }
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -1841,16 +1889,27 @@ int ensure_step_buffers(gade_model *m, int nsteps) {
 // K gated BiCGSTAB iterations on the current step of every instance
 int enqueue_iterations(gade_model *m, double2 *psi, int K) {
     m->gate = m->d_ctl + CT_SKIP;
+    const bool fused = fused_sums(m);
     for (int k = 0; k < K; k++) {
         GADE_TRY(bicg_op<OP_P>(m, m->tmp));
         GADE_TRY(psolve_async(m, m->p, m->y));
-        GADE_TRY(apply_async(m, m->y, m->v, 1.0, 0.0));
-        GADE_TRY(bicg_op<OP_ALPHA>(m, m->tmp));
-        GADE_TRY(bicg_op<OP_S>(m, m->tmp));
-        GADE_TRY(psolve_async(m, m->s, m->z));
-        GADE_TRY(apply_async(m, m->z, m->t, 1.0, 0.0));
-        GADE_TRY(bicg_op<OP_W>(m, m->tmp));
-        GADE_TRY(bicg_op<OP_XR>(m, m->tmp, psi));
+        if (fused) {
+            // the radial sums carry the dots; alpha and w are finished by the vector steps that use them (w's
+            // partials in their own buffer: x/r's reduction reuses d_part while other blocks still read them)
+            GADE_TRY(apply_dot<OP_ALPHA>(m, m->y, m->v, m->d_part, false));
+            GADE_TRY(bicg_op_fold<OP_S>(m, m->d_part));
+            GADE_TRY(psolve_async(m, m->s, m->z));
+            GADE_TRY(apply_dot<OP_W>(m, m->z, m->t, m->d_part2, false));
+            GADE_TRY(bicg_op_fold<OP_XR>(m, m->d_part2, psi));
+        } else {
+            GADE_TRY(apply_async(m, m->y, m->v, 1.0, 0.0));
+            GADE_TRY(bicg_op<OP_ALPHA>(m, m->tmp));
+            GADE_TRY(bicg_op<OP_S>(m, m->tmp));
+            GADE_TRY(psolve_async(m, m->s, m->z));
+            GADE_TRY(apply_async(m, m->z, m->t, 1.0, 0.0));
+            GADE_TRY(bicg_op<OP_W>(m, m->tmp));
+            GADE_TRY(bicg_op<OP_XR>(m, m->tmp, psi));
+        }
     }
     return GADE_OK;
 }

@@ -234,18 +234,24 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh) {
 
 // One BiCGSTAB vector step over the N1 entries of instance blockIdx.y (vectors
 // are instance-major, N1 apart); partial sums to part[instance][block][4].
-template <int OP, class T = double2>
+// FOLD (OP_S, OP_XR): the scalar this step needs (alpha, w) is finished here from the previous reduction's
+// partials fold_part[instance][fold_nparts][4] instead of by its own k_bicg_fin: every block sums them in
+// k_bicg_fin's order (the same value bit for bit) and block 0 records it (and rho for OP_S) in the scalars.
+template <int OP, class T = double2, bool FOLD = false>
 __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const T *__restrict__ b, T *__restrict__ r,
                                                        T *__restrict__ r0, T *__restrict__ p,
                                                        const T *__restrict__ v, T *__restrict__ s,
                                                        const T *__restrict__ t, const T *__restrict__ yv,
                                                        const T *__restrict__ zv, T *__restrict__ x,
-                                                       const double *__restrict__ sc0, double *__restrict__ part0,
-                                                       const int *gate = nullptr, int gstride = 0) {
+                                                       double *__restrict__ sc0, double *__restrict__ part0,
+                                                       const int *gate = nullptr, int gstride = 0,
+                                                       const double *__restrict__ fold_part = nullptr,
+                                                       int fold_nparts = 0) {
+    static_assert(!FOLD || OP == OP_S || OP == OP_XR, "folded finish: alpha into s, w into x/r");
     const int inst = blockIdx.y;
     GADE_GATED_B(gate, gstride, inst);
     __shared__ double sh[4 * 16];
-    const double *sc = sc0 + (size_t)inst * SC_STRIDE;
+    double *sc = sc0 + (size_t)inst * SC_STRIDE;
     double *part = part0 + ((size_t)inst * gridDim.x) * 4;
     const long long o = (long long)inst * N1;
     double acc[4] = {0, 0, 0, 0};
@@ -258,8 +264,44 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const T *__r
         w = make_double2(sc[SC_W], sc[SC_W + 1]);
         beta = cmul(cdiv(rho_new, rho_old), cdiv(alpha, w));
     }
-    if (OP == OP_S || OP == OP_XR) alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
-    if (OP == OP_XR) w = make_double2(sc[SC_W], sc[SC_W + 1]);
+    if constexpr (FOLD) {
+        __shared__ double2 fs;
+        double tot[4] = {0, 0, 0, 0};
+        const double *fp = fold_part + (size_t)inst * fold_nparts * 4;
+        for (int bidx = threadIdx.x; bidx < fold_nparts; bidx += blockDim.x)
+            for (int i = 0; i < 4; i++) tot[i] += fp[(size_t)bidx * 4 + i];
+        block_sum<4>(tot, sh);
+        if (threadIdx.x == 0) {
+            if (OP == OP_S) {  // k_bicg_fin<OP_ALPHA>
+                const double2 rho = make_double2(sc[SC_RHONEXT], sc[SC_RHONEXT + 1]);
+                const double2 al = cdiv(rho, make_double2(tot[0], tot[1]));
+                fs = al;
+                if (blockIdx.x == 0) {
+                    sc[SC_ALPHA] = al.x;
+                    sc[SC_ALPHA + 1] = al.y;
+                    sc[SC_RHO] = rho.x;
+                    sc[SC_RHO + 1] = rho.y;
+                }
+            } else {  // k_bicg_fin<OP_W>
+                double2 ww = make_double2(0, 0);
+                if (tot[0] > 0) ww = make_double2(tot[1] / tot[0], tot[2] / tot[0]);
+                fs = ww;
+                if (blockIdx.x == 0) {
+                    sc[SC_W] = ww.x;
+                    sc[SC_W + 1] = ww.y;
+                }
+            }
+        }
+        __syncthreads();
+        if (OP == OP_S) alpha = fs;
+        if (OP == OP_XR) {
+            alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
+            w = fs;
+        }
+    } else {
+        if (OP == OP_S || OP == OP_XR) alpha = make_double2(sc[SC_ALPHA], sc[SC_ALPHA + 1]);
+        if (OP == OP_XR) w = make_double2(sc[SC_W], sc[SC_W + 1]);
+    }
     // the vector updates run in T, the dot products accumulate in double
     const T betaT = from_f64<T>(beta), wT = from_f64<T>(w), alphaT = from_f64<T>(alpha);
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
@@ -303,6 +345,44 @@ __global__ __launch_bounds__(kRedThreads) void k_bicg(long long N1, const T *__r
         }
     }
     if (OP == OP_P || OP == OP_S) return;
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; i++) part[(size_t)blockIdx.x * 4 + i] = acc[i];
+}
+
+// The last step of an apply fused with the BiCGSTAB reduction that follows it: out = sum over the lgrid
+// multipole partials of the radial phase (in multipole order, as k_radial_sum with beta 0), then for
+// OP_ALPHA the partial r0 . out, for OP_W the partials |out|^2 and out . s (out = t).  The grid and every
+// thread's order of entries are k_bicg's, so out and the partial sums equal the unfused pair's bit for bit.
+template <int OP>
+__global__ __launch_bounds__(kRedThreads) void k_sum_bicg(long long N1, long long N, int lgrid,
+                                                           const double2 *__restrict__ ypart,
+                                                           double2 *__restrict__ out, const double2 *__restrict__ r0,
+                                                           const double2 *__restrict__ s, double *__restrict__ part0,
+                                                           const int *gate, int gstride) {
+    static_assert(OP == OP_ALPHA || OP == OP_W, "fused sums: alpha or w");
+    const int inst = blockIdx.y;
+    GADE_GATED_B(gate, gstride, inst);
+    __shared__ double sh[4 * 16];
+    double *part = part0 + ((size_t)inst * gridDim.x) * 4;
+    const long long o = (long long)inst * N1;
+    double acc[4] = {0, 0, 0, 0};
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = o + j;
+        double2 v = ypart[i];
+        for (int l = 1; l < lgrid; l++) v = cadd(v, ypart[(size_t)l * N + i]);
+        out[i] = v;
+        if (OP == OP_ALPHA) {
+            double2 d = cfmaconj(r0[i], v, make_double2(acc[0], acc[1]));
+            acc[0] = d.x;
+            acc[1] = d.y;
+        } else {
+            acc[0] = fma(v.x, v.x, fma(v.y, v.y, acc[0]));
+            double2 d = cfmaconj(v, s[i], make_double2(acc[1], acc[2]));
+            acc[1] = d.x;
+            acc[2] = d.y;
+        }
+    }
     block_sum<4>(acc, sh);
     if (threadIdx.x == 0)
         for (int i = 0; i < 4; i++) part[(size_t)blockIdx.x * 4 + i] = acc[i];
