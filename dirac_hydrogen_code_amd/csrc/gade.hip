@@ -2078,6 +2078,33 @@ int enqueue_rhs_ax0(gade_model *m, double2 *psi) {
     return apply_async(m, psi, m->r, 1.0, 0.0);
 }
 
+// the rhs, A x0 and the start of the solve (OP_INIT); with x0 = psi and multipole partials, the paired
+// apply's sums and OP_INIT are one kernel (the same results)
+int enqueue_rhs_init(gade_model *m, double2 *psi) {
+    if (m->guess_order != 0 || !fused_sums(m)) {
+        GADE_TRY(enqueue_rhs_ax0(m, psi));
+        return bicg_op<OP_INIT>(m, m->tmp);
+    }
+    {
+        Prof pr(m, "angular", GADE_PROF_ANGULAR);
+        GADE_TRY(launch_angular(m, psi));
+    }
+    {
+        Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
+        GADE_TRY((launch_dia<true, double2>(m, m->lmax, psi, m->d_V, m->d_GD, m->d_DD, 1, -1.0, 0.0, m->tmp,
+                                            m->d_ypart, m->r, m->d_ypart2, 0)));
+    }
+    Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
+    hipLaunchKernelGGL(k_sum2_init, dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream, m->N1, m->N, m->lmax,
+                       (const double2 *)m->d_ypart, (const double2 *)m->d_ypart2, m->tmp, m->r, m->r0, m->d_part,
+                       m->gate, m->gstride);
+    int *ctl = m->ctl_on ? m->d_ctl : nullptr;
+    hipLaunchKernelGGL(k_bicg_fin<OP_INIT>, dim3(m->B), dim3(kRedThreads), 0, m->stream, m->red_blocks, m->d_part,
+                       m->d_sc, m->gate, m->gstride, ctl);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
 int enqueue_step_end(gade_model *m, int step) {
     hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, step,
                        m->steps_cap, m->d_iters, m->d_tols);
@@ -2096,8 +2123,7 @@ int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K,
     m->gate = m->d_ctl + CT_PSKIP;
     GADE_TRY(launch_gsums(m, m->radial_kind != 2));
     if (m->precision == 1) GADE_TRY(mp_convert_gsums(m));
-    GADE_TRY(enqueue_rhs_ax0(m, psi));
-    GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
+    GADE_TRY(enqueue_rhs_init(m, psi));
     GADE_TRY(enqueue_solve(m, psi, K));
     return enqueue_step_end(m, step);
 }
@@ -2124,8 +2150,7 @@ int enqueue_step_window(gade_model *m, double2 *psi, int K, bool per_instance_st
     m->gate = m->d_ctl + CT_PSKIP;
     GADE_TRY(launch_gsums_window(m));
     if (m->precision == 1) GADE_TRY(mp_convert_gsums(m));
-    GADE_TRY(enqueue_rhs_ax0(m, psi));
-    GADE_TRY(bicg_op<OP_INIT>(m, m->tmp));
+    GADE_TRY(enqueue_rhs_init(m, psi));
     GADE_TRY(enqueue_solve(m, psi, K));
     hipLaunchKernelGGL(k_step_end, dim3(m->B), dim3(64), 0, m->stream, m->d_ctl, m->d_gctl, m->d_sc, -1,
                        m->steps_cap, m->d_iters, m->d_tols);

@@ -388,6 +388,41 @@ __global__ __launch_bounds__(kRedThreads) void k_sum_bicg(long long N1, long lon
         for (int i = 0; i < 4; i++) part[(size_t)blockIdx.x * 4 + i] = acc[i];
 }
 
+// The paired apply's two multipole sums fused with OP_INIT: b = sum of ypart (the rhs (S - H) x),
+// ax = sum of ypart2 (A x0 = (S + H) x), r = r0 = b - ax, partials |r|^2 and |b|^2.  Each sum runs in
+// multipole order and the grid and every thread's order of entries are k_bicg's: the same results as two
+// k_radial_sum launches then k_bicg<OP_INIT>, bit for bit.
+[[maybe_unused]] static __global__ __launch_bounds__(kRedThreads) void k_sum2_init(long long N1, long long N, int lgrid,
+                                                            const double2 *__restrict__ ypart,
+                                                            const double2 *__restrict__ ypart2,
+                                                            double2 *__restrict__ b, double2 *__restrict__ r,
+                                                            double2 *__restrict__ r0, double *__restrict__ part0,
+                                                            const int *gate, int gstride) {
+    const int inst = blockIdx.y;
+    GADE_GATED_B(gate, gstride, inst);
+    __shared__ double sh[4 * 16];
+    double *part = part0 + ((size_t)inst * gridDim.x) * 4;
+    const long long o = (long long)inst * N1;
+    double acc[4] = {0, 0, 0, 0};
+    for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < N1; j += (long long)gridDim.x * blockDim.x) {
+        const long long i = o + j;
+        double2 bs = ypart[i], ax = ypart2[i];
+        for (int l = 1; l < lgrid; l++) {
+            bs = cadd(bs, ypart[(size_t)l * N + i]);
+            ax = cadd(ax, ypart2[(size_t)l * N + i]);
+        }
+        b[i] = bs;
+        const double2 ri = csub(bs, ax);
+        r[i] = ri;
+        r0[i] = ri;
+        acc[0] = fma(ri.x, ri.x, fma(ri.y, ri.y, acc[0]));
+        acc[1] = fma(bs.x, bs.x, fma(bs.y, bs.y, acc[1]));
+    }
+    block_sum<4>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 4; i++) part[(size_t)blockIdx.x * 4 + i] = acc[i];
+}
+
 // Sums the partials of instance blockIdx.x in a fixed order and updates its
 // scalars; with ctl, also takes the host loop's decisions for it.
 // INNER (the FP32 inner solve of the mixed-precision form): the only decision is when the inner
