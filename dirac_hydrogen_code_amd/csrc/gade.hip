@@ -524,7 +524,8 @@ __device__ __forceinline__ int dia_swz(int p) {
 
 // GD[l][v][qd][d][ip] = (G_m1, G_m2) of the kappa-folded pair of v, from the
 // DIA-ordered couplings gs[l][j][nn][qd][d][ip] (g of alpha = 2j + (l even))
-// and gts (conjugated transposes).  One thread per (l, qd, d, ip).
+// and gts (conjugated transposes).  One thread per (l, v, qd, d, ip): the pair of sums that multiplies V
+// component v (six loads each).
 // Coefficients: by value (cg, instances b0 + blockIdx.y), or from the device window table
 // cf_dev[slot][B] at slot gctl[GC_WSLOT] (the form a replayed step graph uses).
 __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ gs, const double2 *__restrict__ gts,
@@ -544,32 +545,26 @@ __global__ __launch_bounds__(256) void k_gsums_dia(const double2 *__restrict__ g
     }
     double2 *__restrict__ GD = GD0 + (size_t)b * gd_stride;
     const int S = 4 * nd * nhp;  // entries per (l, j, nn) plane
-    const long long total = (long long)lmax * S;
+    const long long total = (long long)lmax * 4 * S;
     for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
          idx += (long long)gridDim.x * blockDim.x) {
-        const int l = (int)(idx / S);
-        const int e = (int)(idx - (long long)l * S);  // (qd, d, ip)
-        double2 G[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) G[m] = make_double2(0, 0);
+        const int lv = (int)(idx / S);
+        const int e = (int)(idx - (long long)lv * S);  // (qd, d, ip)
+        const int l = lv >> 2, v = lv & 3;
+        // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
+        const double2 *src = v < 2 ? gs : gts;
+        const int n1 = v == 0 ? 0 : v == 1 ? 2 : v == 2 ? 0 : 1, n2 = v == 0 ? 1 : v == 1 ? 3 : v == 2 ? 2 : 3;
+        double2 G1 = make_double2(0, 0), G2 = make_double2(0, 0);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const double2 c = (l % 2 == 0) ? c_even[j] : c_odd[j];
-#pragma unroll
-            for (int nn = 0; nn < 4; nn++) {
-                const size_t o = ((size_t)(l * 3 + j) * 4 + nn) * S + e;
-                G[nn] = cfma(c, gs[o], G[nn]);
-                G[4 + nn] = cfma(c, gts[o], G[4 + nn]);
-            }
+            const size_t o = (size_t)(l * 3 + j) * 4 * S + e;
+            G1 = cfma(c, src[o + (size_t)n1 * S], G1);
+            G2 = cfma(c, src[o + (size_t)n2 * S], G2);
         }
-        // (Gs0,Gs1) (Gs2,Gs3) (Gts0,Gts2) (Gts1,Gts3) pair with Z0, Z2, W0, W1
-        const int m1[4] = {0, 2, 4, 5}, m2[4] = {1, 3, 6, 7};
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const size_t o = ((size_t)(l * 4 + v) * S + e) * 2;
-            GD[o] = G[m1[v]];
-            GD[o + 1] = G[m2[v]];
-        }
+        const size_t o = ((size_t)lv * S + e) * 2;
+        GD[o] = G1;
+        GD[o + 1] = G2;
     }
 }
 
@@ -1217,7 +1212,7 @@ int launch_gsums(gade_model *m, bool ell) {
         hipLaunchKernelGGL(k_gsums, dim3(grid_for(total, 256)), dim3(256), 0, m->stream, m->d_g, m->d_ell_src,
                            m->d_tmap, coef_of(m, 0), m->lmax, m->ellw, m->n, m->nnzb, m->d_GE, m->gate);
     } else {
-        long long total = (long long)m->lmax * 4 * m->nd * m->nhp;
+        long long total = (long long)m->lmax * 16 * m->nd * m->nhp;  // (l, v, qd, d, ip)
         for (int b0 = 0; b0 < m->B; b0 += kCoefGroup) {
             const int nb = std::min(kCoefGroup, m->B - b0);
             CoefGroup cg{};
@@ -2131,7 +2126,7 @@ int enqueue_step(gade_model *m, double2 *psi, const double *ft, int step, int K,
 // the G-sums of the current window slot from the device coefficient table
 int launch_gsums_window(gade_model *m) {
     Prof pr(m, "gsums", GADE_PROF_GSUMS);
-    long long total = (long long)m->lmax * 4 * m->nd * m->nhp;
+    long long total = (long long)m->lmax * 16 * m->nd * m->nhp;  // (l, v, qd, d, ip)
     CoefGroup cg{};
     hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), m->B), dim3(256), 0, m->stream, m->d_gs,
                        m->d_gts, cg, 0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride,
