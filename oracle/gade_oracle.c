@@ -762,16 +762,4 @@ int or_vec_read(const char *fname, int n, ocplx *out, int *perm)
     return 0;
 }
 
-/* ------------------------------------------------------------------ */
-/* callbacks with the signatures of tools/bicgstab.h:31-32, so that the */
-/* reference's own bicgstab() (oracle/_ref) can run on this operator   */
-/* ------------------------------------------------------------------ */
-void or_cb_spmv(const void *mat, ocplx *x, ocplx *out)
-{
-    or_model_spmv_HS((const or_model_t *)mat, x, out);
-}
 
-void or_cb_psolve(const void *mat, const ocplx *rhs, ocplx *x)
-{
-    or_model_psolve((const or_model_t *)mat, rhs, x);
-}

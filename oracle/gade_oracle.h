@@ -96,9 +96,6 @@ void or_reset_workspace(or_model_t *m);
 int or_cn_run(or_model_t *m, double E0, double omega, double cycles, double t0, int nsteps,
               double tol, int maxit, ocplx *x, int *iters_out, double *tol_out, long *usec_out);
 
-/* spmv_fun / precond_fun callbacks (tools/bicgstab.h:31-32) over a model */
-void or_cb_spmv(const void *mat, ocplx *x, ocplx *out);
-void or_cb_psolve(const void *mat, const ocplx *rhs, ocplx *x);
 
 /* psi0.vec (tools/dirac_solver.c:68-108; tools/main.py:151-155) */
 int or_vec_read(const char *fname, int n, ocplx *out, int *perm);

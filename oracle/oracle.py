@@ -76,9 +76,6 @@ def ref():
         path = os.path.join(HERE, "_ref", "libref.so")
         if not os.path.exists(path):
             return None
-        # bicgstab.c declares zdotc_ as returning double complex by value: the
-        # MKL runtime must use the GNU Fortran calling convention for that.
-        os.environ.setdefault("MKL_INTERFACE_LAYER", "GNU,LP64")
         R = C.CDLL(path)
         R.ik.restype = C.c_int
         R.ik.argtypes = [C.c_int]

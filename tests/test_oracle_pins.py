@@ -260,41 +260,6 @@ def test_bicgstab_equals_direct_solve(datadir):
     assert np.abs(x - want).max() <= 1e-10 * np.abs(want).max()
 
 
-@needs_ref
-@pytest.mark.parametrize("tol", [1e-12, 1e-16])
-def test_bicgstab_vs_reference_source(datadir, tol):
-    """The reference's own bicgstab() (tools/bicgstab.c, BLAS-1 from MKL) on the oracle operator."""
-    cfg = "dipole"
-    dt = dt_of(cfg)
-    m = O.Model(datadir(cfg), 1, dt)
-    m.precond()
-    psi, _ = O.read_psi0_vec(datadir(cfg) + "/psi0.vec", m.n)
-    rng = np.random.default_rng(4)
-    x0 = psi + 1e-3 * (rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n))
-
-    class WSP(C.Structure):
-        _fields_ = [(f, C.c_void_p) for f in ("r", "r0", "v", "p", "y", "z", "s", "t")]
-
-    REF.bicgstab.argtypes = [C.c_void_p, C.c_void_p, O.cplx_p, O.cplx_p, C.c_int, C.c_int, C.c_int, C.c_void_p,
-                             C.c_void_p, C.POINTER(WSP), C.POINTER(C.c_int), C.POINTER(C.c_double)]
-    L = O.lib()
-    w = WSP()
-    xr = x0.copy()
-    xo = x0.copy()
-    m.reset_workspace()
-    for step in range(3):  # the workspace carries p and v across calls in both
-        m.timedep(O.pulse(10, 50, 1, dt * (step + 1)))
-        rhs = m.apply(xr, "S") - m.apply(xr, "H")
-        rhs_o = m.apply(xo, "S") - m.apply(xo, "H")
-        it, te = C.c_int(500), C.c_double(tol)
-        REF.bicgstab(C.cast(L.or_cb_spmv, C.c_void_p).value, m.ptr, rhs, xr, m.n, m.n, 0,
-                     C.cast(L.or_cb_psolve, C.c_void_p).value, m.ptr, C.byref(w), C.byref(it), C.byref(te))
-        xo, it2, te2, _ = m.bicgstab(rhs_o, xo, tol=tol)
-        assert it.value == it2
-        assert abs(te.value - te2) <= 1e-6 * te.value + 1e-300
-        assert np.abs(xo - xr).max() <= 1e-13 * np.abs(xr).max()
-
-
 # ---------------------------------------------------------------- whole CN step
 def test_field_free_eigenstate_phase(datadir):
     """E0=0, psi0 an eigenvector of (H0, S): psi_n = ((1 - i dt E/2)/(1 + i dt E/2))^n psi0."""
