@@ -260,6 +260,36 @@ def test_bicgstab_equals_direct_solve(datadir):
     assert np.abs(x - want).max() <= 1e-10 * np.abs(want).max()
 
 
+def test_bicgstab_known_answers(datadir):
+    """Cases whose outcome tools/bicgstab.c:125-295 fixes by its structure.
+    * rhs = 0: x set to zero, 0 iterations, tolerance error 0.
+    * Field free, the block-LU preconditioner is the whole operator (S + i dt/2 H0 is block diagonal): from
+      x0 = 0 with a fresh workspace the first body gives x = A^-1 b and the loop breaks before its increment:
+      0 iterations, three operator applications (r0, v, t), x equal to the dense solve.
+    * The exact solution as x0: 0 iterations after the single initial application."""
+    cfg = "tiny"
+    dt = dt_of(cfg)
+    m = O.Model(datadir(cfg), 2, dt)
+    m.timedep(O.pulse(0.0, 50, 1, 0.05))
+    Ap, Ai, _ = m.hfull_structure()
+    SAp, SAi, SAx = m.S_matrix()
+    A = _dense(SAp, SAi, SAx, m.n) + _dense(Ap, Ai, m.hfull_values(), m.n)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    x0 = rng.standard_normal(m.n) + 1j * rng.standard_normal(m.n)
+    m.reset_workspace()
+    x, it, te, na = m.bicgstab(np.zeros(m.n, complex), x0, tol=1e-12)
+    assert it == 0 and te == 0 and not np.any(x)
+    m.reset_workspace()
+    x, it, te, na = m.bicgstab(b, np.zeros(m.n, complex), tol=1e-12)
+    want = np.linalg.solve(A, b)
+    assert it == 0 and na == 3 and te < 1e-12, (it, na, te)
+    assert np.abs(x - want).max() <= 1e-11 * np.abs(want).max()
+    x, it, te, na = m.bicgstab(b, want, tol=1e-12)
+    assert it == 0 and na == 1 and te < 1e-12, (it, na, te)
+    assert np.array_equal(x, want)
+
+
 # ---------------------------------------------------------------- whole CN step
 def test_field_free_eigenstate_phase(datadir):
     """E0=0, psi0 an eigenvector of (H0, S): psi_n = ((1 - i dt E/2)/(1 + i dt E/2))^n psi0."""
