@@ -956,7 +956,7 @@ void free_all(gade_model *m) {
     void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
                     m->d_hAp, m->d_hAi, m->d_kap, m->d_blk_q, m->d_coef, m->d_diag, m->d_tiles, m->d_tile_q, m->d_tile_k,
                     m->d_V, m->d_ypart, m->d_flag, m->d_dia_src, m->d_gs, m->d_gts, m->d_GD, m->d_DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k,
-                    m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_part2, m->d_cnt,
+                    m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t, m->tmp, m->d_sc, m->d_part, m->d_part2,
                     m->d_fAp, m->d_fAi, m->d_fAx, m->d_SAp, m->d_SAi, m->d_SAx};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1532,13 +1532,12 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
     if ((rc = hip_alloc((void **)&m->d_sc, sizeof(double) * nsc)) ||
         (rc = hip_alloc((void **)&m->d_part, sizeof(double) * 4 * (size_t)B * m->red_blocks)) ||
         (rc = hip_alloc((void **)&m->d_part2, sizeof(double) * 4 * (size_t)B * m->red_blocks)) ||
-        (rc = hip_alloc((void **)&m->d_cnt, sizeof(unsigned) * 4)) ||
         (rc = hip_alloc((void **)&m->d_flag, sizeof(int) * 4)) ||
         (rc = hip_alloc((void **)&m->d_ctl, sizeof(int) * nct)) ||
         (rc = hip_alloc((void **)&m->d_gctl, sizeof(int) * GC_COUNT)) ||
         (rc = hip_alloc((void **)&m->d_nsteps, sizeof(int) * B)))
         return fail(rc);
-    if (hipMemset(m->d_cnt, 0, sizeof(unsigned) * 4) != hipSuccess || hipMemset(m->d_sc, 0, sizeof(double) * nsc) != hipSuccess ||
+    if (hipMemset(m->d_sc, 0, sizeof(double) * nsc) != hipSuccess ||
         hipMemset(m->d_ctl, 0, sizeof(int) * nct) != hipSuccess || hipMemset(m->d_gctl, 0, sizeof(int) * GC_COUNT) != hipSuccess)
         return fail(GADE_EHIP);
     if (hipHostMalloc((void **)&m->h_ctl, sizeof(int) * nct, hipHostMallocDefault) != hipSuccess ||

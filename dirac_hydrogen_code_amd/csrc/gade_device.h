@@ -68,18 +68,6 @@ template <> struct cx<float2> {
     static __device__ __forceinline__ float2 mk(float re, float im) { return make_float2(re, im); }
 };
 template <class T> __device__ __forceinline__ T czero() { return cx<T>::mk(0, 0); }
-// value of lane j (uniform over the wave) of a register, broadcast without a memory access
-__device__ __forceinline__ int lane_get(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
-__device__ __forceinline__ float lane_get(float v, int j) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ __forceinline__ double lane_get(double v, int j) {
-    const long long b = __double_as_longlong(v);
-    return __hiloint2double(__builtin_amdgcn_readlane((int)(b >> 32), j),
-                            __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j));
-}
-__device__ __forceinline__ double2 lane_get(double2 v, int j) { return make_double2(lane_get(v.x, j), lane_get(v.y, j)); }
-__device__ __forceinline__ float2 lane_get(float2 v, int j) { return make_float2(lane_get(v.x, j), lane_get(v.y, j)); }
 __device__ __forceinline__ float2 to_f32(double2 a) { return make_float2((float)a.x, (float)a.y); }
 __device__ __forceinline__ double2 to_f64(float2 a) { return make_double2((double)a.x, (double)a.y); }
 __device__ __forceinline__ double2 to_f64(double2 a) { return a; }

@@ -142,7 +142,6 @@ struct gade_model {
     double *d_sc = nullptr;        // device scalars [B][SC_STRIDE]
     double *d_part = nullptr;      // reduction partials [B][red_blocks][4]
     double *d_part2 = nullptr;     // the same, for w while x/r's reduction reuses d_part
-    unsigned *d_cnt = nullptr;     // last-block ticket
     double *h_sc = nullptr;        // pinned host mirror
     int red_blocks = 0;            // reduction blocks per instance
     // assembled reference formulation
