@@ -195,3 +195,7 @@ def test_pod_basis_leaves_out_empty_snapshots(datadir):
     # same subspace: every principal angle between the two bases is zero (cosines 1)
     cos = np.linalg.svd(Uref.conj().T @ U.T, compute_uv=False)
     assert np.all(np.abs(cos - 1) <= 1e-8), cos.min()
+    # the basis moves to another model of the same couplings unchanged (gade_lg_set_basis / gade_lg_basis)
+    g2 = gade.Propagator(d, 1, dt_of("dipole"))
+    g2.lg_set_basis(U)
+    assert np.array_equal(g2.lg_basis(npod).view(np.float64), U.view(np.float64))
