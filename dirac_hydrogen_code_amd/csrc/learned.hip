@@ -131,6 +131,53 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_bf16(int M, int N, int K, cons
     }
 }
 
+// The whole network for up to 16 rows (inference) in one block of 16 waves: per layer, wave w computes the
+// 16x16 output tile of columns 16w.. over the full k range on the bf16 MFMA (FP32 accumulation), adds the bias
+// and, for the hidden layers, writes GELU(z) in bf16 to LDS as the next layer's input; a barrier between
+// layers.  X bf16 [16][kIn], Y FP32 [16][kOut].
+__device__ __forceinline__ void mlp_tile(const uint16_t *__restrict__ in, int ldin, int K,
+                                         const uint16_t *__restrict__ W, int n0, fl4 &acc) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf4 av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int kk = k0 + 4 * g + j;
+            av[j] = (short)in[r * ldin + kk];
+            bv[j] = (short)W[(size_t)(n0 + r) * K + kk];
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+}
+__global__ __launch_bounds__(1024) void k_mlp_infer(const uint16_t *__restrict__ X, const uint16_t *__restrict__ pbf,
+                                                    const float *__restrict__ p, float *__restrict__ Y) {
+    __shared__ uint16_t act[2][16 * kHid];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int l = 0; l < kLayers; l++) {
+        const int K = l == 0 ? kIn : kHid, N = l == kLayers - 1 ? kOut : kHid;
+        const size_t wo = l == 0 ? P_W1 : l == 1 ? P_W2 : l == 2 ? P_W3 : P_W4;
+        const size_t bo = l == 0 ? P_B1 : l == 1 ? P_B2 : l == 2 ? P_B3 : P_B4;
+        if (16 * w < N) {
+            fl4 acc = {0.f, 0.f, 0.f, 0.f};
+            if (l == 0)
+                mlp_tile(X, kIn, K, pbf + wo, 16 * w, acc);
+            else
+                mlp_tile(act[(l - 1) & 1], kHid, K, pbf + wo, 16 * w, acc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * g + i, n = 16 * w + r;
+                const float v = acc[i] + p[bo + n];
+                if (l < kLayers - 1)
+                    act[l & 1][m * kHid + n] = f2bf(gelu(v));
+                else
+                    Y[m * kOut + n] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // difference stage: V_0 = psi_n, V_k = nabla^k psi_n (k = 1..5), V_6 = e2 (record:
 // psi_{n+1} - (psi_n + V_1 + V_2)), and the partial Gram sums Re(V_i^H V_j).
@@ -138,10 +185,13 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_bf16(int M, int N, int K, cons
 // ring slot (have - k) mod S.  Record (rec = 1, after): psi_n in slot (have - 1),
 // psi_{n-k} in slot (have - 1 - k), psi_{n+1} = psi.
 // ---------------------------------------------------------------------------
+// Inference (REC false) uses only the norms |V_k|^2: the off-diagonal Gram partials are written as zeros.
+template <bool REC>
 __global__ __launch_bounds__(256) void k_lg_diff(long long N1, long long Ntot, const double2 *__restrict__ psi,
                                                  const double2 *__restrict__ hist, const int *__restrict__ ctl0,
-                                                 int rec, double2 *__restrict__ V, double *__restrict__ part0,
+                                                 double2 *__restrict__ V, double *__restrict__ part0,
                                                  const int *gate, int gstride) {
+    constexpr int rec = REC ? 1 : 0;
     const int b = blockIdx.y;
     GADE_GATED_B(gate, gstride, b);
     __shared__ double sh[kNG * 4];
@@ -184,14 +234,20 @@ __global__ __launch_bounds__(256) void k_lg_diff(long long N1, long long Ntot, c
 #pragma unroll
         for (int p = 0; p < kNV; p++)
 #pragma unroll
-            for (int q = p; q < kNV; q++, t++) acc[t] += v[p].x * v[q].x + v[p].y * v[q].y;
+            for (int q = p; q < kNV; q++, t++)
+                if (REC || q == p) acc[t] += v[p].x * v[q].x + v[p].y * v[q].y;
     }
-    // block reduction of the kNG sums
+    // block reduction of the kNG sums (the used ones)
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    {
+        int t = 0;
 #pragma unroll
-    for (int t = 0; t < kNG; t++) {
-        double s = wave_sum(acc[t]);
-        if (lane == 0) sh[wid * kNG + t] = s;
+        for (int p = 0; p < kNV; p++)
+#pragma unroll
+            for (int q = p; q < kNV; q++, t++) {
+                const double s = (REC || q == p) ? wave_sum(acc[t]) : 0.0;
+                if (lane == 0) sh[wid * kNG + t] = s;
+            }
     }
     __syncthreads();
     if (threadIdx.x < kNG) {
@@ -228,7 +284,8 @@ __device__ __forceinline__ int gidx(int p, int q) {  // index of Re(V_p^H V_q), 
 }
 
 // partial[b][c][v][j] = sum over chunk c of conj(UT[k][j]) V_{v0+v}[b][k]: one block per (chunk, b), lanes
-// (j = t % 64, s = t / 64), UT the k-major copy of the basis (coalesced over j), V shared by the 64 lanes
+// (j = t % 64, s = t / 64), UT the k-major FP32 copy of the basis (coalesced over j; the products and sums in
+// FP64), V shared by the 64 lanes
 // Grid (B, nchunk): the instances of one chunk are neighbouring blocks, so the chunk of the basis is read from
 // memory once and from cache for the other instances.
 constexpr int kProjChunk = 256;  // shortest chunk
@@ -236,33 +293,46 @@ inline int proj_chunks(long long N1, int B) {  // chunks per instance: ~4k block
     const long long want = std::max(64, std::min(1024, 4096 / std::max(1, B)));
     return (int)std::max<long long>(1, std::min<long long>(want, (N1 + kProjChunk - 1) / kProjChunk));
 }
+template <int NV>
 __global__ __launch_bounds__(256) void k_lg_proj_part(long long N1, long long Ntot, int npod,
-                                                      const double2 *__restrict__ UT, const double2 *__restrict__ V,
-                                                      int v0, int nv, double2 *__restrict__ part, const int *gate,
+                                                      const float2 *__restrict__ UT, const double2 *__restrict__ V,
+                                                      int v0, double2 *__restrict__ part, const int *gate,
                                                       int gstride, long long clen) {
     const int b = blockIdx.x, c = blockIdx.y, nch = gridDim.y;
     GADE_GATED_B(gate, gstride, b);
-    __shared__ double2 sh[4][kNV][64];
+    __shared__ double2 sh[4][NV][64];
     const int j = threadIdx.x & 63, sl = threadIdx.x >> 6;
-    double2 acc[kNV];
+    double2 acc[NV];
 #pragma unroll
-    for (int v = 0; v < kNV; v++) acc[v] = make_double2(0, 0);
-    const long long k1 = (long long)(c + 1) * clen < N1 ? (long long)(c + 1) * clen : N1;
-    if (j < npod)
+    for (int v = 0; v < NV; v++) acc[v] = make_double2(0, 0);
+    // wave sl takes a contiguous quarter of the chunk; its V values are loaded 64 at a time (one per lane) and
+    // broadcast with lane reads, so only the basis loads are in the inner loop
+    const long long cb = (long long)c * clen, ce = cb + clen < N1 ? cb + clen : N1;
+    const long long q = (clen + 3) / 4;
+    const long long k0 = cb + sl * q, k1 = k0 + q < ce ? k0 + q : ce;
+    const int jj = j < npod ? j : npod - 1;
+    for (long long kb = k0; kb < k1; kb += 64) {
+        const int cnt = (int)(k1 - kb < 64 ? k1 - kb : 64);
+        double2 vv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            vv[v] = j < cnt ? V[(size_t)(v0 + v) * Ntot + (size_t)b * N1 + kb + j] : make_double2(0, 0);
 #pragma unroll 4
-        for (long long k = (long long)c * clen + sl; k < k1; k += 4) {
-            const double2 u = UT[(size_t)k * npod + j];
+        for (int i = 0; i < cnt; i++) {
+            const double2 u = to_f64(UT[(size_t)(kb + i) * npod + jj]);
 #pragma unroll
-            for (int v = 0; v < kNV; v++) {
-                if (v >= nv) break;
-                acc[v] = cfmaconj(u, V[(size_t)(v0 + v) * Ntot + (size_t)b * N1 + k], acc[v]);
+            for (int v = 0; v < NV; v++) {
+                const double2 x = make_double2(__shfl(vv[v].x, i, 64), __shfl(vv[v].y, i, 64));
+                acc[v] = cfmaconj(u, x, acc[v]);
             }
         }
+    }
 #pragma unroll
-    for (int v = 0; v < kNV; v++) sh[sl][v][j] = acc[v];
+    for (int v = 0; v < NV; v++) sh[sl][v][j] = acc[v];
     __syncthreads();
     if (sl == 0 && j < npod)
-        for (int v = 0; v < nv; v++)
+#pragma unroll
+        for (int v = 0; v < NV; v++)
             part[(((size_t)b * nch + c) * kNV + v) * 64 + j] =
                 cadd(cadd(sh[0][v][j], sh[1][v][j]), cadd(sh[2][v][j], sh[3][v][j]));
 }
@@ -298,12 +368,18 @@ __global__ __launch_bounds__(256) void k_lg_colnorm(long long n, const double2 *
     if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
 }
 
-// UT[k][j] = U[j][k]
-__global__ void k_lg_transpose(long long N1, int npod, const double2 *__restrict__ U, double2 *__restrict__ UT) {
+// the per-step copies of the basis, rounded to FP32: U32[j][k] = UT32[k][j] = U[j][k].  Each step reads
+// the basis twice (projections, guess); FP32 halves those reads.  The correction s U c it forms is a small
+// part of the guess and the converged solve at the unchanged tolerance judges every guess, so the rounding
+// (6e-8 relative in U) changes no result beyond the solve's own tolerance.
+__global__ void k_lg_basis32(long long N1, int npod, const double2 *__restrict__ U, float2 *__restrict__ U32,
+                             float2 *__restrict__ UT32) {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < N1 * npod; i += (long long)gridDim.x * blockDim.x) {
         const long long k = i / npod;
         const int j = (int)(i % npod);
-        UT[i] = U[(size_t)j * N1 + k];
+        const float2 u = to_f32(U[(size_t)j * N1 + k]);
+        UT32[i] = u;
+        U32[(size_t)j * N1 + k] = u;
     }
 }
 
@@ -374,7 +450,7 @@ __global__ __launch_bounds__(64) void k_lg_features(const Coef6 *__restrict__ cf
 __global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, double2 *__restrict__ psi,
                                                   double2 *__restrict__ hist, const double2 *__restrict__ V,
                                                   const double *__restrict__ gram, const float *__restrict__ Y,
-                                                  const double2 *__restrict__ U, int npod,
+                                                  const float2 *__restrict__ U, int npod,
                                                   const int *__restrict__ ctl0, const int *__restrict__ oor,
                                                   int *__restrict__ uses, const int *gate, int gstride) {
     const int b = blockIdx.y;
@@ -415,7 +491,7 @@ __global__ __launch_bounds__(256) void k_lg_guess(long long N1, long long Ntot, 
 #pragma unroll
             for (int k = 1; k < 6; k++) x = cadd(x, cscale(V[(size_t)k * Ntot + i], ws[k]));
 #pragma unroll 8
-            for (int q = 0; q < npod; q++) x = cfma(cs[q], U[(size_t)q * N1 + j], x);
+            for (int q = 0; q < npod; q++) x = cfma(cs[q], to_f64(U[(size_t)q * N1 + j]), x);
         } else {
             // Lagrange extrapolation of order p through psi_n..psi_{n-p}
             const double c2[3][3] = {{1, 0, 0}, {2, -1, 0}, {3, -3, 1}};
@@ -619,7 +695,8 @@ __global__ void k_lg_adam(long n, float *__restrict__ p, const float *__restrict
 struct gade_lg {
     int npod = 0;
     double2 *U = nullptr;         // [npod][N1] POD basis (orthonormal columns)
-    double2 *UT = nullptr;        // [N1][npod] the same, k-major (projections)
+    float2 *U32 = nullptr;        // [npod][N1] FP32 copy (guess assembly)
+    float2 *UT32 = nullptr;       // [N1][npod] FP32 copy, k-major (projections)
     double2 *ppart = nullptr;     // [B][nchunk][7][64] projection partials
     int nchunk = 0;
     double2 *V = nullptr;         // [7][N] difference stage vectors
@@ -653,7 +730,7 @@ namespace gade {
 
 void lg_free(gade_lg *lg) {
     if (!lg) return;
-    std::vector<void *> ptrs = {lg->U, lg->UT, lg->ppart, lg->V, lg->part, lg->gram, lg->proj, lg->lres, lg->feat, lg->mean, lg->stdv,
+    std::vector<void *> ptrs = {lg->U, lg->U32, lg->UT32, lg->ppart, lg->V, lg->part, lg->gram, lg->proj, lg->lres, lg->feat, lg->mean, lg->stdv,
                                 lg->zmax, lg->oor, lg->uses,
                                 lg->xin, lg->yi, lg->p, lg->g, lg->m1, lg->m2, lg->pbf, lg->xt, lg->yt, lg->dy,
                                 lg->da, lg->dz, lg->loss};
@@ -672,6 +749,19 @@ void lg_free(gade_lg *lg) {
 }  // namespace gade
 
 namespace {
+
+// the FP32 per-step copies of the model's basis (after the FP64 basis is built or installed)
+int lg_basis32(gade_model *m) {
+    gade_lg *lg = m->lg;
+    const size_t n = (size_t)m->N1;
+    if (!lg->U32) GADE_TRY(hip_alloc((void **)&lg->U32, sizeof(float2) * n * kPod));
+    if (!lg->UT32) GADE_TRY(hip_alloc((void **)&lg->UT32, sizeof(float2) * n * kPod));
+    hipLaunchKernelGGL(k_lg_basis32, dim3(grid_for_lg((long long)n * lg->npod)), dim3(256), 0, m->stream,
+                       (long long)n, lg->npod, (const double2 *)lg->U, lg->U32, lg->UT32);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    return GADE_OK;
+}
 
 int lg_ensure(gade_model *m) {
     if (m->lg) return GADE_OK;
@@ -748,11 +838,17 @@ int lg_train_alloc(gade_lg *lg, int Mt) {
 }
 
 // out[b][v][j] = U_j^H V_{v0+v} for v < nv
+// projections of V_{v0..v0+nv-1} on the basis (nv: 2 at inference, kNV when recording)
 void lg_project(gade_model *m, int v0, int nv, const int *gate, int gstride) {
     gade_lg *lg = m->lg;
     const long long clen = (m->N1 + lg->nchunk - 1) / lg->nchunk;
-    hipLaunchKernelGGL(k_lg_proj_part, dim3(m->B, lg->nchunk), dim3(256), 0, m->stream, m->N1, m->N, lg->npod,
-                       (const double2 *)lg->UT, (const double2 *)lg->V, v0, nv, lg->ppart, gate, gstride, clen);
+    if (nv == 2)
+        hipLaunchKernelGGL(k_lg_proj_part<2>, dim3(m->B, lg->nchunk), dim3(256), 0, m->stream, m->N1, m->N, lg->npod,
+                           (const float2 *)lg->UT32, (const double2 *)lg->V, v0, lg->ppart, gate, gstride, clen);
+    else
+        hipLaunchKernelGGL(k_lg_proj_part<kNV>, dim3(m->B, lg->nchunk), dim3(256), 0, m->stream, m->N1, m->N,
+                           lg->npod, (const float2 *)lg->UT32, (const double2 *)lg->V, v0, lg->ppart, gate, gstride,
+                           clen);
     hipLaunchKernelGGL(k_lg_proj_fin, dim3(m->B, nv, 4), dim3(256), 0, m->stream, lg->nchunk, lg->npod, nv,
                        (const double2 *)lg->ppart, lg->proj, gate, gstride);
 }
@@ -865,8 +961,8 @@ int lg_step_prologue(gade_model *m, double2 *psi) {
     const dim3 gd(lg->nparts, B);
     hipLaunchKernelGGL(k_lg_lres_push, dim3(B), dim3(64), 0, m->stream, (const double *)m->d_sc, lg->lres, m->gate,
                        m->gstride);
-    hipLaunchKernelGGL(k_lg_diff, gd, dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
-                       (const double2 *)m->d_hist, (const int *)m->d_ctl, 0, lg->V, lg->part, m->gate, m->gstride);
+    hipLaunchKernelGGL(k_lg_diff<false>, gd, dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
+                       (const double2 *)m->d_hist, (const int *)m->d_ctl, lg->V, lg->part, m->gate, m->gstride);
     hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(256), 0, m->stream, lg->nparts, (const double *)lg->part,
                        lg->gram, m->gate, m->gstride);
     if (lg->npod > 0) lg_project(m, 3, 2, m->gate, m->gstride);
@@ -877,10 +973,14 @@ int lg_step_prologue(gade_model *m, double2 *psi) {
                        lg->feat, lg->xin, (const float *)lg->zmax, lg->oor, m->gate, m->gstride);
     GADE_HIP(hipGetLastError());
     if (learned) {
-        mlp_forward(lg, m->stream, lg->Mi, lg->xin, lg->zi, lg->ai, lg->yi);
+        if (lg->Mi == 16)
+            hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(1024), 0, m->stream, (const uint16_t *)lg->xin,
+                               (const uint16_t *)lg->pbf, (const float *)lg->p, lg->yi);
+        else
+            mlp_forward(lg, m->stream, lg->Mi, lg->xin, lg->zi, lg->ai, lg->yi);
         hipLaunchKernelGGL(k_lg_guess, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
                            (const double2 *)lg->V, (const double *)lg->gram, (const float *)lg->yi,
-                           (const double2 *)lg->U, lg->npod, (const int *)m->d_ctl, (const int *)lg->oor, lg->uses,
+                           (const float2 *)lg->U32, lg->npod, (const int *)m->d_ctl, (const int *)lg->oor, lg->uses,
                            m->gate, m->gstride);
     } else {
         hipLaunchKernelGGL(k_extrap, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
@@ -904,9 +1004,8 @@ int gade_lg_record(gade_model_t *m, const double *psi, float *samples, double *s
     gade_lg *lg = m->lg;
     const int B = m->B;
     const long long N1 = m->N1, N = m->N;
-    hipLaunchKernelGGL(k_lg_diff, dim3(lg->nparts, B), dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
-                       (const double2 *)m->d_hist, (const int *)m->d_ctl, 1, lg->V, lg->part, (const int *)nullptr,
-                       0);
+    hipLaunchKernelGGL(k_lg_diff<true>, dim3(lg->nparts, B), dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
+                       (const double2 *)m->d_hist, (const int *)m->d_ctl, lg->V, lg->part, (const int *)nullptr, 0);
     hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(256), 0, m->stream, lg->nparts, (const double *)lg->part,
                        lg->gram, (const int *)nullptr, 0);
     if (snaps)
@@ -1044,12 +1143,7 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
     C = nullptr;
     lg->npod = npod;
     cleanup();
-    if (!lg->UT) GADE_TRY(hip_alloc((void **)&lg->UT, sizeof(double2) * (size_t)n * kPod));
-    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for_lg((long long)n * npod)), dim3(256), 0, m->stream, (long long)n,
-                       npod, (const double2 *)lg->U, lg->UT);
-    GADE_HIP(hipGetLastError());
-    GADE_HIP(hipStreamSynchronize(m->stream));
-    return GADE_OK;
+    return lg_basis32(m);
 }
 
 int gade_lg_set_basis(gade_model_t *m, const double *U, int npod) {
@@ -1061,14 +1155,9 @@ int gade_lg_set_basis(gade_model_t *m, const double *U, int npod) {
     if (lg->U) (void)hipFree(lg->U);
     lg->U = nullptr;
     GADE_TRY(hip_alloc((void **)&lg->U, sizeof(double2) * n * npod));
-    if (!lg->UT) GADE_TRY(hip_alloc((void **)&lg->UT, sizeof(double2) * n * kPod));
     GADE_HIP(hipMemcpy(lg->U, U, sizeof(double2) * n * npod, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_lg_transpose, dim3(grid_for_lg((long long)n * npod)), dim3(256), 0, m->stream, (long long)n,
-                       npod, (const double2 *)lg->U, lg->UT);
-    GADE_HIP(hipGetLastError());
-    GADE_HIP(hipStreamSynchronize(m->stream));
     lg->npod = npod;
-    return GADE_OK;
+    return lg_basis32(m);
 }
 
 int gade_lg_basis(gade_model_t *m, double *U) {
@@ -1144,7 +1233,11 @@ int gade_lg_forward(gade_model_t *m, const float *x, int M, float *y) {
     GADE_TRY(lg_train_alloc(lg, M));
     hipLaunchKernelGGL(k_lg_gather_rows, dim3(M), dim3(64), 0, m->stream, x, (const float *)lg->mean,
                        (const float *)lg->stdv, lg->xt);
-    mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
+    if (M == 16)  // the inference network of up to 16 instances
+        hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(1024), 0, m->stream, (const uint16_t *)lg->xt,
+                           (const uint16_t *)lg->pbf, (const float *)lg->p, lg->yt);
+    else
+        mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
     hipError_t e = hipMemcpyAsync(y, lg->yt, sizeof(float) * M * kOut, hipMemcpyDeviceToDevice, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     if (e != hipSuccess) {

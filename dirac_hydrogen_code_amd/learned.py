@@ -23,15 +23,18 @@ import numpy as np
 from . import gade
 
 
-def scan_pulses(seed, B, dt):
+def scan_pulses(seed, B, dt, cep=None):
     """Seeded pulse scan: E0 log-uniform in [1, 100], omega uniform in [25, 75], carrier phase uniform in
-    [0, 2 pi), cycles uniform in 1..15; per-instance step counts ceil(T_b / dt)."""
+    [0, 2 pi) (or the given cep for every pulse; 0 is the reference's own phase), cycles uniform in 1..15;
+    per-instance step counts ceil(T_b / dt)."""
     rng = np.random.default_rng(seed)
     E0 = np.exp(rng.uniform(np.log(1.0), np.log(100.0), B))
     om = rng.uniform(25.0, 75.0, B)
-    cep = rng.uniform(0.0, 2 * np.pi, B)
+    ceps = rng.uniform(0.0, 2 * np.pi, B)
+    if cep is not None:
+        ceps = np.full(B, float(cep))
     cyc = rng.integers(1, 16, B).astype(float)
-    params = np.stack([E0, om, cyc, cep], axis=1)
+    params = np.stack([E0, om, cyc, ceps], axis=1)
     nsteps_b = np.ceil(cyc * 2 * np.pi / om / dt).astype(np.int32)
     return params, nsteps_b
 
@@ -60,14 +63,18 @@ def step_scan(gp, psi0, params, nsteps_b, nsteps, after_step=None, tol=1e-12):
 class Trainer:
     """Builds and trains the learned guess of a batched model ``gp`` (its couplings, dt and batch width)."""
 
-    def __init__(self, gp, psi0, seed=0x5EED, npod=gade.LG_POD, nsnap=400, nsamples=20000, log=None, tol=1e-13):
+    def __init__(self, gp, psi0, seed=0x5EED, npod=gade.LG_POD, nsnap=400, nsamples=20000, log=None, tol=1e-13,
+                 cep=None):
         """``tol``: tolerance of the solves that make the snapshots and samples.  It is tighter than the
         production one, so that a step whose extrapolated guess already meets the production tolerance (it
         ends at 0 iterations there) still gets a solution that differs from the guess: the sample keeps a
-        non-zero extrapolation error and the loss learns to leave such steps alone."""
+        non-zero extrapolation error and the loss learns to leave such steps alone.
+        ``cep``: carrier phase of the training pulses (None: seeded per pulse).  A guess for the reference's
+        own pulses (fixed phase, cep = 0) is trained with cep = 0: at that phase three of the six f_alpha start
+        at zero and the early response is a family of its own, outside what random phases cover."""
         self.gp, self.psi0 = gp, np.asarray(psi0, np.complex128)
         self.seed, self.npod, self.nsnap, self.nsamples, self.tol = seed, npod, nsnap, nsamples, tol
-        self.params, self.nsteps_b = scan_pulses(seed, gp.batch, gp.dt)
+        self.params, self.nsteps_b = scan_pulses(seed, gp.batch, gp.dt, cep)
         self.log = log or (lambda msg: None)
         self.N1 = gp.nblk * gp.blkdim
 

@@ -46,10 +46,10 @@ def net_model(datadir):
     return gade.Propagator(d, 1, dt_of("dipole"))
 
 
-@pytest.mark.parametrize("M", [48, 160])
+@pytest.mark.parametrize("M", [16, 48, 160])
 def test_mlp_forward_matches_oracle(datadir, M):
-    """Up to 64 rows (inference) every tile's k steps are split over four waves; more rows (training) use one
-    wave per tile: both within the bar."""
+    """16 rows (the inference of up to 16 instances): the whole network in one block; up to 64 rows every
+    tile's k steps are split over four waves; more rows (training) use one wave per tile: all within the bar."""
     say(f"mlp forward M={M}")
     rng = np.random.default_rng(11)
     gp = net_model(datadir)
