@@ -103,15 +103,25 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_bf16(int M, int N, int K, cons
     const uint16_t *ap = a + (long)(tm * 16 + r) * sam;
     const uint16_t *bp = b + (long)(tn * 16 + r) * sbn;
     fl4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 16 * w; k0 < K; k0 += 16 * KS) {
-        bf4 av, bv;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long kk = k0 + 4 * g + j;
-            av[j] = (short)ap[kk * sak];
-            bv[j] = (short)bp[kk * sbk];
+    if (sak == 1 && sbk == 1) {
+        // both operands contiguous along k (the forward layers): each lane's four k are one 8-byte load
+        for (int k0 = 16 * w; k0 < K; k0 += 16 * KS) {
+            bf4 av, bv;
+            __builtin_memcpy(&av, reinterpret_cast<const uint2 *>(ap + k0 + 4 * g), sizeof(bf4));
+            __builtin_memcpy(&bv, reinterpret_cast<const uint2 *>(bp + k0 + 4 * g), sizeof(bf4));
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    } else {
+        for (int k0 = 16 * w; k0 < K; k0 += 16 * KS) {
+            bf4 av, bv;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const long kk = k0 + 4 * g + j;
+                av[j] = (short)ap[kk * sak];
+                bv[j] = (short)bp[kk * sbk];
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        }
     }
     if constexpr (KS > 1) {
         __shared__ fl4 red[KS][64];
@@ -128,53 +138,6 @@ __global__ __launch_bounds__(64 * KS) void k_gemm_bf16(int M, int N, int K, cons
         if (EPI != EPI_STORE) v += bias[n];
         c[(long)m * ldc + n] = v;
         if (EPI == EPI_BIAS_GELU) act[(long)m * ldact + n] = f2bf(gelu(v));
-    }
-}
-
-// The whole network for up to 16 rows (inference) in one block of 16 waves: per layer, wave w computes the
-// 16x16 output tile of columns 16w.. over the full k range on the bf16 MFMA (FP32 accumulation), adds the bias
-// and, for the hidden layers, writes GELU(z) in bf16 to LDS as the next layer's input; a barrier between
-// layers.  X bf16 [16][kIn], Y FP32 [16][kOut].
-__device__ __forceinline__ void mlp_tile(const uint16_t *__restrict__ in, int ldin, int K,
-                                         const uint16_t *__restrict__ W, int n0, fl4 &acc) {
-    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf4 av, bv;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int kk = k0 + 4 * g + j;
-            av[j] = (short)in[r * ldin + kk];
-            bv[j] = (short)W[(size_t)(n0 + r) * K + kk];
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    }
-}
-__global__ __launch_bounds__(1024) void k_mlp_infer(const uint16_t *__restrict__ X, const uint16_t *__restrict__ pbf,
-                                                    const float *__restrict__ p, float *__restrict__ Y) {
-    __shared__ uint16_t act[2][16 * kHid];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int l = 0; l < kLayers; l++) {
-        const int K = l == 0 ? kIn : kHid, N = l == kLayers - 1 ? kOut : kHid;
-        const size_t wo = l == 0 ? P_W1 : l == 1 ? P_W2 : l == 2 ? P_W3 : P_W4;
-        const size_t bo = l == 0 ? P_B1 : l == 1 ? P_B2 : l == 2 ? P_B3 : P_B4;
-        if (16 * w < N) {
-            fl4 acc = {0.f, 0.f, 0.f, 0.f};
-            if (l == 0)
-                mlp_tile(X, kIn, K, pbf + wo, 16 * w, acc);
-            else
-                mlp_tile(act[(l - 1) & 1], kHid, K, pbf + wo, 16 * w, acc);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int m = 4 * g + i, n = 16 * w + r;
-                const float v = acc[i] + p[bo + n];
-                if (l < kLayers - 1)
-                    act[l & 1][m * kHid + n] = f2bf(gelu(v));
-                else
-                    Y[m * kOut + n] = v;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -383,39 +346,59 @@ __global__ void k_lg_basis32(long long N1, int npod, const double2 *__restrict__
     }
 }
 
-// log10 residual ring [b][8], newest first: push the previous solve's final residual relative to its
-// requested bound tol |rhs| (0 where the solve ended just at the bound, below 0 the further it went under it):
-// the same scale at any tolerance, so a network trained on solves at one tolerance reads the same inputs at
-// another
-__global__ void k_lg_lres_push(const double *__restrict__ sc0, double *__restrict__ lres, const int *gate,
-                               int gstride) {
+// The inputs of one step, one block of 256 threads per instance:
+// 1. the residual ring [b][8] (newest first) takes the previous solve's final residual relative to its
+//    requested bound tol |rhs| (0 where the solve ended just at the bound, below 0 the further it went under
+//    it): the same scale at any tolerance, so a network trained on solves at one tolerance reads the same
+//    inputs at another;
+// 2. the Gram sums of the difference stage, in k_lg_gram_fin's order (eight interleaved groups of partials,
+//    then the groups);
+// 3. the raw inputs and their standardized bf16 row.  f_alpha(t_{n+1}) comes from the window's coefficient
+//    table (c_alpha = hd * i * f_alpha, f real).  Range gate: oor[b] = 1 when a standardized input lies beyond
+//    *zmax standard deviations of the training inputs (or is not finite), *zmax > 0; such a step takes the
+//    extrapolation instead of the network.
+__global__ __launch_bounds__(256) void k_lg_inputs(const double *__restrict__ sc0, double *__restrict__ lres,
+                                                   int nparts, const double *__restrict__ gpart,
+                                                   double *__restrict__ gram, const Coef6 *__restrict__ cf_dev,
+                                                   const int *__restrict__ gctl, int B, double hd,
+                                                   const double2 *__restrict__ z, int npod,
+                                                   const float *__restrict__ mean, const float *__restrict__ stdv,
+                                                   float *__restrict__ feat, uint16_t *__restrict__ xin,
+                                                   const float *__restrict__ zmax, int *__restrict__ oor,
+                                                   const int *gate, int gstride) {
     const int b = blockIdx.x;
     GADE_GATED_B(gate, gstride, b);
-    if (threadIdx.x != 0) return;
-    const double *sc = sc0 + (size_t)b * SC_STRIDE;
-    double *r = lres + (size_t)b * 8;
-    for (int k = 7; k > 0; k--) r[k] = r[k - 1];
-    const double t = sc[SC_TOL2] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_TOL2]) : 0.0;
-    r[0] = log10(fmax(t, 1e-30));
-}
-
-// raw inputs of instance b and its standardized bf16 row.  f_alpha(t_{n+1}) comes from the window's
-// coefficient table (c_alpha = hd * i * f_alpha, f real).
-// Range gate: oor[b] = 1 when a standardized input lies beyond *zmax standard deviations of the training
-// inputs (or is not finite), *zmax > 0; such a step takes the extrapolation instead of the network.
-// One wave per instance.
-__global__ __launch_bounds__(64) void k_lg_features(const Coef6 *__restrict__ cf_dev, const int *__restrict__ gctl,
-                                                    int B, double hd, const double *__restrict__ lres,
-                                                    const double *__restrict__ gram, const double2 *__restrict__ z,
-                                                    int npod, const float *__restrict__ mean,
-                                                    const float *__restrict__ stdv, float *__restrict__ feat,
-                                                    uint16_t *__restrict__ xin, const float *__restrict__ zmax,
-                                                    int *__restrict__ oor, const int *gate, int gstride) {
-    const int b = blockIdx.x;
-    GADE_GATED_B(gate, gstride, b);
+    __shared__ double gsh[8][32];
+    __shared__ double g[kNG];
+    __shared__ double lr[6];
+    __shared__ float zw[4];
+    if (threadIdx.x == 0) {
+        const double *sc = sc0 + (size_t)b * SC_STRIDE;
+        double *r = lres + (size_t)b * 8;
+        for (int k = 7; k > 0; k--) r[k] = r[k - 1];
+        const double t = sc[SC_TOL2] > 0 ? sqrt(sc[SC_RNORM2] / sc[SC_TOL2]) : 0.0;
+        r[0] = log10(fmax(t, 1e-30));
+        for (int k = 0; k < 6; k++) lr[k] = r[k];
+    }
+    {
+        const int e = threadIdx.x & 31, gi = threadIdx.x >> 5;
+        double acc = 0;
+        if (e < kNG)
+#pragma unroll 4
+            for (int p = gi; p < nparts; p += 8) acc += gpart[((size_t)b * nparts + p) * kNG + e];
+        gsh[gi][e] = acc;
+        __syncthreads();
+        if (gi == 0 && e < kNG) {
+            double t = gsh[0][e];
+#pragma unroll
+            for (int q = 1; q < 8; q++) t += gsh[q][e];
+            g[e] = t;
+            gram[(size_t)b * kNG + e] = t;
+        }
+        __syncthreads();
+    }
     float zabs = 0.f;
     const Coef6 *cf = cf_dev + (size_t)gctl[GC_WSLOT] * B + b;
-    const double *g = gram + (size_t)b * kNG;
     const double n0 = sqrt(fmax(g[gidx(0, 0)], 1e-300));
     const double s = sqrt(g[gidx(3, 3)]);
     const double inv_s = s > 0 ? 1.0 / s : 0.0;
@@ -423,7 +406,7 @@ __global__ __launch_bounds__(64) void k_lg_features(const Coef6 *__restrict__ cf
     for (int t = threadIdx.x; t < kIn; t += blockDim.x) {
         double v = 0;
         if (t < F_LRES) v = cf->c[t - F_F].y / hd;
-        else if (t < F_DN) v = lres[(size_t)b * 8 + (t - F_LRES)];
+        else if (t < F_DN) v = lr[t - F_LRES];
         else if (t < F_Z3) v = 0.5 * log10(fmax(g[gidx(t - F_DN + 1, t - F_DN + 1)], 1e-300)) - log10(n0);
         else if (t < F_Z4) {
             const int q = (t - F_Z3) >> 1;
@@ -441,7 +424,12 @@ __global__ __launch_bounds__(64) void k_lg_features(const Coef6 *__restrict__ cf
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) zabs = fmaxf(zabs, __shfl_xor(zabs, off));
-    if (threadIdx.x == 0) oor[b] = (*zmax > 0.f && !(zabs <= *zmax)) ? 1 : 0;
+    if ((threadIdx.x & 63) == 0) zw[threadIdx.x >> 6] = zabs;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float zm = fmaxf(fmaxf(zw[0], zw[1]), fmaxf(zw[2], zw[3]));
+        oor[b] = (*zmax > 0.f && !(zm <= *zmax)) ? 1 : 0;
+    }
 }
 
 // the guess of instance b into psi (psi_n enters the ring); fallback: order-2 extrapolation (or lower)
@@ -959,25 +947,17 @@ int lg_step_prologue(gade_model *m, double2 *psi) {
     const int B = m->B;
     const long long N1 = m->N1, N = m->N;
     const dim3 gd(lg->nparts, B);
-    hipLaunchKernelGGL(k_lg_lres_push, dim3(B), dim3(64), 0, m->stream, (const double *)m->d_sc, lg->lres, m->gate,
-                       m->gstride);
     hipLaunchKernelGGL(k_lg_diff<false>, gd, dim3(256), 0, m->stream, N1, N, (const double2 *)psi,
                        (const double2 *)m->d_hist, (const int *)m->d_ctl, lg->V, lg->part, m->gate, m->gstride);
-    hipLaunchKernelGGL(k_lg_gram_fin, dim3(B), dim3(256), 0, m->stream, lg->nparts, (const double *)lg->part,
-                       lg->gram, m->gate, m->gstride);
     if (lg->npod > 0) lg_project(m, 3, 2, m->gate, m->gstride);
     const double hd = kSoL * m->h * m->dt / 2;
-    hipLaunchKernelGGL(k_lg_features, dim3(B), dim3(64), 0, m->stream, (const Coef6 *)m->d_cfwin,
-                       (const int *)m->d_gctl, B, hd, (const double *)lg->lres, (const double *)lg->gram,
+    hipLaunchKernelGGL(k_lg_inputs, dim3(B), dim3(256), 0, m->stream, (const double *)m->d_sc, lg->lres, lg->nparts,
+                       (const double *)lg->part, lg->gram, (const Coef6 *)m->d_cfwin, (const int *)m->d_gctl, B, hd,
                        (const double2 *)lg->proj, lg->npod, (const float *)lg->mean, (const float *)lg->stdv,
                        lg->feat, lg->xin, (const float *)lg->zmax, lg->oor, m->gate, m->gstride);
     GADE_HIP(hipGetLastError());
     if (learned) {
-        if (lg->Mi == 16)
-            hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(1024), 0, m->stream, (const uint16_t *)lg->xin,
-                               (const uint16_t *)lg->pbf, (const float *)lg->p, lg->yi);
-        else
-            mlp_forward(lg, m->stream, lg->Mi, lg->xin, lg->zi, lg->ai, lg->yi);
+        mlp_forward(lg, m->stream, lg->Mi, lg->xin, lg->zi, lg->ai, lg->yi);
         hipLaunchKernelGGL(k_lg_guess, dim3(grid_for_lg(N1), B), dim3(256), 0, m->stream, N1, N, psi, m->d_hist,
                            (const double2 *)lg->V, (const double *)lg->gram, (const float *)lg->yi,
                            (const float2 *)lg->U32, lg->npod, (const int *)m->d_ctl, (const int *)lg->oor, lg->uses,
@@ -1233,11 +1213,7 @@ int gade_lg_forward(gade_model_t *m, const float *x, int M, float *y) {
     GADE_TRY(lg_train_alloc(lg, M));
     hipLaunchKernelGGL(k_lg_gather_rows, dim3(M), dim3(64), 0, m->stream, x, (const float *)lg->mean,
                        (const float *)lg->stdv, lg->xt);
-    if (M == 16)  // the inference network of up to 16 instances
-        hipLaunchKernelGGL(k_mlp_infer, dim3(1), dim3(1024), 0, m->stream, (const uint16_t *)lg->xt,
-                           (const uint16_t *)lg->pbf, (const float *)lg->p, lg->yt);
-    else
-        mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
+    mlp_forward(lg, m->stream, M, lg->xt, lg->zt, lg->at, lg->yt);
     hipError_t e = hipMemcpyAsync(y, lg->yt, sizeof(float) * M * kOut, hipMemcpyDeviceToDevice, m->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
     if (e != hipSuccess) {

@@ -48,8 +48,8 @@ def net_model(datadir):
 
 @pytest.mark.parametrize("M", [16, 48, 160])
 def test_mlp_forward_matches_oracle(datadir, M):
-    """16 rows (the inference of up to 16 instances): the whole network in one block; up to 64 rows every
-    tile's k steps are split over four waves; more rows (training) use one wave per tile: all within the bar."""
+    """Up to 64 rows (inference; 16 rows serve up to 16 instances) every tile's k steps are split over four
+    waves; more rows (training) use one wave per tile: both within the bar."""
     say(f"mlp forward M={M}")
     rng = np.random.default_rng(11)
     gp = net_model(datadir)
