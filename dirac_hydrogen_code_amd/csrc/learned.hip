@@ -280,13 +280,20 @@ __global__ __launch_bounds__(256) void k_lg_proj_part(long long N1, long long Nt
 #pragma unroll
         for (int v = 0; v < NV; v++)
             vv[v] = j < cnt ? V[(size_t)(v0 + v) * Ntot + (size_t)b * N1 + kb + j] : make_double2(0, 0);
-#pragma unroll 4
-        for (int i = 0; i < cnt; i++) {
-            const double2 u = to_f64(UT[(size_t)(kb + i) * npod + jj]);
+        // sixteen basis loads in flight before they are used (clamped to the run, zero-weighted beyond it)
+        for (int i0 = 0; i0 < cnt; i0 += 16) {
+            float2 ub[16];
 #pragma unroll
-            for (int v = 0; v < NV; v++) {
-                const double2 x = make_double2(__shfl(vv[v].x, i, 64), __shfl(vv[v].y, i, 64));
-                acc[v] = cfmaconj(u, x, acc[v]);
+            for (int u = 0; u < 16; u++) ub[u] = UT[(size_t)(kb + min(i0 + u, cnt - 1)) * npod + jj];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int i = i0 + u;
+                const double2 uu = i < cnt ? to_f64(ub[u]) : make_double2(0, 0);
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    const double2 x = make_double2(__shfl(vv[v].x, i & 63, 64), __shfl(vv[v].y, i & 63, 64));
+                    acc[v] = cfmaconj(uu, x, acc[v]);
+                }
             }
         }
     }
