@@ -1,0 +1,107 @@
+"""GPU parity at the full production size: NSplines 250, NKappa 16, NMu 8, Nl 8 non-dipole couplings
+(400 blocks x 510 radial functions, N = 204,000; 15-cycle sin^2 pulse, E0 10, omega 50, dt = T/8000).
+
+The oracle runs here only on what it finishes in seconds at this size (one operator application, one
+preconditioner solve, one CN step).  The rest is checked through properties that hold at any size.
+Bars:
+  * fused operator vs the oracle at a mid-pulse time: max-norm relative 1e-13 (S + H, S - H, S)
+  * linearity of the fused operator: A(a x + b y) against a A x + b A y, 1e-13
+  * preconditioner solve vs the oracle: 1e-11; P (P^-1 b) against b: 1e-12
+  * one CN step from psi_0 at mid-pulse vs the oracle's step, both at rel-tol 1e-12: 1e-10
+  * 100 device-driven steps of the timed window: every step at rel-tol 1e-12, S-norm kept to 1e-10
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from dirac_hydrogen_code_amd import gade
+from dirac_hydrogen_code_amd.couplings import CONFIGS, generate
+
+pytestmark = pytest.mark.gpu
+
+NAME = "nondipole_ns250_k16_mu8_l8"
+LMAX, NTIME, E0, OMEGA, CYCLES = 8, 8000, 10.0, 50.0, 15.0
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def relmax(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def full(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp(NAME))
+    say(f"generating {NAME}")
+    generate(d, **CONFIGS[NAME])
+    dt = CYCLES * 2 * np.pi / OMEGA / NTIME
+    say("loading the oracle and the device model")
+    om = O.Model(d, LMAX, dt, nthreads=8)
+    gp = gade.Propagator(d, LMAX, dt)
+    assert gp.n == 204000 and gp.blkdim == 510
+    psi, _ = gade.read_psi0(d, gp.n)
+    return d, dt, om, gp, psi
+
+
+def test_fullsize_apply_and_linearity(full):
+    d, dt, om, gp, psi = full
+    say("full-size fused apply")
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
+    y = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
+    f = gade.pulse(E0, OMEGA, CYCLES, NTIME // 2 * dt)
+    om.timedep(f)
+    gp.set_time(f)
+    S, H = om.apply(x, "S"), om.apply(x, "H")
+    dx, dy = gade.DeviceArray.from_host(x), gade.DeviceArray(gp.n)
+    for cH, want in ((1.0, S + H), (-1.0, S - H), (0.0, S)):
+        gp.apply(dx, dy, cH=cH)
+        assert relmax(dy.get(), want) <= 1e-13, cH
+    # linearity: A(a x + b y) = a A x + b A y
+    a, b = 0.75 - 0.5j, -1.25 + 2.0j
+    gp.apply(dx, dy, cH=1.0)
+    ax = dy.get()
+    dyy = gade.DeviceArray.from_host(y)
+    gp.apply(dyy, dy, cH=1.0)
+    ay = dy.get()
+    dz = gade.DeviceArray.from_host(a * x + b * y)
+    gp.apply(dz, dy, cH=1.0)
+    assert relmax(dy.get(), a * ax + b * ay) <= 1e-13
+
+
+def test_fullsize_psolve(full):
+    d, dt, om, gp, psi = full
+    say("full-size preconditioner solve")
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
+    db, dx = gade.DeviceArray.from_host(b), gade.DeviceArray(gp.n)
+    gp.psolve(db, dx)
+    assert relmax(dx.get(), om.psolve(b)) <= 1e-11
+    gp.set_time(np.zeros(6, complex))  # with f = 0 the fused operator is exactly P
+    dpx = gade.DeviceArray(gp.n)
+    gp.apply(dx, dpx, cH=1.0)
+    assert relmax(dpx.get(), b) <= 1e-12
+
+
+def test_fullsize_cn_step_and_window(full):
+    d, dt, om, gp, psi = full
+    t0 = NTIME // 2 * dt
+    say("full-size CN step at mid-pulse")
+    dpsi = gade.DeviceArray.from_host(psi)
+    gp.cn_step(dpsi, gade.pulse(E0, OMEGA, CYCLES, t0 + dt), tol=1e-12)
+    x_ref, _, _, _ = om.cn_run(psi, E0, OMEGA, CYCLES, 1, t0=t0, tol=1e-12)
+    err = np.linalg.norm(dpsi.get() - x_ref) / np.linalg.norm(x_ref)
+    say(f"  one-step error {err:.3e}")
+    assert err <= 1e-10
+    say("full-size window of 100 device-driven steps")
+    dpsi.upload(psi)
+    s0 = gp.snorm(dpsi)
+    it, tols, _ = gp.cn_run(dpsi, E0, OMEGA, CYCLES, 100, t0=t0, tol=1e-12)
+    assert len(tols) == 100 and np.all(tols < 1e-12)
+    assert abs(gp.snorm(dpsi) - s0) <= 1e-10 * s0
+    say(f"  iterations/step {np.mean(it):.2f}, S-norm change {abs(gp.snorm(dpsi) - s0):.2e}")
