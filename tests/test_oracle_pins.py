@@ -321,6 +321,28 @@ def test_norm_conservation(datadir):
     assert np.all(tols < 1e-12)
 
 
+@pytest.mark.parametrize("cfg", ["dipole", "nondipole_small"])
+def test_cn_time_reversal(datadir, cfg):
+    """CN is its own inverse under dt -> -dt when the backward step sees the same H(t): with the step from t
+    using H(t + dt) on both sides, a model with -dt started at t_k + dt undoes step k exactly.  Five steps of
+    the pulse forward, five back, from a random psi: a sign or time-index slip in the step breaks it."""
+    d = datadir(cfg)
+    dt = dt_of(cfg)
+    lmax = 1 if cfg == "dipole" else 4
+    fwd, bwd = O.Model(d, lmax, dt, nthreads=4), O.Model(d, lmax, -dt, nthreads=4)
+    rng = np.random.default_rng(7)
+    psi = rng.standard_normal(fwd.n) + 1j * rng.standard_normal(fwd.n)
+    psi /= np.linalg.norm(psi)
+    E0, w, cyc, t0, k = 10.0, 50.0, 1.0, 37 * dt, 5
+    x, _, tols_f, _ = fwd.cn_run(psi, E0, w, cyc, k, t0=t0, tol=1e-14)
+    back, _, tols_b, _ = bwd.cn_run(x, E0, w, cyc, k, t0=t0 + (k + 1) * dt, tol=1e-14)
+    assert np.linalg.norm(x - psi) > 1e-3  # the field did act
+    assert np.linalg.norm(back - psi) <= 1e-11
+    # the same backward run started one step early sees the wrong H(t) and does not return
+    wrong, _, _, _ = bwd.cn_run(x, E0, w, cyc, k, t0=t0 + k * dt, tol=1e-14)
+    assert np.linalg.norm(wrong - psi) > 1e3 * np.linalg.norm(back - psi)
+
+
 def test_oracle_rejects_corrupt_file(tmp_path, datadir):
     import shutil
     d = tmp_path / "bad"
