@@ -10,6 +10,7 @@ Bars:
   * one CN step from psi_0 at mid-pulse vs the oracle's step, both at rel-tol 1e-12: 1e-10
   * 100 device-driven steps of the timed window: every step at rel-tol 1e-12, S-norm kept to 1e-10
   * time reversal: five steps of the pulse, then five with -dt seeing the same H(t), return psi to 1e-10
+  * a batched scan of two pulses: each instance equals its own single-instance run bit for bit
 """
 import sys
 
@@ -125,3 +126,31 @@ def test_fullsize_time_reversal(full):
     say(f"  moved {np.linalg.norm(x - psi) / np.linalg.norm(psi):.3e}, returned to {err:.3e}")
     assert np.linalg.norm(x - psi) > 1e-3 * np.linalg.norm(psi)
     assert err <= 1e-10
+
+
+def test_fullsize_batch_equals_single(full):
+    """Two pulses (the reference's carrier phase and another) in one batch at the production shape: each
+    instance's trajectory, iterations and tolerances equal its single-instance run bit for bit."""
+    d, dt, om, gp, psi = full
+    say("full-size batch of two pulses, 10 steps")
+    t0, k = NTIME // 2 * dt, 10
+    params = [(E0, OMEGA, CYCLES, 0.0), (0.5 * E0, 0.9 * OMEGA, CYCLES, 0.7)]
+    gb = gade.Propagator(d, LMAX, dt, batch=2)
+    xb = gade.DeviceArray.from_host(np.concatenate([psi, psi]))
+    it_b, tol_b = gb.cn_run_batch(xb, params, k, t0=t0, tol=1e-12)
+    xs = xb.get().reshape(2, -1)
+    del gb, xb
+    for b in range(2):
+        g1 = gade.Propagator(d, LMAX, dt, batch=1)
+        x1 = gade.DeviceArray.from_host(psi)
+        it1, tol1 = g1.cn_run_batch(x1, [params[b]], k, t0=t0, tol=1e-12)
+        assert np.array_equal(xs[b].view(np.float64), x1.get().view(np.float64)), b
+        assert np.array_equal(it_b[b], it1[0]) and np.array_equal(tol_b[b], tol1[0]), b
+        del g1, x1
+    # the reference's pulse through the single-instance model (its BiCGSTAB workspace, which persists across
+    # calls as the reference's does, cleared first like a fresh model's): the same trajectory
+    gp.reset_workspace()
+    x = gade.DeviceArray.from_host(psi)
+    it, tols, _ = gp.cn_run(x, E0, OMEGA, CYCLES, k, t0=t0, tol=1e-12, timing=False)
+    assert np.array_equal(xs[0].view(np.float64), x.get().view(np.float64))
+    assert not np.array_equal(xs[0], xs[1])
