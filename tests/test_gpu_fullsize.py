@@ -11,6 +11,8 @@ Bars:
   * 100 device-driven steps of the timed window: every step at rel-tol 1e-12, S-norm kept to 1e-10
   * time reversal: five steps of the pulse, then five with -dt seeing the same H(t), return psi to 1e-10
   * a batched scan of two pulses: each instance equals its own single-instance run bit for bit
+  * 20 mid-pulse steps: the step graph equals the launch-by-launch run bit for bit; the extrapolated guess
+    lands within 1e-10 of it and mixed precision within 1e-8 (every step at rel-tol 1e-12)
 """
 import sys
 
@@ -154,3 +156,32 @@ def test_fullsize_batch_equals_single(full):
     it, tols, _ = gp.cn_run(x, E0, OMEGA, CYCLES, k, t0=t0, tol=1e-12, timing=False)
     assert np.array_equal(xs[0].view(np.float64), x.get().view(np.float64))
     assert not np.array_equal(xs[0], xs[1])
+
+
+def test_fullsize_graph_guess_and_mixed(full):
+    """The stepping modes at the production shape, 20 steps from mid-pulse on a fresh model."""
+    d, dt, om, gp, psi = full
+    say("full-size graph / extrapolation / mixed, 20 steps each")
+    gm = gade.Propagator(d, LMAX, dt)
+    t0, k = NTIME // 2 * dt, 20
+
+    def run(**param):
+        for name, v in param.items():
+            gm.set_param(name, v)
+        gm.reset_workspace()
+        x = gade.DeviceArray.from_host(psi)
+        it, tols, _ = gm.cn_run(x, E0, OMEGA, CYCLES, k, t0=t0, tol=1e-12, timing=False)
+        assert np.all(tols < 1e-12)
+        return x.get(), it
+
+    x_ref, it_ref = run(graph=0, window=4, guess_order=0, precision=0)
+    x_g, it_g = run(graph=1)
+    assert np.array_equal(x_g.view(np.float64), x_ref.view(np.float64)) and np.array_equal(it_g, it_ref)
+    nref = np.linalg.norm(x_ref)
+    x_e, it_e = run(graph=0, guess_order=2)
+    err_e = np.linalg.norm(x_e - x_ref) / nref
+    x_m, it_m = run(guess_order=0, precision=1)
+    err_m = np.linalg.norm(x_m - x_ref) / nref
+    say(f"  extrapolation {err_e:.2e} ({np.mean(it_e):.2f} it/step), mixed {err_m:.2e}, plain {np.mean(it_ref):.2f}")
+    assert err_e <= 1e-10
+    assert err_m <= 1e-8
