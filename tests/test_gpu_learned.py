@@ -2,8 +2,9 @@
 CPU oracle (oracle/lg_oracle.py, with the same bf16 roundings), POD orthonormality, training, and the
 guess inside the solver on held-out pulses (the converged solve at the unchanged tolerance judges it).
 
-Bars: forward outputs within 2e-2 of the output scale; loss within 1e-3 relative and every layer's gradient
-within 3e-2 relative (bf16 rounding points may differ where the accumulation orders differ); POD basis
+Bars: forward outputs within 3e-4 of the output scale; loss within 1e-4 relative and every layer's gradient
+within 5e-3 relative (the oracle rounds to bf16 at the same points; the bars are about ten times the observed
+agreement, which differs only where accumulation orders differ); POD basis
 orthonormal to 1e-12; trained validation loss below the extrapolation's (1.0); every step of the held-out
 scan meets the tolerance and the final states agree with the plain guess's to 1e-9."""
 import numpy as np
@@ -68,7 +69,7 @@ def test_mlp_forward_matches_oracle(datadir, M):
     scale = np.abs(yo).max()
     err = np.abs(y - yo).max()
     say(f"  max |y - y_oracle| = {err:.2e} (scale {scale:.2e})")
-    assert err <= 2e-2 * scale
+    assert err <= 3e-4 * scale
 
 
 def test_mlp_gradient_matches_oracle(datadir):
@@ -90,13 +91,13 @@ def test_mlp_gradient_matches_oracle(datadir):
     lo, dY = LG.loss_and_dY(Y, S)
     go = LG.backward(p, cache, dY.astype(np.float32), quantized=True)
     say(f"  loss {loss:.6f} vs oracle {lo:.6f}")
-    assert abs(loss - lo) <= 1e-3 * abs(lo)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
     for k, (fi, fo) in enumerate(LG.LAYERS):
         wo, bo = LG.OFFS[k]
         for a, b, name in ((g[wo:wo + fo * fi], go[wo:wo + fo * fi], "W"), (g[bo:bo + fo], go[bo:bo + fo], "b")):
             rel = np.linalg.norm(a - b) / np.linalg.norm(b)
             say(f"  layer {k + 1} {name}: relative gradient difference {rel:.2e}")
-            assert rel <= 3e-2, (k, name, rel)
+            assert rel <= 5e-3, (k, name, rel)
 
 
 def test_learned_guess_pipeline_and_held_out_scan(datadir):
