@@ -340,3 +340,44 @@ void csr_comm(const sparse_csr_t *sp, int rank, int nranks)
     (void)sp; (void)rank;
     if (nranks != 1) CSR_FAIL("csr_comm: this build runs on one rank (got %d)\n", nranks);
 }
+
+/* tools/csr.c:609-651 on one rank: the partition is the whole matrix, so the local part is a copy with
+   row_beg 0 and row_end nrows and no halo (every column is local, none is removed).  More ranks: exit(1). */
+void csr_get_partition(sparse_csr_t *out, const sparse_csr_t *sp, int rank, int nranks)
+{
+    if (nranks != 1 || rank != 0) CSR_FAIL("csr_get_partition: this build runs on one rank (got %d)\n", nranks);
+    csr_allocate(out, sp->nrows, sp->ncols, sp->nnz);
+    memcpy(out->Ap, sp->Ap, sizeof(csr_index_t) * ((size_t)sp->nrows + 1));
+    memcpy(out->Ai, sp->Ai, sizeof(csr_index_t) * (size_t)sp->nnz);
+    memcpy(out->Ax, sp->Ax, sizeof(csr_data_t) * (size_t)sp->nnz);
+    if (sp->perm) {
+        out->perm = xmalloc(sizeof(csr_index_t) * (size_t)sp->nrows);
+        memcpy(out->perm, sp->perm, sizeof(csr_index_t) * (size_t)sp->nrows);
+    }
+    out->row_cpu_dist = xmalloc(2 * sizeof(csr_index_t));
+    out->row_cpu_dist[0] = 0;
+    out->row_cpu_dist[1] = sp->nrows;
+    out->npart = 1;
+    out->row_beg = 0;
+    out->row_end = sp->nrows;
+}
+
+/* tools/csr.c:427-439 (its non-MPI message) */
+void csr_print_comm_info(sparse_csr_t *sp, int rank, int nranks, int blkdim)
+{
+    (void)sp; (void)rank; (void)nranks; (void)blkdim;
+    printf("no communication with 1 rank");
+}
+
+/* tools/csr.c:215-225: sp_blk becomes a view of block (row, col) of the blocked matrix sp */
+void csr_block_link(sparse_csr_t *sp_blk, sparse_csr_t *sp, csr_index_t row, csr_index_t col)
+{
+    const csr_index_t cp = find_slot(sp, row, col);
+    if (cp == sp->Ap[row + 1] || sp->Ai[cp] != col)
+        CSR_FAIL("cant insert block: (%d,%d) not present in CSR.", row, col);
+    sp_blk->is_link = 1;
+    sp_blk->Ax = sp->Ax + (size_t)cp * sp->blk_nnz;
+    sp_blk->nrows = sp->blk_dim;
+    sp_blk->ncols = sp->blk_dim;
+    sp_blk->nnz = sp->blk_nnz;
+}

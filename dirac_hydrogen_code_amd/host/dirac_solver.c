@@ -80,7 +80,7 @@ static void gpu_HS_spmv_fun(const void *mat, gpu_dense_vec_t *x, gpu_dense_vec_t
 
 static void gpu_LU_precond_fun(const void *precond, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x)
 {
-    precond_lu_solve((const slu_LU_t *)precond, rhs, x);
+    precond_lu_solve((const gade_precond_t *)precond, rhs, x);
 }
 
 int main(int argc, char *argv[])
@@ -196,7 +196,7 @@ int main(int argc, char *argv[])
         gpu_init_model_matrices(24 * lmax, g, NULL, h0);
         gpu_compute_row_col(lmax, H, &Hfull_blk, &Hfull);
         compute_stationary_matrices(h, dt, &submatrix, &S_blk, &S, &Hst_blk, &Hst, h0, s0);
-        slu_LU_t lu = compute_preconditioner(&S, &Hst);
+        gade_precond_t lu = compute_preconditioner(&S, &Hst);
         gpu_blas_init();
         gpu_sparse_csr_t gpuS = {0}, gpuHfull = {0};
         gpu_dense_vec_t xgpu = {0}, rhsgpu = {0};

@@ -77,9 +77,9 @@ static csr_index_t infer_blkdim(const sparse_csr_t *A)
     return n;
 }
 
-slu_LU_t compute_preconditioner(const sparse_csr_t *S, const sparse_csr_t *Hst)
+gade_precond_t compute_preconditioner(const sparse_csr_t *S, const sparse_csr_t *Hst)
 {
-    slu_LU_t out = {0};
+    gade_precond_t out = {0};
     if (S->nnz != Hst->nnz) SOLVER_FAIL("compute_preconditioner: S and Hst patterns differ\n");
     csr_data_t *P = malloc(sizeof(csr_data_t) * (size_t)(Hst->nnz ? Hst->nnz : 1));
     if (!P) SOLVER_FAIL("compute_preconditioner: out of memory\n");
@@ -93,7 +93,7 @@ slu_LU_t compute_preconditioner(const sparse_csr_t *S, const sparse_csr_t *Hst)
     return out;
 }
 
-void precond_lu_solve(const slu_LU_t *lu, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x)
+void precond_lu_solve(const gade_precond_t *lu, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x)
 {
     double *out = (double *)(x->desc_local ? x->x + x->local_offset : x->x);
     if (gade_blocklu_solve(lu->lu, (const double *)rhs->x, out)) SOLVER_FAIL("precond_lu_solve: %s\n", gade_last_error());

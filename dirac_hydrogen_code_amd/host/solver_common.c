@@ -50,6 +50,8 @@ void beoyndDipolePulse_axialPart(beyondDipolePulse_t *pulse, double t, cdouble_t
     out[5] = (E0 / (2 * w) * sin(phi + t * w));
 }
 
+/* the native API's copies live in libgade.so only (libgade_host.so is built with GADE_HOST_LIB) */
+#ifndef GADE_HOST_LIB
 int gade_ik(int i) { return ik(i); }
 double gade_imu(int i) { return imu(i); }
 
@@ -84,3 +86,4 @@ void gade_pulse(double E0, double omega, double cycles, double t, double *ft)
         ft[2 * a + 1] = cimag(out[a]);
     }
 }
+#endif

@@ -63,6 +63,10 @@ void csr_conj_transpose(sparse_csr_t *out, const sparse_csr_t *in);             
 void csr_spmv(csr_index_t row_l, csr_index_t row_u, const sparse_csr_t *sp, const csr_data_t *x, csr_data_t *result); /* tools/csr.c:974-1003 */
 void csr_coo2csr(sparse_csr_t *out, const csr_index_t *rowidx, const csr_index_t *colidx, const csr_data_t *val,
                  csr_index_t matrix_dim, csr_index_t nnz);                                    /* tools/csr.c:1027-1122 */
+/* one rank: the local partition is the whole matrix (tools/csr.c:609-651); exit(1) for more ranks */
+void csr_get_partition(sparse_csr_t *out, const sparse_csr_t *sp, int rank, int nranks);
+void csr_print_comm_info(sparse_csr_t *sp, int rank, int nranks, int blkdim);                   /* tools/csr.c:427-439 */
+void csr_block_link(sparse_csr_t *sp_blk, sparse_csr_t *sp, csr_index_t row, csr_index_t col);  /* tools/csr.c:215-225 */
 /* one rank: the halo exchange of tools/csr.c:858-943 has nothing to do */
 void csr_init_communication(sparse_csr_t *sp, csr_data_t *px, int rank, int nranks);
 void csr_comm(const sparse_csr_t *sp, int rank, int nranks);

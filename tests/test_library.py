@@ -1,5 +1,7 @@
-"""The C-ABI library: it loads, exports every entry point that include/*.h
-declares, and its host-side pieces agree with the oracle (no GPU calls)."""
+"""The C-ABI libraries: libgade.so (HIP: the drop-in gpu_* entry points and the native gade_* API) and
+libgade_host.so (the reference's CPU entry points: csr_*, compute_*, matfree_*, bicgstab, ik/imu, the pulse)
+load, export every entry point include/*.h declares and nothing else, lay out the shared structs as the
+reference's own headers do, and their host-side pieces agree with the oracle (no GPU calls)."""
 import ctypes as C
 import glob
 import os
@@ -12,11 +14,22 @@ import oracle as O
 from dirac_hydrogen_code_amd import gade
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST_LIB = os.path.join(ROOT, "dirac_hydrogen_code_amd", "libgade_host.so")
+# headers whose functions libgade_host.so provides (the rest, and gade_* everywhere, are libgade.so's)
+HOST_HEADERS = ("csr.h", "solver.h", "solver_common.h", "bicgstab.h")
 
 
-def declared_functions():
+def host_lib():
+    if not os.path.exists(HOST_LIB):
+        raise RuntimeError(f"{HOST_LIB} is missing: build with make -C dirac_hydrogen_code_amd")
+    return C.CDLL(HOST_LIB)
+
+
+def declared_functions(headers=None):
     names = set()
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        if headers is not None and os.path.basename(h) not in headers:
+            continue
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         src = re.sub(r"//[^\n]*", "", src)
@@ -33,13 +46,35 @@ def declared_functions():
     return sorted(names)
 
 
+def exported(path):
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", path], check=True, capture_output=True, text=True).stdout
+    return {ln.split()[-1] for ln in out.splitlines() if len(ln.split()) == 3 and ln.split()[1] in "TWDBRVi"}
+
+
 def test_library_exports_every_declared_symbol():
-    L = gade.lib()
+    L, H = gade.lib(), host_lib()
     names = declared_functions()
     assert len(names) > 25
-    missing = [n for n in names if not hasattr(L, n)]
-    assert not missing, missing
+    host_names = set(declared_functions(HOST_HEADERS))
+    for n in names:
+        lib = H if (n in host_names and not n.startswith("gade_")) else L
+        assert hasattr(lib, n), n
     assert set(gade.EXPORTS) <= set(names)
+
+
+def test_export_sets_are_exact():
+    """libgade.so (the replacement of the reference's HIP objects) exports only gpu_* and gade_* names, so a
+    driver that keeps the reference's own csr.o / solver.o / solver_common.o binds its calls to those;
+    libgade_host.so exports the reference's CPU names and no gade_* duplicate."""
+    gpu = exported(gade.LIB_PATH)
+    host = exported(HOST_LIB)
+    stray = sorted(n for n in gpu if not (n.startswith("gpu_") or n.startswith("gade_")))
+    assert not stray, stray
+    want_gpu = {n for n in declared_functions() if n.startswith("gpu_") or n.startswith("gade_")}
+    assert want_gpu <= gpu, sorted(want_gpu - gpu)
+    want_host = {n for n in declared_functions(HOST_HEADERS) if not n.startswith("gade_")}
+    assert want_host == host, (sorted(want_host - host), sorted(host - want_host))
 
 
 def test_index_maps_and_pulse_match_oracle():
@@ -82,7 +117,7 @@ class SparseCSR(C.Structure):
 @pytest.mark.parametrize("cfg,lmax", [("tiny", 2), ("nondipole_small", 4), ("dipole", 1)])
 def test_host_unblock_bitexact_vs_oracle(datadir, cfg, lmax):
     """The product's closed-form csr_unblock_matrix equals the reference's two-pass algorithm bit for bit."""
-    L = gade.lib()
+    L = host_lib()
     d = datadir(cfg)
     Hall, g = SparseCSR(), SparseCSR()
     L.csr_read.argtypes = [C.c_char_p, C.POINTER(SparseCSR)]
@@ -129,3 +164,48 @@ def test_load_rejects_mismatched_pattern(tmp_path, datadir):
     csr_write(str(d / "s1.csr"), Ap, Ai[:-1], Ax[:-1])
     with pytest.raises(gade.GadeError):
         gade.Propagator(str(d), 2, 1e-3)
+
+
+def build_matfree_check(tmp_path):
+    import subprocess
+    exe = str(tmp_path / "matfree_check")
+    lib_dir = os.path.join(ROOT, "dirac_hydrogen_code_amd")
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "abi", "matfree_check.c"), "-o", exe, "-L" + lib_dir, "-lgade_host",
+                    "-lgade", "-Wl,-rpath," + lib_dir, "-lm"], check=True, capture_output=True, text=True)
+    return exe
+
+
+@pytest.mark.parametrize("cfg,lmax", [("tiny", 2), ("nondipole_small", 4)])
+def test_cpu_matfree_and_bicgstab_match_oracle(datadir, tmp_path, cfg, lmax):
+    """The CPU drop-ins of libgade_host.so, called from C as tools/matrix_tool.c calls the reference's:
+    matfree_spmv_H / matfree_spmv_S equal the oracle's assembled H(t) x and S x (the reference's own check,
+    tools/matrix_tool.c:470-496, 560-578) to 1e-13; the CPU bicgstab solves (S + H) z = (S - H) x to its
+    tolerance (true residual from the oracle's operator) and matches the oracle's solve to 1e-9."""
+    import subprocess
+    from conftest import SMALL, dt_of
+    exe = build_matfree_check(tmp_path)
+    d = datadir(cfg)
+    c = SMALL[cfg]
+    dt = dt_of(cfg)
+    t = 0.37 * c["cycles"] * 2 * np.pi / c["omega"]
+    out = str(tmp_path / "out.bin")
+    tol = 1e-13
+    subprocess.run([exe, d, str(lmax), repr(dt), repr(c["intensity"]), repr(c["omega"]), repr(c["cycles"]), repr(t),
+                    repr(tol), out], check=True, capture_output=True, text=True)
+    raw = open(out, "rb").read()
+    n = int(np.frombuffer(raw, np.int32, 1)[0])
+    iters = int(np.frombuffer(raw, np.int32, 1, 4)[0])
+    terr = float(np.frombuffer(raw, np.float64, 1, 8)[0])
+    x, yH, yS, z = (np.frombuffer(raw, np.complex128, n, 16 + 16 * n * k) for k in range(4))
+    m = O.Model(d, lmax, dt)
+    assert m.n == n
+    m.timedep(O.pulse(c["intensity"], c["omega"], c["cycles"], t))
+    hx, sx = m.apply(x, "H"), m.apply(x, "S")
+    assert np.linalg.norm(yH - hx) <= 1e-13 * np.linalg.norm(hx)
+    assert np.linalg.norm(yS - sx) <= 1e-13 * np.linalg.norm(sx)
+    rhs = sx - hx
+    res = np.linalg.norm(m.apply(z, "HS") - rhs) / np.linalg.norm(rhs)
+    assert 0 < iters < 2000 and terr <= tol and res <= 10 * tol, (iters, terr, res)
+    zo, _, _, _ = m.bicgstab(rhs, x, tol=1e-13)
+    assert np.linalg.norm(z - zo) <= 1e-9 * np.linalg.norm(zo)
