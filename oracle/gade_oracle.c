@@ -199,6 +199,7 @@ void or_unblock(int nblk, const int *bAp, const int *bAi, int blkdim, const int 
 /* ------------------------------------------------------------------ */
 struct or_model {
     int lmax, nthreads;
+    int all_cores; /* 1: SpMV rows and preconditioner blocks also run on nthreads (same results) */
     double h, dt;
     ocplx ihdt;
     or_csr_t Hall;
@@ -482,14 +483,34 @@ void or_model_timedep(or_model_t *m, const ocplx ft[6])
     }
 }
 
+/* Rows are independent and each row sums in the same order, so the threaded form of the all-cores
+   baseline gives the serial result bit for bit. */
+static void spmv_rows(const or_model_t *m, const int *Ap, const int *Ai, const ocplx *Ax, const ocplx *x,
+                      ocplx *out)
+{
+    int n = m->nblk * m->blkdim;
+    if (!m->all_cores || m->nthreads <= 1) {
+        or_csr_spmv(0, n, Ap, Ai, Ax, x, out);
+        return;
+    }
+#pragma omp parallel for num_threads(m->nthreads) schedule(static, 256)
+    for (int i = 0; i < n; i++) or_csr_spmv(i, i + 1, Ap, Ai, Ax, x, out);
+}
+
 void or_model_spmv_H(const or_model_t *m, const ocplx *x, ocplx *out)
 {
-    or_csr_spmv(0, m->nblk * m->blkdim, m->Ap, m->Ai, m->Ax, x, out);
+    spmv_rows(m, m->Ap, m->Ai, m->Ax, x, out);
 }
 
 void or_model_spmv_S(const or_model_t *m, const ocplx *x, ocplx *out)
 {
-    or_csr_spmv(0, m->nblk * m->blkdim, m->SAp, m->SAi, m->SAx, x, out);
+    spmv_rows(m, m->SAp, m->SAi, m->SAx, x, out);
+}
+
+void or_model_set_threads(or_model_t *m, int nthreads, int all_cores)
+{
+    if (nthreads > 0) m->nthreads = nthreads;
+    m->all_cores = all_cores ? 1 : 0;
 }
 
 /* tools/dirac_solver.c:115-126 */
@@ -603,29 +624,43 @@ int or_model_precond_setup(or_model_t *m)
     return 0;
 }
 
-void or_model_psolve(const or_model_t *m, const ocplx *rhs, ocplx *x)
+static void psolve_block(const or_model_t *m, int r, const ocplx *rhs, ocplx *x, ocplx *yv)
 {
     int bd = m->blkdim, band = m->band, w = 2 * band + 1;
-    ocplx *yv = malloc(sizeof(ocplx) * bd);
-    for (int r = 0; r < m->nblk; r++) {
-        const ocplx *A = m->lu + (size_t)m->kidx[r] * bd * w;
-        const ocplx *b = rhs + (size_t)r * bd;
-        ocplx *xo = x + (size_t)r * bd;
-        for (int i = 0; i < bd; i++) yv[m->ilv[i]] = b[i];
-        for (int i = 0; i < bd; i++) {
-            ocplx acc = yv[i];
-            for (int j = (i - band > 0 ? i - band : 0); j < i; j++) acc -= A[(size_t)i * w + (j - i + band)] * yv[j];
-            yv[i] = acc;
-        }
-        for (int i = bd - 1; i >= 0; i--) {
-            ocplx acc = yv[i];
-            int jend = i + band < bd - 1 ? i + band : bd - 1;
-            for (int j = i + 1; j <= jend; j++) acc -= A[(size_t)i * w + (j - i + band)] * yv[j];
-            yv[i] = acc / A[(size_t)i * w + band];
-        }
-        for (int i = 0; i < bd; i++) xo[i] = yv[m->ilv[i]];
+    const ocplx *A = m->lu + (size_t)m->kidx[r] * bd * w;
+    const ocplx *b = rhs + (size_t)r * bd;
+    ocplx *xo = x + (size_t)r * bd;
+    for (int i = 0; i < bd; i++) yv[m->ilv[i]] = b[i];
+    for (int i = 0; i < bd; i++) {
+        ocplx acc = yv[i];
+        for (int j = (i - band > 0 ? i - band : 0); j < i; j++) acc -= A[(size_t)i * w + (j - i + band)] * yv[j];
+        yv[i] = acc;
     }
-    free(yv);
+    for (int i = bd - 1; i >= 0; i--) {
+        ocplx acc = yv[i];
+        int jend = i + band < bd - 1 ? i + band : bd - 1;
+        for (int j = i + 1; j <= jend; j++) acc -= A[(size_t)i * w + (j - i + band)] * yv[j];
+        yv[i] = acc / A[(size_t)i * w + band];
+    }
+    for (int i = 0; i < bd; i++) xo[i] = yv[m->ilv[i]];
+}
+
+void or_model_psolve(const or_model_t *m, const ocplx *rhs, ocplx *x)
+{
+    int bd = m->blkdim;
+    if (!m->all_cores || m->nthreads <= 1) {
+        ocplx *yv = malloc(sizeof(ocplx) * bd);
+        for (int r = 0; r < m->nblk; r++) psolve_block(m, r, rhs, x, yv);
+        free(yv);
+        return;
+    }
+#pragma omp parallel num_threads(m->nthreads)
+    {
+        ocplx *yv = malloc(sizeof(ocplx) * bd);
+#pragma omp for schedule(static)
+        for (int r = 0; r < m->nblk; r++) psolve_block(m, r, rhs, x, yv);
+        free(yv);
+    }
 }
 
 /* ------------------------------------------------------------------ */

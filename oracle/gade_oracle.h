@@ -58,6 +58,10 @@ typedef struct or_model or_model_t;
 /* Loads H.csr, H0l/H1l, g, h0, s from dir (tools/dirac_solver.c:223-370) and
    builds Hfull/S structure.  Returns NULL on error (message on stderr). */
 or_model_t *or_model_load(const char *dir, int lmax, double h, double dt, int nthreads);
+/* threads of the per-step assembly (nthreads > 0), and the all-cores baseline: the SpMV rows and the
+   preconditioner blocks also run on them (the reference runs them serially); results are the serial ones
+   bit for bit */
+void or_model_set_threads(or_model_t *m, int nthreads, int all_cores);
 void or_model_free(or_model_t *m);
 void or_model_info(const or_model_t *m, int *nblk, int *blkdim, int *hall_nnz, long long *nnz_full,
                    int *nnz_b);

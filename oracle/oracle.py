@@ -40,6 +40,7 @@ def lib():
         L.or_model_load.restype = C.c_void_p
         L.or_model_load.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int]
         L.or_model_free.argtypes = [C.c_void_p]
+        L.or_model_set_threads.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.or_model_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3 + [C.POINTER(C.c_longlong),
                                                                               C.POINTER(C.c_int)]
         for name in ("or_model_hfull_Ap", "or_model_hfull_Ai", "or_model_hfull_submap",
@@ -96,11 +97,14 @@ def pulse(E0, omega, cycles, t):
 class Model:
     """The reference CPU path on one input directory."""
 
-    def __init__(self, directory, lmax, dt, h=1.0, nthreads=1):
+    def __init__(self, directory, lmax, dt, h=1.0, nthreads=1, all_cores=False):
+        """nthreads: OpenMP threads of the per-step assembly (the reference's only threaded part); with
+        all_cores the SpMV rows and preconditioner blocks use them too (same results bit for bit)."""
         L = lib()
         self.ptr = L.or_model_load(directory.encode(), lmax, h, dt, nthreads)
         if not self.ptr:
             raise RuntimeError(f"oracle failed to load {directory}")
+        L.or_model_set_threads(self.ptr, nthreads, 1 if all_cores else 0)
         nb, bd, hn, nz = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         nf = C.c_longlong()
         L.or_model_info(self.ptr, C.byref(nb), C.byref(bd), C.byref(hn), C.byref(nf), C.byref(nz))

@@ -371,3 +371,18 @@ def test_populations_ground_state_and_completeness(datadir):
     q = O.populations(d, x)
     assert abs(q["bound"] + q["ionised"] + q["negative"] - q["norm"]) <= 1e-11 * q["norm"]
     assert q["ionised"] > 0 and q["negative"] > 0
+
+
+def test_all_cores_baseline_equals_serial(datadir):
+    """The all-cores CPU baseline (SpMV rows and preconditioner blocks on every thread) computes the serial
+    trajectory bit for bit: rows and blocks are independent and each sums in the same order."""
+    cfg = "nondipole_small"
+    d = datadir(cfg)
+    out = []
+    for all_cores in (False, True):
+        m = O.Model(d, 4, dt_of(cfg), nthreads=4, all_cores=all_cores)
+        psi, _ = O.read_psi0_vec(d + "/psi0.vec", m.n)
+        x, iters, tols, _ = m.cn_run(psi, 10.0, 50.0, 2.0, 8, tol=1e-12)
+        out.append((x, iters, tols))
+    assert np.array_equal(out[0][0].view(np.float64), out[1][0].view(np.float64))
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
