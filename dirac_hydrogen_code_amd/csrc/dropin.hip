@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "gade_internal.h"
@@ -147,25 +148,120 @@ void engine_init(long long N) {
 }
 
 template <int OP>
-void eop(long long N, const double2 *b, gpu_solver_workspace_t *w, double2 *x, int off) {
-    const int np = (int)std::min<long long>(g_engine.nparts, (N + kRedThreads - 1) / kRedThreads);
+void eop(const Engine &e, long long N, const double2 *b, gpu_solver_workspace_t *w, double2 *x, int off) {
+    const int np = (int)std::min<long long>(e.nparts, (N + kRedThreads - 1) / kRedThreads);
     hipLaunchKernelGGL(k_bicg<OP>, dim3(np), dim3(kRedThreads), 0, 0, N, b, (double2 *)w->r.x, (double2 *)w->r0.x,
                        (double2 *)w->p.x, (const double2 *)w->v.x, (double2 *)w->s.x, (const double2 *)w->t.x,
-                       (const double2 *)w->y.x + off, (const double2 *)w->z.x + off, x, g_engine.d_sc,
-                       g_engine.d_part);
+                       (const double2 *)w->y.x + off, (const double2 *)w->z.x + off, x, e.d_sc, e.d_part);
     if (OP != OP_P && OP != OP_S)
-        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(1), dim3(kRedThreads), 0, 0, np, g_engine.d_part, g_engine.d_sc);
+        hipLaunchKernelGGL(k_bicg_fin<OP>, dim3(1), dim3(kRedThreads), 0, 0, np, e.d_part, e.d_sc);
     CHECK_HIP(hipGetLastError());
 }
 
-void read_sc() {
-    CHECK_HIP(hipMemcpy(g_engine.h_sc, g_engine.d_sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost));
+void read_sc(const Engine &e) {
+    CHECK_HIP(hipMemcpy(e.h_sc, e.d_sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost));
+}
+
+// The BiCGSTAB of tools/gpu_bicgstab.c:133-307 on the workspace w with the scalars and partials of e.
+void bicgstab_run(gpu_spmv_fun spmv, const void *mat, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x, int nrow,
+                  int local_col_beg, gpu_precond_fun psolve, const void *precond, gpu_solver_workspace_t *wsp,
+                  const Engine &e, int *iters, double *tol_error) {
+    const double tol = *tol_error;
+    const int maxit = *iters;
+    const long long N = nrow;
+    const csr_data_t one(1, 0), zero(0, 0);
+    double2 *xl = (double2 *)(x->x + local_col_beg);
+    const double2 *b = (const double2 *)rhs->x;
+    // r = rhs - A x, r0 = r
+    spmv(mat, x, &wsp->r, one, zero);
+    eop<OP_INIT>(e, N, b, wsp, xl, local_col_beg);
+    read_sc(e);
+    double r0_sq = e.h_sc[SC_R0SQ];
+    const double rhs_sq = e.h_sc[SC_RHSSQ];
+    if (rhs_sq == 0) {
+        CHECK_HIP(hipMemset(x->x, 0, sizeof(csr_data_t) * x->dim));
+        *iters = 0;
+        *tol_error = 0;
+        return;
+    }
+    const double tol2 = tol * tol * rhs_sq, eps2 = 1e-12 * 1e-12;
+    if (r0_sq < tol2) {
+        *iters = 0;
+        *tol_error = std::sqrt(r0_sq / rhs_sq);
+        return;
+    }
+    int i = 0;
+    double rnorm2 = r0_sq, rho_abs = r0_sq;
+    while (i < maxit) {
+        if (rho_abs < eps2 * r0_sq) {
+            spmv(mat, x, &wsp->r, one, zero);
+            eop<OP_RESTART>(e, N, b, wsp, xl, local_col_beg);
+            read_sc(e);
+            r0_sq = e.h_sc[SC_R0SQ];
+        }
+        eop<OP_P>(e, N, b, wsp, xl, local_col_beg);
+        psolve(precond, &wsp->p, &wsp->y);
+        spmv(mat, &wsp->y, &wsp->v, one, zero);
+        eop<OP_ALPHA>(e, N, b, wsp, xl, local_col_beg);
+        eop<OP_S>(e, N, b, wsp, xl, local_col_beg);
+        psolve(precond, &wsp->s, &wsp->z);
+        spmv(mat, &wsp->z, &wsp->t, one, zero);
+        eop<OP_W>(e, N, b, wsp, xl, local_col_beg);
+        eop<OP_XR>(e, N, b, wsp, xl, local_col_beg);
+        read_sc(e);
+        rnorm2 = e.h_sc[SC_RNORM2];
+        rho_abs = std::hypot(e.h_sc[SC_RHONEXT], e.h_sc[SC_RHONEXT + 1]);
+        if (rnorm2 < tol2) break;
+        ++i;
+    }
+    *tol_error = std::sqrt(rnorm2 / rhs_sq);
+    *iters = i;
 }
 
 void alloc_vec(gpu_dense_vec_t *v, int n, int nrow, int off, bool local) {
     if (v->x) return;
     gpu_put_vec(v, nullptr, n);
     if (local) gpu_vec_local_part(v, nrow, off);
+}
+
+// Scratch of gpu_bicgstab_noalloc: the eight BiCGSTAB vectors (kNoAllocRows complex entries each), then the
+// scalars and the reduction partials.  A file-scope device array, so the entry point never allocates; every
+// GPU the library runs on holds its own copy until the process ends.
+constexpr long long kNoAllocRows = 1 << 18;              // 262,144 rows: 32 MiB of vectors
+constexpr int kNoAllocParts = 1024;                       // >= the k_bicg grid for kNoAllocRows rows
+constexpr long long kNoAllocTail = 32 + 2 * kNoAllocParts;  // SC scalars (64 doubles), partials (4 per block)
+static_assert((kNoAllocRows + kRedThreads - 1) / kRedThreads <= kNoAllocParts, "partials of the largest solve");
+static __device__ __align__(256) double2 scratch[8 * kNoAllocRows + kNoAllocTail];
+
+constexpr int kMaxDevices = 64;
+double2 *g_scratch_ptr[kMaxDevices] = {};  // device address of 'scratch' on each device (host-side name: stand-in)
+std::mutex g_scratch_mu;
+
+// the device address of this device's scratch array, looked up once per device with that device current
+int scratch_of_current_device(double2 **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        gade::set_error("gpu_bicgstab_noalloc: no current HIP device (or device index beyond 63)");
+        return GADE_EHIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            gade::set_error(std::string("gpu_bicgstab_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                            hipGetErrorString(e));
+            return GADE_EHIP;
+        }
+        g_scratch_ptr[dev] = (double2 *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GADE_OK;
+}
+
+__global__ __launch_bounds__(256) void k_zero2(double2 *__restrict__ p, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = make_double2(0, 0);
 }
 
 // one model per process, as the reference keeps its matrices in file-static globals
@@ -325,10 +421,7 @@ void gpu_blas_init(void) { engine_init(1); }
 void gpu_bicgstab(gpu_spmv_fun spmv, const void *mat, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x, int nrow, int ncol,
                   int local_col_beg, gpu_precond_fun psolve, const void *precond, gpu_solver_workspace_t *wsp,
                   int *iters, double *tol_error) {
-    const double tol = *tol_error;
-    const int maxit = *iters;
-    const long long N = nrow;
-    engine_init(N);
+    engine_init(nrow);
     alloc_vec(&wsp->v, nrow, nrow, 0, false);
     alloc_vec(&wsp->p, nrow, nrow, 0, false);
     alloc_vec(&wsp->y, ncol, nrow, local_col_beg, true);
@@ -337,53 +430,51 @@ void gpu_bicgstab(gpu_spmv_fun spmv, const void *mat, const gpu_dense_vec_t *rhs
     alloc_vec(&wsp->t, nrow, nrow, 0, false);
     alloc_vec(&wsp->r, nrow, nrow, 0, false);
     alloc_vec(&wsp->r0, nrow, nrow, 0, false);
-    const csr_data_t one(1, 0), zero(0, 0);
-    double2 *xl = (double2 *)(x->x + local_col_beg);
-    const double2 *b = (const double2 *)rhs->x;
-    // r = rhs - A x, r0 = r
-    spmv(mat, x, &wsp->r, one, zero);
-    eop<OP_INIT>(N, b, wsp, xl, local_col_beg);
-    read_sc();
-    double r0_sq = g_engine.h_sc[SC_R0SQ];
-    const double rhs_sq = g_engine.h_sc[SC_RHSSQ];
-    if (rhs_sq == 0) {
-        CHECK_HIP(hipMemset(x->x, 0, sizeof(csr_data_t) * x->dim));
-        *iters = 0;
-        *tol_error = 0;
-        return;
+    bicgstab_run(spmv, mat, rhs, x, nrow, local_col_beg, psolve, precond, wsp, g_engine, iters, tol_error);
+}
+
+int gpu_bicgstab_noalloc(gpu_spmv_fun spmv, const void *mat, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x, int nrow,
+                         int ncol, int local_col_beg, gpu_precond_fun psolve, const void *precond, int *iters,
+                         double *tol_error) {
+    if (!spmv || !psolve || !rhs || !x || !iters || !tol_error || nrow < 0 || ncol < nrow || local_col_beg < 0 ||
+        local_col_beg + nrow > ncol) {
+        gade::set_error("gpu_bicgstab_noalloc: invalid arguments");
+        return GADE_EINVAL;
     }
-    const double tol2 = tol * tol * rhs_sq, eps2 = 1e-12 * 1e-12;
-    if (r0_sq < tol2) {
-        *iters = 0;
-        *tol_error = std::sqrt(r0_sq / rhs_sq);
-        return;
+    if (ncol > kNoAllocRows) {
+        gade::set_error("gpu_bicgstab_noalloc: " + std::to_string(ncol) + " columns exceed the scratch maximum of " +
+                        std::to_string(kNoAllocRows) + " (use gpu_bicgstab)");
+        return GADE_EINVAL;
     }
-    int i = 0;
-    double rnorm2 = r0_sq, rho_abs = r0_sq;
-    while (i < maxit) {
-        if (rho_abs < eps2 * r0_sq) {
-            spmv(mat, x, &wsp->r, one, zero);
-            eop<OP_RESTART>(N, b, wsp, xl, local_col_beg);
-            read_sc();
-            r0_sq = g_engine.h_sc[SC_R0SQ];
-        }
-        eop<OP_P>(N, b, wsp, xl, local_col_beg);
-        psolve(precond, &wsp->p, &wsp->y);
-        spmv(mat, &wsp->y, &wsp->v, one, zero);
-        eop<OP_ALPHA>(N, b, wsp, xl, local_col_beg);
-        eop<OP_S>(N, b, wsp, xl, local_col_beg);
-        psolve(precond, &wsp->s, &wsp->z);
-        spmv(mat, &wsp->z, &wsp->t, one, zero);
-        eop<OP_W>(N, b, wsp, xl, local_col_beg);
-        eop<OP_XR>(N, b, wsp, xl, local_col_beg);
-        read_sc();
-        rnorm2 = g_engine.h_sc[SC_RNORM2];
-        rho_abs = std::hypot(g_engine.h_sc[SC_RHONEXT], g_engine.h_sc[SC_RHONEXT + 1]);
-        if (rnorm2 < tol2) break;
-        ++i;
+    double2 *base = nullptr;
+    GADE_TRY(scratch_of_current_device(&base));
+    // the eight vectors at fixed slots, then the scalars and partials; a fresh workspace is all zeros
+    gpu_solver_workspace_t w;
+    memset(&w, 0, sizeof w);
+    gpu_dense_vec_t *slots[8] = {&w.r, &w.r0, &w.v, &w.p, &w.y, &w.z, &w.s, &w.t};
+    for (int k = 0; k < 8; k++) {
+        const bool local = slots[k] == &w.y || slots[k] == &w.z;
+        slots[k]->x = (gpu_complex_t *)(base + (size_t)k * kNoAllocRows);
+        slots[k]->dim = local ? ncol : nrow;
+        slots[k]->local_dim = local ? nrow : 0;  // as gpu_put_vec leaves a zeroed workspace's vectors
+        slots[k]->local_offset = local ? local_col_beg : 0;
+        slots[k]->desc_local = local ? (void *)slots[k] : nullptr;  // marks the local view, as gpu_vec_local_part
     }
-    *tol_error = std::sqrt(rnorm2 / rhs_sq);
-    *iters = i;
+    double h_sc[SC_STRIDE];
+    Engine e;
+    e.d_sc = (double *)(base + 8 * kNoAllocRows);
+    e.d_part = e.d_sc + 64;
+    e.h_sc = h_sc;
+    e.nparts = kNoAllocParts;
+    for (int k = 0; k < 8; k++) {
+        const long long n = slots[k]->dim;
+        if (n) hipLaunchKernelGGL(k_zero2, dim3((unsigned)std::min<long long>(1024, (n + 255) / 256)), dim3(256), 0, 0,
+                                  (double2 *)slots[k]->x, n);
+    }
+    hipLaunchKernelGGL(k_zero2, dim3(8), dim3(256), 0, 0, (double2 *)e.d_sc, kNoAllocTail);
+    GADE_HIP(hipGetLastError());
+    bicgstab_run(spmv, mat, rhs, x, nrow, local_col_beg, psolve, precond, &w, e, iters, tol_error);
+    return GADE_OK;
 }
 
 // ---------------------------------------------------------------- gpu_solver.h

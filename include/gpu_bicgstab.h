@@ -36,6 +36,18 @@ void gpu_bicgstab(gpu_spmv_fun spmv, const void *mat, const gpu_dense_vec_t *rhs
                   int local_col_beg, gpu_precond_fun psolve, const void *precond, gpu_solver_workspace_t *wsp, int *iters,
                   double *tol_error);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same solve as gpu_bicgstab called with a
+ * fresh (zeroed) workspace, with the workspace, scalars and reduction partials in a fixed device array of the
+ * library instead of allocated memory (32 MiB per GPU, held until the process ends).  It calls no hipMalloc,
+ * hipFree or allocating routine; the callbacks are the caller's.  Limit: ncol <= 262144 (2^18) rows; larger
+ * sizes return GADE_EINVAL (-1) with the reason in gade_last_error() and nothing is launched.  Every call on one
+ * GPU uses that GPU's one array and starts by clearing it: calls that overlap on one GPU (other threads or
+ * streams) race.  Returns 0 on success. */
+int gpu_bicgstab_noalloc(gpu_spmv_fun spmv, const void *mat, const gpu_dense_vec_t *rhs, gpu_dense_vec_t *x, int nrow,
+                         int ncol, int local_col_beg, gpu_precond_fun psolve, const void *precond, int *iters,
+                         double *tol_error);
+
 #ifdef __cplusplus
 }
 #endif
