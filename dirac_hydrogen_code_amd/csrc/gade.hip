@@ -792,6 +792,244 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
 #undef DIA_EMIT
 }
 
+// ---------------------------------------------------------------------------
+// fused operator, phase B on the FP64 matrix cores (v_mfma_f64_4x4x4_4b_f64)
+//
+// One wave owns 16 rows of the banded radial product in two halves of 8 rows, each half of one kappa, one
+// output half h, one multipole l and a run of NIT "i-steps" of 4 consecutive outputs.  The MFMA's four
+// blocks b are four groups of 4 rows; within a block, M = 4 outputs (ii), N = 4 rows (j), K = 4 window
+// positions (kk).  Lanes: A lane = kk*16 + b*4 + ii, B lane = kk*16 + b*4 + j, D lane = ii*16 + b*4 + j
+// (measured on gfx950).  For i-step mi and K-step s (s = 0..4), block b computes
+//   D[ii][j] += sum_kk M[4 mi + ii][d = 4 s + kk - ii] * V[row_j][4 mi + 4 s + kk - 7]
+// over the band d = 0..14 (M = 0 outside it): 20 window positions for the 18 an i-step needs.  Every lane
+// loads and folds its own A entry (M = C1 + kappa C2 with its half's kappa; blocks 0-1 and 2-3 hold equal
+// A values), so the instruction runs without A broadcast (cbsz 0: with cbsz 1 the chained products come
+// out wrong on gfx950).  The B fragment of (mi, s) is the fragment of
+// (mi + 1, s - 1): each wave loads every window fragment once and keeps five in a ring.  Complex products
+// are four real MFMAs: Re += Mr Vr + Mi (-Vi), Im += Mr Vi + Mi Vr.
+// The coefficients come in fragment order: GF[l][v][qd][mi][s][e = kk*4 + ii][C1re, C1im, C2re, C2im]
+// (zero outside the band), so the A loads of a wave are 512 consecutive bytes; the stationary operators
+// D_sel(kappa) likewise as DF[sel][q][qd][mi][s][e] (complex).
+// ---------------------------------------------------------------------------
+constexpr int kMfmaNIT = 8;     // i-steps (4 outputs each) per wave
+constexpr int kMfmaFrag = 16;   // entries per coefficient fragment (kk, ii)
+
+// GF[b][l][v][qd][mi][s][e] from the DIA-ordered couplings: the (G, G') pair of k_gsums_dia at
+// (l, v, qd, d = 4 s + kk - ii, i = 4 mi + ii), zero outside the band
+__global__ __launch_bounds__(256) void k_gsums_frag(const double2 *__restrict__ gs, const double2 *__restrict__ gts,
+                                                    CoefGroup cg, int b0, size_t gf_stride, int lmax, int nd, int nhp,
+                                                    int nmi, double *__restrict__ GF0, const int *gate, int gstride,
+                                                    const Coef6 *__restrict__ cf_dev, const int *__restrict__ gctl,
+                                                    int B) {
+    const int b = b0 + blockIdx.y;
+    GADE_GATED_B(gate, gstride, b);
+    const Coef6 cf = cf_dev ? cf_dev[(size_t)gctl[GC_WSLOT] * B + b] : cg.c[blockIdx.y];
+    double2 c_even[3], c_odd[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        c_even[j] = cf.c[2 * j + 1];
+        c_odd[j] = cf.c[2 * j];
+    }
+    double *__restrict__ GF = GF0 + (size_t)b * gf_stride;
+    const int S = 4 * nd * nhp;                 // entries per (l, j, nn) plane of gs / gts
+    const int F = 4 * nmi * 5 * kMfmaFrag;      // fragment entries per (l, v)
+    const long long total = (long long)lmax * 4 * F;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int lv = (int)(idx / F);
+        int rem = (int)(idx - (long long)lv * F);
+        const int e = rem % kMfmaFrag;
+        rem /= kMfmaFrag;
+        const int sk = rem % 5;
+        rem /= 5;
+        const int mi = rem % nmi;
+        const int qd = rem / nmi;
+        const int ii = e & 3, kk = e >> 2;
+        const int d = 4 * sk + kk - ii, ip = 4 * mi + ii;
+        double2 G1 = make_double2(0, 0), G2 = make_double2(0, 0);
+        if (d >= 0 && d < nd) {
+            const int l = lv >> 2, v = lv & 3;
+            const double2 *src = v < 2 ? gs : gts;
+            const int n1 = v == 0 ? 0 : v == 1 ? 2 : v == 2 ? 0 : 1, n2 = v == 0 ? 1 : v == 1 ? 3 : v == 2 ? 2 : 3;
+            const int ent = (qd * nd + d) * nhp + ip;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double2 c = (l % 2 == 0) ? c_even[j] : c_odd[j];
+                const size_t o = (size_t)(l * 3 + j) * 4 * S + ent;
+                G1 = cfma(c, src[o + (size_t)n1 * S], G1);
+                G2 = cfma(c, src[o + (size_t)n2 * S], G2);
+            }
+        }
+        double *dst = GF + (size_t)idx * 4;
+        dst[0] = G1.x;
+        dst[1] = G1.y;
+        dst[2] = G2.x;
+        dst[3] = G2.y;
+    }
+}
+
+// DF[sel][q][qd][mi][s][e] from DD[sel][q][qd][d][i]
+__global__ void k_build_DF(const double2 *__restrict__ DD, int nq, int nd, int nhp, int nmi, double2 *__restrict__ DF) {
+    const long long total = (long long)3 * nq * 4 * nmi * 5 * kMfmaFrag;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        long long rem = idx;
+        const int e = (int)(rem % kMfmaFrag);
+        rem /= kMfmaFrag;
+        const int sk = (int)(rem % 5);
+        rem /= 5;
+        const int mi = (int)(rem % nmi);
+        const long long sq = rem / nmi;  // (sel * nq + q) * 4 + qd
+        const int ii = e & 3, kk = e >> 2;
+        const int d = 4 * sk + kk - ii, ip = 4 * mi + ii;
+        DF[idx] = (d >= 0 && d < nd) ? DD[((size_t)sq * nd + d) * nhp + ip] : make_double2(0, 0);
+    }
+}
+
+__device__ __forceinline__ double mfma4(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);  // per-block A, no broadcast
+}
+
+template <bool DUAL, int NIT>
+__global__ __launch_bounds__(64) void k_radial_mfma(const double2 *__restrict__ x, const double2 *__restrict__ V,
+                                                    const double *__restrict__ GF, const double2 *__restrict__ DF,
+                                                    const int *__restrict__ grows, const int *__restrict__ gq,
+                                                    const double *__restrict__ gk, const int *__restrict__ gb,
+                                                    int n, int nh, int nmi, int nseg, int lgrid, int nq, int sel,
+                                                    double cH, double beta, double2 *__restrict__ y,
+                                                    double2 *__restrict__ ypart, size_t plane, const int *gate,
+                                                    int gstride, size_t gf_stride, double2 *__restrict__ y2,
+                                                    double2 *__restrict__ ypart2, int sel2) {
+    const int lane = threadIdx.x;
+    int rest = blockIdx.x;
+    const int l = rest % lgrid;
+    rest /= lgrid;
+    const int seg = rest % nseg;
+    rest /= nseg;
+    const int h = rest & 1;
+    const int g = rest >> 1;
+    const int inst = gb[g];
+    GADE_GATED_B(gate, gstride, inst);
+    const int jj = lane & 3, blk = (lane >> 2) & 3, kk = lane >> 4;
+    const int half = blk >> 1;
+    const int row = grows[g * 16 + blk * 4 + jj];      // this lane's B / D row, -1 pads
+    const double kap = gk[g * 2 + half];
+    const int q = gq[g * 2 + half];
+    const int e = kk * 4 + jj;                         // this lane's A entry (jj plays ii)
+    const int mi0 = seg * NIT;
+    const size_t frag = (size_t)5 * kMfmaFrag;         // entries per i-step
+    GF += (size_t)inst * gf_stride;
+
+    double accR[NIT], accI[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; it++) accR[it] = accI[it] = 0.0;
+
+    // acc += M * window over the i-steps of this wave; A entries from coef (fragment order, 'pair': C1 + kap C2
+    // from 4 doubles, else one complex), window values src[0..nh) of this lane's row (nullptr: zero)
+    auto sweep = [&](const double *coef, bool pair, const double2 *src) {
+        double bR[5], bI[5], bN[5];
+        auto load_b = [&](int m, int slot) {
+            const int pos = 4 * m + kk - 7;
+            double2 v = make_double2(0.0, 0.0);
+            if (src && pos >= 0 && pos < nh) v = src[pos];
+            bR[slot] = v.x;
+            bI[slot] = v.y;
+            bN[slot] = -v.y;
+        };
+#pragma unroll
+        for (int k = 0; k < 4; k++) load_b(mi0 + k, k);
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            load_b(mi0 + it + 4, (it + 4) % 5);
+            const int mi = mi0 + it;
+#pragma unroll
+            for (int sk = 0; sk < 5; sk++) {
+                const size_t ent = (size_t)mi * frag + (size_t)sk * kMfmaFrag + e;
+                double mr, mv;
+                if (pair) {
+                    const double4 c = *(const double4 *)(coef + ent * 4);
+                    mr = fma(kap, c.z, c.x);
+                    mv = fma(kap, c.w, c.y);
+                } else {
+                    const double2 c = *(const double2 *)(coef + ent * 2);
+                    mr = c.x;
+                    mv = c.y;
+                }
+                const int sl = (it + sk) % 5;
+                accR[it] = mfma4(mr, bR[sl], accR[it]);
+                accR[it] = mfma4(mv, bN[sl], accR[it]);
+                accI[it] = mfma4(mr, bI[sl], accI[it]);
+                accI[it] = mfma4(mv, bR[sl], accI[it]);
+            }
+        }
+    };
+
+    // D lanes: output row 'row', position 4 (mi0 + it) + (lane >> 4) of half h
+    const int ii_out = lane >> 4;
+    auto emit = [&](double2 *YO, double2 *YPO) {
+        if (row < 0) return;
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int ip = 4 * (mi0 + it) + ii_out;
+            if (ip >= nh) continue;
+            const size_t o = (size_t)row * n + h * nh + ip;
+            double2 r = make_double2(accR[it], accI[it]);
+            if (YPO) {
+                YPO[(size_t)l * plane + o] = r;
+            } else {
+                if (beta != 0.0) r = cadd(r, cscale(YO[o], beta));
+                YO[o] = r;
+            }
+        }
+    };
+
+    if (cH != 0.0) {
+        for (int v = 0; v < 4; v++)
+            for (int hp = 0; hp < 2; hp++) {
+                const double *coef = GF + ((size_t)((l * 4 + v) * 4 + 2 * h + hp) * nmi) * frag * 4;
+                const double2 *src = row >= 0 ? V + (size_t)(l * 4 + v) * plane + (size_t)row * n + hp * nh : nullptr;
+                sweep(coef, true, src);
+            }
+    }
+    if constexpr (DUAL) emit(y2, ypart2);
+    if (cH != 0.0 && cH != 1.0) {
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            accR[it] *= cH;
+            accI[it] *= cH;
+        }
+    }
+    const size_t qplane = (size_t)4 * nmi * frag;  // DF entries per (sel, q)
+    if (l == 0) {
+        for (int hp = 0; hp < 2; hp++) {
+            const double2 *coef = DF + ((size_t)sel * nq + q) * qplane + (size_t)(2 * h + hp) * nmi * frag;
+            sweep((const double *)coef, false, row >= 0 ? x + (size_t)row * n + hp * nh : nullptr);
+        }
+    }
+    emit(y, ypart);
+    if constexpr (DUAL) {
+        if (l == 0) {
+            // reload the parked second output (written by this lane above), then add D_sel2 x
+#pragma unroll
+            for (int it = 0; it < NIT; it++) {
+                const int ip = 4 * (mi0 + it) + ii_out;
+                accR[it] = accI[it] = 0.0;
+                if (row >= 0 && ip < nh) {
+                    const size_t o = (size_t)row * n + h * nh + ip;
+                    const double2 p = ypart2 ? ypart2[o] : y2[o];
+                    accR[it] = p.x;
+                    accI[it] = p.y;
+                }
+            }
+            for (int hp = 0; hp < 2; hp++) {
+                const double2 *coef = DF + ((size_t)sel2 * nq + q) * qplane + (size_t)(2 * h + hp) * nmi * frag;
+                sweep((const double *)coef, false, row >= 0 ? x + (size_t)row * n + hp * nh : nullptr);
+            }
+            emit(y2, ypart2);
+        }
+    }
+}
+
 // per-block S-norm contributions pop[r] = x_r^H S(k_r) x_r
 __global__ __launch_bounds__(256) void k_snorm(const double2 *__restrict__ x, const double2 *__restrict__ D,
                                                const int *__restrict__ ell_col, const int *__restrict__ blk_q,
@@ -951,6 +1189,7 @@ unsigned grid_for(long long total, int threads, unsigned cap = 65535u * 4) {
 }
 
 void free_all(gade_model *m) {
+    if (m->stream) (void)hipStreamSynchronize(m->stream);  // no launch may still read what is freed below
     gade::blocklu_free(m->pre);
     m->pre = nullptr;
     void *ptrs[] = {m->d_ell_lcol, m->d_locj, m->d_nloc, m->d_ell_col, m->d_ell_src, m->d_tmap, m->d_ell_pos, m->d_g, m->d_s, m->d_h0, m->d_GE, m->d_D,
@@ -977,7 +1216,8 @@ void free_all(gade_model *m) {
                     (void *)m->lu32[2], (void *)m->lu32[3], (void *)m->mp_r, (void *)m->mp_r0, (void *)m->mp_p,
                     (void *)m->mp_v, (void *)m->mp_y, (void *)m->mp_z, (void *)m->mp_s, (void *)m->mp_t,
                     (void *)m->mp_d, (void *)m->d_mp_ax, (void *)m->d_sc32, (void *)m->d_aoff, (void *)m->d_ack,
-                    (void *)m->d_acf, (void *)m->d_acf32})
+                    (void *)m->d_acf, (void *)m->d_acf32, (void *)m->d_GF, (void *)m->d_DF, (void *)m->d_mg_rows,
+                    (void *)m->d_mg_q, (void *)m->d_mg_b, (void *)m->d_mg_k})
         if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
@@ -1057,6 +1297,22 @@ int launch_dia(gade_model *m, int lgrid, const T *x, const T *V, const T *GD, co
     return GADE_OK;
 }
 
+// FP64 banded radial phase: the MFMA kernel when enabled, else k_radial_dia with the model's tile shape
+template <bool DUAL>
+int launch_dia64(gade_model *m, int lgrid, const double2 *x, int sel, double cH, double beta, double2 *y,
+                 double2 *yp, double2 *y2, double2 *yp2, int sel2) {
+    if (!m->radial_mfma)
+        return launch_dia<DUAL, double2>(m, lgrid, x, m->d_V, m->d_GD, m->d_DD, sel, cH, beta, y, yp, y2, yp2, sel2);
+    const int nseg = m->nmi / kMfmaNIT;
+    const dim3 grid((unsigned)m->nmg * 2 * nseg * lgrid);
+    hipLaunchKernelGGL((k_radial_mfma<DUAL, kMfmaNIT>), grid, dim3(64), 0, m->stream, x, (const double2 *)m->d_V,
+                       (const double *)m->d_GF, (const double2 *)m->d_DF, m->d_mg_rows, m->d_mg_q, m->d_mg_k,
+                       m->d_mg_b, m->n, m->nh, m->nmi, nseg, lgrid, m->nq, sel, cH, beta, y, yp,
+                       (size_t)m->nblk * m->n, m->gate, m->gstride, m->gf_stride, y2, yp2, sel2);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
 // y2 (banded kernel only): also y2 = (D_sel2 + G-part) x from the same pass, beta 0
 int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double cH, double beta,
                   double2 *y2 = nullptr, int sel2 = 0) {
@@ -1067,12 +1323,10 @@ int launch_radial(gade_model *m, const double2 *x, double2 *y, int sel, double c
         double2 *yp2 = y2 && yp ? m->d_ypart2 : nullptr;  // allocated with the model when lmax > 1
         if (y2) {
             Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
-            GADE_TRY((launch_dia<true, double2>(m, lgrid, x, m->d_V, m->d_GD, m->d_DD, sel, cH, 0.0, y, yp, y2, yp2,
-                                                sel2)));
+            GADE_TRY((launch_dia64<true>(m, lgrid, x, sel, cH, 0.0, y, yp, y2, yp2, sel2)));
         } else {
             Prof pr(m, "radial", GADE_PROF_RADIAL);
-            GADE_TRY((launch_dia<false, double2>(m, lgrid, x, m->d_V, m->d_GD, m->d_DD, sel, cH, beta, y, yp,
-                                                 (double2 *)nullptr, (double2 *)nullptr, 0)));
+            GADE_TRY((launch_dia64<false>(m, lgrid, x, sel, cH, beta, y, yp, nullptr, nullptr, 0)));
         }
         if (yp) {
             Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
@@ -1194,6 +1448,53 @@ int setup_dia(gade_model *m) {
     return GADE_OK;
 }
 
+// MFMA form of the banded radial phase: per instance, the rows of each kappa in halves of 8 (padded), two
+// halves per wave (one instance per wave); the coefficient fragments GF and the stationary fragments DF.
+int setup_mfma(gade_model *m) {
+    m->mfma_ok = false;
+    if (!m->dia_ok || m->nhp % 4 || (m->nhp / 4) % kMfmaNIT) return GADE_OK;
+    m->nmi = m->nhp / 4;
+    std::vector<int> rows, qs, bs;
+    std::vector<double> ks;
+    for (int b = 0; b < m->B; b++) {
+        std::vector<std::pair<int, std::vector<int>>> halves;  // (q, up to 8 rows)
+        for (int q = 0; q < m->nq; q++) {
+            std::vector<int> rq;
+            for (int r = b * m->nblk1; r < (b + 1) * m->nblk1; r++)
+                if (m->blk_q[r] == q) rq.push_back(r);
+            for (size_t i = 0; i < rq.size(); i += 8)
+                halves.emplace_back(q, std::vector<int>(rq.begin() + i, rq.begin() + std::min(rq.size(), i + 8)));
+        }
+        for (size_t w = 0; w < halves.size(); w += 2) {
+            for (int hf = 0; hf < 2; hf++) {
+                const bool have = w + hf < halves.size();
+                const auto &hv = halves[have ? w + hf : w];
+                for (int i = 0; i < 8; i++) rows.push_back(have && i < (int)hv.second.size() ? hv.second[i] : -1);
+                qs.push_back(hv.first);
+                ks.push_back((double)m->kap_q[hv.first]);
+            }
+            bs.push_back(b);
+        }
+    }
+    m->nmg = (int)bs.size();
+    GADE_TRY(upload(&m->d_mg_rows, rows.data(), rows.size()));
+    GADE_TRY(upload(&m->d_mg_q, qs.data(), qs.size()));
+    GADE_TRY(upload(&m->d_mg_k, ks.data(), ks.size()));
+    GADE_TRY(upload(&m->d_mg_b, bs.data(), bs.size()));
+    m->gf_stride = (size_t)m->lmax * 4 * 4 * m->nmi * 5 * kMfmaFrag * 4;
+    GADE_TRY(hip_alloc((void **)&m->d_GF, sizeof(double) * m->gf_stride * m->B));
+    GADE_HIP(hipMemset(m->d_GF, 0, sizeof(double) * m->gf_stride * m->B));  // f = 0 until the first set_time
+    const long long tdf = (long long)3 * m->nq * 4 * m->nmi * 5 * kMfmaFrag;
+    GADE_TRY(hip_alloc((void **)&m->d_DF, sizeof(double2) * tdf));
+    hipLaunchKernelGGL(k_build_DF, dim3(grid_for(tdf, 256)), dim3(256), 0, m->stream, m->d_DD, m->nq, m->nd, m->nhp,
+                       m->nmi, m->d_DF);
+    GADE_HIP(hipGetLastError());
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    m->mfma_ok = true;
+    m->radial_mfma = 1;
+    return GADE_OK;
+}
+
 // G-sums of the current f(t) in the layout of the active radial kernel
 // pulse coefficients c_alpha = SoL * (i h dt/2) * f_alpha of instance b
 Coef6 coef_of(const gade_model *m, int b) {
@@ -1204,6 +1505,10 @@ Coef6 coef_of(const gade_model *m, int b) {
     return cf;
 }
 
+// the DIA-layout G-sums feed k_radial_dia (FP64 when the MFMA kernel is off) and the FP32 copies of the
+// mixed-precision form
+bool dia_gsums_needed(const gade_model *m) { return !m->radial_mfma || m->precision == 1; }
+
 // G-sums of the current f(t) of every instance in the layout of the active radial kernel
 int launch_gsums(gade_model *m, bool ell) {
     Prof pr(m, "gsums", GADE_PROF_GSUMS);
@@ -1213,13 +1518,19 @@ int launch_gsums(gade_model *m, bool ell) {
                            m->d_tmap, coef_of(m, 0), m->lmax, m->ellw, m->n, m->nnzb, m->d_GE, m->gate);
     } else {
         long long total = (long long)m->lmax * 16 * m->nd * m->nhp;  // (l, v, qd, d, ip)
+        const long long totf = (long long)m->lmax * 16 * m->nmi * 5 * kMfmaFrag;
         for (int b0 = 0; b0 < m->B; b0 += kCoefGroup) {
             const int nb = std::min(kCoefGroup, m->B - b0);
             CoefGroup cg{};
             for (int k = 0; k < nb; k++) cg.c[k] = coef_of(m, b0 + k);
-            hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), nb), dim3(256), 0, m->stream, m->d_gs,
-                               m->d_gts, cg, b0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride,
-                               (const Coef6 *)nullptr, (const int *)nullptr, m->B);
+            if (dia_gsums_needed(m))
+                hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), nb), dim3(256), 0, m->stream,
+                                   m->d_gs, m->d_gts, cg, b0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate,
+                                   m->gstride, (const Coef6 *)nullptr, (const int *)nullptr, m->B);
+            if (m->radial_mfma)
+                hipLaunchKernelGGL(k_gsums_frag, dim3(grid_for(totf, 256, 4096), nb), dim3(256), 0, m->stream,
+                                   m->d_gs, m->d_gts, cg, b0, m->gf_stride, m->lmax, m->nd, m->nhp, m->nmi, m->d_GF,
+                                   m->gate, m->gstride, (const Coef6 *)nullptr, (const int *)nullptr, m->B);
         }
     }
     GADE_HIP(hipGetLastError());
@@ -1516,6 +1827,7 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
     }
     if ((rc = build_tiles(m))) return fail(rc);
     if ((rc = setup_dia(m))) return fail(rc);
+    if ((rc = setup_mfma(m))) return fail(rc);
     m->radial_kind = m->dia_ok ? 2 : 1;
     if (B > 1 && !m->dia_ok) {
         gade::set_error("batched models need a banded radial pattern (2x2 bands of half-width <= 7)");
@@ -1674,8 +1986,7 @@ int apply_dot(gade_model *m, const double2 *x, double2 *out, double *part, bool 
     }
     {
         Prof pr(m, "radial", GADE_PROF_RADIAL);
-        GADE_TRY((launch_dia<false, double2>(m, m->lmax, x, m->d_V, m->d_GD, m->d_DD, 0, 1.0, 0.0, out, m->d_ypart,
-                                             (double2 *)nullptr, (double2 *)nullptr, 0)));
+        GADE_TRY((launch_dia64<false>(m, m->lmax, x, 0, 1.0, 0.0, out, m->d_ypart, nullptr, nullptr, 0)));
     }
     Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
     hipLaunchKernelGGL(k_sum_bicg<OP>, dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream, m->N1, m->N,
@@ -1856,6 +2167,8 @@ struct GateScope {
 namespace gade {
 // the step graph holds device addresses of the model's buffers: drop it before any of them is replaced
 void drop_step_graph(gade_model *m) {
+    // a replay may still be running on the model's stream: let it finish before the executable goes
+    if (m->graph_exec && m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->graph_exec) (void)hipGraphExecDestroy(m->graph_exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
     m->graph_exec = nullptr;
@@ -2085,8 +2398,7 @@ int enqueue_rhs_init(gade_model *m, double2 *psi) {
     }
     {
         Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
-        GADE_TRY((launch_dia<true, double2>(m, m->lmax, psi, m->d_V, m->d_GD, m->d_DD, 1, -1.0, 0.0, m->tmp,
-                                            m->d_ypart, m->r, m->d_ypart2, 0)));
+        GADE_TRY((launch_dia64<true>(m, m->lmax, psi, 1, -1.0, 0.0, m->tmp, m->d_ypart, m->r, m->d_ypart2, 0)));
     }
     Prof pr(m, "radial_sum", GADE_PROF_RADIAL_SUM);
     hipLaunchKernelGGL(k_sum2_init, dim3(m->red_blocks, m->B), dim3(kRedThreads), 0, m->stream, m->N1, m->N, m->lmax,
@@ -2127,9 +2439,16 @@ int launch_gsums_window(gade_model *m) {
     Prof pr(m, "gsums", GADE_PROF_GSUMS);
     long long total = (long long)m->lmax * 16 * m->nd * m->nhp;  // (l, v, qd, d, ip)
     CoefGroup cg{};
-    hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), m->B), dim3(256), 0, m->stream, m->d_gs,
-                       m->d_gts, cg, 0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride,
-                       (const Coef6 *)m->d_cfwin, m->d_gctl, m->B);
+    if (dia_gsums_needed(m))
+        hipLaunchKernelGGL(k_gsums_dia, dim3(grid_for(total, 256, 4096), m->B), dim3(256), 0, m->stream, m->d_gs,
+                           m->d_gts, cg, 0, m->gd_stride, m->lmax, m->nd, m->nhp, m->d_GD, m->gate, m->gstride,
+                           (const Coef6 *)m->d_cfwin, m->d_gctl, m->B);
+    if (m->radial_mfma) {
+        const long long totf = (long long)m->lmax * 16 * m->nmi * 5 * kMfmaFrag;
+        hipLaunchKernelGGL(k_gsums_frag, dim3(grid_for(totf, 256, 4096), m->B), dim3(256), 0, m->stream, m->d_gs,
+                           m->d_gts, cg, 0, m->gf_stride, m->lmax, m->nd, m->nhp, m->nmi, m->d_GF, m->gate,
+                           m->gstride, (const Coef6 *)m->d_cfwin, m->d_gctl, m->B);
+    }
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -2953,6 +3272,19 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->precision = value;
         return GADE_OK;
     }
+    if (!strcmp(name, "radial_mfma")) {
+        // 1: the FP64 banded radial products on the matrix cores (default when the pattern allows), 0: VALU
+        if ((value != 0 && value != 1) || (value == 1 && !m->mfma_ok)) {
+            gade::set_error(value == 1 ? "the radial pattern does not fit the MFMA kernel" : "radial_mfma must be 0 or 1");
+            return GADE_EINVAL;
+        }
+        if (value != m->radial_mfma) {
+            drop_step_graph(m);
+            m->radial_mfma = value;
+            if (m->radial_kind == 2) GADE_TRY(launch_gsums(m, false));  // the current f(t) in the kernel's layout
+        }
+        return GADE_OK;
+    }
     if (!strcmp(name, "mp_inner_iters")) {
         if (value < 1 || value > 64) {
             gade::set_error("mp_inner_iters must be 1..64");
@@ -3058,6 +3390,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "precision")) {
         *value = m->precision;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "radial_mfma")) {
+        *value = m->radial_mfma;
         return GADE_OK;
     }
     if (!strcmp(name, "dia_rows")) {

@@ -127,6 +127,15 @@ struct gade_model {
     double *d_dtile_k = nullptr;
     int *d_dtile_b = nullptr;      // instance of each tile (tiles never mix instances)
     int dia_r = 8, dia_p = 2;      // banded tile shape: rows per tile, outputs per lane
+    // FP64 banded radial phase on the matrix cores: waves of two 8-row halves of one kappa each
+    bool mfma_ok = false;          // the banded pattern fits the MFMA kernel (set up)
+    int radial_mfma = 0;           // 1: FP64 radial products on MFMA (k_radial_mfma), 0: k_radial_dia
+    int nmi = 0, nmg = 0;          // i-steps of 4 outputs per half; MFMA wave groups
+    size_t gf_stride = 0;          // GF doubles per instance
+    double *d_GF = nullptr;        // [B][lmax][4][4][nmi][5][16][4] G-sum pairs in MFMA fragment order
+    double2 *d_DF = nullptr;       // [3][nq][4][nmi][5][16] diagonal operators in fragment order
+    int *d_mg_rows = nullptr, *d_mg_q = nullptr, *d_mg_b = nullptr;  // [nmg][16], [nmg][2], [nmg]
+    double *d_mg_k = nullptr;      // [nmg][2] kappa of each half
     int ang_lb = 4;                // multipoles per angular block
     int psolve_ch = 16;            // preconditioner level height (16 or 32 rows)
     size_t gd_stride = 0;          // GD entries per instance

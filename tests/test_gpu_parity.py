@@ -36,15 +36,23 @@ def pair(datadir, cfg, ntime=200):
     return d, dt, O.Model(d, LMAX[cfg], dt, nthreads=4), gade.Propagator(d, LMAX[cfg], dt)
 
 
-RADIAL = {"ell": 1, "dia": 2}
+# radial phase kernels: general ELL, banded on the vector units, banded on the FP64 matrix cores (the default)
+RADIAL = {"ell": (1, 0), "dia": (2, 0), "mfma": (2, 1)}
+
+
+def use_radial(gp, kind):
+    radial, mfma = RADIAL[kind]
+    gp.set_param("radial", radial)
+    if radial == 2:
+        gp.set_param("radial_mfma", mfma)
+    assert gp.get_param("radial") == radial and (radial == 1 or gp.get_param("radial_mfma") == mfma)
 
 
 @pytest.mark.parametrize("kind", sorted(RADIAL))
 @pytest.mark.parametrize("cfg", CFGS)
 def test_fused_apply_matches_oracle(datadir, cfg, kind):
     d, dt, om, gp = pair(datadir, cfg)
-    gp.set_param("radial", RADIAL[kind])
-    assert gp.get_param("radial") == RADIAL[kind]
+    use_radial(gp, kind)
     say(f"fused apply {cfg} n={gp.n} radial={kind}")
     rng = np.random.default_rng(0)
     x = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
@@ -177,7 +185,7 @@ def test_tile_width_invariance(datadir):
     x = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
     dx, dy = gade.DeviceArray.from_host(x), gade.DeviceArray(gp.n)
     gp.set_time(gade.pulse(10, 50, 2, 0.03))
-    gp.set_param("radial", RADIAL["ell"])
+    use_radial(gp, "ell")
     outs = []
     for r in (1, 2, 4, 8):
         gp.set_param("tile_r", r)
@@ -189,8 +197,9 @@ def test_tile_width_invariance(datadir):
 
 
 def test_radial_kernels_agree_and_switch(datadir):
-    """The banded and the ELL radial kernels give the same operator (1e-14), and switching the kernel
-    mid-trajectory re-derives the coefficients of the current f(t) in the new layout."""
+    """The banded radial kernels (matrix cores and vector units) and the ELL kernel give the same operator
+    (1e-14), and switching the kernel mid-trajectory re-derives the coefficients of the current f(t) in the
+    new layout."""
     cfg = "nondipole_small"
     d, dt, om, gp = pair(datadir, cfg)
     say("radial kernels agree")
@@ -200,15 +209,17 @@ def test_radial_kernels_agree_and_switch(datadir):
     f = gade.pulse(10, 50, 2, 0.07)
     gp.set_time(f)
     res = {}
-    for kind in ("dia", "ell", "dia"):
-        gp.set_param("radial", RADIAL[kind])
+    for kind in ("dia", "ell", "mfma", "dia", "mfma"):
+        use_radial(gp, kind)
         for cH in (1.0, -1.0, 0.0):
             gp.apply(dx, dy, cH=cH)
             res.setdefault((kind, cH), []).append(dy.get())
     for cH in (1.0, -1.0, 0.0):
-        a, b = res[("dia", cH)], res[("ell", cH)][0]
+        a, b, c = res[("dia", cH)], res[("ell", cH)][0], res[("mfma", cH)]
         assert np.array_equal(a[0].view(np.float64), a[1].view(np.float64))
+        assert np.array_equal(c[0].view(np.float64), c[1].view(np.float64))
         assert relmax(a[0], b) <= 1e-14, cH
+        assert relmax(c[0], b) <= 1e-14, cH
     om.timedep(f)
     assert relmax(res[("dia", 1.0)][0], om.apply(x, "S") + om.apply(x, "H")) <= 1e-13
 
@@ -417,3 +428,24 @@ def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
         for a, b in zip(out[shape][1], ref[1]):
             assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64)), shape
             assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), shape
+
+
+@pytest.mark.parametrize("cfg", CFGS)
+def test_mfma_radial_is_default_and_steps_agree(datadir, cfg):
+    """The matrix-core radial kernel is the default FP64 form; CN steps with it and with the vector-unit
+    kernel (device-driven stepping with the paired rhs / A x0 apply) agree to 1e-11 over 12 steps, every step
+    at the tolerance."""
+    d, dt, om, gp = pair(datadir, cfg)
+    assert gp.get_param("radial") == 2 and gp.get_param("radial_mfma") == 1
+    say(f"mfma vs dia steps {cfg}")
+    psi0, _ = gade.read_psi0(d, gp.n)
+    out = []
+    for mfma in (1, 0):
+        g = gade.Propagator(d, LMAX[cfg], dt)
+        g.set_param("radial_mfma", mfma)
+        x = gade.DeviceArray.from_host(psi0)
+        it, tols, _ = g.cn_run(x, 10.0, 50.0, 1.0, 12, tol=1e-12, timing=False)
+        assert np.all(tols <= 1e-12)
+        out.append(x.get())
+    assert np.linalg.norm(out[0] - out[1]) <= 1e-11 * np.linalg.norm(out[1])
+
