@@ -17,14 +17,16 @@ The propagator consumes the files that the reference's offline generator
 
 The angular matrices follow the reference's formulas exactly (Wigner-3j
 selection rules and phases of spnrbasis::bdpK/L/M/N), with a Wigner-3j of our
-own (Racah's formula in exact integer arithmetic).  The radial matrices are a
-structure-faithful stand-in: a degree-7 B-spline dual-kinetic-balance basis on
-the same knots (src/dumpBdpCouplings.cpp:194-198, src/splinehandler.cpp:26-52,
-first and last spline dropped), with overlap and field-free Dirac H0 built by
-Gauss-Legendre quadrature and interaction integrals that use spherical Bessel
-functions j_l(k r).  Their values are this project's own and are not pinned
-against the reference's dkbbasis integrals; the sparsity pattern, the
-kappa-polynomial structure (s0..s2, h00..h03, g0..g3) and the file formats are.
+own (Racah's formula in exact integer arithmetic).  The radial matrices are this
+project's own quadrature of the reference's construction: a degree-7 B-spline
+dual-kinetic-balance basis on the same knots (src/dumpBdpCouplings.cpp:194-198,
+src/splinehandler.cpp:26-52, first and last spline dropped) with the P / Q
+kappa-parts of src/dkbbasis.cpp (dkbCache, bdpalphmat), overlap and field-free
+Dirac H0 by Gauss-Legendre quadrature, and interaction integrals P^T W C_alpha Q
+with the spherical Bessel kernels of src/potential.h:470-497 (one wave number per
+alpha pair).  The field-free spectrum is the Dirac hydrogen spectrum (1s to
+1e-11); the values are not compared with files the reference wrote (it ships
+none, and dumpCouplings cannot be built here).
 """
 from __future__ import annotations
 
@@ -232,18 +234,19 @@ def radial_matrices(nsplines, rbox, Z, omega, cycles, nl, nq=12):
     c = SOL
     inv_r = 1.0 / x
     z0 = np.zeros_like(B)
-    # kappa-polynomial components of the P and Q parts of the basis functions
-    # (H below is [[V, c(-d/dr + k/r)], [c(d/dr + k/r), V - 2c^2]]):
-    # upper (large) functions: P = B, Q = (B' + kappa B/r)/(2c)
-    # lower (small) functions: P = (B' - kappa B/r)/(2c), Q = B
+    # kappa-polynomial components of the P and Q parts of the basis functions, as the reference's dual
+    # kinetically balanced basis (src/dkbbasis.cpp: dkbCache, with P the ul = 1 and Q the ul = -1 part):
+    # large functions: P = B,                    Q = (B' + kappa B/r)/(2c)
+    # small functions: P = (-B' + kappa B/r)/(2c), Q = B
+    # (H below is [[V, c(-d/dr + k/r)], [c(d/dr + k/r), V - 2c^2]])
     Br = B * inv_r[:, None]
     dBr = dB * inv_r[:, None] - B * (inv_r ** 2)[:, None]
-    P0 = np.hstack([B, dB / (2 * c)])
-    P1 = np.hstack([z0, -Br / (2 * c)])
+    P0 = np.hstack([B, -dB / (2 * c)])
+    P1 = np.hstack([z0, Br / (2 * c)])
     Q0 = np.hstack([dB / (2 * c), B])
     Q1 = np.hstack([Br / (2 * c), z0])
-    dP0 = np.hstack([dB, ddB / (2 * c)])
-    dP1 = np.hstack([z0, -dBr / (2 * c)])
+    dP0 = np.hstack([dB, -ddB / (2 * c)])
+    dP1 = np.hstack([z0, dBr / (2 * c)])
     dQ0 = np.hstack([ddB / (2 * c), dB])
     dQ1 = np.hstack([dBr / (2 * c), z0])
     V = -Z * inv_r
@@ -269,17 +272,19 @@ def radial_matrices(nsplines, rbox, Z, omega, cycles, nl, nq=12):
             h[p + q] += ip(Pp[p], A[q]) + ip(Qp[p], C[q])
     s = [0.5 * (m + m.T) for m in s]
     h = [0.5 * (m + m.T) for m in h]
-    # interaction integrals: alpha pairs share a wave number (cf. tools/solver_common.c:53-65)
+    # interaction integrals g^n_{alpha,l} = P^T W C_alpha(l) Q over the four kappa parts (src/dkbbasis.cpp:
+    # 780-820): C_alpha(l)(r) = j_l(k_alpha r) with the spherical Bessel function j_l and one wave number per
+    # pair alpha = 2j, 2j+1 (src/potential.h:470-497): k = 2 pi/(c T) + omega/c, -(2 pi/(c T) - omega/c),
+    # omega/c
     from scipy.special import spherical_jn
     T = cycles * 2 * math.pi / omega
-    kw = [(2 * math.pi / T + omega) / c, (2 * math.pi / T - omega) / c, omega / c]
+    kw = [2 * math.pi / (c * T) + omega / c, -(2 * math.pi / (c * T) - omega / c), omega / c]
     g = [[[None] * nl for _ in range(6)] for _ in range(4)]
     for alpha in range(6):
         k = kw[alpha // 2]
         for l in range(nl):
+            # j_l(-z) = (-1)^l j_l(z) for a negative wave number
             jl = spherical_jn(l, abs(k) * x) * (np.sign(k) ** l if k != 0 else 1.0)
-            if alpha % 2:
-                jl = jl * (1.0 + 0.25 * x / rbox)
             g[0][alpha][l] = ip(P0, Q0, jl)
             g[1][alpha][l] = ip(P1, Q0, jl)
             g[2][alpha][l] = ip(P0, Q1, jl)
