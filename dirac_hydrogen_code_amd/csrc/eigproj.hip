@@ -3,14 +3,20 @@
 // eigenpairs H0(k) v = E S(k) v (S-orthonormal), and for every block r
 // population_j = |v_j^H S(k_r) psi_r|^2, summed into the classes
 // E < -c^2 (negative continuum), -c^2 <= E < 0 (bound) and E >= 0 (ionised).
-// The eigenpairs are computed once per model on the device (rocSOLVER
-// zhegvd, batched over kappa); the projection is a hand-written kernel.
+// The eigenpairs are computed once per model on the host by this library's
+// dense solver (gade_eig_dense: Cholesky, Householder tridiagonalization,
+// implicit QL), one thread per distinct kappa, on the real parts of H0(k) and
+// S(k) as the reference does (src/hamiltonian.h:906-943); the projections run
+// in hand-written kernels.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gade_device.h"
@@ -19,44 +25,6 @@
 using gade::set_error;
 
 namespace {
-
-// dense column-major H0(k_q), S(k_q) [q][col][row] from the radial CSR values
-__global__ void k_dense_ops(const double2 *__restrict__ s, const double2 *__restrict__ h0,
-                            const int *__restrict__ ell_src, const int *__restrict__ ell_col,
-                            const int *__restrict__ kq, int nq, int n, int ellw, int nnzb, double2 *__restrict__ A,
-                            double2 *__restrict__ B) {
-    const long long total = (long long)nq * n;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const int q = (int)(idx / n), a = (int)(idx % n);
-        const double k = kq[q];
-        const size_t base = (size_t)q * n * n;
-        for (int sl = 0; sl < ellw; sl++) {
-            const int src = ell_src[(size_t)sl * n + a];
-            if (src < 0) continue;
-            const int c = ell_col[(size_t)sl * n + a];
-            const double2 S = cadd(cadd(s[src], cscale(s[nnzb + src], k)), cscale(s[2 * nnzb + src], k * k));
-            const double2 H = cadd(cadd(cadd(h0[src], cscale(h0[nnzb + src], k)), cscale(h0[2 * nnzb + src], k * k)),
-                                   cscale(h0[3 * nnzb + src], k * k * k));
-            A[base + (size_t)c * n + a] = H;
-            B[base + (size_t)c * n + a] = S;
-        }
-    }
-}
-
-// VT[q][i][j] = conj(v_j[i]) from the column-major eigenvectors A[q][j][i]
-__global__ void k_conj_transpose(const double2 *__restrict__ A, int n, int nq, double2 *__restrict__ VT) {
-    const long long total = (long long)nq * n * n;
-    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
-         idx += (long long)gridDim.x * blockDim.x) {
-        const long long q = idx / ((long long)n * n);
-        const int rem = (int)(idx % ((long long)n * n));
-        const int i = rem / n, j = rem % n;
-        double2 v = A[q * n * n + (size_t)j * n + i];
-        v.y = -v.y;
-        VT[idx] = v;
-    }
-}
 
 // per block r: w = S(k_r) psi_r (staged in LDS), c_j = sum_i conj(v_j[i]) w_i,
 // out[r] = (sum over classes of |c_j|^2, then psi_r^H S psi_r)
@@ -101,6 +69,36 @@ __global__ __launch_bounds__(256) void k_eigproj(const double2 *__restrict__ x, 
         for (int k = 0; k < 4; k++) out[(size_t)r * 4 + k] = acc[k];
 }
 
+// per block r: c_j = v_j^H S(k_r) psi_r for every eigenpair j of its kappa (the amplitudes behind the
+// populations); out [nblk][n] complex
+__global__ __launch_bounds__(256) void k_eigcoef(const double2 *__restrict__ x, const double2 *__restrict__ D,
+                                                 const int *__restrict__ ell_col, const int *__restrict__ blk_q,
+                                                 const double2 *__restrict__ VT, int n, int ellw, int nq,
+                                                 double2 *__restrict__ out) {
+    extern __shared__ double2 w[];
+    const int r = blockIdx.x;
+    const int q = blk_q[r];
+    const double2 *Sq = D + ((size_t)2 * nq + q) * ellw * n;
+    const double2 *xr = x + (size_t)r * n;
+    for (int a = threadIdx.x; a < n; a += blockDim.x) {
+        double2 sx = make_double2(0, 0);
+        for (int sl = 0; sl < ellw; sl++) sx = cfma(Sq[(size_t)sl * n + a], xr[ell_col[(size_t)sl * n + a]], sx);
+        w[a] = sx;
+    }
+    __syncthreads();
+    const double2 *V = VT + (size_t)q * n * n;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        double2 c0 = make_double2(0, 0), c1 = make_double2(0, 0);
+        int i = 0;
+        for (; i + 1 < n; i += 2) {
+            c0 = cfma(V[(size_t)i * n + j], w[i], c0);
+            c1 = cfma(V[(size_t)(i + 1) * n + j], w[i + 1], c1);
+        }
+        if (i < n) c0 = cfma(V[(size_t)i * n + j], w[i], c0);
+        out[(size_t)r * n + j] = cadd(c0, c1);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -108,82 +106,60 @@ extern "C" {
 int gade_eig_setup(gade_model_t *m) {
     if (!m) return GADE_EINVAL;
     if (m->d_evec) return GADE_OK;
-    const int n = m->n, nq = m->nq;
+    const int n = m->n, nq = m->nq, nnzb = m->nnzb;
     const size_t nn = (size_t)n * n;
-    double2 *A = nullptr, *B = nullptr;
-    double *Ework = nullptr;
-    int *d_kq = nullptr, *info = nullptr;
-    int rc = GADE_OK;
-    bool ok = false;
-    auto cleanup = [&]() {
-        for (void *p : {(void *)A, (void *)B, (void *)Ework, (void *)d_kq, (void *)info})
-            if (p) (void)hipFree(p);
-        if (!ok) {
-            for (void *p : {(void *)m->d_eval, (void *)m->d_evec})
-                if (p) (void)hipFree(p);
-            m->d_eval = nullptr;
-            m->d_evec = nullptr;
+    // radial values of s0..s2 and h00..h03 (real parts, as the reference's .real())
+    std::vector<double2> hs((size_t)3 * nnzb), hh((size_t)4 * nnzb);
+    GADE_HIP(hipMemcpy(hs.data(), m->d_s, sizeof(double2) * hs.size(), hipMemcpyDeviceToHost));
+    GADE_HIP(hipMemcpy(hh.data(), m->d_h0, sizeof(double2) * hh.size(), hipMemcpyDeviceToHost));
+    std::vector<double> E((size_t)nq * n), V((size_t)nq * nn);
+    std::vector<int> rcs(nq, GADE_OK);
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+        std::vector<double> A(nn), B(nn);
+        for (int q = next++; q < nq; q = next++) {
+            const double k = m->kap_q[q];
+            std::fill(A.begin(), A.end(), 0.0);
+            std::fill(B.begin(), B.end(), 0.0);
+            for (int i = 0; i < n; i++)
+                for (int p = m->rad_Ap[i]; p < m->rad_Ap[i + 1]; p++) {
+                    const int c = m->rad_Ai[p];
+                    B[(size_t)i * n + c] = hs[p].x + k * hs[nnzb + p].x + k * k * hs[2 * nnzb + p].x;
+                    A[(size_t)i * n + c] =
+                        hh[p].x + k * hh[nnzb + p].x + k * k * hh[2 * nnzb + p].x + k * k * k * hh[3 * nnzb + p].x;
+                }
+            rcs[q] = gade_eig_dense(n, A.data(), B.data(), E.data() + (size_t)q * n, V.data() + (size_t)q * nn);
         }
     };
-    if ((rc = gade::hip_alloc((void **)&A, sizeof(double2) * nq * nn)) ||
-        (rc = gade::hip_alloc((void **)&B, sizeof(double2) * nq * nn)) ||
-        (rc = gade::hip_alloc((void **)&Ework, sizeof(double) * nq * n)) ||
-        (rc = gade::hip_alloc((void **)&d_kq, sizeof(int) * nq)) ||
-        (rc = gade::hip_alloc((void **)&info, sizeof(int) * nq)) ||
-        (rc = gade::hip_alloc((void **)&m->d_eval, sizeof(double) * nq * n)) ||
-        (rc = gade::hip_alloc((void **)&m->d_evec, sizeof(double2) * nq * nn))) {
-        cleanup();
-        return rc;
-    }
-    hipError_t e = hipMemcpy(d_kq, m->kap_q.data(), sizeof(int) * nq, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemsetAsync(A, 0, sizeof(double2) * nq * nn, m->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(B, 0, sizeof(double2) * nq * nn, m->stream);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_dense_ops, dim3((nq * n + 255) / 256), dim3(256), 0, m->stream, m->d_s, m->d_h0,
-                           m->d_ell_src, m->d_ell_col, d_kq, nq, n, m->ellw, m->nnzb, A, B);
-        e = hipGetLastError();
-    }
-    if (e != hipSuccess) {
-        cleanup();
-        set_error(std::string("gade_eig_setup: ") + hipGetErrorString(e));
-        return GADE_EHIP;
-    }
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) {
-        cleanup();
-        set_error("gade_eig_setup: rocblas_create_handle failed");
-        return GADE_EHIP;
-    }
-    rocblas_set_stream(h, m->stream);
-    rocblas_status st = rocsolver_zhegvd_strided_batched(
-        h, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, n, (rocblas_double_complex *)A, n,
-        (rocblas_stride)nn, (rocblas_double_complex *)B, n, (rocblas_stride)nn, m->d_eval, (rocblas_stride)n, Ework,
-        (rocblas_stride)n, info, nq);
-    std::vector<int> info_h(nq, 0);
-    e = hipStreamSynchronize(m->stream);
-    if (e == hipSuccess) e = hipMemcpy(info_h.data(), info, sizeof(int) * nq, hipMemcpyDeviceToHost);
-    rocblas_destroy_handle(h);
-    if (st != rocblas_status_success || e != hipSuccess) {
-        cleanup();
-        set_error("gade_eig_setup: rocsolver_zhegvd failed");
-        return GADE_EHIP;
-    }
+    const int nthr = std::max(1, std::min<int>(nq, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthr; t++) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
     for (int q = 0; q < nq; q++)
-        if (info_h[q] != 0) {
-            cleanup();
+        if (rcs[q] != GADE_OK) {
             set_error("gade_eig_setup: generalized eigenproblem of kappa " + std::to_string(m->kap_q[q]) +
-                      " failed (info " + std::to_string(info_h[q]) + "); S must be Hermitian positive definite");
-            return GADE_EINVAL;
+                      (rcs[q] == GADE_EINVAL ? " failed: S is not positive definite" : " did not converge"));
+            return rcs[q];
         }
-    hipLaunchKernelGGL(k_conj_transpose, dim3(4096), dim3(256), 0, m->stream, A, n, nq, m->d_evec);
-    e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
-    ok = e == hipSuccess;
-    cleanup();
-    if (e != hipSuccess) {
+    // VT[q][i][j] = conj(v_j[i]); the eigenvectors are real
+    std::vector<double2> vt((size_t)nq * nn);
+    for (size_t idx = 0; idx < vt.size(); idx++) vt[idx] = make_double2(V[idx], 0.0);
+    double *d_eval = nullptr;
+    double2 *d_evec = nullptr;
+    GADE_TRY(gade::hip_alloc((void **)&d_eval, sizeof(double) * nq * n));
+    int rc = gade::hip_alloc((void **)&d_evec, sizeof(double2) * vt.size());
+    hipError_t e = hipSuccess;
+    if (rc == GADE_OK) e = hipMemcpy(d_eval, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice);
+    if (rc == GADE_OK && e == hipSuccess) e = hipMemcpy(d_evec, vt.data(), sizeof(double2) * vt.size(), hipMemcpyHostToDevice);
+    if (rc != GADE_OK || e != hipSuccess) {
+        (void)hipFree(d_eval);
+        if (d_evec) (void)hipFree(d_evec);
+        if (rc != GADE_OK) return rc;
         set_error(std::string("gade_eig_setup: ") + hipGetErrorString(e));
         return GADE_EHIP;
     }
+    m->d_eval = d_eval;
+    m->d_evec = d_evec;
     return GADE_OK;
 }
 
@@ -226,6 +202,67 @@ int gade_populations(gade_model_t *m, const double *x, double *norm, double *bou
     if (bound) *bound = tb;
     if (ionised) *ionised = ti;
     if (per_block) memcpy(per_block, out.data(), sizeof(double) * out.size());
+    return GADE_OK;
+}
+
+int gade_psiev(gade_model_t *m, const double *x, double *psiev) {
+    if (!m || !x || !psiev) return GADE_EINVAL;
+    GADE_TRY(gade_eig_setup(m));
+    const size_t cnt = (size_t)m->nblk * m->n;
+    double2 *d_out = nullptr;
+    GADE_TRY(gade::hip_alloc((void **)&d_out, sizeof(double2) * cnt));
+    hipLaunchKernelGGL(k_eigcoef, dim3(m->nblk), dim3(256), sizeof(double2) * m->n, m->stream, (const double2 *)x,
+                       m->d_D, m->d_ell_col, m->d_blk_q, m->d_evec, m->n, m->ellw, m->nq, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(psiev, d_out, sizeof(double2) * cnt, hipMemcpyDeviceToHost, m->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) {
+        set_error(std::string("gade_psiev: ") + hipGetErrorString(e));
+        return GADE_EHIP;
+    }
+    // psiev_j = psi^H S v_j = conj(v_j^H S psi)  (src/hamiltonian.h:1318: psiseg.adjoint() * skev)
+    for (size_t i = 0; i < cnt; i++) psiev[2 * i + 1] = -psiev[2 * i + 1];
+    return GADE_OK;
+}
+
+// dense_write of src/dumpEigvals.cpp:87-108: int64 rows, int64 cols, then the column-major data
+static int dense_write(const std::string &fname, int64_t rows, int64_t cols, const void *data, size_t elt) {
+    FILE *fd = fopen(fname.c_str(), "wb");
+    if (!fd) {
+        set_error("cant open " + fname);
+        return GADE_EIO;
+    }
+    bool ok = fwrite(&rows, sizeof rows, 1, fd) == 1 && fwrite(&cols, sizeof cols, 1, fd) == 1 &&
+              fwrite(data, elt, (size_t)(rows * cols), fd) == (size_t)(rows * cols);
+    ok = (fclose(fd) == 0) && ok;
+    if (!ok) {
+        set_error("write failed: " + fname);
+        return GADE_EIO;
+    }
+    return GADE_OK;
+}
+
+int gade_eig_write(gade_model_t *m, const char *dir) {
+    if (!m || !dir) return GADE_EINVAL;
+    GADE_TRY(gade_eig_setup(m));
+    const int n = m->n, nq = m->nq;
+    const size_t nn = (size_t)n * n;
+    std::vector<double> E((size_t)nq * n);
+    std::vector<double2> VT((size_t)nq * nn), evc(nn);
+    GADE_HIP(hipMemcpy(E.data(), m->d_eval, sizeof(double) * E.size(), hipMemcpyDeviceToHost));
+    GADE_HIP(hipMemcpy(VT.data(), m->d_evec, sizeof(double2) * VT.size(), hipMemcpyDeviceToHost));
+    for (int q = 0; q < nq; q++) {
+        const int kappa = m->kap_q[q];
+        GADE_TRY(dense_write(std::string(dir) + "/evl" + std::to_string(kappa) + ".mat", n, 1, E.data() + (size_t)q * n,
+                             sizeof(double)));
+        // column j = eigenvector j: evc[j * n + i] = v_j[i] = conj(VT[q][i][j])
+        const double2 *V = VT.data() + (size_t)q * nn;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) evc[(size_t)j * n + i] = make_double2(V[(size_t)i * n + j].x, -V[(size_t)i * n + j].y);
+        GADE_TRY(dense_write(std::string(dir) + "/evc" + std::to_string(kappa) + ".mat", n, n, evc.data(),
+                             sizeof(double2)));
+    }
     return GADE_OK;
 }
 

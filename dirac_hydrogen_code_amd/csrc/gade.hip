@@ -2739,6 +2739,100 @@ int gade_reset_workspace(gade_model_t *m) {
     return GADE_OK;
 }
 
+// ---- checkpoint / resume (tools/dirac_solver.c:203-209 only shifts the start time; this restores the state)
+// File: "GADECKP1", int32 [version, B, nblk, blkdim, step, precision, guess_order, has_hist], double time,
+// then complex128 psi [B N1], the eight BiCGSTAB vectors r r0 v p y z s t [B N1] each (p and v carry into
+// the next solve, tools/gpu_bicgstab.c:36-47), double scalars [B][SC_STRIDE], int32 control words
+// [B][CT_STRIDE] and, with has_hist, the guess history [kHistSlots][B N1].
+namespace {
+constexpr char kCkpMagic[8] = {'G', 'A', 'D', 'E', 'C', 'K', 'P', '1'};
+
+int ckp_io(FILE *fd, bool save, void *host, size_t bytes) {
+    const size_t k = save ? fwrite(host, 1, bytes, fd) : fread(host, 1, bytes, fd);
+    if (k != bytes) {
+        gade::set_error(save ? "checkpoint: write failed" : "checkpoint: file is truncated");
+        return GADE_EIO;
+    }
+    return GADE_OK;
+}
+
+// copies one device array through the host in either direction
+int ckp_dev(FILE *fd, bool save, void *dev, size_t bytes, std::vector<char> &buf) {
+    buf.resize(bytes);
+    if (save) GADE_HIP(hipMemcpy(buf.data(), dev, bytes, hipMemcpyDeviceToHost));
+    GADE_TRY(ckp_io(fd, save, buf.data(), bytes));
+    if (!save) GADE_HIP(hipMemcpy(dev, buf.data(), bytes, hipMemcpyHostToDevice));
+    return GADE_OK;
+}
+
+int checkpoint(gade_model *m, double2 *psi, int *step, double *time, const char *file, bool save) {
+    if (m->guess_order == GADE_GUESS_LEARNED || m->lg_features) {
+        gade::set_error("checkpoint: the learned guess's state is not saved; checkpoint runs without it");
+        return GADE_EINVAL;
+    }
+    GADE_HIP(hipStreamSynchronize(m->stream));
+    FILE *fd = fopen(file, save ? "wb" : "rb");
+    if (!fd) {
+        gade::set_error(std::string("checkpoint: cannot open ") + file);
+        return GADE_EIO;
+    }
+    struct Closer {
+        FILE *f;
+        ~Closer() { if (f) fclose(f); }
+    } closer{fd};
+    char magic[8];
+    int hdr[8] = {1, m->B, m->nblk1, m->n, step ? *step : 0, m->precision, m->guess_order, m->d_hist ? 1 : 0};
+    double t = time ? *time : 0;
+    if (save) {
+        GADE_TRY(ckp_io(fd, true, (void *)kCkpMagic, 8));
+        GADE_TRY(ckp_io(fd, true, hdr, sizeof hdr));
+        GADE_TRY(ckp_io(fd, true, &t, sizeof t));
+    } else {
+        GADE_TRY(ckp_io(fd, false, magic, 8));
+        int fh[8];
+        GADE_TRY(ckp_io(fd, false, fh, sizeof fh));
+        GADE_TRY(ckp_io(fd, false, &t, sizeof t));
+        if (memcmp(magic, kCkpMagic, 8) || fh[0] != 1) {
+            gade::set_error(std::string("checkpoint: ") + file + " is not a checkpoint of this library");
+            return GADE_EINVAL;
+        }
+        if (fh[1] != hdr[1] || fh[2] != hdr[2] || fh[3] != hdr[3] || fh[5] != hdr[5] || fh[6] != hdr[6]) {
+            gade::set_error("checkpoint: saved for batch " + std::to_string(fh[1]) + ", " + std::to_string(fh[2]) +
+                            " x " + std::to_string(fh[3]) + " blocks, precision " + std::to_string(fh[5]) +
+                            ", guess order " + std::to_string(fh[6]) + "; the model differs");
+            return GADE_EINVAL;
+        }
+        if (fh[7] && !m->d_hist) GADE_TRY(hip_alloc((void **)&m->d_hist, sizeof(double2) * kHistSlots * m->N));
+        hdr[7] = fh[7];
+        if (step) *step = fh[4];
+        if (time) *time = t;
+    }
+    std::vector<char> buf;
+    const size_t vb = sizeof(double2) * m->N;
+    GADE_TRY(ckp_dev(fd, save, psi, vb, buf));
+    double2 *ws[] = {m->r, m->r0, m->v, m->p, m->y, m->z, m->s, m->t};
+    for (double2 *w : ws) GADE_TRY(ckp_dev(fd, save, w, vb, buf));
+    GADE_TRY(ckp_dev(fd, save, m->d_sc, sizeof(double) * m->B * SC_STRIDE, buf));
+    GADE_TRY(ckp_dev(fd, save, m->d_ctl, sizeof(int) * m->B * CT_STRIDE, buf));
+    if (hdr[7]) GADE_TRY(ckp_dev(fd, save, m->d_hist, vb * kHistSlots, buf));
+    if (save && fflush(fd) != 0) {
+        gade::set_error("checkpoint: write failed");
+        return GADE_EIO;
+    }
+    return GADE_OK;
+}
+}  // namespace
+
+int gade_checkpoint_save(gade_model_t *m, const double *psi, int step, double time, const char *file) {
+    if (!m || !psi || !file) return GADE_EINVAL;
+    return checkpoint(m, (double2 *)psi, &step, &time, file, true);
+}
+
+int gade_checkpoint_load(gade_model_t *m, double *psi, int *step, double *time, const char *file) {
+    if (!m || !psi || !file) return GADE_EINVAL;
+    return checkpoint(m, (double2 *)psi, step, time, file, false);
+}
+
 int gade_extrapolate(gade_model_t *m, double *psi, int order) {
     if (!m || !psi || order < 0 || order > kMaxGuessOrder) return GADE_EINVAL;
     const int saved = m->guess_order;

@@ -28,6 +28,7 @@ EXPORTS = (
     "gade_set_time_batch", "gade_pulse_cep", "gade_autotune", "gade_lg_record", "gade_lg_pod_build",
     "gade_lg_basis", "gade_lg_set_norm", "gade_lg_init", "gade_lg_set_params", "gade_lg_get_params",
     "gade_lg_forward", "gade_lg_grad", "gade_lg_train", "gade_lg_set_basis", "gade_lg_uses",
+    "gade_psiev", "gade_eig_write", "gade_checkpoint_save", "gade_checkpoint_load", "gade_eig_dense",
 )
 
 # learned initial guess (include/gade.h)
@@ -82,6 +83,10 @@ def lib():
             "gade_assembled_get": [vp, vp, vp, vp, vp], "gade_profile": [vp, i32],
             "gade_profile_get": [vp, C.c_char_p, C.POINTER(f64), C.POINTER(C.c_longlong)],
             "gade_set_param": [vp, C.c_char_p, i32], "gade_get_param": [vp, C.c_char_p, C.POINTER(i32)],
+            "gade_psiev": [vp, vp, vp], "gade_eig_write": [vp, C.c_char_p],
+            "gade_checkpoint_save": [vp, vp, i32, f64, C.c_char_p],
+            "gade_checkpoint_load": [vp, vp, C.POINTER(i32), C.POINTER(f64), C.c_char_p],
+            "gade_eig_dense": [i32, vp, vp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -334,6 +339,25 @@ class Propagator:
         if per_block:
             out["per_block"] = pb
         return out
+
+    def psiev(self, x: DeviceArray):
+        """Eigen-state amplitudes psi_r^H S v_j per block r and eigenpair j of its kappa: complex [nblk][blkdim]."""
+        out = np.zeros((self.nblk, self.blkdim), np.complex128)
+        check(lib().gade_psiev(self.ptr, x.ptr, out.ctypes.data))
+        return out
+
+    def eig_write(self, directory):
+        """evl<kappa>.mat / evc<kappa>.mat of every distinct kappa into directory."""
+        check(lib().gade_eig_write(self.ptr, directory.encode()))
+
+    def checkpoint_save(self, x: DeviceArray, step, time, path):
+        check(lib().gade_checkpoint_save(self.ptr, x.ptr, int(step), float(time), path.encode()))
+
+    def checkpoint_load(self, x: DeviceArray, path):
+        """Restores psi and the solver state; returns (step, time)."""
+        st, t = C.c_int(), C.c_double()
+        check(lib().gade_checkpoint_load(self.ptr, x.ptr, C.byref(st), C.byref(t), path.encode()))
+        return st.value, t.value
 
     def eig_values(self):
         """(kappa[nkappa], energies[nkappa][blkdim]) of H0(k) v = E S(k) v."""

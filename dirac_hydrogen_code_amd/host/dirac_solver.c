@@ -12,6 +12,17 @@
  * time += dt (the reference's `while (time <= maxtime)` runs Ntime or Ntime+1
  * steps depending on rounding).  Outputs: x<step>.out (int32 nrows, int32
  * ncols=1, complex128[], un-permuted), runstats.txt ("iters, tol, usec" per step).
+ *
+ * Checkpoint (-k file, with -f): written at every output with psi, the step, the accumulated time and the
+ * solver state (gade_checkpoint_save); with -s k > 1 the run resumes from it (its step must be k - 1) and
+ * continues bit for bit as the uninterrupted run (the reference's -s only shifts the start time and restarts
+ * from psi0.vec, tools/dirac_solver.c:203-209).  Output windows end at the multiples of -w and at the last
+ * step; the time is accumulated step by step across windows.
+ *
+ * Post-processing (-x file, then exit): the eigen-projection of a saved state, as src/process_output.cpp:
+ * 320-355 and src/dumpEigvals.cpp:239-252 produce it: evl<kappa>.mat / evc<kappa>.mat for every kappa and
+ * psiev_<file> with, per angular state th (kept states in original order), "evl[th] = npy.array((...))" and
+ * "psiev[th] = npy.array((...))", psiev_j = psi_th^H S v_j (src/hamiltonian.h:1333-1364).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -25,6 +36,7 @@
 #include "gpu_solver.h"
 #include "gpu_sparse.h"
 #include "solver.h"
+#include "solver_common.h"
 
 #define FAIL(...)                     \
     do {                              \
@@ -50,6 +62,48 @@ static void vec_read(const char *fname, int n, csr_data_t *out, int *perm)
     fclose(fd);
 }
 
+/* src/hamiltonian.h:1338: Eigen IOFormat(FullPrecision, DontAlignCols, ", ", "\n", "(", "),", " = npy.array((\n",
+   "\n))\n"); complex entries print as (re,im) */
+static void write_psiev(const char *fname, gade_model_t *m, const double *xdev, const int *hall_perm, int nblk, int n)
+{
+    double *psiev = malloc(sizeof(double) * 2 * (size_t)nblk * n);
+    int nk = 0, kap[256];
+    gade_model_info(m, NULL, NULL, NULL, NULL, NULL, &nk);
+    double *E = malloc(sizeof(double) * (size_t)nk * n);
+    if (nk > 256 || gade_psiev(m, xdev, psiev) || gade_eig_values(m, &nk, kap, E))
+        FAIL("eigen-projection: %s\n", gade_last_error());
+    /* blocks in the order of their original angular index */
+    int *order = malloc(sizeof(int) * (size_t)nblk);
+    for (int r = 0; r < nblk; r++) order[r] = r;
+    for (int a = 1; a < nblk; a++)
+        for (int b = a; b > 0 && hall_perm && hall_perm[order[b - 1]] > hall_perm[order[b]]; b--) {
+            int t = order[b]; order[b] = order[b - 1]; order[b - 1] = t;
+        }
+    FILE *fd = fopen(fname, "w");
+    if (!fd) FAIL("cant open %s\n", fname);
+    for (int th = 0; th < nblk; th++) {
+        const int r = order[th];
+        const int k = ik(hall_perm ? hall_perm[r] : r);
+        int q = 0;
+        while (q < nk && kap[q] != k) q++;
+        if (q == nk) FAIL("eigen-projection: kappa %d of block %d has no eigenpairs\n", k, r);
+        fprintf(fd, "evl[%d] = npy.array((\n", th);
+        for (int j = 0; j < n; j++) fprintf(fd, "%s(%.17g),", j ? "\n" : "", E[(size_t)q * n + j]);
+        fprintf(fd, "\n))\n\n");
+        fprintf(fd, "psiev[%d] = npy.array((\n(", th);
+        for (int j = 0; j < n; j++) {
+            const double *c = psiev + 2 * ((size_t)r * n + j);
+            fprintf(fd, "%s(%.17g,%.17g)", j ? ", " : "", c[0], c[1]);
+        }
+        fprintf(fd, "),\n))\n\n");
+    }
+    fclose(fd);
+    free(order);
+    free(E);
+    free(psiev);
+    printf("saved %s\n", fname);
+}
+
 static void write_out(const char *dir, int step, const csr_data_t *x, const int *perm, int n)
 {
     char fname[4096];
@@ -64,6 +118,20 @@ static void write_out(const char *dir, int step, const csr_data_t *x, const int 
     fclose(fd);
     free(xo);
     printf("saving output file %s\n", fname);
+}
+
+/* runstats.txt, rewritten at every output as the reference does (tools/dirac_solver.c:622-628): one line
+   "iters, tol, usec" per step from step 1; steps before a resumed run's first step are zeros */
+static void write_stats(const char *dir, int first_step, int done, const int *iters, const double *tols,
+                        const long *usec)
+{
+    char fname[4096];
+    snprintf(fname, sizeof fname, "%s/runstats.txt", dir);
+    FILE *fs = fopen(fname, "w");
+    if (!fs) FAIL("cant open %s\n", fname);
+    for (int s = 1; s < first_step; s++) fprintf(fs, "%d, %e, %ld\n", 0, 0.0, 0L);
+    for (int s = 0; s < done; s++) fprintf(fs, "%d, %e, %ld\n", iters[s], tols[s], usec[s]);
+    fclose(fs);
 }
 
 typedef struct {
@@ -87,8 +155,8 @@ int main(int argc, char *argv[])
 {
     int opt, lmax = 1, maxit = 500, fused = 0, write_every = 2000, iter = 1;
     double maxtime = 0, intensity = 0, omega = 50, cycles = 1, ntime = 0, tol = 1e-12;
-    const char *dir = ".";
-    while ((opt = getopt(argc, argv, "t:l:i:o:c:n:s:e:m:d:w:f")) != -1) {
+    const char *dir = ".", *xfile = NULL, *ckfile = NULL;
+    while ((opt = getopt(argc, argv, "t:l:i:o:c:n:s:e:m:d:w:fx:k:")) != -1) {
         switch (opt) {
         case 't': maxtime = atof(optarg); break;
         case 'l': lmax = atoi(optarg); break;
@@ -102,10 +170,13 @@ int main(int argc, char *argv[])
         case 'd': dir = optarg; break;
         case 'w': write_every = atoi(optarg); break;
         case 'f': fused = 1; break;
-        default: FAIL("Usage: %s -t time -l lmax -i intensity -o omega -c cycles -n Ntime [-s step -e tol -m maxit -d dir -w every -f]\n", argv[0]);
+        case 'x': xfile = optarg; break;
+        case 'k': ckfile = optarg; break;
+        default: FAIL("Usage: %s -t time -l lmax -i intensity -o omega -c cycles -n Ntime [-s step -e tol -m maxit -d dir -w every -f -x file -k file]\n", argv[0]);
         }
     }
     if (ntime <= 0 || maxtime <= 0 || iter < 1) FAIL("need -t maxtime > 0, -n Ntime > 0 and -s >= 1\n");
+    if (ckfile && !fused) FAIL("-k (checkpoint / resume) runs with the fused operator: add -f\n");
     const double dt = maxtime / ntime, h = 1;
     const int nsteps = (int)ntime - (iter - 1);
     printf("Parameters:\n - time       %f\n - lmax       %d\n - intensity  %lf\n - omega      %lf\n - cycles     %lf\n"
@@ -123,6 +194,32 @@ int main(int argc, char *argv[])
     int *perm = malloc(sizeof(int) * (size_t)n);
     snprintf(fname, sizeof fname, "%s/psi0.vec", dir);
     vec_read(fname, n, psi, perm);
+    if (xfile) {
+        /* post-processing of a saved state: read it (original order), permute into the solver's block order */
+        gade_model_t *m = NULL;
+        if (gade_model_load(dir, lmax, h, dt, NULL, &m)) FAIL("gade_model_load: %s\n", gade_last_error());
+        FILE *fx = fopen(xfile, "rb");
+        if (!fx) FAIL("cant open %s\n", xfile);
+        int hdr[2];
+        csr_data_t *xo = malloc(sizeof(csr_data_t) * (size_t)n);
+        if (fread(hdr, sizeof(int), 2, fx) != 2 || hdr[0] != n || hdr[1] != 1 ||
+            fread(xo, sizeof(csr_data_t), (size_t)n, fx) != (size_t)n)
+            FAIL("wrong file format in %s\n", xfile);
+        fclose(fx);
+        for (int i = 0; i < n; i++) psi[i] = xo[perm[i]];
+        double *dx = NULL;
+        if (gade_malloc((void **)&dx, sizeof(csr_data_t) * (size_t)n) || gade_memcpy_h2d(dx, psi, sizeof(csr_data_t) * (size_t)n))
+            FAIL("%s\n", gade_last_error());
+        if (gade_eig_write(m, dir)) FAIL("gade_eig_write: %s\n", gade_last_error());
+        const char *base = strrchr(xfile, '/');
+        base = base ? base + 1 : xfile;
+        snprintf(fname, sizeof fname, "%s/psiev_%s", dir, base);
+        write_psiev(fname, m, dx, Hall.perm, Hall.nrows, blkdim);
+        gade_free(dx);
+        gade_model_destroy(m);
+        free(xo);
+        return 0;
+    }
     int *itersarr = calloc((size_t)nsteps, sizeof(int));
     double *tolsarr = calloc((size_t)nsteps, sizeof(double));
     long *timearr = calloc((size_t)nsteps, sizeof(long));
@@ -137,17 +234,33 @@ int main(int argc, char *argv[])
         if (gade_malloc((void **)&dpsi, sizeof(csr_data_t) * (size_t)n) ||
             gade_memcpy_h2d(dpsi, psi, sizeof(csr_data_t) * (size_t)n))
             FAIL("%s\n", gade_last_error());
+        if (ckfile && iter > 1) {
+            /* resume: psi, step, accumulated time and the solver state of the checkpoint */
+            int done = 0;
+            if (gade_checkpoint_load(m, dpsi, &done, &time, ckfile)) FAIL("resume: %s\n", gade_last_error());
+            if (done != iter - 1) FAIL("resume: %s holds step %d, -s %d needs step %d\n", ckfile, done, iter, iter - 1);
+            printf("resumed from %s at step %d, time %.17g\n", ckfile, done, time);
+        }
         double *usec = calloc((size_t)nsteps, sizeof(double));
-        for (int s0 = 0; s0 < nsteps; s0 += write_every) {
-            int cnt = nsteps - s0 < write_every ? nsteps - s0 : write_every;
+        const int last = iter + nsteps - 1;
+        for (int s0 = 0; s0 < nsteps;) {
+            /* windows end at the output steps (multiples of write_every) and at the last step */
+            const int first = iter + s0;
+            int end = (first + write_every - 1) / write_every * write_every;
+            if (end > last) end = last;
+            const int cnt = end - first + 1;
             if (gade_cn_run(m, dpsi, intensity, omega, cycles, time, cnt, tol, maxit, itersarr + s0, tolsarr + s0,
                             usec + s0))
                 FAIL("gade_cn_run: %s\n", gade_last_error());
-            time += cnt * dt;
+            for (int k = 0; k < cnt; k++) time = time + dt; /* as gade_cn_run accumulated it (tools/dirac_solver.c:509) */
+            for (int k = 0; k < cnt; k++) timearr[s0 + k] = (long)usec[s0 + k];
+            s0 += cnt;
             if (gade_memcpy_d2h(psi, dpsi, sizeof(csr_data_t) * (size_t)n)) FAIL("%s\n", gade_last_error());
-            write_out(dir, iter + s0 + cnt - 1, psi, perm, n);
+            write_out(dir, end, psi, perm, n);
+            write_stats(dir, iter, s0, itersarr, tolsarr, timearr);
+            if (ckfile && gade_checkpoint_save(m, dpsi, end, time, ckfile))
+                FAIL("checkpoint: %s\n", gade_last_error());
         }
-        for (int s = 0; s < nsteps; s++) timearr[s] = (long)usec[s];
         free(usec);
         gade_free(dpsi);
         gade_model_destroy(m);
@@ -234,10 +347,6 @@ int main(int argc, char *argv[])
         }
     }
     printf("Total runtime: %.3f s\n", (now_us() - t_all) * 1e-6);
-    snprintf(fname, sizeof fname, "%s/runstats.txt", dir);
-    FILE *fs = fopen(fname, "w");
-    if (!fs) FAIL("cant open %s\n", fname);
-    for (int s = 0; s < nsteps; s++) fprintf(fs, "%d, %e, %ld\n", itersarr[s], tolsarr[s], timearr[s]);
-    fclose(fs);
+    write_stats(dir, iter, nsteps, itersarr, tolsarr, timearr);
     return 0;
 }

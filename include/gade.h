@@ -205,6 +205,16 @@ int gade_lg_uses(gade_model_t *m, int *counts);
 int gade_autotune(gade_model_t *m, const double *psi_dev, const double *params, double t0, int nsteps,
                   double tol, int maxit, int reps, double *report, int *ncand, int *chosen);
 
+/* Checkpoint / resume of a CN run (the reference's -s flag only shifts the start time,
+ * tools/dirac_solver.c:203-209, and restarts from psi0.vec).  gade_checkpoint_save writes psi_dev, the step
+ * number and the accumulated time (exact bits) with every piece of solver state that carries from one step to
+ * the next (the BiCGSTAB workspace, scalars, control words and the extrapolation history); after
+ * gade_checkpoint_load into a model of the same couplings, batch, precision and guess order, gade_cn_run from
+ * t0 = *time continues the run bit for bit as if it had not stopped.  The learned guess's state is not saved
+ * (GADE_EINVAL while it is active).  No reference counterpart for the state. */
+int gade_checkpoint_save(gade_model_t *m, const double *psi_dev, int step, double time, const char *file);
+int gade_checkpoint_load(gade_model_t *m, double *psi_dev, int *step, double *time, const char *file);
+
 /* Pulse coefficients f_alpha(t), alpha = 0..5 (tools/solver_common.c:46-65),
  * as 12 doubles (complex with zero imaginary part). */
 void gade_pulse(double E0, double omega, double cycles, double t, double *ft);
@@ -236,10 +246,24 @@ int gade_extrapolate(gade_model_t *m, double *psi_dev, int order);
  * per_block (optional) receives [nblk][4]: negative continuum (E < -c^2),
  * bound, ionised, psi_r^H S psi_r. */
 int gade_eig_setup(gade_model_t *m);
+/* The dense solver behind gade_eig_setup (host, no reference counterpart; the reference uses Eigen's
+ * GeneralizedSelfAdjointEigenSolver, src/hamiltonian.h:872-943): real symmetric A, symmetric positive
+ * definite B, both n x n row-major; E [n] ascending eigenvalues, V [n][n] row-major with column j the
+ * eigenvector of E[j], V^T B V = I, each column's largest-magnitude entry positive.  GADE_EINVAL when B is
+ * not positive definite, GADE_ENOTCONV when the iteration does not converge. */
+int gade_eig_dense(int n, const double *A, const double *B, double *E, double *V);
 /* distinct kappa values [nkappa] and energies [nkappa][blkdim] (ascending per kappa) */
 int gade_eig_values(gade_model_t *m, int *nkappa, int *kappa, double *energies);
 int gade_populations(gade_model_t *m, const double *x_dev, double *norm, double *bound, double *ionised,
                      double *per_block);
+/* Eigen-state amplitudes of a state, as src/hamiltonian.h:1246-1331 (eigProj) computes them for
+ * savePsievs: psiev [nblk][blkdim] complex (host), psiev[r][j] = psi_r^H S(k_r) v_j over the eigenpairs
+ * of block r's kappa (E ascending, gade_eig_values). */
+int gade_psiev(gade_model_t *m, const double *x_dev, double *psiev);
+/* The eigenpair files of src/dumpEigvals.cpp:239-252 for every distinct kappa of the model, in dir:
+ * evl<kappa>.mat (int64 rows = blkdim, int64 cols = 1, double energies ascending) and evc<kappa>.mat
+ * (int64 blkdim, int64 blkdim, complex128 column-major: column j the S-normalized eigenvector of energy j). */
+int gade_eig_write(gade_model_t *m, const char *dir);
 
 /* Assembled reference formulation (kept as the plain-FP64 HBM comparator):
  * builds Hfull Ap/Ai/Ai_sub_map on the device (tools/csr.c:653-743), assembles

@@ -255,3 +255,26 @@ def populations(directory, psi, csq=137.035999084 ** 2):
         out[r] = [p[E < -csq].sum(), p[(E >= -csq) & (E < 0)].sum(), p[E >= 0].sum(), np.vdot(psi[r], w).real]
     return dict(norm=out[:, 3].sum(), bound=out[:, 1].sum(), ionised=out[:, 2].sum(), negative=out[:, 0].sum(),
                 per_block=out, energies={k: v[0] for k, v in cache.items()})
+
+
+def eigenpairs(directory):
+    """Per kappa of the H.csr blocks: the generalized eigenpairs of the real parts of H0(k), S(k)
+    (src/hamiltonian.h:906-943 takes .real() of both) from scipy's eigh, E ascending, v^T S v = 1, each
+    eigenvector's largest-magnitude entry made positive.  Returns {kappa: (E, V, S)} with V[:, j] = v_j."""
+    import scipy.linalg as sl
+    s = [_dense(os.path.join(directory, f"s{i}.csr")).real for i in range(3)]
+    h = [_dense(os.path.join(directory, f"h0{i}.csr")).real for i in range(4)]
+    nblk, _, _, _, perm = _read_csr(os.path.join(directory, "H.csr"))
+    perm = np.arange(nblk) if perm is None else perm
+    out = {}
+    for r in range(nblk):
+        k = lib().or_ik(int(perm[r]))
+        if k in out:
+            continue
+        S = s[0] + k * s[1] + k * k * s[2]
+        H = h[0] + k * h[1] + k * k * h[2] + k ** 3 * h[3]
+        E, V = sl.eigh(H, S)
+        sign = np.sign(V[np.argmax(np.abs(V), axis=0), np.arange(V.shape[1])])
+        out[k] = (E, V * sign, S)
+    return out
+

@@ -209,3 +209,51 @@ def test_cpu_matfree_and_bicgstab_match_oracle(datadir, tmp_path, cfg, lmax):
     assert 0 < iters < 2000 and terr <= tol and res <= 10 * tol, (iters, terr, res)
     zo, _, _, _ = m.bicgstab(rhs, x, tol=1e-13)
     assert np.linalg.norm(z - zo) <= 1e-9 * np.linalg.norm(zo)
+
+
+def gen_eig(A, B):
+    L = gade.lib()
+    n = A.shape[0]
+    E = np.zeros(n)
+    V = np.zeros((n, n))
+    L.gade_eig_dense.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    A = np.ascontiguousarray(A, np.float64)
+    B = np.ascontiguousarray(B, np.float64)
+    rc = L.gade_eig_dense(n, A.ctypes.data, B.ctypes.data, E.ctypes.data, V.ctypes.data)
+    return rc, E, V
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (7, 2), (60, 3), (200, 4)])
+def test_dense_generalized_eigensolver_matches_scipy(n, seed):
+    """gade_eig_dense (host setup of the populations and evl / evc files) against scipy.linalg.eigh on random
+    symmetric-definite pencils: eigenvalues to 1e-12 of the spectrum's scale, A V = B V diag(E) and
+    V^T B V = I to 1e-11, eigenvectors equal to scipy's up to sign (well-separated random spectra)."""
+    from scipy.linalg import eigh
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    G = rng.standard_normal((n, n))
+    B = G @ G.T + n * np.eye(n)
+    rc, E, V = gen_eig(A, B)
+    assert rc == 0
+    Er, Vr = eigh(A, B)
+    scale = np.abs(Er).max()
+    assert np.abs(E - Er).max() <= 1e-12 * scale
+    assert np.abs(A @ V - B @ V * E).max() <= 1e-11 * scale * np.abs(B).max()
+    assert np.abs(V.T @ B @ V - np.eye(n)).max() <= 1e-11
+    for j in range(n):
+        s = np.sign(Vr[np.argmax(np.abs(Vr[:, j])), j])
+        assert np.abs(V[:, j] - s * Vr[:, j]).max() <= 1e-8 * np.abs(Vr[:, j]).max()
+        assert V[np.argmax(np.abs(V[:, j])), j] > 0
+
+
+def test_dense_eigensolver_known_answers():
+    """Diagonal pencils give the sorted ratios and unit vectors scaled by 1/sqrt(b); a repeated eigenvalue
+    keeps an orthonormal (B-orthonormal) basis; an indefinite B is refused."""
+    rc, E, V = gen_eig(np.diag([3.0, -1.0, 2.0]), np.diag([1.0, 4.0, 2.0]))
+    assert rc == 0 and np.allclose(E, [-0.25, 1.0, 3.0], rtol=0, atol=1e-15)
+    assert np.allclose(np.abs(V), np.array([[0, 0, 1], [0.5, 0, 0], [0, 2 ** -0.5, 0]]), atol=1e-15)
+    rc, E, V = gen_eig(np.diag([1.0, 1.0, 5.0]), np.eye(3))
+    assert rc == 0 and np.allclose(E, [1, 1, 5]) and np.allclose(V.T @ V, np.eye(3), atol=1e-15)
+    rc, _, _ = gen_eig(np.eye(2), np.diag([1.0, -1.0]))
+    assert rc == -1
