@@ -516,8 +516,10 @@ constexpr int kDiaBW = 7;
 // that lanes reading positions 2*lane+t hit distinct banks
 // (P = 2); for P = 4 (lanes reading positions 4*lane+t) the low two bits are XORed instead.  Both are
 // conflict-free for the sliding reads and for the stride-1 staging writes (64 banks, 16 lanes per pass).
+// With one output per lane (P = 1) the lanes read consecutive positions: no swizzle.
 template <int P>
 __device__ __forceinline__ int dia_swz(int p) {
+    if constexpr (P == 1) return p;
     if constexpr (P == 4) return p ^ (((p >> 2) ^ (p >> 4)) & 3);
     return p ^ (((p >> 3) ^ (p >> 4)) & 1);
 }
@@ -611,8 +613,10 @@ __global__ void k_build_DD(const double2 *__restrict__ D, const int *__restrict_
 
 // DUAL: a second output y2/ypart2 = (D_sel2 + G-part) x from the same G sweeps (beta must be 0); with
 // cH = -1, sel = 1, sel2 = 0 this is the CN pair (S - H) x, (S + H) x, each bit-identical to its own launch.
-template <int BW, bool DUAL, class T, int R, int P>
-__global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, const T *__restrict__ V,
+// HW = 2: one workgroup of two waves serves both output halves h of a tile (wave h), staging each window once
+// for both; HW = 1: one wave per (tile, h).
+template <int BW, bool DUAL, class T, int R, int P, int HW = 1>
+__global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x, const T *__restrict__ V,
                                                    const T *__restrict__ GD, const T *__restrict__ DD,
                                                    const int *__restrict__ tiles, const int *__restrict__ tile_q,
                                                    const double *__restrict__ tile_k, int nblk, int n, int nh,
@@ -624,14 +628,14 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
     constexpr int SEG = 64 * P, ND = 2 * BW + 1, W = SEG + 2 * BW;
     using Rt = typename cx<T>::R;
     __shared__ T sm[R * W];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     int rest = blockIdx.x;
     const int l = rest % lgrid;
     rest /= lgrid;
     const int seg = rest % nseg;
     rest /= nseg;
-    const int h = rest & 1;
-    const int tile = rest >> 1;
+    const int h = HW == 2 ? (int)(threadIdx.x >> 6) : (rest & 1);
+    const int tile = HW == 2 ? rest : rest >> 1;
     const int inst = tile_b[tile];
     GADE_GATED_B(gate, gstride, inst);
     GD += (size_t)inst * gd_stride;  // this instance's G-sums
@@ -653,14 +657,14 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
     // window of half hp of 'src' for every row of the tile: position p <-> half index i0 - BW + p.
     // All loads of the window are issued before the first LDS write.
     auto stage = [&](const T *src, int hp) {
-        constexpr int NC = (W + 63) / 64;
+        constexpr int NT = 64 * HW, NC = (W + NT - 1) / NT;
         T t[R][NC];
 #pragma unroll
         for (int i = 0; i < R; i++) {
             const T *rp = src + (size_t)(rows[i] >= 0 ? rows[i] : 0) * n + hp * nh;
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const int p = c * 64 + lane;
+                const int p = c * NT + (int)threadIdx.x;
                 const int b = i0 - BW + p;
                 t[i][c] = czero<T>();
                 if (rows[i] >= 0 && p < W && b >= 0 && b < nh) t[i][c] = rp[b];
@@ -671,7 +675,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
         for (int i = 0; i < R; i++)
 #pragma unroll
             for (int c = 0; c < NC; c++) {
-                const int p = c * 64 + lane;
+                const int p = c * NT + (int)threadIdx.x;
                 if (p < W) sm[i * W + dia_swz<P>(p)] = t[i][c];
             }
         __syncthreads();
@@ -679,7 +683,7 @@ __global__ __launch_bounds__(64) void k_radial_dia(const T *__restrict__ x, cons
     // acc[i][j] += sum_d M_d[i0 + P*lane + j] * window[P*lane + j + d]; with two coefficient
     // streams M = C1 + kap*C2 (pairs), else M = C1.  Coefficients are fetched PF diagonals ahead.
     auto sweep = [&](const T *C, bool pair) {
-        constexpr int PF = P >= 4 ? 2 : 3;  // coefficient diagonals in flight
+        constexpr int PF = P >= 4 ? 2 : P == 2 ? 3 : (R <= 8 ? 8 : 4);  // coefficient diagonals in flight
         const int stride = pair ? 2 : 1;
         const T *cp = C + (size_t)(i0 + P * lane) * stride;
         T w[R][P];
@@ -1272,6 +1276,12 @@ int launch_angular(gade_model *m, const double2 *x) {
     return launch_angular_t<double2>(m, x, m->d_coef, m->d_acf, m->d_V);
 }
 
+// tile shapes (rows per tile, outputs per lane) with an instantiated banded kernel: rows x outputs <= 16
+// complex accumulators per lane
+bool dia_shape_ok(int r, int p) {
+    return (r == 4 && (p == 1 || p == 2 || p == 4)) || (r == 8 && (p == 1 || p == 2)) || (r == 16 && p == 1);
+}
+
 // the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
 // each output in the same order, so all give the same results bit for bit
 template <bool DUAL, class T>
@@ -1279,20 +1289,30 @@ int launch_dia(gade_model *m, int lgrid, const T *x, const T *V, const T *GD, co
                double beta, T *y, T *yp, T *y2, T *yp2, int sel2) {
     const int P = m->dia_p;
     const int nseg = (m->nh + 64 * P - 1) / (64 * P);
-    const dim3 grid((unsigned)m->ndtiles * 2 * nseg * lgrid);
-#define GADE_DIA(R_, P_)                                                                                        \
-    hipLaunchKernelGGL((k_radial_dia<kDiaBW, DUAL, T, R_, P_>), grid, dim3(64), 0, m->stream, x, V, GD, DD,    \
-                       m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, nseg, lgrid, m->nq, \
-                       sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b, m->gd_stride, y2, yp2, sel2)
+    const int hw = m->dia_hw;
+    const dim3 grid((unsigned)m->ndtiles * (2 / hw) * nseg * lgrid);
+#define GADE_DIA1(R_, P_, HW_)                                                                                    \
+    hipLaunchKernelGGL((k_radial_dia<kDiaBW, DUAL, T, R_, P_, HW_>), grid, dim3(64 * HW_), 0, m->stream, x, V, GD, \
+                       DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, nseg, lgrid,     \
+                       m->nq, sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b, m->gd_stride, y2, yp2, sel2)
+#define GADE_DIA(R_, P_)          \
+    if (hw == 2)                  \
+        GADE_DIA1(R_, P_, 2);     \
+    else                          \
+        GADE_DIA1(R_, P_, 1)
     switch (m->dia_r * 10 + P) {
         case 82: GADE_DIA(8, 2); break;
         case 42: GADE_DIA(4, 2); break;
         case 44: GADE_DIA(4, 4); break;
+        case 81: GADE_DIA(8, 1); break;
+        case 41: GADE_DIA(4, 1); break;
+        case 161: GADE_DIA(16, 1); break;
         default:
-            gade::set_error("unsupported banded tile shape (rows, outputs per lane): (8,2), (4,2) or (4,4)");
+            gade::set_error("unsupported banded tile shape (rows, outputs per lane)");
             return GADE_EINVAL;
     }
 #undef GADE_DIA
+#undef GADE_DIA1
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -1491,7 +1511,6 @@ int setup_mfma(gade_model *m) {
     GADE_HIP(hipGetLastError());
     GADE_HIP(hipStreamSynchronize(m->stream));
     m->mfma_ok = true;
-    m->radial_mfma = 1;
     return GADE_OK;
 }
 
@@ -2943,21 +2962,30 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         times[step] = time;
     }
     struct Cand {
-        int window, iter_batch, graph, dia_r, dia_p, ang_lb;
+        int window, iter_batch, graph, dia_r, dia_p, ang_lb, dia_hw;
         double ms;
     };
     // stage 1: kernel shapes (banded tile shape x angular multipole group) with the model's stepping;
     // stage 2: stepping with the best shapes
+    // (the FP64 MFMA radial kernel has one shape: only the angular group is tuned then; the tile shape still
+    // matters to the FP32 kernels of the mixed-precision form)
     std::vector<Cand> cands;
+    const bool one_tile = m->radial_mfma && m->precision == 0;
     if (m->radial_kind == 2)
-        for (auto rp : {std::pair<int, int>{8, 2}, {4, 2}, {4, 4}})
-            for (int lb : {4, 2, 8})
-                if (ang_lds_ok(m, lb)) cands.push_back({m->window, m->iter_batch, m->use_graph ? 1 : 0, rp.first, rp.second, lb, 0.0});
+        for (auto rp : {std::pair<int, int>{8, 2}, {4, 2}, {4, 4}, {8, 1}, {16, 1}, {4, 1}}) {
+            if (one_tile && (rp.first != m->dia_r || rp.second != m->dia_p)) continue;
+            for (int hw : {1, 2}) {
+                if (one_tile && hw != m->dia_hw) continue;
+                for (int lb : {4, 2, 8})
+                    if (ang_lds_ok(m, lb))
+                        cands.push_back({m->window, m->iter_batch, m->use_graph ? 1 : 0, rp.first, rp.second, lb, hw, 0.0});
+            }
+        }
     const size_t nshape = cands.size();
     for (int g : {1, 0}) {
         if (g && (m->radial_kind != 2 || !m->use_graph)) continue;  // graph 0 on the model excludes graphs
         for (int w : {2, 4, 8, 16, 32})
-            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, -1, 0.0});
+            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, -1, -1, 0.0});
     }
     const size_t N = (size_t)m->N, B = (size_t)m->B;
     // the solver state a run starts from (BiCGSTAB workspace, which carries over between solves, scalars,
@@ -3000,12 +3028,13 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         return GADE_OK;
     };
     const int sv_window = m->window, sv_batch = m->iter_batch, sv_profile = m->profile, sv_k = m->last_k;
-    const int sv_r = m->dia_r, sv_p = m->dia_p, sv_lb = m->ang_lb;
+    const int sv_r = m->dia_r, sv_p = m->dia_p, sv_lb = m->ang_lb, sv_hw = m->dia_hw;
     const bool sv_graph = m->use_graph;
-    auto set_shape = [&](int r, int pp, int lb) -> int {
-        if (lb != m->ang_lb) {
+    auto set_shape = [&](int r, int pp, int lb, int hw) -> int {
+        if (lb != m->ang_lb || hw != m->dia_hw) {
             drop_step_graph(m);
             m->ang_lb = lb;
+            m->dia_hw = hw;
         }
         if (!m->dia_ok || (r == m->dia_r && pp == m->dia_p)) return GADE_OK;
         drop_step_graph(m);
@@ -3021,7 +3050,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         m->use_graph = sv_graph;
         m->profile = sv_profile;
         m->last_k = sv_k;
-        (void)set_shape(sv_r, sv_p, sv_lb);
+        (void)set_shape(sv_r, sv_p, sv_lb, sv_hw);
         const int rc = copy_state(false);
         (void)hipStreamSynchronize(m->stream);
         (void)hipFree(work);
@@ -3042,7 +3071,8 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         if (c == nshape && nshape > 0) {
             for (size_t k = 1; k < nshape; k++)
                 if (cands[k].ms < cands[best_shape].ms) best_shape = k;
-            if (int rc = set_shape(cands[best_shape].dia_r, cands[best_shape].dia_p, cands[best_shape].ang_lb)) {
+            if (int rc = set_shape(cands[best_shape].dia_r, cands[best_shape].dia_p, cands[best_shape].ang_lb,
+                                   cands[best_shape].dia_hw)) {
                 restore();
                 return rc;
             }
@@ -3051,7 +3081,8 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             cands[c].dia_r = m->dia_r;
             cands[c].dia_p = m->dia_p;
             cands[c].ang_lb = m->ang_lb;
-        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p, cands[c].ang_lb)) {
+            cands[c].dia_hw = m->dia_hw;
+        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p, cands[c].ang_lb, cands[c].dia_hw)) {
             restore();
             return rc;
         }
@@ -3093,7 +3124,8 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             gade::set_error("gade_autotune: candidate window " + std::to_string(cands[c].window) + " iter_batch " +
                             std::to_string(cands[c].iter_batch) + " graph " + std::to_string(cands[c].graph) +
                             " tile " + std::to_string(cands[c].dia_r) + "x" + std::to_string(cands[c].dia_p) +
-                            " angular group " + std::to_string(cands[c].ang_lb) +
+                            " halves " + std::to_string(cands[c].dia_hw) + " angular group " +
+                            std::to_string(cands[c].ang_lb) +
                             " changed the trajectory");
             return GADE_EINVAL;
         }
@@ -3105,7 +3137,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
     m->window = cands[pick].window;
     m->iter_batch = cands[pick].iter_batch;
     m->use_graph = cands[pick].graph != 0;
-    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p, cands[pick].ang_lb));
+    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p, cands[pick].ang_lb, cands[pick].dia_hw));
     if (report)
         for (size_t c = 0; c < cands.size() && c < GADE_TUNE_MAX; c++) {
             double *r = report + (size_t)GADE_TUNE_COLS * c;
@@ -3116,6 +3148,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             r[4] = cands[c].dia_p;
             r[5] = cands[c].ang_lb;
             r[6] = cands[c].ms;
+            r[7] = cands[c].dia_hw;
         }
     if (ncand) *ncand = (int)std::min<size_t>(cands.size(), GADE_TUNE_MAX);
     if (chosen) *chosen = (int)pick;
@@ -3340,11 +3373,22 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->ang_lb = value;
         return GADE_OK;
     }
+    if (!strcmp(name, "dia_halves")) {
+        // output halves per banded workgroup: 1 (one wave per half) or 2 (two waves share each window)
+        if (value != 1 && value != 2) {
+            gade::set_error("dia_halves must be 1 or 2");
+            return GADE_EINVAL;
+        }
+        if (value != m->dia_hw) drop_step_graph(m);
+        m->dia_hw = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "dia_rows") || !strcmp(name, "dia_outputs")) {
         const bool rows = name[4] == 'r';
         const int r = rows ? value : m->dia_r, pp = rows ? m->dia_p : value;
-        if (!((r == 8 && pp == 2) || (r == 4 && pp == 2) || (r == 4 && pp == 4))) {
-            gade::set_error("banded tile shape (dia_rows, dia_outputs) must be (8,2), (4,2) or (4,4)");
+        if (!dia_shape_ok(r, pp)) {
+            gade::set_error("banded tile shape (dia_rows, dia_outputs) must be (4,1), (4,2), (4,4), (8,1), (8,2) "
+                            "or (16,1)");
             return GADE_EINVAL;
         }
         if (!m->dia_ok) {
@@ -3367,7 +3411,8 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         return GADE_OK;
     }
     if (!strcmp(name, "radial_mfma")) {
-        // 1: the FP64 banded radial products on the matrix cores (default when the pattern allows), 0: VALU
+        // 1: the FP64 banded radial products on the matrix cores (when the pattern allows), 0: VALU (default:
+        // on gfx950 the FP64 matrix rate equals the vector rate, and the 4x4x4 band form carries zero padding)
         if ((value != 0 && value != 1) || (value == 1 && !m->mfma_ok)) {
             gade::set_error(value == 1 ? "the radial pattern does not fit the MFMA kernel" : "radial_mfma must be 0 or 1");
             return GADE_EINVAL;
@@ -3504,6 +3549,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "psolve_chunk")) {
         *value = m->pre ? m->pre->ch : m->psolve_ch;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "dia_halves")) {
+        *value = m->dia_hw;
         return GADE_OK;
     }
     if (!strcmp(name, "dia_outputs")) {

@@ -127,9 +127,10 @@ struct gade_model {
     double *d_dtile_k = nullptr;
     int *d_dtile_b = nullptr;      // instance of each tile (tiles never mix instances)
     int dia_r = 8, dia_p = 2;      // banded tile shape: rows per tile, outputs per lane
+    int dia_hw = 1;                // output halves per banded workgroup (2: both halves share each window)
     // FP64 banded radial phase on the matrix cores: waves of two 8-row halves of one kappa each
     bool mfma_ok = false;          // the banded pattern fits the MFMA kernel (set up)
-    int radial_mfma = 0;           // 1: FP64 radial products on MFMA (k_radial_mfma), 0: k_radial_dia
+    int radial_mfma = 0;           // 1: FP64 radial products on MFMA (k_radial_mfma), 0: k_radial_dia (default)
     int nmi = 0, nmg = 0;          // i-steps of 4 outputs per half; MFMA wave groups
     size_t gf_stride = 0;          // GF doubles per instance
     double *d_GF = nullptr;        // [B][lmax][4][4][nmi][5][16][4] G-sum pairs in MFMA fragment order

@@ -138,18 +138,20 @@ int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, co
 /* Autotuner of the device-driven stepping (no reference counterpart): times nsteps probe steps from
  * psi_dev (left unchanged) with the pulses params [batch][4] (as gade_cn_run_batch, t0 the start time),
  * best of reps runs per candidate.  Stage 1 (banded radial kernel): the tile shapes (dia_rows,
- * dia_outputs) in {(8,2), (4,2), (4,4)} x the angular multipole group ang_lb in {4, 2, 8}, with the
- * model's stepping; stage 2, with the fastest shapes:
+ * dia_outputs) in {(8,2), (4,2), (4,4), (8,1), (16,1), (4,1)} x output halves per workgroup dia_halves in
+ * {1, 2} x the angular multipole group ang_lb in {4, 2, 8}, with the model's stepping (with the FP64 MFMA
+ * radial kernel in FP64 only ang_lb varies); stage 2, with the fastest shapes:
  * window in {2,4,8,16,32}, iter_batch in {0,1,2,4}, graph in {0,1} (the graph only with the banded
  * kernel and when the model's "graph" parameter is 1).  The fastest stage-2 candidate is kept.  Every candidate
  * must give the same trajectory, iterations and tolerances bit for bit, else GADE_EINVAL with the
  * message naming it.  report [GADE_TUNE_MAX][GADE_TUNE_COLS] (NULL allowed) receives (window,
- * iter_batch, graph, dia_rows, dia_outputs, ang_lb, ms) per candidate and *ncand their count; the chosen index
+ * iter_batch, graph, dia_rows, dia_outputs, ang_lb, ms, dia_halves) per candidate and *ncand their count; the
+ * chosen index
  * goes to *chosen.  Every probe starts from the
  * model's current solver state (BiCGSTAB workspace, scalars, guess histories), which is restored on
  * return: a run after tuning equals the same run without it. */
-#define GADE_TUNE_MAX 64
-#define GADE_TUNE_COLS 7
+#define GADE_TUNE_MAX 96
+#define GADE_TUNE_COLS 8
 
 /* ---- learned initial guess (no reference counterpart) ----------------------------------------------
  * guess_order GADE_GUESS_LEARNED: x0 = x2 + s sum_k w_k D_k/|D_k| + s U c per step, with D_k the backward

@@ -383,24 +383,29 @@ def test_extrapolated_guess_same_solution_fewer_iterations(datadir):
 
 @pytest.mark.parametrize("cfg,batch", [("nondipole_small", 1), ("dipole", 3)])
 def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
-    """Every banded tile shape (rows per tile, outputs per lane), angular multipole group and angular form
-    (coupling lists or Hall rows) sums each output in the same order: applies (both signs, FP64) and whole
-    device-driven trajectories (paired apply, FP32 inner solves of the mixed form) are bit-identical across
-    shapes."""
+    """Every banded tile shape (rows per tile, outputs per lane, one or both output halves per workgroup),
+    angular multipole group and angular form (coupling lists or Hall rows) sums each output in the same
+    order: applies (both signs, FP64) and whole device-driven trajectories (paired apply, FP32 inner solves of
+    the mixed form) are bit-identical across shapes."""
     d = datadir(cfg)
     dt = dt_of(cfg)
     say(f"tile shapes {cfg} batch={batch}")
     rng = np.random.default_rng(3)
     out = {}
-    shapes = ((8, 2, 4, 1), (8, 2, 4, 0), (4, 2, 2, 1), (4, 2, 2, 0), (4, 4, 8, 1), (4, 4, 8, 0))
+    # (rows per tile, outputs per lane, angular group, coupling lists, output halves per workgroup)
+    shapes = ((8, 2, 4, 1, 1), (8, 2, 4, 0, 2), (4, 2, 2, 1, 2), (4, 2, 2, 0, 1), (4, 4, 8, 1, 1), (4, 4, 8, 0, 2),
+              (8, 1, 4, 1, 2), (16, 1, 2, 0, 2), (4, 1, 8, 1, 1), (16, 1, 4, 1, 1))
     for shape in shapes:
         gp = gade.Propagator(d, LMAX[cfg], dt, batch=batch)
         if gp.get_param("radial") != 2:
             pytest.skip("radial pattern not banded")
+        gp.set_param("dia_outputs", 1)  # (rows, 1) is a valid shape for every row count
         gp.set_param("dia_rows", shape[0])
         gp.set_param("dia_outputs", shape[1])
         gp.set_param("ang_lb", shape[2])
         gp.set_param("ang_list", shape[3])
+        gp.set_param("dia_halves", shape[4])
+        assert gp.get_param("dia_halves") == shape[4]
         gp.set_time_batch(np.array([gade.pulse(10.0 + 5 * b, 50.0, 1.0, 0.013) for b in range(batch)]))
         n = gp.n * batch
         x = rng.standard_normal(n) + 1j * rng.standard_normal(n) if not out else out["x"]
@@ -431,12 +436,12 @@ def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
 
 
 @pytest.mark.parametrize("cfg", CFGS)
-def test_mfma_radial_is_default_and_steps_agree(datadir, cfg):
-    """The matrix-core radial kernel is the default FP64 form; CN steps with it and with the vector-unit
-    kernel (device-driven stepping with the paired rhs / A x0 apply) agree to 1e-11 over 12 steps, every step
-    at the tolerance."""
+def test_mfma_radial_steps_agree(datadir, cfg):
+    """The matrix-core radial kernel is available (the vector-unit kernel is the default FP64 form); CN steps
+    with it and with the vector-unit kernel (device-driven stepping with the paired rhs / A x0 apply) agree to
+    1e-11 over 12 steps, every step at the tolerance."""
     d, dt, om, gp = pair(datadir, cfg)
-    assert gp.get_param("radial") == 2 and gp.get_param("radial_mfma") == 1
+    assert gp.get_param("radial") == 2 and gp.get_param("radial_mfma") == 0
     say(f"mfma vs dia steps {cfg}")
     psi0, _ = gade.read_psi0(d, gp.n)
     out = []
