@@ -21,8 +21,6 @@
 // sample therefore holds the small projected quantities of that form (G, a, P, q,
 // beta) next to its inputs, and no full vector is touched while training.
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include <cmath>
 #include <cstdint>
@@ -37,11 +35,16 @@
 using gade::hip_alloc;
 using gade::set_error;
 
+namespace gade {
+int svd_left_jacobi(int k, const double *Rcm, double *S, double *Ucm);  // host/eigsolve.cpp
+}
+
 namespace {
 
 constexpr int kIn = GADE_LG_IN, kHid = GADE_LG_HIDDEN, kOut = GADE_LG_OUT, kPod = GADE_LG_POD;
 constexpr int kSample = GADE_LG_SAMPLE;
 constexpr int kNV = 7;  // vectors of the difference stage: D_0..D_5, e2 (record)
+constexpr int kHhChunks = 64;  // row chunks of a reflector's dot products
 constexpr int kNG = kNV * (kNV + 1) / 2;  // Gram entries Re(V_i^H V_j), i <= j
 // sample layout (floats)
 constexpr int S_FEAT = 0, S_G = kIn, S_A = S_G + 36, S_P = S_A + 6, S_Q = S_P + kPod * 6 * 2, S_BETA = S_Q + kPod * 2,
@@ -336,6 +339,84 @@ __global__ __launch_bounds__(256) void k_lg_colnorm(long long n, const double2 *
     for (long long i = threadIdx.x; i < n; i += blockDim.x) acc[0] += x[i].x * x[i].x + x[i].y * x[i].y;
     block_sum<1>(acc, sh);
     if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR of the snapshot matrix for the POD (n x k, column-major), in the LAPACK convention
+// (zgeqr2 / zlarfg): H_j = I - tau_j v_j v_j^H with v_j = (0 .. 0, 1, A[j+1:n][j]) stored below the diagonal
+// and A[j][j] = beta real; the trailing columns take H_j^H = I - conj(tau_j) v_j v_j^H.  Q C = H_0 H_1 ..
+// H_{k-1} C applies the reflectors last to first.  Every sum runs in a fixed order (chunk partials added
+// in chunk order), so the factorisation is deterministic.
+// ---------------------------------------------------------------------------
+
+// reflector of column j: one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_hh_reflector(double2 *__restrict__ A, long long n, int j,
+                                                       double2 *__restrict__ tau) {
+    __shared__ double sh[4 * 16];
+    __shared__ double2 scale_sh;
+    double2 *x = A + (size_t)j * n;
+    double acc[1] = {0};
+    for (long long r = j + 1 + threadIdx.x; r < n; r += blockDim.x) acc[0] += x[r].x * x[r].x + x[r].y * x[r].y;
+    block_sum<1>(acc, sh);
+    if (threadIdx.x == 0) {
+        const double2 al = x[j];
+        double2 t = make_double2(0, 0), sc = make_double2(0, 0);
+        if (acc[0] > 0 || al.y != 0) {
+            const double beta = -copysign(sqrt(al.x * al.x + al.y * al.y + acc[0]), al.x);
+            t = make_double2((beta - al.x) / beta, -al.y / beta);
+            sc = cdiv(make_double2(1, 0), make_double2(al.x - beta, al.y));
+            x[j] = make_double2(beta, 0);
+        }
+        tau[j] = t;
+        scale_sh = sc;
+    }
+    __syncthreads();
+    const double2 sc = scale_sh;
+    if (sc.x != 0 || sc.y != 0)
+        for (long long r = j + 1 + threadIdx.x; r < n; r += blockDim.x) x[r] = cmul(x[r], sc);
+}
+
+// part[c][ch] = sum over the rows of chunk ch (rows >= j) of conj(v_r) M[r][c0 + c]
+__global__ __launch_bounds__(256) void k_hh_dots(const double2 *__restrict__ v, long long n, int j,
+                                                 const double2 *__restrict__ M, int c0, long long rows_per, int nch,
+                                                 double2 *__restrict__ part) {
+    __shared__ double sh[4 * 16];
+    const int c = blockIdx.x, ch = blockIdx.y;
+    const double2 *mc = M + (size_t)(c0 + c) * n;
+    const long long r0 = j + (long long)ch * rows_per, r1 = min(n, r0 + rows_per);
+    double acc[2] = {0, 0};
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const double2 vr = r == j ? make_double2(1, 0) : v[r];
+        const double2 mr = mc[r];
+        acc[0] += vr.x * mr.x + vr.y * mr.y;
+        acc[1] += vr.x * mr.y - vr.y * mr.x;
+    }
+    block_sum<2>(acc, sh);
+    if (threadIdx.x == 0) part[(size_t)c * nch + ch] = make_double2(acc[0], acc[1]);
+}
+
+// M[r][c0 + c] -= v_r w_c for rows r >= j, w_c = t sum_ch part[c][ch], t = conj(tau_j) (R) or tau_j (Q C)
+__global__ __launch_bounds__(256) void k_hh_update(const double2 *__restrict__ v, long long n, int j,
+                                                   double2 *__restrict__ M, int c0, const double2 *__restrict__ part,
+                                                   int nch, const double2 *__restrict__ tau, int conj_tau,
+                                                   long long rows_per) {
+    __shared__ double2 w_sh;
+    const int c = blockIdx.x;
+    if (threadIdx.x == 0) {
+        double2 sum = make_double2(0, 0);
+        for (int ch = 0; ch < nch; ch++) sum = cadd(sum, part[(size_t)c * nch + ch]);
+        double2 t = tau[j];
+        if (conj_tau) t.y = -t.y;
+        w_sh = cmul(t, sum);
+    }
+    __syncthreads();
+    const double2 w = w_sh;
+    double2 *mc = M + (size_t)(c0 + c) * n;
+    const long long r0 = j + (long long)blockIdx.y * rows_per, r1 = min(n, r0 + rows_per);
+    for (long long r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const double2 vr = r == j ? make_double2(1, 0) : v[r];
+        mc[r] = csub(mc[r], cmul(vr, w));
+    }
 }
 
 // the per-step copies of the basis, rounded to FP32: U32[j][k] = UT32[k][j] = U[j][k].  Each step reads
@@ -1014,6 +1095,24 @@ int gade_lg_record(gade_model_t *m, const double *psi, float *samples, double *s
     return GADE_OK;
 }
 
+// M[:, c0 .. c0+nc) <- H_j^H M (conj_tau) or H_j M with the reflector of column j of A (n rows)
+static int hh_apply(gade_model *m, const double2 *A, long long n, int j, double2 *M, int c0, int nc,
+                    double2 *part, const double2 *tau, int conj_tau) {
+    if (nc <= 0) return GADE_OK;
+    const long long rows = n - j;
+    const int nch = std::max(1, std::min(kHhChunks, (2048 + nc - 1) / nc));
+    const long long per = (rows + nch - 1) / nch;
+    const double2 *v = A + (size_t)j * n;
+    hipLaunchKernelGGL(k_hh_dots, dim3(nc, nch), dim3(256), 0, m->stream, v, n, j, (const double2 *)M, c0, per, nch,
+                       part);
+    const int nrb = std::max(1, std::min(256, (4096 + nc - 1) / nc));
+    const long long per_b = (rows + nrb - 1) / nrb;
+    hipLaunchKernelGGL(k_hh_update, dim3(nc, nrb), dim3(256), 0, m->stream, v, n, j, M, c0, (const double2 *)part,
+                       nch, tau, conj_tau, per_b);
+    GADE_HIP(hipGetLastError());
+    return GADE_OK;
+}
+
 int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod, double *sv) {
     if (!m || !snaps || npod <= 0 || npod > kPod || nsnap < npod) return GADE_EINVAL;
     GADE_TRY(lg_ensure(m));
@@ -1021,17 +1120,14 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
     const int n = (int)m->N1;
     if (nsnap > n) return GADE_EINVAL;
     gade::drop_step_graph(m);  // the graph reads the basis being replaced
-    double2 *A = nullptr, *tau = nullptr, *R = nullptr, *Sv = nullptr, *Uq = nullptr, *C = nullptr;
-    double *S = nullptr, *E = nullptr;
-    int *info = nullptr;
+    double2 *A = nullptr, *tau = nullptr, *part = nullptr, *C = nullptr;
     auto cleanup = [&]() {
-        for (void *q : {(void *)A, (void *)tau, (void *)R, (void *)Sv, (void *)Uq, (void *)C, (void *)S, (void *)E,
-                        (void *)info})
+        for (void *q : {(void *)A, (void *)tau, (void *)part, (void *)C})
             if (q) (void)hipFree(q);
     };
     int rc = GADE_OK;
-    // snapshot columns that are zero (unused slots) or not finite are left out: exact zero columns make R
-    // singular in a way the bidiagonal SVD does not converge on
+    // snapshot columns that are zero (unused slots) or not finite are left out: they carry nothing and an
+    // exact zero column only adds a zero singular value
     std::vector<int> keep;
     {
         double *d_nrm = nullptr;
@@ -1056,72 +1152,53 @@ int gade_lg_pod_build(gade_model_t *m, const double *snaps, int nsnap, int npod,
     }
     const int nused = (int)keep.size();
     const size_t ns = (size_t)nused;
+    const int ncol = std::max(nused, npod);
     if ((rc = hip_alloc((void **)&A, sizeof(double2) * n * ns)) || (rc = hip_alloc((void **)&tau, sizeof(double2) * ns)) ||
-        (rc = hip_alloc((void **)&R, sizeof(double2) * ns * ns)) || (rc = hip_alloc((void **)&Sv, sizeof(double2) * ns * ns)) ||
-        (rc = hip_alloc((void **)&Uq, sizeof(double2) * ns * ns)) || (rc = hip_alloc((void **)&C, sizeof(double2) * n * npod)) ||
-        (rc = hip_alloc((void **)&S, sizeof(double) * ns)) || (rc = hip_alloc((void **)&E, sizeof(double) * ns)) ||
-        (rc = hip_alloc((void **)&info, sizeof(int)))) {
+        (rc = hip_alloc((void **)&part, sizeof(double2) * (size_t)ncol * kHhChunks)) ||
+        (rc = hip_alloc((void **)&C, sizeof(double2) * n * npod))) {
         cleanup();
         return rc;
     }
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) {
-        cleanup();
-        set_error("gade_lg_pod_build: rocblas_create_handle failed");
-        return GADE_EHIP;
-    }
-    rocblas_set_stream(h, m->stream);
-    // snapshots [col][n] are the column-major n x nsnap matrix X; X = Q R; R = Ur S Vr^H; U = Q Ur[:, :npod]
+    // snapshots [col][n] are the column-major n x nused matrix X: X = Q R (Householder, own kernels),
+    // R = Ur S Vr^H (one-sided Jacobi on the host), U = Q Ur[:, :npod]
     bool ok = true;
     for (size_t c = 0; c < ns && ok; c++)
         ok = hipMemcpyAsync(A + c * n, (const double2 *)snaps + (size_t)keep[c] * n, sizeof(double2) * n,
                             hipMemcpyDeviceToDevice, m->stream) == hipSuccess;
-    ok = ok && rocsolver_zgeqrf(h, n, nused, (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau) ==
-                   rocblas_status_success;
-    // R: upper triangle of the first nused rows (only those rows come back: a strided copy)
+    for (int j = 0; j < nused && ok && rc == GADE_OK; j++) {
+        hipLaunchKernelGGL(k_hh_reflector, dim3(1), dim3(1024), 0, m->stream, A, (long long)n, j, tau);
+        rc = hh_apply(m, A, n, j, A, j + 1, nused - j - 1, part, tau, 1);
+    }
+    // R: upper triangle of the first nused rows (a strided copy), below the diagonal the reflectors
     std::vector<double2> Rh(ns * ns, make_double2(0, 0));
-    if (ok) {
+    if (ok && rc == GADE_OK)
         ok = hipMemcpy2DAsync(Rh.data(), sizeof(double2) * ns, A, sizeof(double2) * n, sizeof(double2) * ns, ns,
                               hipMemcpyDeviceToHost, m->stream) == hipSuccess &&
              hipStreamSynchronize(m->stream) == hipSuccess;
-    }
-    if (ok) {
+    std::vector<double> Sh(ns);
+    std::vector<double2> Uh(ns * ns);
+    int svd_rc = GADE_OK;
+    if (ok && rc == GADE_OK) {
         for (size_t c = 0; c < ns; c++)
-            for (size_t r = c + 1; r < ns; r++) Rh[c * ns + r] = make_double2(0, 0);  // below the diagonal: reflectors
-        ok = hipMemcpy(R, Rh.data(), sizeof(double2) * ns * ns, hipMemcpyHostToDevice) == hipSuccess;
+            for (size_t r = c + 1; r < ns; r++) Rh[c * ns + r] = make_double2(0, 0);
+        svd_rc = gade::svd_left_jacobi(nused, (const double *)Rh.data(), Sh.data(), (double *)Uh.data());
     }
-    ok = ok && rocsolver_zgesvd(h, rocblas_svect_all, rocblas_svect_none, nused, nused, (rocblas_double_complex *)R,
-                                nused, S, (rocblas_double_complex *)Uq, nused, (rocblas_double_complex *)Sv, nused, E,
-                                rocblas_outofplace, info) == rocblas_status_success;
     // C = [Ur[:, :npod]; 0] (n x npod), then U = Q C
-    std::vector<double2> Uh, Ch;
-    if (ok) {
-        Uh.resize(ns * ns);
-        ok = hipMemcpyAsync(Uh.data(), Uq, sizeof(double2) * ns * ns, hipMemcpyDeviceToHost, m->stream) == hipSuccess &&
-             hipStreamSynchronize(m->stream) == hipSuccess;
-    }
-    if (ok) {
-        Ch.assign((size_t)n * npod, make_double2(0, 0));
+    if (ok && rc == GADE_OK && svd_rc == GADE_OK) {
+        std::vector<double2> Ch((size_t)n * npod, make_double2(0, 0));
         for (int c = 0; c < npod; c++)
             for (size_t r = 0; r < ns; r++) Ch[(size_t)c * n + r] = Uh[(size_t)c * ns + r];
         ok = hipMemcpy(C, Ch.data(), sizeof(double2) * n * npod, hipMemcpyHostToDevice) == hipSuccess;
+        for (int j = nused - 1; j >= 0 && ok && rc == GADE_OK; j--) rc = hh_apply(m, A, n, j, C, 0, npod, part, tau, 0);
+        if (ok && rc == GADE_OK) ok = hipStreamSynchronize(m->stream) == hipSuccess;
     }
-    ok = ok && rocsolver_zunmqr(h, rocblas_side_left, rocblas_operation_none, n, npod, nused,
-                                (rocblas_double_complex *)A, n, (rocblas_double_complex *)tau,
-                                (rocblas_double_complex *)C, n) == rocblas_status_success;
-    int info_h = 0;
-    if (ok)
-        ok = hipMemcpy(&info_h, info, sizeof(int), hipMemcpyDeviceToHost) == hipSuccess && info_h == 0 &&
-             hipStreamSynchronize(m->stream) == hipSuccess;
-    rocblas_destroy_handle(h);
-    if (!ok) {
+    if (!ok || rc != GADE_OK || svd_rc != GADE_OK) {
         cleanup();
-        set_error("gade_lg_pod_build: rocSOLVER QR / SVD failed (info " + std::to_string(info_h) + ")");
-        return GADE_EHIP;
+        set_error(svd_rc != GADE_OK ? "gade_lg_pod_build: the SVD of R did not converge"
+                                    : "gade_lg_pod_build: QR of the snapshots failed");
+        return rc != GADE_OK ? rc : svd_rc != GADE_OK ? svd_rc : GADE_EHIP;
     }
     if (sv) {  // singular values of the used snapshots, then zeros for the columns left out
-        std::vector<double> Sh(ns);
-        (void)hipMemcpy(Sh.data(), S, sizeof(double) * ns, hipMemcpyDeviceToHost);
         memcpy(sv, Sh.data(), sizeof(double) * ns);
         for (int c = nused; c < nsnap; c++) sv[c] = 0.0;
     }

@@ -10,8 +10,11 @@
 // depend on the order of the iteration's sign choices.  Setup only: it runs once per model.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <numeric>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -206,3 +209,130 @@ extern "C" int gade_eig_dense(int n, const double *A, const double *B, double *E
     }
     return GADE_OK;
 }
+
+// ---------------------------------------------------------------------------------------------------------
+// Singular values and left singular vectors of a complex k x k matrix R (column-major), for the POD of the
+// learned initial guess: one-sided (Hestenes) Jacobi on the columns of A = R^H, A <- A G_pq, with the same
+// rotations accumulated in J (J = I at the start).  At convergence R^H J = W has orthogonal columns, so
+// R = J W^H: the singular values are the column norms of W and the left singular vectors the columns of J
+// (Demmel and Veselic, SIAM J. Matrix Anal. Appl. 13 (1992) 1204-1245: high relative accuracy, also for
+// the small singular values of graded R).  Rotation of the pair (p, q): gamma = a_p^H a_q = |gamma| e,
+// zeta = (|a_q|^2 - |a_p|^2) / (2 |gamma|), t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1/sqrt(1 + t^2),
+// s = c t; a_p <- c a_p - s conj(e) a_q, a_q <- s a_p + c conj(e) a_q.  Pairs are visited in round-robin
+// (tournament) order: the k/2 pairs of a round touch disjoint columns and run on host threads; every thread
+// count gives the same result.  Sweeps end when no pair has |gamma| > 4 eps |a_p| |a_q|.
+// Output: S [k] descending, U [k][k] column-major (column j for S[j]).  GADE_ENOTCONV after 60 sweeps.
+namespace gade {
+int svd_left_jacobi(int k, const double *Rcm, double *S, double *Ucm) {
+    if (k <= 0 || !Rcm || !S || !Ucm) return GADE_EINVAL;
+    const int kp = k + (k & 1);  // an odd order gets a zero column that never rotates
+    std::vector<double> A((size_t)2 * kp * k, 0.0), J((size_t)2 * kp * k, 0.0);
+    auto a = [&](int col) { return &A[(size_t)2 * k * col]; };
+    auto jc = [&](int col) { return &J[(size_t)2 * k * col]; };
+    for (int c = 0; c < k; c++)  // A[:, c] = conj(R[c, :])
+        for (int r = 0; r < k; r++) {
+            const double *z = Rcm + 2 * ((size_t)r * k + c);
+            a(c)[2 * r] = z[0];
+            a(c)[2 * r + 1] = -z[1];
+        }
+    for (int c = 0; c < k; c++) jc(c)[2 * c] = 1.0;
+    const double tol = 4 * std::numeric_limits<double>::epsilon();
+    auto rotate = [&](int p, int q) -> int {
+        if (p >= k || q >= k) return 0;
+        double al = 0, be = 0, gr = 0, gi = 0;
+        const double *x = a(p), *y = a(q);
+        for (int r = 0; r < k; r++) {
+            const double xr = x[2 * r], xi = x[2 * r + 1], yr = y[2 * r], yi = y[2 * r + 1];
+            al += xr * xr + xi * xi;
+            be += yr * yr + yi * yi;
+            gr += xr * yr + xi * yi;  // conj(x) y
+            gi += xr * yi - xi * yr;
+        }
+        const double g = std::hypot(gr, gi);
+        if (g == 0 || al == 0 || be == 0 || g <= tol * std::sqrt(al) * std::sqrt(be)) return 0;
+        const double er = gr / g, ei = -gi / g;  // conj(e)
+        const double z = (be - al) / (2 * g);
+        const double t = (z >= 0 ? 1.0 : -1.0) / (std::fabs(z) + std::sqrt(1 + z * z));
+        const double c = 1 / std::sqrt(1 + t * t), s = c * t;
+        for (double *M : {&A[0], &J[0]}) {
+            double *u = M + (size_t)2 * k * p, *v = M + (size_t)2 * k * q;
+            for (int r = 0; r < k; r++) {
+                const double ur = u[2 * r], ui = u[2 * r + 1];
+                const double wr = er * v[2 * r] - ei * v[2 * r + 1], wi = er * v[2 * r + 1] + ei * v[2 * r];
+                u[2 * r] = c * ur - s * wr;
+                u[2 * r + 1] = c * ui - s * wi;
+                v[2 * r] = s * ur + c * wr;
+                v[2 * r + 1] = s * ui + c * wi;
+            }
+        }
+        return 1;
+    };
+    const int npair = kp / 2;
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    // circle method: player 0 fixed, the others rotate; round rd pairs slot i with slot kp - 1 - i
+    auto pair_of = [&](int rd, int i, int &p, int &q) {
+        auto player = [&](int slot) { return slot == 0 ? 0 : 1 + (slot - 1 + rd) % (kp - 1); };
+        p = player(i);
+        q = player(kp - 1 - i);
+        if (p > q) std::swap(p, q);
+    };
+    // the threads stay up for the whole iteration and meet at a barrier after every round
+    const int T = npair >= 2 * nth ? nth : 1;
+    std::mutex mu;
+    std::condition_variable cv;
+    int arrived = 0, generation = 0;
+    auto barrier = [&]() {
+        std::unique_lock<std::mutex> lk(mu);
+        const int g = generation;
+        if (++arrived == T) {
+            arrived = 0;
+            generation++;
+            cv.notify_all();
+        } else {
+            cv.wait(lk, [&] { return generation != g; });
+        }
+    };
+    std::vector<int> rot(T, 0);
+    bool done = false;
+    auto run = [&](int tid) {
+        for (int sw = 0; sw < 60; sw++) {
+            rot[tid] = 0;
+            for (int rd = 0; rd < kp - 1; rd++) {
+                for (int i = tid; i < npair; i += T) {
+                    int p, q;
+                    pair_of(rd, i, p, q);
+                    rot[tid] += rotate(p, q);
+                }
+                barrier();
+            }
+            if (tid == 0) {
+                int total = 0;
+                for (int v : rot) total += v;
+                done = total == 0;
+            }
+            barrier();  // every thread reads 'done' after thread 0 wrote it
+            if (done) return;
+            barrier();  // and before thread 0 clears the counts of the next sweep
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; t++) th.emplace_back(run, t);
+    run(0);
+    for (auto &x : th) x.join();
+    if (!done) return GADE_ENOTCONV;
+    std::vector<double> nrm(k);
+    for (int c = 0; c < k; c++) {
+        double s2 = 0;
+        for (int r = 0; r < 2 * k; r++) s2 += a(c)[r] * a(c)[r];
+        nrm[c] = std::sqrt(s2);
+    }
+    std::vector<int> order(k);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nrm[x] > nrm[y]; });
+    for (int j = 0; j < k; j++) {
+        S[j] = nrm[order[j]];
+        memcpy(Ucm + (size_t)2 * k * j, jc(order[j]), sizeof(double) * 2 * k);
+    }
+    return GADE_OK;
+}
+}  // namespace gade

@@ -77,6 +77,18 @@ def test_export_sets_are_exact():
     assert want_host == host, (sorted(want_host - host), sorted(host - want_host))
 
 
+def test_no_vendor_math_libraries():
+    """The libraries carry their own dense linear algebra (eigensolver, QR, SVD): neither needs rocBLAS,
+    rocSOLVER, hipBLAS or hipSOLVER at load time."""
+    import subprocess
+    for lib in (gade.LIB_PATH, HOST_LIB):
+        dyn = subprocess.run(["readelf", "-d", lib], capture_output=True, text=True, check=True).stdout
+        needed = [ln.split("[")[1].split("]")[0] for ln in dyn.splitlines() if "(NEEDED)" in ln]
+        assert needed, lib
+        bad = [n for n in needed if any(v in n for v in ("rocblas", "rocsolver", "hipblas", "hipsolver"))]
+        assert not bad, (lib, bad)
+
+
 def test_index_maps_and_pulse_match_oracle():
     L = gade.lib()
     for i in range(2 * 41 * 40):
