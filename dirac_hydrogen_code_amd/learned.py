@@ -23,17 +23,21 @@ import numpy as np
 from . import gade
 
 
-def scan_pulses(seed, B, dt, cep=None):
+PULSE_RANGES = dict(E0=(1.0, 100.0), omega=(25.0, 75.0), cycles=(1, 15))
+
+
+def scan_pulses(seed, B, dt, cep=None, ranges=None):
     """Seeded pulse scan: E0 log-uniform in [1, 100], omega uniform in [25, 75], carrier phase uniform in
-    [0, 2 pi) (or the given cep for every pulse; 0 is the reference's own phase), cycles uniform in 1..15;
-    per-instance step counts ceil(T_b / dt)."""
+    [0, 2 pi) (or the given cep for every pulse; 0 is the reference's own phase), cycles uniform in 1..15
+    (other ranges through ``ranges``, keys E0, omega, cycles); per-instance step counts ceil(T_b / dt)."""
+    r = dict(PULSE_RANGES, **(ranges or {}))
     rng = np.random.default_rng(seed)
-    E0 = np.exp(rng.uniform(np.log(1.0), np.log(100.0), B))
-    om = rng.uniform(25.0, 75.0, B)
+    E0 = np.exp(rng.uniform(np.log(r["E0"][0]), np.log(r["E0"][1]), B))
+    om = rng.uniform(r["omega"][0], r["omega"][1], B)
     ceps = rng.uniform(0.0, 2 * np.pi, B)
     if cep is not None:
         ceps = np.full(B, float(cep))
-    cyc = rng.integers(1, 16, B).astype(float)
+    cyc = rng.integers(r["cycles"][0], r["cycles"][1] + 1, B).astype(float)
     params = np.stack([E0, om, cyc, ceps], axis=1)
     nsteps_b = np.ceil(cyc * 2 * np.pi / om / dt).astype(np.int32)
     return params, nsteps_b
@@ -64,17 +68,20 @@ class Trainer:
     """Builds and trains the learned guess of a batched model ``gp`` (its couplings, dt and batch width)."""
 
     def __init__(self, gp, psi0, seed=0x5EED, npod=gade.LG_POD, nsnap=400, nsamples=20000, log=None, tol=1e-13,
-                 cep=None):
+                 cep=None, ranges=None):
         """``tol``: tolerance of the solves that make the snapshots and samples.  It is tighter than the
         production one, so that a step whose extrapolated guess already meets the production tolerance (it
         ends at 0 iterations there) still gets a solution that differs from the guess: the sample keeps a
         non-zero extrapolation error and the loss learns to leave such steps alone.
         ``cep``: carrier phase of the training pulses (None: seeded per pulse).  A guess for the reference's
         own pulses (fixed phase, cep = 0) is trained with cep = 0: at that phase three of the six f_alpha start
-        at zero and the early response is a family of its own, outside what random phases cover."""
+        at zero and the early response is a family of its own, outside what random phases cover.
+        ``ranges``: E0 / omega / cycles ranges of the training pulses (scan_pulses).  Short pulses put the
+        peak of their envelope within the first steps the samples cover, so the guess also sees the strong-field
+        regime of a long pulse's middle."""
         self.gp, self.psi0 = gp, np.asarray(psi0, np.complex128)
         self.seed, self.npod, self.nsnap, self.nsamples, self.tol = seed, npod, nsnap, nsamples, tol
-        self.params, self.nsteps_b = scan_pulses(seed, gp.batch, gp.dt, cep)
+        self.params, self.nsteps_b = scan_pulses(seed, gp.batch, gp.dt, cep, ranges)
         self.log = log or (lambda msg: None)
         self.N1 = gp.nblk * gp.blkdim
 
@@ -83,7 +90,11 @@ class Trainer:
         self.gp.set_param("lg_features", 1 if features else 0)
 
     def build_pod(self, snap_every=20, snap_first=10):
-        """Snapshots of the normalized extrapolation defect, then the POD basis; returns singular values."""
+        """Snapshots of the normalized extrapolation defect, then the POD basis; returns singular values.
+        snap_every 0: spread the snapshots of each instance over the steps the samples cover."""
+        if snap_every <= 0:
+            span = min(int(np.median(self.nsteps_b)), self.nsamples // max(1, self.gp.batch))
+            snap_every = max(20, (span - snap_first) // max(1, self.nsnap // max(1, self.gp.batch)))
         gp, B = self.gp, self.gp.batch
         self._mode(False)
         snaps = gade.DeviceArray(self.nsnap * self.N1)
@@ -134,12 +145,20 @@ class Trainer:
         return buf, valid
 
     def standardize(self, rows):
-        """Input mean / standard deviation over the given samples (used inputs; padding: 0 / 1)."""
+        """Input mean / standard deviation over the given samples (used inputs; padding: 0 / 1).  The POD
+        coordinates of the third and fourth differences share one scale: a coordinate's deviation is at least
+        1 % of the largest in its block.  The weakest POD modes hardly vary over the training steps, and without
+        the floor a state whose history excites them a little (a longer pulse's middle) would read as 100 sigma
+        out of range and send every step to the fallback."""
         F = self.samples_host[rows, :gade.LG_IN].astype(np.float64)
         mean = F.mean(0)
         std = F.std(0)
         used = np.arange(gade.LG_IN) < gade.LG_FEATURES
         std = np.where(used & (std > 1e-6 * (np.abs(mean) + 1)), std, 1.0)
+        z0 = gade.LG_FEATURES - 4 * gade.LG_POD  # the z3 block, then the z4 block, 2 * LG_POD reals each
+        for b in (z0, z0 + 2 * gade.LG_POD):
+            blk = std[b:b + 2 * gade.LG_POD]
+            std[b:b + 2 * gade.LG_POD] = np.maximum(blk, 1e-2 * blk.max())
         mean = np.where(used, mean, 0.0)
         self.gp.lg_set_norm(mean, std)
         self.mean, self.std = mean, std
