@@ -178,5 +178,27 @@ int main(int argc, char **argv)
     fwrite(yS, sizeof(csr_data_t), n, fo);
     fwrite(z, sizeof(csr_data_t), n, fo);
     fclose(fo);
+    /* everything this program owns goes back, so a leak-checking build reports only the library's own */
+    void *wv[] = {wsp.r, wsp.r0, wsp.v, wsp.p, wsp.y, wsp.z, wsp.s, wsp.t};
+    for (int i = 0; i < 8; i++) free(wv[i]);
+    free(P.lu);
+    free(P.piv);
+    free(x);
+    free(yH);
+    free(yS);
+    free(rhs);
+    free(z);
+    csr_free(&Hall);
+    csr_free(&Sdiag);
+    for (int i = 0; i < 2 * lmax; i++) csr_free(H + i);
+    for (int i = 0; i < 24 * lmax; i++) {
+        csr_free(g + i);
+        csr_free(gt + i);
+    }
+    for (int k = 0; k < 4; k++) csr_free(h0 + k);
+    for (int k = 0; k < 3; k++) csr_free(s0 + k);
+    free(H);
+    free(g);
+    free(gt);
     return 0;
 }
