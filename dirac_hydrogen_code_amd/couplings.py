@@ -226,29 +226,48 @@ def radial_pattern(ns: int, half_band: int = 7):
     return np.array(Ap, np.int32), np.array(Ai, np.int32)
 
 
-def radial_matrices(nsplines, rbox, Z, omega, cycles, nl, nq=12):
-    """Overlap s0..s2, field-free h00..h03 and interaction g[n][alpha][l] (dense)."""
+def dkb_parts(nsplines, rbox, nq=12):
+    """Quadrature points x, weights w and the kappa-polynomial parts of the radial basis functions at x:
+    P(kappa) = P0 + kappa P1 and Q(kappa) = Q0 + kappa Q1 (columns: the 2 (nsplines - 1) functions), with
+    their r-derivatives, as the reference's dual kinetically balanced basis (src/dkbbasis.cpp: dkbCache, with P
+    the ul = 1 and Q the ul = -1 part):
+      large functions: P = B,                    Q = (B' + kappa B/r)/(2c)
+      small functions: P = (-B' + kappa B/r)/(2c), Q = B
+    Returns dict(x, w, P0, P1, Q0, Q1, dP0, dP1, dQ0, dQ1)."""
     t = np.array([rbox / nsplines * i for i in range(nsplines + 1)])
     x, w = _gauss_legendre_points(t, nq)
     B, dB, ddB = _bspline_cache(t, 7, x)
     c = SOL
     inv_r = 1.0 / x
     z0 = np.zeros_like(B)
-    # kappa-polynomial components of the P and Q parts of the basis functions, as the reference's dual
-    # kinetically balanced basis (src/dkbbasis.cpp: dkbCache, with P the ul = 1 and Q the ul = -1 part):
-    # large functions: P = B,                    Q = (B' + kappa B/r)/(2c)
-    # small functions: P = (-B' + kappa B/r)/(2c), Q = B
-    # (H below is [[V, c(-d/dr + k/r)], [c(d/dr + k/r), V - 2c^2]])
     Br = B * inv_r[:, None]
     dBr = dB * inv_r[:, None] - B * (inv_r ** 2)[:, None]
-    P0 = np.hstack([B, -dB / (2 * c)])
-    P1 = np.hstack([z0, Br / (2 * c)])
-    Q0 = np.hstack([dB / (2 * c), B])
-    Q1 = np.hstack([Br / (2 * c), z0])
-    dP0 = np.hstack([dB, -ddB / (2 * c)])
-    dP1 = np.hstack([z0, dBr / (2 * c)])
-    dQ0 = np.hstack([ddB / (2 * c), dB])
-    dQ1 = np.hstack([dBr / (2 * c), z0])
+    return dict(x=x, w=w,
+                P0=np.hstack([B, -dB / (2 * c)]), P1=np.hstack([z0, Br / (2 * c)]),
+                Q0=np.hstack([dB / (2 * c), B]), Q1=np.hstack([Br / (2 * c), z0]),
+                dP0=np.hstack([dB, -ddB / (2 * c)]), dP1=np.hstack([z0, dBr / (2 * c)]),
+                dQ0=np.hstack([ddB / (2 * c), dB]), dQ1=np.hstack([dBr / (2 * c), z0]))
+
+
+def bessel_kernel(alpha, l, omega, cycles, x):
+    """C_alpha(l)(r) = j_l(k_alpha r), one wave number per pair alpha = 2j, 2j+1 (src/potential.h:470-497):
+    k = 2 pi/(c T) + omega/c, -(2 pi/(c T) - omega/c), omega/c; j_l(-z) = (-1)^l j_l(z)."""
+    from scipy.special import spherical_jn
+    c = SOL
+    T = cycles * 2 * math.pi / omega
+    k = [2 * math.pi / (c * T) + omega / c, -(2 * math.pi / (c * T) - omega / c), omega / c][alpha // 2]
+    return spherical_jn(l, abs(k) * x) * (np.sign(k) ** l if k != 0 else 1.0)
+
+
+def radial_matrices(nsplines, rbox, Z, omega, cycles, nl, nq=12):
+    """Overlap s0..s2, field-free h00..h03 and interaction g[n][alpha][l] (dense)."""
+    d = dkb_parts(nsplines, rbox, nq)
+    x, w = d["x"], d["w"]
+    P0, P1, Q0, Q1 = d["P0"], d["P1"], d["Q0"], d["Q1"]
+    dP0, dP1, dQ0, dQ1 = d["dP0"], d["dP1"], d["dQ0"], d["dQ1"]
+    c = SOL
+    inv_r = 1.0 / x
+    # (H below is [[V, c(-d/dr + k/r)], [c(d/dr + k/r), V - 2c^2]])
     V = -Z * inv_r
 
     def ip(f, g, weight=None):
@@ -273,18 +292,11 @@ def radial_matrices(nsplines, rbox, Z, omega, cycles, nl, nq=12):
     s = [0.5 * (m + m.T) for m in s]
     h = [0.5 * (m + m.T) for m in h]
     # interaction integrals g^n_{alpha,l} = P^T W C_alpha(l) Q over the four kappa parts (src/dkbbasis.cpp:
-    # 780-820): C_alpha(l)(r) = j_l(k_alpha r) with the spherical Bessel function j_l and one wave number per
-    # pair alpha = 2j, 2j+1 (src/potential.h:470-497): k = 2 pi/(c T) + omega/c, -(2 pi/(c T) - omega/c),
-    # omega/c
-    from scipy.special import spherical_jn
-    T = cycles * 2 * math.pi / omega
-    kw = [2 * math.pi / (c * T) + omega / c, -(2 * math.pi / (c * T) - omega / c), omega / c]
+    # 780-820), with the Bessel kernels of bessel_kernel: <P(k_r)| C |Q(k_c)> = g0 + k_r g1 + k_c g2 + k_r k_c g3
     g = [[[None] * nl for _ in range(6)] for _ in range(4)]
     for alpha in range(6):
-        k = kw[alpha // 2]
         for l in range(nl):
-            # j_l(-z) = (-1)^l j_l(z) for a negative wave number
-            jl = spherical_jn(l, abs(k) * x) * (np.sign(k) ** l if k != 0 else 1.0)
+            jl = bessel_kernel(alpha, l, omega, cycles, x)
             g[0][alpha][l] = ip(P0, Q0, jl)
             g[1][alpha][l] = ip(P1, Q0, jl)
             g[2][alpha][l] = ip(P0, Q1, jl)

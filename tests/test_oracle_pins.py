@@ -209,6 +209,78 @@ def test_matfree_equals_assembled(datadir, cfg):
             assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
 
 
+def test_coupling_blocks_follow_basis_kappas(datadir):
+    """Which kappa multiplies which coupling part (tools/solver.c:372-381): every block of the oracle's
+    assembled G-part equals the interaction matrix element of the kappa-specific basis functions, computed
+    by quadrature from the basis itself (couplings.dkb_parts: P(k) = P0 + k P1, Q(k) = Q0 + k Q1):
+      H_rc - H_rc(f = 0) = c (i h dt/2) f_alpha sum_{l, l % 2 != alpha % 2}
+                           [H0l[r,c] <P(k_r)| j_l |Q(k_c)> + H1l[r,c] <Q(k_r)| j_l |P(k_c)>]
+    on the radial pattern.  Swapping k_r and k_c consistently in both terms (which the Hermiticity pin cannot
+    see) changes blocks with k_r != k_c by O(1)."""
+    from conftest import SMALL
+    from dirac_hydrogen_code_amd.couplings import dkb_parts, bessel_kernel
+    cfg, lmax = "nondipole_small", 4
+    c = SMALL[cfg]
+    d = datadir(cfg)
+    dt = dt_of(cfg)
+    m = O.Model(d, lmax, dt)
+    nb, n = m.nblk, m.blkdim
+    _, _, _, _, perm = O._read_csr(os.path.join(d, "H.csr"))
+    perm = np.arange(nb) if perm is None else perm
+    kap = np.array([O.lib().or_ik(int(p)) for p in perm], float)
+    H0l = [csr_read(os.path.join(d, f"H0l{l}.csr")) for l in range(lmax)]
+    H1l = [csr_read(os.path.join(d, f"H1l{l}.csr")) for l in range(lmax)]
+
+    def ang(M, i, j):
+        seg = slice(M["Ap"][i], M["Ap"][i + 1])
+        hit = np.nonzero(M["Ai"][seg] == j)[0]
+        return complex(M["Ax"][seg][hit[0]]) if len(hit) else 0.0
+
+    parts = dkb_parts(c["nsplines"], c["rbox"])
+    x, w = parts["x"], parts["w"]
+    Ap, Ai, _ = m.hfull_structure()
+    m.timedep(np.zeros(6, complex))
+    A0 = m.hfull_values().copy()
+    # block pairs: some with different kappas, some with equal, the diagonal
+    rng = np.random.default_rng(4)
+    pairs = [(r, r) for r in range(0, nb, 7)]
+    for r in range(nb):
+        for cc in range(nb):
+            if r != cc and kap[r] != kap[cc] and any(ang(H0l[l], perm[r], perm[cc]) != 0 or
+                                                      ang(H1l[l], perm[r], perm[cc]) != 0 for l in range(lmax)):
+                pairs.append((r, cc))
+    pairs = [pairs[i] for i in rng.permutation(len(pairs))[:24]]
+    assert sum(kap[r] != kap[cc] for r, cc in pairs) >= 8
+    for alpha in (1, 2):
+        f = np.zeros(6, complex)
+        f[alpha] = 1.0
+        m.timedep(f)
+        dA = m.hfull_values() - A0
+        fac = SOL * 1j * 1.0 * dt / 2
+        for r, cc in pairs:
+            P_r, Q_r = parts["P0"] + kap[r] * parts["P1"], parts["Q0"] + kap[r] * parts["Q1"]
+            P_c, Q_c = parts["P0"] + kap[cc] * parts["P1"], parts["Q0"] + kap[cc] * parts["Q1"]
+            want = np.zeros((n, n), complex)
+            for l in range(lmax):
+                if l % 2 == alpha % 2:
+                    continue
+                jw = (w * bessel_kernel(alpha, l, c["omega"], c["cycles"], x))[:, None]
+                want += ang(H0l[l], perm[r], perm[cc]) * (P_r.T @ (jw * Q_c))
+                want += ang(H1l[l], perm[r], perm[cc]) * (Q_r.T @ (jw * P_c))
+            want *= fac
+            got = np.zeros((n, n), complex)
+            on = np.zeros((n, n), bool)
+            for a in range(n):
+                row = r * n + a
+                seg = slice(Ap[row], Ap[row + 1])
+                cols = Ai[seg]
+                sel = (cols >= cc * n) & (cols < (cc + 1) * n)
+                got[a, cols[sel] - cc * n] = dA[seg][sel]
+                on[a, cols[sel] - cc * n] = True
+            scale = max(np.abs(want[on]).max(), 1e-300)
+            assert np.abs(got[on] - want[on]).max() <= 1e-11 * scale, (alpha, r, cc, kap[r], kap[cc])
+
+
 def test_row_col_gathers(datadir):
     """tools/gpu_solver.cu:156-188 gathers: orig indices via the H.csr perm, kappa via ik."""
     d = datadir("nondipole_small")
