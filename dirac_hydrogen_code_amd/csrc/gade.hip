@@ -152,6 +152,15 @@ __global__ void k_build_D(const double2 *__restrict__ s, const double2 *__restri
 // accumulators stay small; the row's coefficients are staged in LDS and the
 // x loads of four block columns are in flight together.
 // ---------------------------------------------------------------------------
+// XCD-aware workgroup order: workgroup b of a 1-D grid of G runs on XCD b % 8 (round-robin dispatch over the 8
+// XCDs, each with its own L2); the logical index returned gives XCD j the contiguous range [j G/8, (j+1) G/8)
+// (remainder spread over the first XCDs), so an XCD works through a few neighbouring slices of the work, and
+// their data, at a time.
+__device__ __forceinline__ int xcd_order(int b, int G) {
+    const int j = b & 7, k = b >> 3, q = G >> 3, r = G & 7;
+    return j * q + min(j, r) + k;
+}
+
 constexpr int kAngThreads = 256;
 constexpr int kAngLB = 4;  // default multipoles per block
 
@@ -227,24 +236,28 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const T *__restrict__ x
     }
 }
 
-// Phase A from the coupling lists: per (row, multipole, part) only the columns with a non-zero H0l (part 0:
-// Z0, Z2) or H1l (part 1: W0, W1) coefficient, in increasing column order, padded to a multiple of
-// kAngListU entries with zero coefficients (on the list's last column).  The sums start at +0 and round to
-// nearest, so a padded term (+0 or -0 added) leaves them unchanged: every V entry is the same sum of the same
-// terms in the same order as in k_angular, bit for bit, without testing coefficients.  The list entries are
-// read at addresses uniform over the block.  A block's lists (its multipole group) are one contiguous stream;
-// the next group's x loads are issued before the current group is summed.  Lists are per instance-local row;
-// x is offset to the instance.
+// Phase A from the coupling lists: per (row, multipole) only the columns with a non-zero H0l or H1l
+// coefficient, in increasing column order, each with both coefficients (h0 for Z0, Z2; h1 for W0, W1; either
+// may be 0), padded to a multiple of kAngListU entries with zero coefficients (on the list's last column).  The
+// sums start at +0 and round to nearest, so they never hold -0, and a zero-coefficient term (+0 or -0 added)
+// leaves them unchanged: every V entry is the same sum of the same non-zero terms in the same order as in
+// k_angular, bit for bit, without testing coefficients.  Each x value is loaded once for the four sums.  The
+// list entries are read at addresses uniform over the block.  A block's lists (its multipole group) are one
+// contiguous stream; the next group's x loads are issued before the current group is summed.  Lists are per
+// instance-local row; x is offset to the instance.
 constexpr int kAngListU = 4;
 template <class T>
 __device__ __forceinline__ void ang_list_group(const T (&xv)[kAngListU], const int2 *__restrict__ ck,
-                                               const T *__restrict__ cf, T &s0, T &s1) {
+                                               const T *__restrict__ cf, T &z0, T &z2, T &w0, T &w1) {
     using Rt = typename cx<T>::R;
 #pragma unroll
     for (int u = 0; u < kAngListU; u++) {
-        const T h = cf[u];
-        s0 = cfma(h, xv[u], s0);
-        s1 = cfma(h, cscale(xv[u], (Rt)ck[u].y), s1);
+        const T h0 = cf[2 * u], h1 = cf[2 * u + 1];
+        const T kx = cscale(xv[u], (Rt)ck[u].y);
+        z0 = cfma(h0, xv[u], z0);
+        z2 = cfma(h0, kx, z2);
+        w0 = cfma(h1, xv[u], w0);
+        w1 = cfma(h1, kx, w1);
     }
 }
 
@@ -263,23 +276,21 @@ __global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restric
     const int a = blockIdx.y * blockDim.x + threadIdx.x;
     const int l0 = blockIdx.z * LB;
     const int nlb = min(LB, lmax - l0);
-    const int *off = aoff + ((size_t)rl * lmax + l0) * 2;
+    const int *off = aoff + (size_t)rl * lmax + l0;
     const T *xi = x + (size_t)inst * nblk1 * n;
     const int ac = a < n ? a : n - 1;
     T z0[LB], z2[LB], w0[LB], w1[LB];
 #pragma unroll
     for (int l = 0; l < LB; l++) z0[l] = z2[l] = w0[l] = w1[l] = czero<T>();
-    const int E0 = off[0], E1 = off[2 * nlb];
+    const int E0 = off[0], E1 = off[nlb];
     if (E1 > E0) {
         T cur[kAngListU];
 #pragma unroll
         for (int u = 0; u < kAngListU; u++) cur[u] = xi[(size_t)ack[E0 + u].x * n + ac];
         int g = E0;
 #pragma unroll
-        for (int li = 0; li < 2 * LB; li++) {
-            if ((li >> 1) >= nlb) break;
-            T &s0 = (li & 1) ? w0[li >> 1] : z0[li >> 1];
-            T &s1 = (li & 1) ? w1[li >> 1] : z2[li >> 1];
+        for (int li = 0; li < LB; li++) {
+            if (li >= nlb) break;
             const int e = off[li + 1];
             // two groups per pass so the two register sets alternate (no copies that wait on the loads)
             for (; g < e; g += 2 * kAngListU) {
@@ -287,7 +298,7 @@ __global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restric
 #pragma unroll
                 for (int u = 0; u < kAngListU; u++)
                     nxt[u] = xi[(size_t)ack[min(g + kAngListU + u, E1 - 1)].x * n + ac];
-                ang_list_group(cur, ack + g, acf + g, s0, s1);
+                ang_list_group(cur, ack + g, acf + 2 * g, z0[li], z2[li], w0[li], w1[li]);
                 if (g + kAngListU >= e) {  // odd number of groups: the loaded group opens the next list
 #pragma unroll
                     for (int u = 0; u < kAngListU; u++) cur[u] = nxt[u];
@@ -297,7 +308,8 @@ __global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restric
 #pragma unroll
                 for (int u = 0; u < kAngListU; u++)
                     cur[u] = xi[(size_t)ack[min(g + 2 * kAngListU + u, E1 - 1)].x * n + ac];
-                ang_list_group(nxt, ack + g + kAngListU, acf + g + kAngListU, s0, s1);
+                ang_list_group(nxt, ack + g + kAngListU, acf + 2 * (g + kAngListU), z0[li], z2[li], w0[li],
+                               w1[li]);
             }
         }
     }
@@ -615,27 +627,33 @@ __global__ void k_build_DD(const double2 *__restrict__ D, const int *__restrict_
 // cH = -1, sel = 1, sel2 = 0 this is the CN pair (S - H) x, (S + H) x, each bit-identical to its own launch.
 // HW = 2: one workgroup of two waves serves both output halves h of a tile (wave h), staging each window once
 // for both; HW = 1: one wave per (tile, h).
-template <int BW, bool DUAL, class T, int R, int P, int HW = 1>
-__global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x, const T *__restrict__ V,
-                                                   const T *__restrict__ GD, const T *__restrict__ DD,
-                                                   const int *__restrict__ tiles, const int *__restrict__ tile_q,
-                                                   const double *__restrict__ tile_k, int nblk, int n, int nh,
-                                                   int nhp, int nseg, int lgrid, int nq, int sel, double cH,
-                                                   double beta, T *__restrict__ y,
-                                                   T *__restrict__ ypart, const int *gate, int gstride,
-                                                   const int *__restrict__ tile_b, size_t gd_stride,
-                                                   T *__restrict__ y2, T *__restrict__ ypart2, int sel2) {
+// The G-part sums the source halves hp and parts v (Z0, Z2, W0, W1) in the order (hp, v).
+// FUSED: no V input.  The windows of V_{l,v} (all four parts of a half hp) are formed from x by the coupling
+// lists of phase A (aoff, ack, acf: the lists of k_angular_list), with the same terms in the same order as
+// k_angular_list forms V, so the results are bit-identical to the two-kernel form and V never goes to HBM.
+template <int BW, bool DUAL, class T, int R, int P, int HW = 1, bool FUSED = false>
+__global__ __launch_bounds__(64 * HW) void k_radial_dia(
+    const T *__restrict__ x, const T *__restrict__ V, const T *__restrict__ GD, const T *__restrict__ DD,
+    const int *__restrict__ tiles, const int *__restrict__ tile_q, const double *__restrict__ tile_k, int nblk, int n,
+    int nh, int nhp, int nseg, int lgrid, int nq, int sel, double cH, double beta, T *__restrict__ y,
+    T *__restrict__ ypart, const int *gate, int gstride, const int *__restrict__ tile_b, size_t gd_stride,
+    T *__restrict__ y2, T *__restrict__ ypart2, int sel2, const int *__restrict__ aoff = nullptr,
+    const int2 *__restrict__ ack = nullptr, const T *__restrict__ acf = nullptr, int lmax = 0, int nblk1 = 0) {
     constexpr int SEG = 64 * P, ND = 2 * BW + 1, W = SEG + 2 * BW;
+    constexpr int NT = 64 * HW;  // threads of the workgroup
     using Rt = typename cx<T>::R;
-    __shared__ T sm[R * W];
+    __shared__ T sm[(FUSED ? 4 : 1) * R * W];  // FUSED: the windows of the four parts
     const int lane = threadIdx.x & 63;
-    int rest = blockIdx.x;
+    // (segment, multipole, tile [, half]) with tiles fastest, in XCD order: an XCD works through a few
+    // (segment, multipole) slices at a time, whose G-sums and x windows stay in its L2
+    const int nt2 = (int)gridDim.x / (nseg * lgrid);
+    int rest = xcd_order((int)blockIdx.x, (int)gridDim.x);
+    const int t2 = rest % nt2;
+    rest /= nt2;
     const int l = rest % lgrid;
-    rest /= lgrid;
-    const int seg = rest % nseg;
-    rest /= nseg;
-    const int h = HW == 2 ? (int)(threadIdx.x >> 6) : (rest & 1);
-    const int tile = HW == 2 ? rest : rest >> 1;
+    const int seg = rest / lgrid;
+    const int h = HW == 2 ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : (t2 & 1);
+    const int tile = HW == 2 ? t2 : t2 >> 1;
     const int inst = tile_b[tile];
     GADE_GATED_B(gate, gstride, inst);
     GD += (size_t)inst * gd_stride;  // this instance's G-sums
@@ -657,7 +675,7 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
     // window of half hp of 'src' for every row of the tile: position p <-> half index i0 - BW + p.
     // All loads of the window are issued before the first LDS write.
     auto stage = [&](const T *src, int hp) {
-        constexpr int NT = 64 * HW, NC = (W + NT - 1) / NT;
+        constexpr int NC = (W + NT - 1) / NT;
         T t[R][NC];
 #pragma unroll
         for (int i = 0; i < R; i++) {
@@ -680,9 +698,71 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
             }
         __syncthreads();
     };
-    // acc[i][j] += sum_d M_d[i0 + P*lane + j] * window[P*lane + j + d]; with two coefficient
-    // streams M = C1 + kap*C2 (pairs), else M = C1.  Coefficients are fetched PF diagonals ahead.
-    auto sweep = [&](const T *C, bool pair) {
+    // FUSED: the windows of V_{l,v} (v = 0..3) for half hp of every row into sm[v][i][p], each entry the sums
+    // Z0 = sum_e h0_e x_{c_e}, Z2 = sum_e h0_e (k_{c_e} x_{c_e}), W0, W1 likewise with h1, over the row's list
+    // (l) in list order from +0, as k_angular_list forms them (positions outside the half are 0).  Wave h of a
+    // two-wave workgroup takes rows h, h + 2, ...; each lane positions lane + 64c.  The list entries are
+    // wave-uniform (scalar loads); each x value is loaded once for the four sums.
+    auto gather = [&](int hp) {
+        constexpr int NCG = (W + 63) / 64;
+        constexpr int U = kAngListU;
+        const int w0 = HW == 2 ? h : 0;
+        const T *xh = x + (size_t)inst * nblk1 * n + hp * nh;
+        int bc[NCG];
+#pragma unroll
+        for (int c = 0; c < NCG; c++) bc[c] = min(max(i0 - BW + c * 64 + lane, 0), nh - 1);
+#pragma unroll
+        for (int k = 0; k < R / HW; k++) {
+            const int i = w0 + HW * k;
+            T z[4][NCG];
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+#pragma unroll
+                for (int c = 0; c < NCG; c++) z[v][c] = czero<T>();
+            if (rows[i] >= 0) {
+                const int *off = aoff + (size_t)(rows[i] - inst * nblk1) * lmax + l;
+                const int e1 = off[1];
+#pragma unroll 2
+                for (int e = off[0]; e < e1; e += U) {
+                    T xv[U][NCG];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const T *xc = xh + (size_t)ack[e + u].x * n;
+#pragma unroll
+                        for (int c = 0; c < NCG; c++) xv[u][c] = xc[bc[c]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const T h0 = acf[2 * (e + u)], h1 = acf[2 * (e + u) + 1];
+                        const Rt kc = (Rt)ack[e + u].y;
+#pragma unroll
+                        for (int c = 0; c < NCG; c++) {
+                            const T kx = cscale(xv[u][c], kc);
+                            z[0][c] = cfma(h0, xv[u][c], z[0][c]);
+                            z[1][c] = cfma(h0, kx, z[1][c]);
+                            z[2][c] = cfma(h1, xv[u][c], z[2][c]);
+                            z[3][c] = cfma(h1, kx, z[3][c]);
+                        }
+                    }
+                }
+            }
+            if (k == 0) __syncthreads();  // the previous sweeps' reads of sm are done
+#pragma unroll
+            for (int c = 0; c < NCG; c++) {
+                const int p = c * 64 + lane;
+                const int b = i0 - BW + p;
+                const bool ok = rows[i] >= 0 && b >= 0 && b < nh;
+                if (p < W) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++) sm[(v * R + i) * W + dia_swz<P>(p)] = ok ? z[v][c] : czero<T>();
+                }
+            }
+        }
+        __syncthreads();
+    };
+    // acc[i][j] += sum_d M_d[i0 + P*lane + j] * win[i][P*lane + j + d] (win: R windows of W); with two
+    // coefficient streams M = C1 + kap*C2 (pairs), else M = C1.  Coefficients are fetched PF diagonals ahead.
+    auto sweep = [&](const T *win, const T *C, bool pair) {
         constexpr int PF = P >= 4 ? 2 : P == 2 ? 3 : (R <= 8 ? 8 : 4);  // coefficient diagonals in flight
         const int stride = pair ? 2 : 1;
         const T *cp = C + (size_t)(i0 + P * lane) * stride;
@@ -690,7 +770,7 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
 #pragma unroll
         for (int i = 0; i < R; i++)
 #pragma unroll
-            for (int j = 0; j < P; j++) w[i][j] = sm[i * W + dia_swz<P>(P * lane + j)];
+            for (int j = 0; j < P; j++) w[i][j] = win[i * W + dia_swz<P>(P * lane + j)];
         T c1[PF][P], c2[PF][P];
 #pragma unroll
         for (int d = 0; d < PF - 1; d++)
@@ -723,7 +803,7 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
                 if (d + 1 < ND) {
 #pragma unroll
                     for (int j = 0; j + 1 < P; j++) w[i][j] = w[i][j + 1];
-                    w[i][P - 1] = sm[i * W + dia_swz<P>(P * lane + P + d)];
+                    w[i][P - 1] = win[i * W + dia_swz<P>(P * lane + P + d)];
                 }
             }
         }
@@ -740,19 +820,25 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
             if (YPO) {                                                        \
                 (YPO)[(size_t)l * plane + o] = acc[i][j];                     \
             } else {                                                          \
-                T r_ = acc[i][j];                                       \
-                if (beta != 0.0) r_ = cadd(r_, cscale((YO)[o], (Rt)beta));        \
+                T r_ = acc[i][j];                                             \
+                if (beta != 0.0) r_ = cadd(r_, cscale((YO)[o], (Rt)beta));    \
                 (YO)[o] = r_;                                                 \
             }                                                                 \
         }                                                                     \
     }
 
     if (cH != 0.0) {
-        for (int v = 0; v < 4; v++) {
-            for (int hp = 0; hp < 2; hp++) {
+        // k = (hp, v): one copy of the sweep in the code (its own loops are unrolled)
+#pragma unroll 1
+        for (int k = 0; k < 8; k++) {
+            const int hp = k >> 2, v = k & 3;
+            if constexpr (FUSED) {
+                if (v == 0) gather(hp);
+            } else {
                 stage(V + (size_t)(l * 4 + v) * plane, hp);
-                sweep(GD + ((size_t)(l * 4 + v) * S + (size_t)(2 * h + hp) * ND * nhp) * 2, true);
             }
+            sweep(FUSED ? sm + v * R * W : sm, GD + ((size_t)(l * 4 + v) * S + (size_t)(2 * h + hp) * ND * nhp) * 2,
+                  true);
         }
     }
     // the second output's unscaled G-part: final for l > 0, parked in its own output slot for l = 0
@@ -768,7 +854,7 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
     if (l == 0) {
         for (int hp = 0; hp < 2; hp++) {
             stage(x, hp);
-            sweep(DD + ((size_t)(sel * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
+            sweep(sm, DD + ((size_t)(sel * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
         }
     }
     DIA_EMIT(y, ypart)
@@ -788,7 +874,7 @@ __global__ __launch_bounds__(64 * HW) void k_radial_dia(const T *__restrict__ x,
                 }
             for (int hp = 0; hp < 2; hp++) {
                 stage(x, hp);
-                sweep(DD + ((size_t)(sel2 * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
+                sweep(sm, DD + ((size_t)(sel2 * nq + q) * S + (size_t)(2 * h + hp) * ND * nhp), false);
             }
             DIA_EMIT(y2, ypart2)
         }
@@ -1276,6 +1362,16 @@ int launch_angular(gade_model *m, const double2 *x) {
     return launch_angular_t<double2>(m, x, m->d_coef, m->d_acf, m->d_V);
 }
 
+template <class T>
+bool ang_in_radial(const gade_model *m);
+
+// phase A of an apply into V, unless the banded radial kernel forms its windows from x itself
+int angular_phase(gade_model *m, const double2 *x) {
+    if (ang_in_radial<double2>(m)) return GADE_OK;
+    Prof pr(m, "angular", GADE_PROF_ANGULAR);
+    return launch_angular(m, x);
+}
+
 // tile shapes (rows per tile, outputs per lane) with an instantiated banded kernel: rows x outputs <= 16
 // complex accumulators per lane
 bool dia_shape_ok(int r, int p) {
@@ -1284,26 +1380,53 @@ bool dia_shape_ok(int r, int p) {
 
 // the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
 // each output in the same order, so all give the same results bit for bit
+// phase A inside the banded radial kernel (its windows formed from x and the coupling lists, no V): the
+// FP64 form when the MFMA kernel is off, the FP32 form of the mixed solve always (it has no MFMA kernel)
+template <class T>
+bool ang_in_radial(const gade_model *m) {
+    return m->ang_fused && m->radial_kind == 2 && (sizeof(T) == sizeof(float2) || !m->radial_mfma);
+}
+
+// tile shapes with a fused form: one row's window sums per lane fit the registers for up to 2 outputs per lane
+bool fused_shape_ok(int r, int p) { return dia_shape_ok(r, p) && p <= 2; }
+
 template <bool DUAL, class T>
 int launch_dia(gade_model *m, int lgrid, const T *x, const T *V, const T *GD, const T *DD, int sel, double cH,
                double beta, T *y, T *yp, T *y2, T *yp2, int sel2) {
     const int P = m->dia_p;
     const int nseg = (m->nh + 64 * P - 1) / (64 * P);
+    const bool fused = ang_in_radial<T>(m);
     const int hw = m->dia_hw;
     const dim3 grid((unsigned)m->ndtiles * (2 / hw) * nseg * lgrid);
-#define GADE_DIA1(R_, P_, HW_)                                                                                    \
-    hipLaunchKernelGGL((k_radial_dia<kDiaBW, DUAL, T, R_, P_, HW_>), grid, dim3(64 * HW_), 0, m->stream, x, V, GD, \
-                       DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, nseg, lgrid,     \
-                       m->nq, sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b, m->gd_stride, y2, yp2, sel2)
-#define GADE_DIA(R_, P_)          \
-    if (hw == 2)                  \
-        GADE_DIA1(R_, P_, 2);     \
-    else                          \
-        GADE_DIA1(R_, P_, 1)
+    const T *acf = sizeof(T) == sizeof(float2) ? (const T *)m->d_acf32 : (const T *)m->d_acf;
+    if (fused && !fused_shape_ok(m->dia_r, P)) {
+        gade::set_error("the fused angular + radial form needs at most 2 outputs per lane");
+        return GADE_EINVAL;
+    }
+#define GADE_DIA1(R_, P_, HW_, F_)                                                                               \
+    hipLaunchKernelGGL((k_radial_dia<kDiaBW, DUAL, T, R_, P_, HW_, F_>), grid, dim3(64 * HW_), 0, m->stream, x, V, \
+                       GD, DD, m->d_dtiles, m->d_dtile_q, m->d_dtile_k, m->nblk, m->n, m->nh, m->nhp, nseg, lgrid, \
+                       m->nq, sel, cH, beta, y, yp, m->gate, m->gstride, m->d_dtile_b, m->gd_stride, y2, yp2, sel2, \
+                       m->d_aoff, (const int2 *)m->d_ack, acf, m->lmax, m->nblk1)
+#define GADE_DIA(R_, P_)                 \
+    if (hw == 2 && fused) {              \
+        GADE_DIA1(R_, P_, 2, true);      \
+    } else if (fused) {                  \
+        GADE_DIA1(R_, P_, 1, true);      \
+    } else if (hw == 2) {                \
+        GADE_DIA1(R_, P_, 2, false);     \
+    } else {                             \
+        GADE_DIA1(R_, P_, 1, false);     \
+    }
     switch (m->dia_r * 10 + P) {
         case 82: GADE_DIA(8, 2); break;
         case 42: GADE_DIA(4, 2); break;
-        case 44: GADE_DIA(4, 4); break;
+        case 44:
+            if (hw == 2)
+                GADE_DIA1(4, 4, 2, false);
+            else
+                GADE_DIA1(4, 4, 1, false);
+            break;
         case 81: GADE_DIA(8, 1); break;
         case 41: GADE_DIA(4, 1); break;
         case 161: GADE_DIA(16, 1); break;
@@ -1736,35 +1859,33 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
                 cp[0] = v0.real(); cp[1] = v0.imag(); cp[2] = v1.real(); cp[3] = v1.imag();
             }
         }
-    // coupling lists of phase A (one instance's rows): per row, multipole and part, the columns whose
-    // coefficient is non-zero, in the Hall row's (increasing column) order
+    // coupling lists of phase A (one instance's rows): per row and multipole, the columns whose H0l or H1l
+    // coefficient is non-zero, in the Hall row's (increasing column) order, each with both coefficients (h0,
+    // h1; one of them may be 0)
     std::vector<int2> ack;
     std::vector<double> acf;
-    m->h_aoff.assign((size_t)m->nblk * lmax * 2 + 1, 0);
+    m->h_aoff.assign((size_t)m->nblk * lmax + 1, 0);
     for (int r = 0; r < m->nblk; r++)
-        for (int l = 0; l < lmax; l++)
-            for (int w = 0; w < 2; w++) {
-                m->h_aoff[((size_t)r * lmax + l) * 2 + w] = (int)ack.size();
-                const size_t first = ack.size();
-                for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++) {
-                    const double *cp = &coef[((size_t)p * lmax + l) * 4 + 2 * w];
-                    if (cp[0] != 0.0 || cp[1] != 0.0) {
-                        const int c = m->hall_Ai[p];
-                        ack.push_back(make_int2(c, m->kap_blk[c]));
-                        acf.push_back(cp[0]);
-                        acf.push_back(cp[1]);
-                    }
-                }
-                while ((ack.size() - first) % kAngListU) {  // zero terms on the last column
-                    ack.push_back(ack.back());
-                    acf.push_back(0.0);
-                    acf.push_back(0.0);
+        for (int l = 0; l < lmax; l++) {
+            m->h_aoff[(size_t)r * lmax + l] = (int)ack.size();
+            const size_t first = ack.size();
+            for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++) {
+                const double *cp = &coef[((size_t)p * lmax + l) * 4];
+                if (cp[0] != 0.0 || cp[1] != 0.0 || cp[2] != 0.0 || cp[3] != 0.0) {
+                    const int c = m->hall_Ai[p];
+                    ack.push_back(make_int2(c, m->kap_blk[c]));
+                    acf.insert(acf.end(), cp, cp + 4);
                 }
             }
+            while ((ack.size() - first) % kAngListU) {  // zero terms on the last column
+                ack.push_back(ack.back());
+                acf.insert(acf.end(), 4, 0.0);
+            }
+        }
     m->h_aoff.back() = (int)ack.size();
     if (ack.empty()) {  // no coupling: one unread entry keeps the uploads non-empty
         ack.push_back(make_int2(0, 0));
-        acf.assign(2, 0.0);
+        acf.assign(4, 0.0);
     }
     // B instances: the Hall structure, kappa maps and angular coefficients repeat with row offsets
     m->N1 = (long long)m->nblk1 * n;
@@ -1922,10 +2043,7 @@ static int apply_async(gade_model *m, const double2 *x, double2 *y, double cH, d
         gade::set_error("gade_apply: cH must be +1, -1 or 0");
         return GADE_EINVAL;
     }
-    if (sel != 2) {
-        Prof pr(m, "angular", GADE_PROF_ANGULAR);
-        GADE_TRY(launch_angular(m, x));
-    }
+    if (sel != 2) GADE_TRY(angular_phase(m, x));
     return launch_radial(m, x, y, sel, cH, beta);
 }
 
@@ -1936,10 +2054,7 @@ static int apply_pair_async(gade_model *m, const double2 *x, double2 *ym, double
         GADE_TRY(apply_async(m, x, ym, -1.0, 0.0));
         return apply_async(m, x, yp, 1.0, 0.0);
     }
-    {
-        Prof pr(m, "angular", GADE_PROF_ANGULAR);
-        GADE_TRY(launch_angular(m, x));
-    }
+    GADE_TRY(angular_phase(m, x));
     return launch_radial(m, x, ym, 1, -1.0, 0.0, yp, 0);
 }
 
@@ -1999,10 +2114,7 @@ bool fused_sums(const gade_model *m) { return m->radial_kind == 2 && m->lmax >= 
 
 template <int OP>
 int apply_dot(gade_model *m, const double2 *x, double2 *out, double *part, bool fin) {
-    {
-        Prof pr(m, "angular", GADE_PROF_ANGULAR);
-        GADE_TRY(launch_angular(m, x));
-    }
+    GADE_TRY(angular_phase(m, x));
     {
         Prof pr(m, "radial", GADE_PROF_RADIAL);
         GADE_TRY((launch_dia64<false>(m, m->lmax, x, 0, 1.0, 0.0, out, m->d_ypart, nullptr, nullptr, 0)));
@@ -2274,7 +2386,7 @@ int mp_setup(gade_model *m) {
     GADE_TRY(hip_alloc((void **)&m->d_GD32, sizeof(float2) * m->gd_stride * m->B));
     GADE_TRY(hip_alloc((void **)&m->d_DD32, sizeof(float2) * nDD));
     GADE_TRY(hip_alloc((void **)&m->d_coef32, sizeof(float2) * nco));
-    const size_t nacf = (size_t)m->h_aoff.back();
+    const size_t nacf = 2 * (size_t)m->h_aoff.back();  // (h0, h1) per list entry
     GADE_TRY(hip_alloc((void **)&m->d_acf32, sizeof(float2) * std::max<size_t>(nacf, 1)));
     for (auto &f : m->lu32) GADE_TRY(hip_alloc((void **)&f, sizeof(float2) * nlu));
     for (float2 **v : {&m->mp_r, &m->mp_r0, &m->mp_p, &m->mp_v, &m->mp_y, &m->mp_z, &m->mp_s, &m->mp_t, &m->mp_d})
@@ -2296,7 +2408,7 @@ int mp_setup(gade_model *m) {
 
 // y = (S + H) x in FP32 (the FP32 G-sums of the current step)
 int apply32_async(gade_model *m, const float2 *x, float2 *y) {
-    {
+    if (!ang_in_radial<float2>(m)) {
         Prof pr(m, "angular_f32", GADE_PROF_ANGULAR);
         GADE_TRY(launch_angular_t<float2>(m, x, (const float2 *)m->d_coef32, (const float2 *)m->d_acf32, m->d_V32));
     }
@@ -2411,10 +2523,7 @@ int enqueue_rhs_init(gade_model *m, double2 *psi) {
         GADE_TRY(enqueue_rhs_ax0(m, psi));
         return bicg_op<OP_INIT>(m, m->tmp);
     }
-    {
-        Prof pr(m, "angular", GADE_PROF_ANGULAR);
-        GADE_TRY(launch_angular(m, psi));
-    }
+    GADE_TRY(angular_phase(m, psi));
     {
         Prof pr(m, "radial_pair", GADE_PROF_RADIAL);
         GADE_TRY((launch_dia64<true>(m, m->lmax, psi, 1, -1.0, 0.0, m->tmp, m->d_ypart, m->r, m->d_ypart2, 0)));
@@ -2962,15 +3071,17 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         times[step] = time;
     }
     struct Cand {
-        int window, iter_batch, graph, dia_r, dia_p, ang_lb, dia_hw;
+        int window, iter_batch, graph, dia_r, dia_p, ang_lb, dia_hw, fused;
         double ms;
     };
     // stage 1: kernel shapes (banded tile shape x angular multipole group) with the model's stepping;
     // stage 2: stepping with the best shapes
     // (the FP64 MFMA radial kernel has one shape: only the angular group is tuned then; the tile shape still
     // matters to the FP32 kernels of the mixed-precision form)
+    // (the fused angular + radial form: the shapes with up to 2 outputs per lane, the angular group unused)
     std::vector<Cand> cands;
     const bool one_tile = m->radial_mfma && m->precision == 0;
+    const int g0 = m->use_graph ? 1 : 0;
     if (m->radial_kind == 2)
         for (auto rp : {std::pair<int, int>{8, 2}, {4, 2}, {4, 4}, {8, 1}, {16, 1}, {4, 1}}) {
             if (one_tile && (rp.first != m->dia_r || rp.second != m->dia_p)) continue;
@@ -2978,14 +3089,16 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
                 if (one_tile && hw != m->dia_hw) continue;
                 for (int lb : {4, 2, 8})
                     if (ang_lds_ok(m, lb))
-                        cands.push_back({m->window, m->iter_batch, m->use_graph ? 1 : 0, rp.first, rp.second, lb, hw, 0.0});
+                        cands.push_back({m->window, m->iter_batch, g0, rp.first, rp.second, lb, hw, 0, 0.0});
+                if (!one_tile && fused_shape_ok(rp.first, rp.second))
+                    cands.push_back({m->window, m->iter_batch, g0, rp.first, rp.second, m->ang_lb, hw, 1, 0.0});
             }
         }
     const size_t nshape = cands.size();
     for (int g : {1, 0}) {
         if (g && (m->radial_kind != 2 || !m->use_graph)) continue;  // graph 0 on the model excludes graphs
         for (int w : {2, 4, 8, 16, 32})
-            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, -1, -1, 0.0});
+            for (int k : {0, 1, 2, 4}) cands.push_back({w, k, g, -1, -1, -1, -1, -1, 0.0});
     }
     const size_t N = (size_t)m->N, B = (size_t)m->B;
     // the solver state a run starts from (BiCGSTAB workspace, which carries over between solves, scalars,
@@ -3028,13 +3141,14 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         return GADE_OK;
     };
     const int sv_window = m->window, sv_batch = m->iter_batch, sv_profile = m->profile, sv_k = m->last_k;
-    const int sv_r = m->dia_r, sv_p = m->dia_p, sv_lb = m->ang_lb, sv_hw = m->dia_hw;
+    const int sv_r = m->dia_r, sv_p = m->dia_p, sv_lb = m->ang_lb, sv_hw = m->dia_hw, sv_fused = m->ang_fused;
     const bool sv_graph = m->use_graph;
-    auto set_shape = [&](int r, int pp, int lb, int hw) -> int {
-        if (lb != m->ang_lb || hw != m->dia_hw) {
+    auto set_shape = [&](int r, int pp, int lb, int hw, int fused) -> int {
+        if (lb != m->ang_lb || hw != m->dia_hw || fused != m->ang_fused) {
             drop_step_graph(m);
             m->ang_lb = lb;
             m->dia_hw = hw;
+            m->ang_fused = fused;
         }
         if (!m->dia_ok || (r == m->dia_r && pp == m->dia_p)) return GADE_OK;
         drop_step_graph(m);
@@ -3050,7 +3164,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
         m->use_graph = sv_graph;
         m->profile = sv_profile;
         m->last_k = sv_k;
-        (void)set_shape(sv_r, sv_p, sv_lb, sv_hw);
+        (void)set_shape(sv_r, sv_p, sv_lb, sv_hw, sv_fused);
         const int rc = copy_state(false);
         (void)hipStreamSynchronize(m->stream);
         (void)hipFree(work);
@@ -3072,7 +3186,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             for (size_t k = 1; k < nshape; k++)
                 if (cands[k].ms < cands[best_shape].ms) best_shape = k;
             if (int rc = set_shape(cands[best_shape].dia_r, cands[best_shape].dia_p, cands[best_shape].ang_lb,
-                                   cands[best_shape].dia_hw)) {
+                                   cands[best_shape].dia_hw, cands[best_shape].fused)) {
                 restore();
                 return rc;
             }
@@ -3082,7 +3196,9 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             cands[c].dia_p = m->dia_p;
             cands[c].ang_lb = m->ang_lb;
             cands[c].dia_hw = m->dia_hw;
-        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p, cands[c].ang_lb, cands[c].dia_hw)) {
+            cands[c].fused = m->ang_fused;
+        } else if (int rc = set_shape(cands[c].dia_r, cands[c].dia_p, cands[c].ang_lb, cands[c].dia_hw,
+                                      cands[c].fused)) {
             restore();
             return rc;
         }
@@ -3125,7 +3241,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
                             std::to_string(cands[c].iter_batch) + " graph " + std::to_string(cands[c].graph) +
                             " tile " + std::to_string(cands[c].dia_r) + "x" + std::to_string(cands[c].dia_p) +
                             " halves " + std::to_string(cands[c].dia_hw) + " angular group " +
-                            std::to_string(cands[c].ang_lb) +
+                            std::to_string(cands[c].ang_lb) + " fused " + std::to_string(cands[c].fused) +
                             " changed the trajectory");
             return GADE_EINVAL;
         }
@@ -3137,7 +3253,8 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
     m->window = cands[pick].window;
     m->iter_batch = cands[pick].iter_batch;
     m->use_graph = cands[pick].graph != 0;
-    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p, cands[pick].ang_lb, cands[pick].dia_hw));
+    GADE_TRY(set_shape(cands[pick].dia_r, cands[pick].dia_p, cands[pick].ang_lb, cands[pick].dia_hw,
+                       cands[pick].fused));
     if (report)
         for (size_t c = 0; c < cands.size() && c < GADE_TUNE_MAX; c++) {
             double *r = report + (size_t)GADE_TUNE_COLS * c;
@@ -3149,6 +3266,7 @@ int gade_autotune(gade_model_t *m, const double *psi, const double *params, doub
             r[5] = cands[c].ang_lb;
             r[6] = cands[c].ms;
             r[7] = cands[c].dia_hw;
+            r[8] = cands[c].fused;
         }
     if (ncand) *ncand = (int)std::min<size_t>(cands.size(), GADE_TUNE_MAX);
     if (chosen) *chosen = (int)pick;
@@ -3373,6 +3491,17 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         m->ang_lb = value;
         return GADE_OK;
     }
+    if (!strcmp(name, "ang_fused")) {
+        // 1: the banded radial kernel forms each V window from x and the coupling lists (no angular kernel, no
+        // V in HBM; bit-identical results), 0: the angular kernel writes V and the radial kernel reads it
+        if ((value != 0 && value != 1) || (value == 1 && !fused_shape_ok(m->dia_r, m->dia_p))) {
+            gade::set_error("ang_fused must be 0 or 1 (1 needs a tile shape with at most 2 outputs per lane)");
+            return GADE_EINVAL;
+        }
+        if (value != m->ang_fused) drop_step_graph(m);
+        m->ang_fused = value;
+        return GADE_OK;
+    }
     if (!strcmp(name, "dia_halves")) {
         // output halves per banded workgroup: 1 (one wave per half) or 2 (two waves share each window)
         if (value != 1 && value != 2) {
@@ -3389,6 +3518,10 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
         if (!dia_shape_ok(r, pp)) {
             gade::set_error("banded tile shape (dia_rows, dia_outputs) must be (4,1), (4,2), (4,4), (8,1), (8,2) "
                             "or (16,1)");
+            return GADE_EINVAL;
+        }
+        if (m->ang_fused && !fused_shape_ok(r, pp)) {
+            gade::set_error("the fused angular + radial form (ang_fused 1) has no (4,4) tile shape");
             return GADE_EINVAL;
         }
         if (!m->dia_ok) {
@@ -3553,6 +3686,10 @@ int gade_get_param(const gade_model_t *m, const char *name, int *value) {
     }
     if (!strcmp(name, "dia_halves")) {
         *value = m->dia_hw;
+        return GADE_OK;
+    }
+    if (!strcmp(name, "ang_fused")) {
+        *value = m->ang_fused;
         return GADE_OK;
     }
     if (!strcmp(name, "dia_outputs")) {

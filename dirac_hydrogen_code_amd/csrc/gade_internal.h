@@ -138,6 +138,7 @@ struct gade_model {
     int *d_mg_rows = nullptr, *d_mg_q = nullptr, *d_mg_b = nullptr;  // [nmg][16], [nmg][2], [nmg]
     double *d_mg_k = nullptr;      // [nmg][2] kappa of each half
     int ang_lb = 4;                // multipoles per angular block
+    int ang_fused = 0;             // 1: the banded radial kernel forms its V windows from x (no V in HBM)
     int psolve_ch = 16;            // preconditioner level height (16 or 32 rows)
     size_t gd_stride = 0;          // GD entries per instance
     double2 *d_V = nullptr;        // [lmax][4][nblk][n]

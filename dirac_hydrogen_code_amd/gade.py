@@ -39,7 +39,7 @@ LG_SAMPLE_BETA = LG_IN + 36 + 6 + 12 * LG_POD + 2 * LG_POD  # index of |e2|^2 / 
 LG_SAMPLE_VALID = LG_SAMPLE_BETA + 1                             # index of a sample's valid flag
 
 GADE_TUNE_MAX = 96
-GADE_TUNE_COLS = 8
+GADE_TUNE_COLS = 9
 
 _lib = None
 
@@ -215,14 +215,16 @@ class Propagator:
     def autotune(self, psi: DeviceArray, params, nsteps, t0=0.0, tol=1e-12, maxit=500, reps=2):
         """Time every kernel shape and stepping candidate on nsteps probe steps from psi (unchanged), keep the
         fastest; every candidate must reproduce the same trajectory bit for bit.
-        Returns (chosen, [dict(window, iter_batch, graph, dia_rows, dia_outputs, ang_lb, ms, dia_halves)])."""
+        Returns (chosen, [dict(window, iter_batch, graph, dia_rows, dia_outputs, ang_lb, ms, dia_halves,
+        ang_fused)])."""
         params = np.ascontiguousarray(params, np.float64).reshape(self.batch, 4)
         rep = np.zeros((GADE_TUNE_MAX, GADE_TUNE_COLS), np.float64)
         nc, ch = C.c_int(), C.c_int()
         check(lib().gade_autotune(self.ptr, psi.ptr, params.ctypes.data, t0, nsteps, tol, maxit, reps,
                                   rep.ctypes.data, C.byref(nc), C.byref(ch)))
         rows = [dict(window=int(r[0]), iter_batch=int(r[1]), graph=int(r[2]), dia_rows=int(r[3]),
-                     dia_outputs=int(r[4]), ang_lb=int(r[5]), ms=float(r[6]), dia_halves=int(r[7]))
+                     dia_outputs=int(r[4]), ang_lb=int(r[5]), ms=float(r[6]), dia_halves=int(r[7]),
+                     ang_fused=int(r[8]))
                 for r in rep[:nc.value]]
         return ch.value, rows
 

@@ -139,19 +139,20 @@ int gade_cn_run_batch(gade_model_t *m, double *psi_dev, const double *params, co
  * psi_dev (left unchanged) with the pulses params [batch][4] (as gade_cn_run_batch, t0 the start time),
  * best of reps runs per candidate.  Stage 1 (banded radial kernel): the tile shapes (dia_rows,
  * dia_outputs) in {(8,2), (4,2), (4,4), (8,1), (16,1), (4,1)} x output halves per workgroup dia_halves in
- * {1, 2} x the angular multipole group ang_lb in {4, 2, 8}, with the model's stepping (with the FP64 MFMA
+ * {1, 2} x the angular multipole group ang_lb in {4, 2, 8}, and the fused angular + radial form
+ * (ang_fused 1) for the shapes with up to 2 outputs per lane, with the model's stepping (with the FP64 MFMA
  * radial kernel in FP64 only ang_lb varies); stage 2, with the fastest shapes:
  * window in {2,4,8,16,32}, iter_batch in {0,1,2,4}, graph in {0,1} (the graph only with the banded
  * kernel and when the model's "graph" parameter is 1).  The fastest stage-2 candidate is kept.  Every candidate
  * must give the same trajectory, iterations and tolerances bit for bit, else GADE_EINVAL with the
  * message naming it.  report [GADE_TUNE_MAX][GADE_TUNE_COLS] (NULL allowed) receives (window,
- * iter_batch, graph, dia_rows, dia_outputs, ang_lb, ms, dia_halves) per candidate and *ncand their count; the
+ * iter_batch, graph, dia_rows, dia_outputs, ang_lb, ms, dia_halves, ang_fused) per candidate and *ncand their count; the
  * chosen index
  * goes to *chosen.  Every probe starts from the
  * model's current solver state (BiCGSTAB workspace, scalars, guess histories), which is restored on
  * return: a run after tuning equals the same run without it. */
 #define GADE_TUNE_MAX 96
-#define GADE_TUNE_COLS 8
+#define GADE_TUNE_COLS 9
 
 /* ---- learned initial guess (no reference counterpart) ----------------------------------------------
  * guess_order GADE_GUESS_LEARNED: x0 = x2 + s sum_k w_k D_k/|D_k| + s U c per step, with D_k the backward
@@ -298,6 +299,10 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
  *   "guess_order"  initial guess of each CN step: 0 psi_n (default, the
  *             reference), 1..3 polynomial extrapolation (gade_extrapolate),
  *             GADE_GUESS_LEARNED the learned guess;
+ *   "ang_fused"  1: the banded radial kernel forms its windows of the angular
+ *             phase from x and the coupling lists itself (no angular kernel, no V in
+ *             HBM; tile shapes with up to 2 outputs per lane); 0 (default): the
+ *             angular kernel writes V, the radial kernel reads it;
  *   "lg_zmax"  range gate of the learned guess's inputs (gade_lg_uses).
  * Results do not depend on the choice. */
 int gade_set_param(gade_model_t *m, const char *name, int value);

@@ -100,16 +100,16 @@ def test_autotune_keeps_trajectory(datadir, cfg, batch):
         it1, tol1 = gp.cn_run_batch(x, params, 5, tol=1e-12)
         if tune:
             chosen, rows = gp.autotune(x, params, 6, t0=5 * dt, reps=1)
-            # kernel shapes: 6 tile shapes x 2 output-half groupings x 3 angular groups, only the angular groups
-            # with the FP64 MFMA radial kernel (one shape); then 5 windows x 4 iteration batches x graph on / off
-            shapes = 3 if gp.get_param("radial_mfma") and gp.get_param("precision") == 0 else 36
+            # kernel shapes: 6 tile shapes x 2 output-half groupings x (3 angular groups + the fused form for the
+            # 5 shapes with up to 2 outputs per lane), only the angular groups with the FP64 MFMA radial kernel
+            # (one shape); then 5 windows x 4 iteration batches x graph on / off
+            shapes = 3 if gp.get_param("radial_mfma") and gp.get_param("precision") == 0 else 46
             assert len(rows) == (shapes + 40 if gp.get_param("radial") == 2 else 20)
+            assert sum(r["ang_fused"] for r in rows[:shapes]) == (10 if shapes == 46 else 0)
             assert all(r["ms"] > 0 for r in rows)
             c = rows[chosen]
-            assert (gp.get_param("window"), gp.get_param("iter_batch"), gp.get_param("graph"),
-                    gp.get_param("dia_rows"), gp.get_param("dia_outputs"), gp.get_param("ang_lb"),
-                    gp.get_param("dia_halves")) == (c["window"], c["iter_batch"], c["graph"], c["dia_rows"],
-                                                    c["dia_outputs"], c["ang_lb"], c["dia_halves"])
+            keys = ("window", "iter_batch", "graph", "dia_rows", "dia_outputs", "ang_lb", "dia_halves", "ang_fused")
+            assert tuple(gp.get_param(k) for k in keys) == tuple(c[k] for k in keys)
             say(f"  chose {c} of {len(rows)}")
         it2, tol2 = gp.cn_run_batch(x, params, 10, t0=5 * dt, tol=1e-12)
         out[tune] = (x.get(), it1, it2, tol2)

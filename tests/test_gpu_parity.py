@@ -36,16 +36,19 @@ def pair(datadir, cfg, ntime=200):
     return d, dt, O.Model(d, LMAX[cfg], dt, nthreads=4), gade.Propagator(d, LMAX[cfg], dt)
 
 
-# radial phase kernels: general ELL, banded on the vector units, banded on the FP64 matrix cores (the default)
-RADIAL = {"ell": (1, 0), "dia": (2, 0), "mfma": (2, 1)}
+# radial phase kernels: general ELL, banded on the vector units (the default), banded on the FP64 matrix cores,
+# banded with the angular phase inside (ang_fused)
+RADIAL = {"ell": (1, 0, 0), "dia": (2, 0, 0), "mfma": (2, 1, 0), "fused": (2, 0, 1)}
 
 
 def use_radial(gp, kind):
-    radial, mfma = RADIAL[kind]
+    radial, mfma, fused = RADIAL[kind]
     gp.set_param("radial", radial)
     if radial == 2:
         gp.set_param("radial_mfma", mfma)
+        gp.set_param("ang_fused", fused)
     assert gp.get_param("radial") == radial and (radial == 1 or gp.get_param("radial_mfma") == mfma)
+    assert radial == 1 or gp.get_param("ang_fused") == fused
 
 
 @pytest.mark.parametrize("kind", sorted(RADIAL))
@@ -198,8 +201,9 @@ def test_tile_width_invariance(datadir):
 
 def test_radial_kernels_agree_and_switch(datadir):
     """The banded radial kernels (matrix cores and vector units) and the ELL kernel give the same operator
-    (1e-14), and switching the kernel mid-trajectory re-derives the coefficients of the current f(t) in the
-    new layout."""
+    (1e-14), the banded kernel with the angular phase inside (ang_fused) equals the two-kernel form bit for bit,
+    and switching the kernel mid-trajectory re-derives the coefficients of the current f(t) in the new
+    layout."""
     cfg = "nondipole_small"
     d, dt, om, gp = pair(datadir, cfg)
     say("radial kernels agree")
@@ -209,7 +213,7 @@ def test_radial_kernels_agree_and_switch(datadir):
     f = gade.pulse(10, 50, 2, 0.07)
     gp.set_time(f)
     res = {}
-    for kind in ("dia", "ell", "mfma", "dia", "mfma"):
+    for kind in ("dia", "ell", "mfma", "fused", "dia", "mfma"):
         use_radial(gp, kind)
         for cH in (1.0, -1.0, 0.0):
             gp.apply(dx, dy, cH=cH)
@@ -217,6 +221,7 @@ def test_radial_kernels_agree_and_switch(datadir):
     for cH in (1.0, -1.0, 0.0):
         a, b, c = res[("dia", cH)], res[("ell", cH)][0], res[("mfma", cH)]
         assert np.array_equal(a[0].view(np.float64), a[1].view(np.float64))
+        assert np.array_equal(a[0].view(np.float64), res[("fused", cH)][0].view(np.float64))
         assert np.array_equal(c[0].view(np.float64), c[1].view(np.float64))
         assert relmax(a[0], b) <= 1e-14, cH
         assert relmax(c[0], b) <= 1e-14, cH
@@ -384,17 +389,18 @@ def test_extrapolated_guess_same_solution_fewer_iterations(datadir):
 @pytest.mark.parametrize("cfg,batch", [("nondipole_small", 1), ("dipole", 3)])
 def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
     """Every banded tile shape (rows per tile, outputs per lane, one or both output halves per workgroup),
-    angular multipole group and angular form (coupling lists or Hall rows) sums each output in the same
-    order: applies (both signs, FP64) and whole device-driven trajectories (paired apply, FP32 inner solves of
-    the mixed form) are bit-identical across shapes."""
+    angular multipole group and angular form (coupling lists, Hall rows, or the lists inside the radial kernel:
+    ang_fused) sums each output in the same order: applies (both signs, FP64) and whole device-driven
+    trajectories (paired apply, FP32 inner solves of the mixed form) are bit-identical across shapes."""
     d = datadir(cfg)
     dt = dt_of(cfg)
     say(f"tile shapes {cfg} batch={batch}")
     rng = np.random.default_rng(3)
     out = {}
-    # (rows per tile, outputs per lane, angular group, coupling lists, output halves per workgroup)
-    shapes = ((8, 2, 4, 1, 1), (8, 2, 4, 0, 2), (4, 2, 2, 1, 2), (4, 2, 2, 0, 1), (4, 4, 8, 1, 1), (4, 4, 8, 0, 2),
-              (8, 1, 4, 1, 2), (16, 1, 2, 0, 2), (4, 1, 8, 1, 1), (16, 1, 4, 1, 1))
+    # (rows per tile, outputs per lane, angular group, coupling lists, output halves per workgroup, fused)
+    shapes = ((8, 2, 4, 1, 1, 0), (8, 2, 4, 0, 2, 0), (4, 2, 2, 1, 2, 0), (4, 2, 2, 0, 1, 0), (4, 4, 8, 1, 1, 0),
+              (4, 4, 8, 0, 2, 0), (8, 1, 4, 1, 2, 0), (16, 1, 2, 0, 2, 0), (4, 1, 8, 1, 1, 0), (16, 1, 4, 1, 1, 0),
+              (8, 2, 4, 1, 2, 1), (8, 1, 4, 1, 1, 1), (4, 2, 2, 1, 1, 1), (4, 1, 4, 1, 2, 1), (16, 1, 4, 1, 2, 1))
     for shape in shapes:
         gp = gade.Propagator(d, LMAX[cfg], dt, batch=batch)
         if gp.get_param("radial") != 2:
@@ -405,7 +411,8 @@ def test_banded_tile_shapes_bit_identical(datadir, cfg, batch):
         gp.set_param("ang_lb", shape[2])
         gp.set_param("ang_list", shape[3])
         gp.set_param("dia_halves", shape[4])
-        assert gp.get_param("dia_halves") == shape[4]
+        gp.set_param("ang_fused", shape[5])
+        assert gp.get_param("dia_halves") == shape[4] and gp.get_param("ang_fused") == shape[5]
         gp.set_time_batch(np.array([gade.pulse(10.0 + 5 * b, 50.0, 1.0, 0.013) for b in range(batch)]))
         n = gp.n * batch
         x = rng.standard_normal(n) + 1j * rng.standard_normal(n) if not out else out["x"]
