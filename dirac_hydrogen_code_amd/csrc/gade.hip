@@ -162,7 +162,7 @@ __device__ __forceinline__ int xcd_order(int b, int G) {
 }
 
 constexpr int kAngThreads = 256;
-constexpr int kAngLB = 4;  // default multipoles per block
+constexpr int kAngLB = 2;  // default multipoles per block
 
 template <int LB, class T = double2>
 __global__ __launch_bounds__(kAngThreads) void k_angular(const T *__restrict__ x, const int *__restrict__ hAp,
@@ -512,15 +512,14 @@ __global__ __launch_bounds__(256) void k_radial_sum(const T *__restrict__ ypart,
 // radial product is a sum over d of coefficient[d][i] * window[i+d]: no column
 // indices, and the window of a wave is one contiguous run staged in LDS.
 //
-// One wave owns kDiaR row blocks of one kappa, one output half h and
-// kDiaSeg = 64*kDiaP consecutive outputs (kDiaP per lane, so each staged
-// window value is read once per lane and reused across the diagonals), and
-// one multipole l (the fastest part of blockIdx, so with 8 multipoles each
-// XCD streams a single multipole's coefficients from its own L2).  The
-// per-multipole results go to ypart and are summed in a fixed order.
+// One wave owns R row blocks of one kappa (kDiaR by default), one output half h,
+// 64*P consecutive outputs (P per lane, so each staged window value is read
+// once per lane and reused across the diagonals) and one multipole l; the grid
+// is ordered so that each XCD works through a few (segment, multipole) slices
+// at a time.  The per-multipole results go to ypart and are summed in a fixed
+// order.
 // ---------------------------------------------------------------------------
 constexpr int kDiaR = 8;        // default rows per tile
-constexpr int kDiaP = 2;        // default outputs per lane
 constexpr int kDiaSegMax = 256; // longest output run of a wave (64 lanes x 4 outputs); nhp is a multiple of it
 constexpr int kDiaBW = 7;
 
@@ -1557,7 +1556,6 @@ int setup_dia(gade_model *m) {
             if (std::abs(m->rad_Ai[k] % nh - a % nh) > kDiaBW) return GADE_OK;
     m->nh = nh;
     m->nd = 2 * kDiaBW + 1;
-    m->nseg = (nh + 64 * kDiaP - 1) / (64 * kDiaP);
     m->nhp = (nh + kDiaSegMax - 1) / kDiaSegMax * kDiaSegMax;
     const int S = 4 * m->nd * m->nhp;
     std::vector<int> dia_src(S, -1);

@@ -118,7 +118,7 @@ struct gade_model {
     // banded (DIA) radial phase: quadrant/diagonal-major coefficients, tiles of 8 rows of one kappa
     bool dia_ok = false;           // radial pattern is a 2x2 block of bands of half-width <= 7
     int radial_kind = 0;           // 1: ELL kernel, 2: DIA kernel (active)
-    int nh = 0, nhp = 0, nd = 0, nseg = 0, ndtiles = 0;
+    int nh = 0, nhp = 0, nd = 0, ndtiles = 0;
     int *d_dia_src = nullptr;      // [4][nd][nhp] radial CSR index or -1
     double2 *d_gs = nullptr, *d_gts = nullptr;  // [lmax][3][4][4][nd][nhp] DIA-ordered couplings
     double2 *d_GD = nullptr;       // [B][lmax][4][4][nd][nhp][2] kappa-folded G-sum pairs
@@ -126,7 +126,7 @@ struct gade_model {
     int *d_dtiles = nullptr, *d_dtile_q = nullptr;
     double *d_dtile_k = nullptr;
     int *d_dtile_b = nullptr;      // instance of each tile (tiles never mix instances)
-    int dia_r = 8, dia_p = 2;      // banded tile shape: rows per tile, outputs per lane
+    int dia_r = 8, dia_p = 1;      // banded tile shape: rows per tile, outputs per lane
     int dia_hw = 1;                // output halves per banded workgroup (2: both halves share each window)
     // FP64 banded radial phase on the matrix cores: waves of two 8-row halves of one kappa each
     bool mfma_ok = false;          // the banded pattern fits the MFMA kernel (set up)
@@ -137,7 +137,7 @@ struct gade_model {
     double2 *d_DF = nullptr;       // [3][nq][4][nmi][5][16] diagonal operators in fragment order
     int *d_mg_rows = nullptr, *d_mg_q = nullptr, *d_mg_b = nullptr;  // [nmg][16], [nmg][2], [nmg]
     double *d_mg_k = nullptr;      // [nmg][2] kappa of each half
-    int ang_lb = 4;                // multipoles per angular block
+    int ang_lb = 2;                // multipoles per angular block
     int ang_fused = 0;             // 1: the banded radial kernel forms its V windows from x (no V in HBM)
     int psolve_ch = 16;            // preconditioner level height (16 or 32 rows)
     size_t gd_stride = 0;          // GD entries per instance

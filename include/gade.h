@@ -299,6 +299,17 @@ int gade_assembled_get(gade_model_t *m, int *Ap, int *Ai, int *Ai_sub_map, doubl
  *   "guess_order"  initial guess of each CN step: 0 psi_n (default, the
  *             reference), 1..3 polynomial extrapolation (gade_extrapolate),
  *             GADE_GUESS_LEARNED the learned guess;
+ *   "dia_rows", "dia_outputs"  banded tile shape: rows per tile and outputs per
+ *             lane, (8,1) by default; (4,1), (4,2), (4,4), (8,2), (16,1) also;
+ *   "dia_halves"  1 (default): one wave per output half of a tile; 2: both halves
+ *             in one two-wave workgroup sharing each staged window;
+ *   "ang_lb"  multipoles per angular block (2 by default; 4, 8);
+ *   "ang_list"  1 (default): angular phase over the coupling lists; 0: Hall rows;
+ *   "radial_mfma"  1: FP64 banded products on the matrix cores (0 by default);
+ *   "psolve_chunk"  preconditioner level height, 16 (default) or 32 rows;
+ *   "precision"  0 (default): FP64 solve; 1: FP32 inner solves with FP64 defect
+ *             correction (gade_cn_run / gade_cn_run_batch);
+ *   "graph"  1 (default): device-driven windows replay a captured step graph;
  *   "ang_fused"  1: the banded radial kernel forms its windows of the angular
  *             phase from x and the coupling lists itself (no angular kernel, no V in
  *             HBM; tile shapes with up to 2 outputs per lane); 0 (default): the
