@@ -1413,7 +1413,9 @@ int launch_dia(gade_model *m, int lgrid, const T *x, const T *V, const T *GD, co
     const int P = m->dia_p;
     const int nseg = (m->nh + 64 * P - 1) / (64 * P);
     const bool fused = ang_in_radial<T>(m);
-    const int hw = m->dia_hw;
+    // the paired (DUAL) two-kernel form runs one wave per output half whatever the model's choice: with both
+    // halves in one workgroup it measured 30-40 % slower at every shape (its two accumulator sets per wave)
+    const int hw = DUAL && !fused ? 1 : m->dia_hw;
     const dim3 grid((unsigned)m->ndtiles * (2 / hw) * nseg * lgrid);
     const T *acf = sizeof(T) == sizeof(float2) ? (const T *)m->d_acf32 : (const T *)m->d_acf;
     if (fused && !fused_shape_ok(m->dia_r, P)) {
