@@ -13,6 +13,8 @@ Bars:
   * a batched scan of two pulses: each instance equals its own single-instance run bit for bit
   * 20 mid-pulse steps: the step graph equals the launch-by-launch run bit for bit; the extrapolated guess
     lands within 1e-10 of it and mixed precision within 1e-8 (every step at rel-tol 1e-12)
+  * the fused angular + radial form (ang_fused): its applies match the oracle to 1e-13 and the two-kernel
+    form bit for bit, and its 20-step graph run equals the two-kernel run bit for bit
 """
 import sys
 
@@ -76,6 +78,17 @@ def test_fullsize_apply_and_linearity(full):
     dz = gade.DeviceArray.from_host(a * x + b * y)
     gp.apply(dz, dy, cH=1.0)
     assert relmax(dy.get(), a * ax + b * ay) <= 1e-13
+    # the fused angular + radial form: the same operator, bit for bit
+    two = {}
+    for fused in (0, 1):
+        gp.set_param("ang_fused", fused)
+        for cH in (1.0, -1.0, 0.0):
+            gp.apply(dx, dy, cH=cH)
+            two[(fused, cH)] = dy.get()
+    gp.set_param("ang_fused", 0)
+    for cH, want in ((1.0, S + H), (-1.0, S - H), (0.0, S)):
+        assert relmax(two[(1, cH)], want) <= 1e-13, cH
+        assert np.array_equal(two[(1, cH)].view(np.float64), two[(0, cH)].view(np.float64)), cH
 
 
 def test_fullsize_psolve(full):
@@ -177,6 +190,9 @@ def test_fullsize_graph_guess_and_mixed(full):
     x_ref, it_ref = run(graph=0, window=4, guess_order=0, precision=0)
     x_g, it_g = run(graph=1)
     assert np.array_equal(x_g.view(np.float64), x_ref.view(np.float64)) and np.array_equal(it_g, it_ref)
+    x_f, it_f = run(ang_fused=1)
+    assert np.array_equal(x_f.view(np.float64), x_ref.view(np.float64)) and np.array_equal(it_f, it_ref)
+    gm.set_param("ang_fused", 0)
     nref = np.linalg.norm(x_ref)
     x_e, it_e = run(graph=0, guess_order=2)
     err_e = np.linalg.norm(x_e - x_ref) / nref
