@@ -1395,8 +1395,6 @@ bool dia_shape_ok(int r, int p) {
     return (r == 4 && (p == 1 || p == 2 || p == 4)) || (r == 8 && (p == 1 || p == 2)) || (r == 16 && p == 1);
 }
 
-// the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
-// each output in the same order, so all give the same results bit for bit
 // phase A inside the banded radial kernel (its windows formed from x and the coupling lists, no V): the
 // FP64 form when the MFMA kernel is off, the FP32 form of the mixed solve always (it has no MFMA kernel)
 template <class T>
@@ -1407,6 +1405,8 @@ bool ang_in_radial(const gade_model *m) {
 // tile shapes with a fused form: one row's window sums per lane fit the registers for up to 2 outputs per lane
 bool fused_shape_ok(int r, int p) { return dia_shape_ok(r, p) && p <= 2; }
 
+// the banded radial kernel with the model's tile shape (rows per tile, outputs per lane); every shape sums
+// each output in the same order, so all give the same results bit for bit
 template <bool DUAL, class T>
 int launch_dia(gade_model *m, int lgrid, const T *x, const T *V, const T *GD, const T *DD, int sel, double cH,
                double beta, T *y, T *yp, T *y2, T *yp2, int sel2) {
