@@ -236,61 +236,35 @@ __global__ __launch_bounds__(kAngThreads) void k_angular(const T *__restrict__ x
     }
 }
 
-// Phase A from the coupling lists.  Two list sets are built once at load, both per instance-local row, in
-// increasing column order, padded to a multiple of kAngListU entries with zero coefficients (on the list's
-// last column):
-//  * per (row, multipole): the columns with a non-zero H0l or H1l coefficient, each with both (h0 for Z0, Z2;
-//    h1 for W0, W1; either may be 0); the fused radial kernel forms its windows from these;
-//  * per (row, group of LB multipoles of one parity): the union of the members' lists, each entry with the
-//    (h0, h1) of every member; k_angular_list runs over these, loading each x value once for the group (the
-//    couplings of same-parity multipoles share most columns; those of opposite parity share none).
-// The sums start at +0 and round to nearest, so they never hold -0, and a zero-coefficient term (+0 or -0
-// added) leaves them unchanged: every V entry is the same sum of the same non-zero terms in the same order as
-// in k_angular, bit for bit, without testing coefficients.
+// Phase A from the coupling lists: per (row, multipole) only the columns with a non-zero H0l or H1l
+// coefficient, in increasing column order, each with both coefficients (h0 for Z0, Z2; h1 for W0, W1; either
+// may be 0), padded to a multiple of kAngListU entries with zero coefficients (on the list's last column).  The
+// sums start at +0 and round to nearest, so they never hold -0, and a zero-coefficient term (+0 or -0 added)
+// leaves them unchanged: every V entry is the same sum of the same non-zero terms in the same order as in
+// k_angular, bit for bit, without testing coefficients.  Each x value is loaded once for the four sums.  The
+// list entries are read at addresses uniform over the block.  A block's lists (its multipole group) are one
+// contiguous stream; the next group's x loads are issued before the current group is summed.  Lists are per
+// instance-local row; x is offset to the instance.
 constexpr int kAngListU = 4;
-
-// multipole groups of the angular kernel: per parity (even l first) its multipoles in increasing order, in
-// groups of LB; group g has members l0 + 2j, j < count
-inline int ang_groups(int lmax, int LB) { return ((lmax + 1) / 2 + LB - 1) / LB + (lmax / 2 + LB - 1) / LB; }
-__host__ __device__ inline int ang_group_l0(int lmax, int LB, int g) {
-    const int ge = ((lmax + 1) / 2 + LB - 1) / LB;
-    return g < ge ? 2 * LB * g : 1 + 2 * LB * (g - ge);
-}
-__host__ __device__ inline int ang_group_count(int lmax, int LB, int g) {
-    const int l0 = ang_group_l0(lmax, LB, g);
-    return min(LB, (lmax - l0 + 1) / 2);
-}
-inline void ang_group_members(int lmax, int LB, int g, int *ls, int *nl) {
-    const int l0 = ang_group_l0(lmax, LB, g);
-    *nl = ang_group_count(lmax, LB, g);
-    for (int j = 0; j < *nl; j++) ls[j] = l0 + 2 * j;
-}
-
-// the four sums of every member of the group over kAngListU entries (cf: [entry][LB][h0, h1])
-template <int LB, class T>
+template <class T>
 __device__ __forceinline__ void ang_list_group(const T (&xv)[kAngListU], const int2 *__restrict__ ck,
-                                               const T *__restrict__ cf, int nlb, T (&z0)[LB], T (&z2)[LB],
-                                               T (&w0)[LB], T (&w1)[LB]) {
+                                               const T *__restrict__ cf, T &z0, T &z2, T &w0, T &w1) {
     using Rt = typename cx<T>::R;
 #pragma unroll
     for (int u = 0; u < kAngListU; u++) {
+        const T h0 = cf[2 * u], h1 = cf[2 * u + 1];
         const T kx = cscale(xv[u], (Rt)ck[u].y);
-#pragma unroll
-        for (int j = 0; j < LB; j++) {
-            if (j >= nlb) break;
-            const T h0 = cf[(u * LB + j) * 2], h1 = cf[(u * LB + j) * 2 + 1];
-            z0[j] = cfma(h0, xv[u], z0[j]);
-            z2[j] = cfma(h0, kx, z2[j]);
-            w0[j] = cfma(h1, xv[u], w0[j]);
-            w1[j] = cfma(h1, kx, w1[j]);
-        }
+        z0 = cfma(h0, xv[u], z0);
+        z2 = cfma(h0, kx, z2);
+        w0 = cfma(h1, xv[u], w0);
+        w1 = cfma(h1, kx, w1);
     }
 }
 
 template <int LB, class T = double2>
-__global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restrict__ x, const int *__restrict__ goff,
-                                                              const int2 *__restrict__ gck,
-                                                              const T *__restrict__ gcf, int n, int lmax,
+__global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restrict__ x, const int *__restrict__ aoff,
+                                                              const int2 *__restrict__ ack,
+                                                              const T *__restrict__ acf, int n, int lmax,
                                                               int nblk, T *__restrict__ V, const int *gate,
                                                               int gstride, int nblk1) {
     // rows are the fastest block index: the blocks in flight together write different rows of the V planes
@@ -300,41 +274,55 @@ __global__ __launch_bounds__(kAngThreads) void k_angular_list(const T *__restric
     GADE_GATED_B(gate, gstride, inst);
     const int rl = r - inst * nblk1;
     const int a = blockIdx.y * blockDim.x + threadIdx.x;
-    const int G = gridDim.z, g = blockIdx.z;
-    const int l0 = ang_group_l0(lmax, LB, g), nlb = ang_group_count(lmax, LB, g);
+    const int l0 = blockIdx.z * LB;
+    const int nlb = min(LB, lmax - l0);
+    const int *off = aoff + (size_t)rl * lmax + l0;
     const T *xi = x + (size_t)inst * nblk1 * n;
     const int ac = a < n ? a : n - 1;
-    constexpr int U = kAngListU;
     T z0[LB], z2[LB], w0[LB], w1[LB];
 #pragma unroll
-    for (int j = 0; j < LB; j++) z0[j] = z2[j] = w0[j] = w1[j] = czero<T>();
-    const int E0 = goff[(size_t)rl * G + g], E1 = goff[(size_t)rl * G + g + 1];
+    for (int l = 0; l < LB; l++) z0[l] = z2[l] = w0[l] = w1[l] = czero<T>();
+    const int E0 = off[0], E1 = off[nlb];
     if (E1 > E0) {
-        // two groups of entries per pass so the two register sets alternate (no copies that wait on the loads)
-        T cur[U];
+        T cur[kAngListU];
 #pragma unroll
-        for (int u = 0; u < U; u++) cur[u] = xi[(size_t)gck[E0 + u].x * n + ac];
-        for (int e = E0; e < E1; e += 2 * U) {
-            T nxt[U];
+        for (int u = 0; u < kAngListU; u++) cur[u] = xi[(size_t)ack[E0 + u].x * n + ac];
+        int g = E0;
 #pragma unroll
-            for (int u = 0; u < U; u++) nxt[u] = xi[(size_t)gck[min(e + U + u, E1 - 1)].x * n + ac];
-            ang_list_group<LB>(cur, gck + e, gcf + (size_t)2 * LB * e, nlb, z0, z2, w0, w1);
-            if (e + U >= E1) break;
+        for (int li = 0; li < LB; li++) {
+            if (li >= nlb) break;
+            const int e = off[li + 1];
+            // two groups per pass so the two register sets alternate (no copies that wait on the loads)
+            for (; g < e; g += 2 * kAngListU) {
+                T nxt[kAngListU];
 #pragma unroll
-            for (int u = 0; u < U; u++) cur[u] = xi[(size_t)gck[min(e + 2 * U + u, E1 - 1)].x * n + ac];
-            ang_list_group<LB>(nxt, gck + e + U, gcf + (size_t)2 * LB * (e + U), nlb, z0, z2, w0, w1);
+                for (int u = 0; u < kAngListU; u++)
+                    nxt[u] = xi[(size_t)ack[min(g + kAngListU + u, E1 - 1)].x * n + ac];
+                ang_list_group(cur, ack + g, acf + 2 * g, z0[li], z2[li], w0[li], w1[li]);
+                if (g + kAngListU >= e) {  // odd number of groups: the loaded group opens the next list
+#pragma unroll
+                    for (int u = 0; u < kAngListU; u++) cur[u] = nxt[u];
+                    g += kAngListU;
+                    break;
+                }
+#pragma unroll
+                for (int u = 0; u < kAngListU; u++)
+                    cur[u] = xi[(size_t)ack[min(g + 2 * kAngListU + u, E1 - 1)].x * n + ac];
+                ang_list_group(nxt, ack + g + kAngListU, acf + 2 * (g + kAngListU), z0[li], z2[li], w0[li],
+                               w1[li]);
+            }
         }
     }
     if (a >= n) return;
     const size_t plane = (size_t)nblk * n;
 #pragma unroll
-    for (int j = 0; j < LB; j++) {
-        if (j >= nlb) break;
-        size_t base = (size_t)((l0 + 2 * j) * 4) * plane + (size_t)r * n + a;
-        V[base] = z0[j];
-        V[base + plane] = z2[j];
-        V[base + 2 * plane] = w0[j];
-        V[base + 3 * plane] = w1[j];
+    for (int l = 0; l < LB; l++) {
+        if (l >= nlb) break;
+        size_t base = (size_t)((l0 + l) * 4) * plane + (size_t)r * n + a;
+        V[base] = z0[l];
+        V[base + plane] = z2[l];
+        V[base + 2 * plane] = w0[l];
+        V[base + 3 * plane] = w1[l];
     }
 }
 
@@ -1320,9 +1308,6 @@ void free_all(gade_model *m) {
                     (void *)m->d_acf, (void *)m->d_acf32, (void *)m->d_GF, (void *)m->d_DF, (void *)m->d_mg_rows,
                     (void *)m->d_mg_q, (void *)m->d_mg_b, (void *)m->d_mg_k})
         if (p) hipFree(p);
-    for (int gi = 0; gi < 3; gi++)
-        for (void *p : {(void *)m->d_goff[gi], (void *)m->d_gck[gi], (void *)m->d_gcf[gi], (void *)m->d_gcf32[gi]})
-            if (p) hipFree(p);
     resolve_pending(m);
     for (hipEvent_t e : m->event_pool) (void)hipEventDestroy(e);
     m->event_pool.clear();
@@ -1336,16 +1321,13 @@ bool ang_lds_ok(const gade_model *m, int lb) {
 // the angular kernel with the model's multipole group size (2, 4 or 8 multipoles per block; each V entry is
 // summed over the row's couplings in the same order for every group size)
 template <class T>
-int launch_angular_t(gade_model *m, const T *x, const T *coef, T *V) {
+int launch_angular_t(gade_model *m, const T *x, const T *coef, const T *acf, T *V) {
     const int LB = m->ang_lb;
     dim3 grid(m->nblk, (m->n + kAngThreads - 1) / kAngThreads, (m->lmax + LB - 1) / LB);
     if (m->ang_list) {
-        const int gi = LB == 2 ? 0 : LB == 4 ? 1 : 2;
-        const T *gcf = sizeof(T) == sizeof(float2) ? (const T *)m->d_gcf32[gi] : (const T *)m->d_gcf[gi];
-        const dim3 gridl(m->nblk, grid.y, m->ang_ngroups[gi]);
-#define GADE_ANGL(LB_)                                                                                         \
-    hipLaunchKernelGGL((k_angular_list<LB_, T>), gridl, dim3(kAngThreads), 0, m->stream, x, m->d_goff[gi],       \
-                       m->d_gck[gi], gcf, m->n, m->lmax, m->nblk, V, m->gate, m->gstride, m->nblk1)
+#define GADE_ANGL(LB_)                                                                                        \
+    hipLaunchKernelGGL((k_angular_list<LB_, T>), grid, dim3(kAngThreads), 0, m->stream, x, m->d_aoff, m->d_ack, \
+                       acf, m->n, m->lmax, m->nblk, V, m->gate, m->gstride, m->nblk1)
         switch (LB) {
             case 2: GADE_ANGL(2); break;
             case 4: GADE_ANGL(4); break;
@@ -1376,7 +1358,7 @@ int launch_angular_t(gade_model *m, const T *x, const T *coef, T *V) {
 }
 
 int launch_angular(gade_model *m, const double2 *x) {
-    return launch_angular_t<double2>(m, x, m->d_coef, m->d_V);
+    return launch_angular_t<double2>(m, x, m->d_coef, m->d_acf, m->d_V);
 }
 
 template <class T>
@@ -1905,52 +1887,6 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
         ack.push_back(make_int2(0, 0));
         acf.assign(4, 0.0);
     }
-    // grouped lists of the angular kernel: groups of LB multipoles of one parity (their couplings share many
-    // columns, while multipoles of opposite parity share none), each x value loaded once for the group
-    std::vector<int> goff[3];
-    std::vector<int2> gck[3];
-    std::vector<double> gcf[3];
-    for (int gi = 0; gi < 3; gi++) {
-        const int LB = 2 << gi;
-        const int G = ang_groups(lmax, LB);
-        m->ang_ngroups[gi] = G;
-        goff[gi].assign((size_t)m->nblk * G + 1, 0);
-        for (int r = 0; r < m->nblk; r++)
-            for (int g = 0; g < G; g++) {
-                int ls[8], nl = 0;
-                ang_group_members(lmax, LB, g, ls, &nl);
-                goff[gi][(size_t)r * G + g] = (int)gck[gi].size();
-                const size_t first = gck[gi].size();
-                for (int p = m->hall_Ap[r]; p < m->hall_Ap[r + 1]; p++) {
-                    bool any = false;
-                    for (int j = 0; j < nl; j++) {
-                        const double *cp = &coef[((size_t)p * lmax + ls[j]) * 4];
-                        any = any || cp[0] != 0.0 || cp[1] != 0.0 || cp[2] != 0.0 || cp[3] != 0.0;
-                    }
-                    if (!any) continue;
-                    const int c = m->hall_Ai[p];
-                    gck[gi].push_back(make_int2(c, m->kap_blk[c]));
-                    for (int j = 0; j < LB; j++) {
-                        if (j < nl) {
-                            const double *cp = &coef[((size_t)p * lmax + ls[j]) * 4];
-                            gcf[gi].insert(gcf[gi].end(), cp, cp + 4);
-                        } else {
-                            gcf[gi].insert(gcf[gi].end(), 4, 0.0);
-                        }
-                    }
-                }
-                while ((gck[gi].size() - first) % kAngListU) {  // zero terms on the last column
-                    gck[gi].push_back(gck[gi].back());
-                    gcf[gi].insert(gcf[gi].end(), (size_t)4 * LB, 0.0);
-                }
-            }
-        goff[gi].back() = (int)gck[gi].size();
-        if (gck[gi].empty()) {
-            gck[gi].push_back(make_int2(0, 0));
-            gcf[gi].assign((size_t)4 * LB, 0.0);
-        }
-        m->gcf_n[gi] = gcf[gi].size() / 2;
-    }
     // B instances: the Hall structure, kappa maps and angular coefficients repeat with row offsets
     m->N1 = (long long)m->nblk1 * n;
     if (B > 1) {
@@ -2008,11 +1944,6 @@ int gade_model_create_batch(const gade_inputs_t *in, int B, double h, double dt,
         (rc = upload(&m->d_acf, (const double2 *)acf.data(), acf.size() / 2)) ||
         (rc = upload(&m->d_diag, diag.data(), diag.size())))
         return fail(rc);
-    for (int gi = 0; gi < 3; gi++)
-        if ((rc = upload(&m->d_goff[gi], goff[gi].data(), goff[gi].size())) ||
-            (rc = upload(&m->d_gck[gi], gck[gi].data(), gck[gi].size())) ||
-            (rc = upload(&m->d_gcf[gi], (const double2 *)gcf[gi].data(), gcf[gi].size() / 2)))
-            return fail(rc);
     if ((rc = hip_alloc((void **)&m->d_GE, sizeof(double2) * (size_t)lmax * 8 * ellw * n)) ||
         (rc = hip_alloc((void **)&m->d_D, sizeof(double2) * (size_t)3 * m->nq * ellw * n)) ||
         (rc = hip_alloc((void **)&m->d_V, sizeof(double2) * (size_t)lmax * 4 * m->N)) ||
@@ -2465,10 +2396,6 @@ int mp_setup(gade_model *m) {
     conv(m->d_DD, m->d_DD32, nDD);
     conv(m->d_coef, m->d_coef32, nco);
     if (nacf) conv(m->d_acf, m->d_acf32, nacf);
-    for (int gi = 0; gi < 3; gi++) {
-        GADE_TRY(hip_alloc((void **)&m->d_gcf32[gi], sizeof(float2) * m->gcf_n[gi]));
-        conv(m->d_gcf[gi], m->d_gcf32[gi], m->gcf_n[gi]);
-    }
     conv(m->pre->d_Linv, m->lu32[0], nlu);
     conv(m->pre->d_Uinv, m->lu32[1], nlu);
     conv(m->pre->d_Loff, m->lu32[2], nlu);
@@ -2483,7 +2410,7 @@ int mp_setup(gade_model *m) {
 int apply32_async(gade_model *m, const float2 *x, float2 *y) {
     if (!ang_in_radial<float2>(m)) {
         Prof pr(m, "angular_f32", GADE_PROF_ANGULAR);
-        GADE_TRY(launch_angular_t<float2>(m, x, (const float2 *)m->d_coef32, m->d_V32));
+        GADE_TRY(launch_angular_t<float2>(m, x, (const float2 *)m->d_coef32, (const float2 *)m->d_acf32, m->d_V32));
     }
     const int lgrid = m->lmax;
     float2 *yp = lgrid > 1 ? m->d_ypart32 : nullptr;

@@ -101,21 +101,13 @@ struct gade_model {
     // Hall and angular coefficients
     int *d_hAp = nullptr, *d_hAi = nullptr, *d_kap = nullptr, *d_blk_q = nullptr;
     double2 *d_coef = nullptr;     // [hnnz][lmax][2]
-    // the same couplings as lists (one instance's rows): per (row, multipole) the columns with a non-zero
-    // H0l or H1l coefficient in increasing column order; entries (column, kappa) and (H0l, H1l) values
-    std::vector<int> h_aoff;       // [nblk1 * lmax + 1]
+    // the same couplings as lists (one instance's rows): per (row, multipole, part 0 H0l / 1 H1l) the
+    // non-zero coefficients in increasing column order; entries (column, kappa) and values
+    std::vector<int> h_aoff;       // [nblk1 * lmax * 2 + 1]
     int *d_aoff = nullptr;
     int2 *d_ack = nullptr;
     double2 *d_acf = nullptr;
     float2 *d_acf32 = nullptr;     // FP32 copy (mixed precision)
-    // per (row, group of LB multipoles of one parity; LB = 2, 4, 8 at index 0, 1, 2): the union of the group's
-    // lists, each entry with the (H0l, H1l) values of every member (0 where it has none)
-    int ang_ngroups[3] = {0, 0, 0};
-    size_t gcf_n[3] = {0, 0, 0};   // complex values per grouped list set
-    int *d_goff[3] = {nullptr, nullptr, nullptr};
-    int2 *d_gck[3] = {nullptr, nullptr, nullptr};
-    double2 *d_gcf[3] = {nullptr, nullptr, nullptr};
-    float2 *d_gcf32[3] = {nullptr, nullptr, nullptr};
     int ang_list = 1;              // phase A from the coupling lists (1) or from the Hall rows (0)
     int *d_diag = nullptr;         // [hnnz] 1 when orig_row == orig_col
     // kappa-grouped row tiles for the radial phase
