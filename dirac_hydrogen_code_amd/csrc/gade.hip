@@ -2438,7 +2438,7 @@ int psolve32_async(gade_model *m, const float2 *b, float2 *x) {
                            lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
                            lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
     else
-        hipLaunchKernelGGL((k_psolve_q<2, float2, 32>), dim3(lu->norder), dim3(128), shq, m->stream, b, x,
+        hipLaunchKernelGGL((k_psolve_q<2, float2, 32, 2>), dim3(lu->norder), dim3(64), shq, m->stream, b, x,
                            lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
                            lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
     GADE_HIP(hipGetLastError());
@@ -3879,14 +3879,14 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
         return GADE_EINVAL;
     }
     if (quad) {
-        // one block per workgroup, four lanes per row
+        // one wave per block: 16-row levels with four lanes per row, 32-row levels with two
         const size_t shq = sizeof(double2) * ((size_t)lu->nch * lu->ch + lu->ch);
         if (lu->ch == 16)
             hipLaunchKernelGGL((k_psolve_q<4, double2, 16>), dim3(lu->norder), dim3(64), shq, stream, b, x,
                                lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff,
                                lu->n, lu->band, lu->nch, gate, gstride, nblk1);
         else
-            hipLaunchKernelGGL((k_psolve_q<2, double2, 32>), dim3(lu->norder), dim3(128), shq, stream, b, x,
+            hipLaunchKernelGGL((k_psolve_q<2, double2, 32, 2>), dim3(lu->norder), dim3(64), shq, stream, b, x,
                                lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff,
                                lu->n, lu->band, lu->nch, gate, gstride, nblk1);
         GADE_HIP(hipGetLastError());

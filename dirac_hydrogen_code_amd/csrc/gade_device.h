@@ -839,23 +839,33 @@ __device__ __forceinline__ T quad_sum(T v) {
     v.y += dpp_quad_swap(v.y, 1);
     return v;
 }
+// sum over the L (2 or 4) neighbouring lanes that share a row
+template <int L, class T>
+__device__ __forceinline__ T lanes_sum(T v) {
+    if constexpr (L == 4) return quad_sum(v);
+    v.x += dpp_quad_swap(v.x, 0);
+    v.y += dpp_quad_swap(v.y, 0);
+    return v;
+}
 
-// x = P^{-1} b with CH-row levels (CH = 16 or 32) and one block per workgroup of 4*CH lanes: lane = 4*row
-// + k, and each of the four lanes of a row takes a quarter of the band coupling (terms 4k..4k+3) and of the
-// level inverse (terms (CH/4)k ..), summed over the quad with DPP.  b is staged in LDS; the factors of level
-// ch+PF are loaded as soon as the register slot of level ch is free, so PF levels of loads are in flight.
-// Requires band <= 16 (coupling rows padded to CH, zero beyond band).
-template <int PF, class T = double2, int CH = 16>
-__global__ __launch_bounds__(4 * CH) void k_psolve_q(const T *__restrict__ b, T *__restrict__ xo,
+// x = P^{-1} b with CH-row levels (CH = 16 or 32) and one block per workgroup of LPR*CH lanes: lane = LPR*row
+// + k, and each of the LPR lanes of a row takes 1/LPR of the band coupling (terms (16/LPR)k..) and of the level
+// inverse (terms (CH/LPR)k ..), summed over the lanes of the row with DPP.  LPR 4: 16-row levels in one wave
+// or 32-row levels in two; LPR 2: 32-row levels in one wave (half the levels of 16-row ones, twice the terms
+// per lane).  b is staged in LDS; the factors of level ch+PF are loaded as soon as the register slot of level
+// ch is free, so PF levels of loads are in flight.  Requires band <= 16 (coupling rows padded to CH, zero
+// beyond band).
+template <int PF, class T = double2, int CH = 16, int LPR = 4>
+__global__ __launch_bounds__(LPR * CH) void k_psolve_q(const T *__restrict__ b, T *__restrict__ xo,
                                                      const int *__restrict__ order, const int *__restrict__ blk_q,
                                                      int ilv, const T *__restrict__ Linv, const T *__restrict__ Uinv,
                                                      const T *__restrict__ Loff, const T *__restrict__ Uoff, int n,
                                                      int band, int nch, const int *gate = nullptr, int gstride = 0,
                                                      int nblk1 = 1) {
-    constexpr int NT = 4 * CH, QI = CH / 4;  // lanes; inverse terms per lane
+    constexpr int NT = LPR * CH, QI = CH / LPR, CT = 16 / LPR;  // lanes; inverse and coupling terms per lane
     extern __shared__ __align__(16) char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
-    const int lane = threadIdx.x, i = lane >> 2, k = lane & 3;
+    const int lane = threadIdx.x, i = lane / LPR, k = lane % LPR;
     const int r = order[blockIdx.x];
     if (r < 0) return;  // padding slot (whole workgroup)
     GADE_GATED_B(gate, gstride, r / nblk1);
@@ -879,24 +889,24 @@ __global__ __launch_bounds__(4 * CH) void k_psolve_q(const T *__restrict__ b, T 
     }
     __syncthreads();
     const size_t cs = (size_t)CH * CH;
-    T fc[PF][4], fv[PF][QI];
-    // factors of level ch into slot s: coupling rows 4k..4k+3 and inverse columns QI*k.. of row i
-    auto load = [&](const T *Off, const T *Inv, int ch, T (&c)[4], T (&v)[QI]) {
-        const T *po = Off + ((size_t)q * nch + ch) * cs + (size_t)(4 * k) * CH + i;
+    T fc[PF][CT], fv[PF][QI];
+    // factors of level ch into slot s: coupling rows CT*k.. and inverse columns QI*k.. of row i
+    auto load = [&](const T *Off, const T *Inv, int ch, T (&c)[CT], T (&v)[QI]) {
+        const T *po = Off + ((size_t)q * nch + ch) * cs + (size_t)(CT * k) * CH + i;
         const T *pi = Inv + ((size_t)q * nch + ch) * cs + (size_t)(QI * k) * CH + i;
 #pragma unroll
-        for (int u = 0; u < 4; u++) c[u] = po[(size_t)u * CH];
+        for (int u = 0; u < CT; u++) c[u] = po[(size_t)u * CH];
 #pragma unroll
         for (int u = 0; u < QI; u++) v[u] = pi[(size_t)u * CH];
     };
     // one level: val = b_i - sum_t C[t][i] y[cbase + t];  y_i = sum_j Inv[j][i] val_j
-    auto level = [&](int ch, int cbase, bool couple, const T (&c)[4], const T (&v)[QI]) {
+    auto level = [&](int ch, int cbase, bool couple, const T (&c)[CT], const T (&v)[QI]) {
         const int gi = ch * CH + i;
         T s = czero<T>();
         if (couple) {
 #pragma unroll
-            for (int u = 0; u < 4; u++) s = cfma(c[u], yv[cbase + 4 * k + u], s);
-            s = quad_sum(s);
+            for (int u = 0; u < CT; u++) s = cfma(c[u], yv[cbase + CT * k + u], s);
+            s = lanes_sum<LPR>(s);
         }
         const T val = csub(yv[gi], s);
         if (k == 0) acc[i] = val;
@@ -904,7 +914,7 @@ __global__ __launch_bounds__(4 * CH) void k_psolve_q(const T *__restrict__ b, T 
         T y = czero<T>();
 #pragma unroll
         for (int u = 0; u < QI; u++) y = cfma(v[u], acc[QI * k + u], y);
-        y = quad_sum(y);
+        y = lanes_sum<LPR>(y);
         if (k == 0) yv[gi] = y;
         __syncthreads();
     };
