@@ -98,7 +98,14 @@ def test_fullsize_psolve(full):
     b = rng.standard_normal(gp.n) + 1j * rng.standard_normal(gp.n)
     db, dx = gade.DeviceArray.from_host(b), gade.DeviceArray(gp.n)
     gp.psolve(db, dx)
-    assert relmax(dx.get(), om.psolve(b)) <= 1e-11
+    want = om.psolve(b)
+    assert relmax(dx.get(), want) <= 1e-11
+    # 32-row levels (one wave per block, two lanes per row): the same solve to rounding
+    gp.set_param("psolve_chunk", 32)
+    dx32 = gade.DeviceArray(gp.n)
+    gp.psolve(db, dx32)
+    gp.set_param("psolve_chunk", 16)
+    assert relmax(dx32.get(), want) <= 1e-11
     gp.set_time(np.zeros(6, complex))  # with f = 0 the fused operator is exactly P
     dpx = gade.DeviceArray(gp.n)
     gp.apply(dx, dpx, cH=1.0)
