@@ -1302,7 +1302,7 @@ void free_all(gade_model *m) {
                     (void *)m->d_hist, (void *)m->d_gctl, (void *)m->d_nsteps, (void *)m->d_dtile_b,
                     (void *)m->d_ypart2, (void *)m->d_V32, (void *)m->d_ypart32, (void *)m->d_GD32,
                     (void *)m->d_DD32, (void *)m->d_coef32, (void *)m->lu32[0], (void *)m->lu32[1],
-                    (void *)m->lu32[2], (void *)m->lu32[3], (void *)m->mp_r, (void *)m->mp_r0, (void *)m->mp_p,
+                    (void *)m->lu32[2], (void *)m->lu32[3], (void *)m->tw32, (void *)m->mp_r, (void *)m->mp_r0, (void *)m->mp_p,
                     (void *)m->mp_v, (void *)m->mp_y, (void *)m->mp_z, (void *)m->mp_s, (void *)m->mp_t,
                     (void *)m->mp_d, (void *)m->d_mp_ax, (void *)m->d_sc32, (void *)m->d_aoff, (void *)m->d_ack,
                     (void *)m->d_acf, (void *)m->d_acf32, (void *)m->d_GF, (void *)m->d_DF, (void *)m->d_mg_rows,
@@ -2367,6 +2367,7 @@ int mp_setup(gade_model *m) {
         hipLaunchKernelGGL(k_to_f32, dim3(grid_for((long long)n, 256, 8192), 1), dim3(256), 0, m->stream,
                            (long long)n, (size_t)0, src, dst, (const int *)nullptr, 0);
     };
+    const size_t ntw = (size_t)m->nq * m->pre->nch * 4 * 256;
     if (m->mp_ready) {
         // the preconditioner was rebuilt: its FP32 copies only
         for (auto &f : m->lu32) GADE_TRY(hip_alloc((void **)&f, sizeof(float2) * nlu));
@@ -2374,6 +2375,10 @@ int mp_setup(gade_model *m) {
         conv(m->pre->d_Uinv, m->lu32[1], nlu);
         conv(m->pre->d_Loff, m->lu32[2], nlu);
         conv(m->pre->d_Uoff, m->lu32[3], nlu);
+        if (m->pre->d_TW) {
+            GADE_TRY(hip_alloc((void **)&m->tw32, sizeof(float2) * ntw));
+            conv(m->pre->d_TW, m->tw32, ntw);
+        }
         GADE_HIP(hipGetLastError());
         GADE_HIP(hipStreamSynchronize(m->stream));
         return GADE_OK;
@@ -2400,6 +2405,10 @@ int mp_setup(gade_model *m) {
     conv(m->pre->d_Uinv, m->lu32[1], nlu);
     conv(m->pre->d_Loff, m->lu32[2], nlu);
     conv(m->pre->d_Uoff, m->lu32[3], nlu);
+    if (m->pre->d_TW) {
+        GADE_TRY(hip_alloc((void **)&m->tw32, sizeof(float2) * ntw));
+        conv(m->pre->d_TW, m->tw32, ntw);
+    }
     GADE_HIP(hipGetLastError());
     GADE_HIP(hipStreamSynchronize(m->stream));
     m->mp_ready = true;
@@ -2432,15 +2441,18 @@ int apply32_async(gade_model *m, const float2 *x, float2 *y) {
 int psolve32_async(gade_model *m, const float2 *b, float2 *x) {
     Prof pr(m, "psolve_f32", GADE_PROF_PSOLVE);
     const gade_blocklu *lu = m->pre;
+    if (lu->d_TW) {
+        hipLaunchKernelGGL((k_psolve_tw<4, float2>), dim3(lu->norder), dim3(128), sizeof(float2) * (size_t)lu->nch * 16,
+                           m->stream, b, x, lu->d_order, lu->d_blk_q, lu->ilv, (const float2 *)m->tw32, lu->n,
+                           lu->nch, lu->tw_m, m->gate, m->gstride, m->nblk1);
+        GADE_HIP(hipGetLastError());
+        return GADE_OK;
+    }
+    // 32-row levels (mixed precision needs 16- or 32-row levels and a band of at most 16)
     const size_t shq = sizeof(float2) * ((size_t)lu->nch * lu->ch + lu->ch);
-    if (lu->ch == 16)
-        hipLaunchKernelGGL((k_psolve_q<4, float2, 16>), dim3(lu->norder), dim3(64), shq, m->stream, b, x,
-                           lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
-                           lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
-    else
-        hipLaunchKernelGGL((k_psolve_q<2, float2, 32, 2>), dim3(lu->norder), dim3(64), shq, m->stream, b, x,
-                           lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
-                           lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
+    hipLaunchKernelGGL((k_psolve_q<2, float2, 32, 2>), dim3(lu->norder), dim3(64), shq, m->stream, b, x,
+                       lu->d_order, lu->d_blk_q, lu->ilv, m->lu32[0], m->lu32[1], m->lu32[2], m->lu32[3], lu->n,
+                       lu->band, lu->nch, m->gate, m->gstride, m->nblk1);
     GADE_HIP(hipGetLastError());
     return GADE_OK;
 }
@@ -3469,6 +3481,8 @@ int gade_set_param(gade_model_t *m, const char *name, int value) {
                 if (f) (void)hipFree(f);
                 f = nullptr;
             }
+            if (m->tw32) (void)hipFree(m->tw32);
+            m->tw32 = nullptr;
             m->psolve_ch = value;
         }
         return GADE_OK;
@@ -3756,11 +3770,148 @@ namespace gade {
 
 void blocklu_free(gade_blocklu *lu) {
     if (!lu) return;
-    void *ptrs[] = {lu->d_AB, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->d_blk_q, lu->d_wblocks, lu->d_flag,
-                    lu->d_order};
+    void *ptrs[] = {lu->d_AB,      lu->d_Linv, lu->d_Uinv, lu->d_Loff,  lu->d_Uoff, lu->d_blk_q,
+                    lu->d_wblocks, lu->d_flag, lu->d_order, lu->d_TW};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete lu;
+}
+
+// Twisted (burn-at-both-ends) block factorization of each distinct block over 16-row levels c = 0..nch-1 of the
+// interleaved band (band <= 16, so level c couples only to c-1 and c+1): P = tridiag(A_c, D_c, B_c), rows
+// beyond n identity.  Top levels c < m: S_c = D_c - A_c S_{c-1}^-1 B_{c-1}; bottom levels c > m:
+// T_c = D_c - B_c T_{c+1}^-1 A_{c+1}; the meeting level m: M = D_m - A_m S_{m-1}^-1 B_{m-1} - B_m T_{m+1}^-1 A_{m+1}.
+// Per level four 16 x 16 matrices (column-major: entry (i, t) at t*16 + i):
+//   X_c = A_c S_{c-1}^-1 (0 < c <= m), B_c T_{c+1}^-1 (m < c < nch-1);   Y_c = S_c^-1, M^-1, T_c^-1;
+//   Z_c = S_c^-1 B_c (c < m), T_c^-1 A_c (c > m);   W_m = B_m T_{m+1}^-1 (zero elsewhere).
+// The solve (k_psolve_tw) runs g_c = b_c - X_c g_{c-1} down to m-1 and h_c = b_c - X_c h_{c+1} up to m+1, then
+// x_m = Y_m (b_m - X_m g_{m-1} - W_m h_{m+1}), then x_c = Y_c g_c - Z_c x_{c+1} (c < m) and
+// x_c = Y_c h_c - Z_c x_{c-1} (c > m).  Inverses by Gauss-Jordan with partial pivoting.
+int twisted_factors(const std::vector<double2> &AB, int nq, int n, int band, std::vector<double2> &out,
+                    int *mout) {
+    using cd = std::complex<double>;
+    constexpr int C = 16;
+    const int nch = (n + C - 1) / C, w = 2 * band + 1, m = (nch - 1) / 2;
+    struct Mat {
+        cd a[C][C];
+    };
+    auto zero = [] {
+        Mat r;
+        for (auto &row : r.a)
+            for (auto &v : row) v = 0.0;
+        return r;
+    };
+    auto mul = [&](const Mat &x, const Mat &y) {
+        Mat r = zero();
+        for (int i = 0; i < C; i++)
+            for (int k = 0; k < C; k++) {
+                const cd xv = x.a[i][k];
+                if (xv == 0.0) continue;
+                for (int j = 0; j < C; j++) r.a[i][j] += xv * y.a[k][j];
+            }
+        return r;
+    };
+    auto sub = [](Mat x, const Mat &y) {
+        for (int i = 0; i < C; i++)
+            for (int j = 0; j < C; j++) x.a[i][j] -= y.a[i][j];
+        return x;
+    };
+    bool singular = false;
+    auto inv = [&](const Mat &x) {
+        cd g[C][2 * C];
+        for (int i = 0; i < C; i++)
+            for (int j = 0; j < 2 * C; j++) g[i][j] = j < C ? x.a[i][j] : cd(i == j - C ? 1.0 : 0.0);
+        for (int k = 0; k < C; k++) {
+            int p = k;
+            for (int i = k + 1; i < C; i++)
+                if (std::abs(g[i][k]) > std::abs(g[p][k])) p = i;
+            if (g[p][k] == 0.0) {
+                singular = true;
+                return zero();
+            }
+            if (p != k)
+                for (int j = 0; j < 2 * C; j++) std::swap(g[p][j], g[k][j]);
+            const cd d = 1.0 / g[k][k];
+            for (int j = 0; j < 2 * C; j++) g[k][j] *= d;
+            for (int i = 0; i < C; i++)
+                if (i != k && g[i][k] != 0.0) {
+                    const cd f = g[i][k];
+                    for (int j = 0; j < 2 * C; j++) g[i][j] -= f * g[k][j];
+                }
+        }
+        Mat r;
+        for (int i = 0; i < C; i++)
+            for (int j = 0; j < C; j++) r.a[i][j] = g[i][C + j];
+        return r;
+    };
+    out.assign((size_t)nq * nch * 4 * C * C, make_double2(0, 0));
+    std::vector<Mat> D(nch), A(nch), B(nch), Si(nch), Ti(nch);
+    for (int q = 0; q < nq; q++) {
+        // entry (gi, gj) of block q, identity beyond n
+        auto at = [&](int gi, int gj) -> cd {
+            if (gi >= n || gj >= n) return cd(gi == gj ? 1.0 : 0.0);
+            if (gi < 0 || gj < 0 || std::abs(gi - gj) > band) return 0.0;
+            const double2 v = AB[((size_t)q * n + gi) * w + (gj - gi + band)];
+            return cd(v.x, v.y);
+        };
+        for (int c = 0; c < nch; c++)
+            for (int i = 0; i < C; i++)
+                for (int j = 0; j < C; j++) {
+                    D[c].a[i][j] = at(c * C + i, c * C + j);
+                    A[c].a[i][j] = c > 0 ? at(c * C + i, (c - 1) * C + j) : 0.0;
+                    B[c].a[i][j] = c + 1 < nch ? at(c * C + i, (c + 1) * C + j) : 0.0;
+                }
+        auto put = [&](int c, int slot, const Mat &x) {
+            double2 *o = out.data() + (((size_t)q * nch + c) * 4 + slot) * C * C;
+            for (int i = 0; i < C; i++)
+                for (int t = 0; t < C; t++) o[t * C + i] = make_double2(x.a[i][t].real(), x.a[i][t].imag());
+        };
+        for (int c = 0; c < m; c++) {
+            if (c > 0) {
+                const Mat X = mul(A[c], Si[c - 1]);
+                put(c, 0, X);
+                Si[c] = inv(sub(D[c], mul(X, B[c - 1])));
+            } else {
+                Si[c] = inv(D[c]);
+            }
+        }
+        for (int c = nch - 1; c > m; c--) {
+            if (c < nch - 1) {
+                const Mat X = mul(B[c], Ti[c + 1]);
+                put(c, 0, X);
+                Ti[c] = inv(sub(D[c], mul(X, A[c + 1])));
+            } else {
+                Ti[c] = inv(D[c]);
+            }
+        }
+        Mat M = D[m];
+        if (m >= 1) {
+            const Mat X = mul(A[m], Si[m - 1]);
+            put(m, 0, X);
+            M = sub(M, mul(X, B[m - 1]));
+        }
+        if (m + 1 < nch) {
+            const Mat Wm = mul(B[m], Ti[m + 1]);
+            put(m, 3, Wm);
+            M = sub(M, mul(Wm, A[m + 1]));
+        }
+        put(m, 1, inv(M));
+        for (int c = 0; c < nch; c++) {
+            if (c < m) {
+                put(c, 1, Si[c]);
+                put(c, 2, mul(Si[c], B[c]));
+            } else if (c > m) {
+                put(c, 1, Ti[c]);
+                put(c, 2, mul(Ti[c], A[c]));
+            }
+        }
+        if (singular) {
+            gade::set_error("preconditioner: singular level block in the twisted factorization");
+            return GADE_EPIVOT;
+        }
+    }
+    *mout = m;
+    return GADE_OK;
 }
 
 int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const int *blk_q,
@@ -3801,6 +3952,11 @@ int blocklu_build(int nblk, int n, const int *rAp, const int *rAi, int nq, const
             }
     int rc;
     if ((rc = upload(&lu->d_AB, AB.data(), AB.size()))) return fail(rc);
+    if (CH == 16 && lu->band <= 16) {
+        std::vector<double2> tw;
+        if ((rc = twisted_factors(AB, nq, n, lu->band, tw, &lu->tw_m))) return fail(rc);
+        if ((rc = upload(&lu->d_TW, tw.data(), tw.size()))) return fail(rc);
+    }
     const size_t cs = (size_t)CH * CH;
     if ((rc = hip_alloc((void **)&lu->d_Linv, sizeof(double2) * nq * lu->nch * cs)) ||
         (rc = hip_alloc((void **)&lu->d_Uinv, sizeof(double2) * nq * lu->nch * cs)) ||
@@ -3873,22 +4029,26 @@ int blocklu_solve_async(const gade_blocklu *lu, const double2 *b, double2 *x, hi
         gade::set_error("preconditioner not set up");
         return GADE_EINVAL;
     }
+    if (lu->d_TW) {
+        // twisted factorization: one block per workgroup of two waves, meeting at level tw_m
+        const size_t sh = sizeof(double2) * (size_t)lu->nch * 16;
+        hipLaunchKernelGGL((k_psolve_tw<4, double2>), dim3(lu->norder), dim3(128), sh, stream, b, x, lu->d_order,
+                           lu->d_blk_q, lu->ilv, lu->d_TW, lu->n, lu->nch, lu->tw_m, gate, gstride, nblk1);
+        GADE_HIP(hipGetLastError());
+        return GADE_OK;
+    }
     const bool quad = (lu->ch == 16 || lu->ch == 32) && lu->band <= 16;
     if (!quad && gstride != 0) {
         gade::set_error("gated (batched) preconditioner solves need 16- or 32-row levels and a band of at most 16");
         return GADE_EINVAL;
     }
     if (quad) {
-        // one wave per block: 16-row levels with four lanes per row, 32-row levels with two
+        // 32-row levels (16-row ones with this band have the twisted factors above): one wave per block, two
+        // lanes per row
         const size_t shq = sizeof(double2) * ((size_t)lu->nch * lu->ch + lu->ch);
-        if (lu->ch == 16)
-            hipLaunchKernelGGL((k_psolve_q<4, double2, 16>), dim3(lu->norder), dim3(64), shq, stream, b, x,
-                               lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff,
-                               lu->n, lu->band, lu->nch, gate, gstride, nblk1);
-        else
-            hipLaunchKernelGGL((k_psolve_q<2, double2, 32, 2>), dim3(lu->norder), dim3(64), shq, stream, b, x,
-                               lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff,
-                               lu->n, lu->band, lu->nch, gate, gstride, nblk1);
+        hipLaunchKernelGGL((k_psolve_q<2, double2, 32, 2>), dim3(lu->norder), dim3(64), shq, stream, b, x,
+                           lu->d_order, lu->d_blk_q, lu->ilv, lu->d_Linv, lu->d_Uinv, lu->d_Loff, lu->d_Uoff, lu->n,
+                           lu->band, lu->nch, gate, gstride, nblk1);
         GADE_HIP(hipGetLastError());
         return GADE_OK;
     }

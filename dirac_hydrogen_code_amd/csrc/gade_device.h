@@ -852,9 +852,9 @@ __device__ __forceinline__ T lanes_sum(T v) {
 // + k, and each of the LPR lanes of a row takes 1/LPR of the band coupling (terms (16/LPR)k..) and of the level
 // inverse (terms (CH/LPR)k ..), summed over the lanes of the row with DPP.  LPR 4: 16-row levels in one wave
 // or 32-row levels in two; LPR 2: 32-row levels in one wave (half the levels of 16-row ones, twice the terms
-// per lane).  b is staged in LDS; the factors of level ch+PF are loaded as soon as the register slot of level
-// ch is free, so PF levels of loads are in flight.  Requires band <= 16 (coupling rows padded to CH, zero
-// beyond band).
+// per lane), the form launched for 32-row levels (16-row ones use the twisted form, k_psolve_tw).  b is
+// staged in LDS; the factors of level ch+PF are loaded as soon as the register slot of level ch is free, so
+// PF levels of loads are in flight.  Requires band <= 16 (coupling rows padded to CH, zero beyond band).
 template <int PF, class T = double2, int CH = 16, int LPR = 4>
 __global__ __launch_bounds__(LPR * CH) void k_psolve_q(const T *__restrict__ b, T *__restrict__ xo,
                                                      const int *__restrict__ order, const int *__restrict__ blk_q,
@@ -945,6 +945,133 @@ __global__ __launch_bounds__(LPR * CH) void k_psolve_q(const T *__restrict__ b, 
         }
     }
     for (int c = lane; c < n; c += NT) xo[(size_t)r * n + natural_index(c, n, ilv)] = yv[c];
+}
+
+// LDS exchange inside one wave: a wave's LDS operations complete in issue order, so ordering the compiler's
+// memory operations is all that is needed between a lane's write and another lane's read
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// x = P^{-1} b from the twisted factorization over 16-row levels (twisted_factors): one block per workgroup of
+// two waves.  Wave 0 runs the top levels (g_c = b_c - X_c g_{c-1}, c = 1..m-1), wave 1 the bottom ones
+// (h_c = b_c - X_c h_{c+1}, c = nch-2..m+1) at the same time; after a barrier wave 0 solves the meeting level
+// x_m = Y_m (b_m - X_m g_{m-1} - W_m h_{m+1}); after another, both run their back sweeps outwards
+// (x_c = Y_c g_c - Z_c x_{c+1} above m, x_c = Y_c h_c - Z_c x_{c-1} below).  So each wave's dependency chain is
+// about half the levels of one sweep pair, with no workgroup barrier inside a sweep.  Per level, lane 4i + k of
+// a wave takes terms 4k..4k+3 of row i, summed over the quad with DPP; b, g/h and x share one LDS array in
+// place (a level reads its neighbour's values and then writes its own rows).  The factors of the level PF
+// steps ahead load as soon as a register slot is free; the meeting level's factors load at the start.
+template <int PF, class T = double2>
+__global__ __launch_bounds__(128) void k_psolve_tw(const T *__restrict__ b, T *__restrict__ xo,
+                                                   const int *__restrict__ order, const int *__restrict__ blk_q,
+                                                   int ilv, const T *__restrict__ F, int n, int nch, int m,
+                                                   const int *gate = nullptr, int gstride = 0, int nblk1 = 1) {
+    constexpr int CH = 16, MS = CH * CH;
+    extern __shared__ __align__(16) char smem_raw[];
+    T *v = reinterpret_cast<T *>(smem_raw);  // [nch*16], interleaved order
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, i = lane >> 2, k = lane & 3;
+    const int r = order[blockIdx.x];
+    if (r < 0) return;  // padding slot (whole workgroup)
+    GADE_GATED_B(gate, gstride, r / nblk1);
+    const int q = blk_q[r];
+    const int ntot = nch * CH;
+    // terms 4k..4k+3 of row i of matrix j of level c
+    const T *Fq = F + (size_t)q * nch * 4 * MS + (size_t)(4 * k) * CH + i;
+    auto ld4 = [&](int c, int j, T (&f)[4]) {
+        const T *p = Fq + ((size_t)c * 4 + j) * MS;
+#pragma unroll
+        for (int u = 0; u < 4; u++) f[u] = p[(size_t)u * CH];
+    };
+    // this wave's levels: nf forward steps, then nb back steps
+    const int nf = wave == 0 ? max(m - 1, 0) : max(nch - 2 - m, 0);
+    const int nb = wave == 0 ? m : nch - 1 - m;
+    const int dir = wave == 0 ? -1 : 1;  // neighbour of a forward level (the back sweep uses -dir)
+    auto fwd_level = [&](int s) { return wave == 0 ? s + 1 : nch - 2 - s; };
+    auto back_level = [&](int s) { return wave == 0 ? m - 1 - s : m + 1 + s; };
+    T fm[3][4];  // meeting level: X_m, W_m, Y_m (wave 0)
+    if (wave == 0) {
+        ld4(m, 0, fm[0]);
+        ld4(m, 3, fm[1]);
+        ld4(m, 1, fm[2]);
+    }
+    T ff[PF][4];
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (u < nf) ld4(fwd_level(u), 0, ff[u]);
+    const T *bb = b + (size_t)r * n;
+    for (int c = threadIdx.x; c < ntot; c += 128) v[c] = c < n ? bb[natural_index(c, n, ilv)] : czero<T>();
+    __syncthreads();
+    // forward: v_c -= X_c v_{c+dir}
+    for (int s0 = 0; s0 < nf; s0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int s = s0 + u;
+            if (s >= nf) break;
+            const int c = fwd_level(s), nbr = (c + dir) * CH;
+            T t = czero<T>();
+#pragma unroll
+            for (int e = 0; e < 4; e++) t = cfma(ff[u][e], v[nbr + 4 * k + e], t);
+            t = quad_sum(t);
+            if (k == 0) v[c * CH + i] = csub(v[c * CH + i], t);
+            wave_lds_sync();
+            if (s + PF < nf) ld4(fwd_level(s + PF), 0, ff[u]);
+        }
+    }
+    T fb[PF][2][4];  // back steps: Y_c, Z_c
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (u < nb) {
+            ld4(back_level(u), 1, fb[u][0]);
+            ld4(back_level(u), 2, fb[u][1]);
+        }
+    __syncthreads();
+    if (wave == 0) {
+        // meeting level: v_m = Y_m (v_m - X_m v_{m-1} - W_m v_{m+1})
+        T t = czero<T>();
+        if (m >= 1) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) t = cfma(fm[0][e], v[(m - 1) * CH + 4 * k + e], t);
+        }
+        if (m + 1 < nch) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) t = cfma(fm[1][e], v[(m + 1) * CH + 4 * k + e], t);
+        }
+        t = quad_sum(t);
+        if (k == 0) v[m * CH + i] = csub(v[m * CH + i], t);
+        wave_lds_sync();
+        T y = czero<T>();
+#pragma unroll
+        for (int e = 0; e < 4; e++) y = cfma(fm[2][e], v[m * CH + 4 * k + e], y);
+        y = quad_sum(y);
+        if (k == 0) v[m * CH + i] = y;
+    }
+    __syncthreads();
+    // back: v_c = Y_c v_c - Z_c v_{c-dir}
+    for (int s0 = 0; s0 < nb; s0 += PF) {
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int s = s0 + u;
+            if (s >= nb) break;
+            const int c = back_level(s), nbr = (c - dir) * CH;
+            T t = czero<T>();
+#pragma unroll
+            for (int e = 0; e < 4; e++) t = cfma(fb[u][0][e], v[c * CH + 4 * k + e], t);
+#pragma unroll
+            for (int e = 0; e < 4; e++) t = cfma(cx<T>::mk(-fb[u][1][e].x, -fb[u][1][e].y), v[nbr + 4 * k + e], t);
+            t = quad_sum(t);
+            if (k == 0) v[c * CH + i] = t;
+            wave_lds_sync();
+            if (s + PF < nb) {
+                ld4(back_level(s + PF), 1, fb[u][0]);
+                ld4(back_level(s + PF), 2, fb[u][1]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < n; c += 128) xo[(size_t)r * n + natural_index(c, n, ilv)] = v[c];
 }
 
 // x = P^{-1} b: one wave serves 64/CH blocks (preferably of one kappa), lane =

@@ -54,6 +54,10 @@ struct gade_blocklu {
     int *d_blk_q = nullptr, *d_wblocks = nullptr, *d_flag = nullptr;
     int *d_order = nullptr;  // one block per wave, -1 pads; blocks of one factor on one XCD
     int norder = 0;
+    // twisted factorization over 16-row levels (k_psolve_tw): [q][level][X, Y, Z, W][16 x 16, column-major],
+    // meeting level tw_m; null when the levels are not 16 rows or the band exceeds 16
+    double2 *d_TW = nullptr;
+    int tw_m = 0;
 };
 
 struct gade_lg;  // learned initial guess (learned.hip)
@@ -202,6 +206,7 @@ struct gade_model {
     bool mp_ready = false;
     float2 *d_V32 = nullptr, *d_ypart32 = nullptr, *d_GD32 = nullptr, *d_DD32 = nullptr, *d_coef32 = nullptr;
     float2 *lu32[4] = {};          // Linv, Uinv, Loff, Uoff
+    float2 *tw32 = nullptr;        // FP32 copy of the twisted factors (when the preconditioner has them)
     float2 *mp_r = nullptr, *mp_r0 = nullptr, *mp_p = nullptr, *mp_v = nullptr, *mp_y = nullptr, *mp_z = nullptr,
            *mp_s = nullptr, *mp_t = nullptr, *mp_d = nullptr;
     double2 *d_mp_ax = nullptr;    // A x of the outer residual
