@@ -1001,8 +1001,21 @@ __global__ __launch_bounds__(128) void k_psolve_tw(const T *__restrict__ b, T *_
 #pragma unroll
     for (int u = 0; u < PF; u++)
         if (u < nf) ld4(fwd_level(u), 0, ff[u]);
+    // b: eight loads per thread in flight before the LDS writes
     const T *bb = b + (size_t)r * n;
-    for (int c = threadIdx.x; c < ntot; c += 128) v[c] = c < n ? bb[natural_index(c, n, ilv)] : czero<T>();
+    for (int c0 = 0; c0 < ntot; c0 += 128 * 8) {
+        T t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = c0 + 128 * u + (int)threadIdx.x;
+            t[u] = c < n ? bb[natural_index(c, n, ilv)] : czero<T>();
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = c0 + 128 * u + (int)threadIdx.x;
+            if (c < ntot) v[c] = t[u];
+        }
+    }
     __syncthreads();
     // forward: v_c -= X_c v_{c+dir}
     for (int s0 = 0; s0 < nf; s0 += PF) {
